@@ -1,0 +1,246 @@
+/*
+ * granite_hip.h -- C ABI of the MI355X trajectory engine (libgranite_hip.so).
+ *
+ * This is the boundary between the pygranite-compatible host layer (the `pygranite` Python
+ * module in pygranite_amd/, or any other C/C++/FFI caller) and the gfx950 HIP kernels. It
+ * carries plain pointers, sizes and POD descriptors only; no exceptions cross it. Every
+ * function returns 0 (GR_OK) on success or a negative GR_E* code; gr_last_error() returns the
+ * calling thread's last message.
+ *
+ * The reference (pygranite 1.5.0) has no C ABI: its boundary is the pybind11 module
+ * (source/pygranite.cpp:3-152) over one C++ class that owns CUDA textures and launches one
+ * kernel per time step. Each entry point below names the reference code it replaces.
+ *
+ * Ownership: device memory is owned by the context (fields, jobs). Host pointers are borrowed
+ * for the duration of the call. A context, and every field and job created from it, is used
+ * by one submitting thread at a time.
+ */
+#ifndef GRANITE_HIP_H
+#define GRANITE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_ABI_VERSION 1
+
+/* error codes */
+#define GR_OK 0
+#define GR_E_INVALID -1   /* bad argument / unsupported combination */
+#define GR_E_HIP -2       /* HIP runtime error */
+#define GR_E_NOMEM -3     /* allocation failed */
+#define GR_E_PARSE -4     /* expression did not parse / validate */
+#define GR_E_STATE -5     /* call out of order (e.g. capacity exceeded) */
+
+/* limits (source/granite/Constants.hpp:7-13, Language.hpp:75) */
+#define GR_MAX_VOLUMES 8
+#define GR_MAX_PROGRAMS 4
+#define GR_MAX_CONSTANTS 4
+#define GR_MAX_CODE 64
+#define GR_MAX_STACK 8
+#define GR_ABORT_VALUE 1e-9f
+
+/* storage types of field slices */
+typedef enum { GR_F32 = 0, GR_F16 = 1, GR_BF16 = 2 } gr_dtype;
+
+/* enum values match source/granite/IntegratorSettings.hpp:16-76 and AbortReason.hpp:8-15 */
+typedef enum { GR_EULER = 0, GR_RK4 = 1 } gr_integrator;
+typedef enum { GR_BLOCK = 0, GR_LOOP_X = 1, GR_LOOP_Y = 2, GR_LOOP_XY = 3 } gr_border;
+typedef enum {
+    GR_CURV_OFF = 0,
+    GR_CURV_TOTAL = 1,
+    GR_CURV_FAST_TOTAL = 2,
+    GR_CURV_INDIVIDUAL_AND_TOTAL = 3
+} gr_curvature;
+typedef enum { GR_ABORT_TIME = 0, GR_ABORT_LENGTH = 1, GR_ABORT_FIT_LENGTH = 2 } gr_abort_mode;
+typedef enum { GR_MODE_OFF = 0, GR_MODE_CONSTANT = 1, GR_MODE_DYNAMIC = 2 } gr_mode;
+typedef enum {
+    GR_STATUS_TIME = 0,
+    GR_STATUS_LENGTH = 1,
+    GR_STATUS_DOMAIN = 2,
+    GR_STATUS_TOPOGRAPHY = 3,
+    GR_STATUS_WIND = 4
+} gr_status;
+/* texture-weight emulation: exact FP32 weights, or weights quantised to 1/256 */
+typedef enum { GR_WEIGHTS_EXACT = 0, GR_WEIGHTS_Q8_ROUND = 1, GR_WEIGHTS_Q8_TRUNC = 2 } gr_weights;
+
+typedef struct gr_ctx gr_ctx;
+typedef struct gr_field gr_field;
+typedef struct gr_job gr_job;
+
+/* ---- library / context ---------------------------------------------------------------- */
+
+int gr_abi_version(void);
+/* message of the calling thread's last failed call ("" if none) */
+const char * gr_last_error(void);
+/* number of visible HIP devices (0 without a GPU; never fails) */
+int gr_device_count(int * count);
+
+/* Creates a context on `device`: one compute stream, one copy stream.
+ * Replaces the implicit device 0 + default stream of TrajectoryIntegrator.tpp:787-796. */
+int gr_ctx_create(int device, gr_ctx ** out);
+int gr_ctx_destroy(gr_ctx * ctx);
+int gr_ctx_synchronize(gr_ctx * ctx);
+/* device name and HBM bytes of the context's GPU */
+int gr_ctx_info(gr_ctx * ctx, char * name, size_t name_len, uint64_t * hbm_bytes,
+                int * compute_units);
+
+/* ---- fields (wind, additional volumes, topography) ---------------------------------------
+ * A field holds `nslots` resident time slices of an nx*ny*nz grid with `ncomp` components,
+ * stored node-interleaved in `dtype`. Replaces the CUDA texture objects and arrays of
+ * TrajectoryIntegrator.cpp:444-609 (loadTexture3D / loadTexture2D) and the back/front/cache
+ * rotation of TrajectoryIntegrator.tpp:894-902. Addressing: 0 = border (zero outside, the
+ * reference's 3D textures), 1 = clamp to edge (the reference's 2D textures). */
+typedef struct {
+    uint32_t nx, ny, nz; /* nz = 1 for 2D */
+    uint32_t ncomp;      /* 1..3 */
+    uint32_t dtype;      /* gr_dtype */
+    uint32_t nslots;     /* resident slices, >= 1 */
+    uint32_t address;    /* 0 border, 1 clamp */
+    uint32_t reserved;
+} gr_field_desc;
+
+int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_field ** out);
+int gr_field_destroy(gr_field * field);
+/* Uploads one slice: comps[c] points to nz*ny*nx host floats (x fastest). The host data is
+ * staged into pinned memory before return; the H2D copy and the on-device FP32 -> dtype
+ * conversion run on the copy stream. With async = 0 the call returns after the slot is
+ * resident. Kernels that read the slot wait for it on the device. */
+int gr_field_upload(gr_field * field, uint32_t slot, const float * const * comps, int async);
+/* Same from device memory (comps[c] are device pointers, FP32). */
+int gr_field_upload_device(gr_field * field, uint32_t slot, const float * const * dcomps);
+/* Blocks until the slot's last upload finished. */
+int gr_field_wait(gr_field * field, uint32_t slot);
+int gr_field_bytes(const gr_field * field, uint64_t * bytes_per_slot);
+
+/* ---- expressions -------------------------------------------------------------------------
+ * Compiles one AdditionalCompute program (grammar of TrajectoryIntegrator.cpp:10-26,
+ * identifier binding cpp:744-768, validation tpp:653-697) into postfix bytecode for the
+ * wavefront-uniform interpreter fused into the step kernel (replaces flatten cpp:770-798 and
+ * the recursive device interpret hpp:539-627). `names` are the loader's constant keys
+ * (bound to C0..C3) and volume keys (bound to F0..F7). */
+typedef struct {
+    uint32_t n;                  /* number of instructions */
+    uint32_t depth;              /* maximum stack depth */
+    uint32_t code[GR_MAX_CODE];  /* op | (arg << 8) */
+    float imm[GR_MAX_CODE];      /* immediates referenced by OP_CONST */
+} gr_program;
+
+#define GR_EXPR_ALLOW_ABS 1u /* accept abs(...) (extension) */
+typedef struct {
+    uint32_t dim;           /* 2 or 3: z and w are rejected in 2D */
+    uint32_t curvature;     /* cv needs GR_CURV_INDIVIDUAL_AND_TOTAL */
+    uint32_t nconst;        /* constant names, bound to C0.. */
+    uint32_t nvol;          /* volume names, bound to F0.. */
+    const char * const * const_names;
+    const char * const * vol_names;
+    uint32_t flags;
+} gr_expr_env;
+
+int gr_expr_compile(const char * src, const gr_expr_env * env, gr_program * out);
+/* CPU evaluation of compiled bytecode on a 19-slot reference vector (for checking). */
+int gr_expr_eval_host(const gr_program * prog, const float * ref19, float * out);
+
+/* ---- jobs ----------------------------------------------------------------------------------
+ * A job is one particle set advanced through windows of a field. It replaces compute_ /
+ * computeLoop_ / computeLoopGPU_ (TrajectoryIntegrator.tpp:467-995) and the SimulationData
+ * kernel argument (SimulationData.hpp:21-70). Particle state stays resident in HBM between
+ * calls. */
+typedef struct {
+    uint32_t dim;            /* 2 or 3 */
+    uint32_t integrator;     /* gr_integrator */
+    uint32_t border;         /* gr_border (2D only; 3D is always Block) */
+    uint32_t curvature;      /* gr_curvature */
+    uint32_t abort_mode;     /* gr_abort_mode */
+    uint32_t uplift_mode;    /* gr_mode (3D only) */
+    uint32_t windfield_mode; /* 0 Constant, 1 Dynamic */
+    uint32_t constants_mode; /* gr_mode */
+    uint32_t volume_mode;    /* gr_mode */
+    uint32_t reverse;        /* Reverse */
+    uint32_t weights;        /* gr_weights */
+    uint32_t time64;         /* accumulate simulation time in FP64 (precision form) */
+    uint32_t kahan;          /* Kahan-compensated positions (precision form) */
+    uint32_t local_coords;   /* keep positions relative to `offset` (FP64 geo-origin) */
+    uint32_t nvol;           /* additional volumes (<= 8) */
+    uint32_t nconst;         /* constants (<= 4) */
+    float dt;                /* DeltaT */
+    float dtd;               /* DataTimeDistance */
+    float sphere_radius;     /* > 0 enables sphere coordinates (2D) */
+    float max_length;        /* MaximumLength */
+    float grid_scale[3];     /* expanded GridScale (tpp:521-551) */
+    float domain[3];         /* grid * GridScale (tpp:526-546) */
+    double offset[3];        /* expanded Offset (tpp:553-574) */
+    uint64_t save_interval;  /* SaveInterval (0 = only last) */
+    uint32_t nprog;          /* number of programs */
+    uint32_t reserved;
+    gr_program prog[GR_MAX_PROGRAMS];
+} gr_job_desc;
+
+/* One window of the host schedule: the slots to read and the time stamps
+ * (TrajectoryIntegrator.tpp:933-969), and the steps to run (tpp:973-994). */
+typedef struct {
+    gr_field * wind;          /* required */
+    uint32_t wind_back, wind_front;
+    gr_field * volumes;       /* additional volumes as one field with nvol components, or NULL */
+    uint32_t vol_back, vol_front;
+    gr_field * topography;    /* 2D scalar field (clamp), or NULL */
+    uint32_t uplift_back, uplift_front;   /* particle-data slots */
+    uint32_t const_back, const_front;     /* particle-data slots */
+    float time_back, time_front;          /* COUNTER*DTD, (COUNTER+1)*DTD in FP32 */
+    double time_back64, time_front64;     /* same in FP64 (time64 jobs) */
+    float time_start;                     /* data.Time at the first step */
+    double time_start64;
+    uint64_t counter_start;               /* _simulation_counter at the first step */
+    uint32_t nsteps;                      /* steps to run */
+    uint32_t reserved;
+} gr_window;
+
+/* output kinds for gr_job_read / gr_job_read_saved */
+typedef enum {
+    GR_OUT_POSITIONS = 0,       /* N x dim float (global coordinates) */
+    GR_OUT_STATUS = 1,          /* N int32 (gr_status) */
+    GR_OUT_TOTAL_CURVATURE = 2, /* N float */
+    GR_OUT_LENGTHS = 3,         /* N float */
+    GR_OUT_CURVATURE = 4,       /* N float (per-step curvature) */
+    GR_OUT_ABORT_STEP = 5,      /* N int32: step counter at which the status left Time, or -1 */
+    GR_OUT_EXPR0 = 8,           /* + e: N float, latest value of program e */
+    GR_OUT_VOLUME0 = 16         /* + k: N float, latest value of volume k */
+} gr_output;
+
+/* Creates a job for `n` particles; `seeds` are n*dim host floats (AoS, the last row of the
+ * reference's TrajectorySet, cpp:424-442). `save_capacity` = saved columns held on the device
+ * between reads. */
+int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds, uint64_t n,
+                  uint32_t save_capacity, gr_job ** out);
+int gr_job_destroy(gr_job * job);
+/* Per-particle data with slots: kind 0 = uplift (n*dim floats), 1 + k = constant k
+ * (n floats). Replaces the uplift/constant buffers of TrajectoryIntegrator.cpp:195-245 and
+ * 305-359. */
+int gr_job_set_particle_data(gr_job * job, uint32_t kind, uint32_t slot, const float * host);
+/* Runs w->nsteps steps on the compute stream (asynchronously). Saved columns (steps whose
+ * counter % save_interval == 0) go to the device column ring. */
+int gr_job_advance(gr_job * job, const gr_window * w);
+/* Reads the current per-particle state (blocking). */
+int gr_job_read(gr_job * job, uint32_t kind, void * dst, uint64_t bytes);
+/* Copies saved columns [first, first + count) of `kind` (positions, curvature, expressions,
+ * volumes) to host memory laid out [count][N(][dim)] (blocking). Columns are numbered by
+ * save index since job creation; a column stays readable until `save_capacity` newer
+ * columns were written. */
+int gr_job_read_saved(gr_job * job, uint32_t kind, uint64_t first, uint64_t count, void * dst);
+/* Number of particles still at status Time (replaces count_if, ToolKernel.tpp:4-25). */
+int gr_job_count_alive(gr_job * job, uint64_t * alive);
+/* Number of saved columns written so far. */
+int gr_job_saved_columns(gr_job * job, uint64_t * cols);
+/* Device time of the last gr_job_advance, measured with HIP events on the compute stream. */
+int gr_job_last_advance_ms(gr_job * job, float * ms);
+/* Block size used for the job's step kernel (0 = automatic). */
+int gr_job_set_launch(gr_job * job, uint32_t block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRANITE_HIP_H */

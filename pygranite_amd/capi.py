@@ -1,0 +1,297 @@
+"""ctypes binding of the C ABI in include/granite_hip.h (libgranite_hip.so).
+
+This is the layer tests and the benchmark drive directly: it exposes fields (resident slices),
+jobs (resident particle state) and windows exactly as the C ABI defines them, without the
+pygranite host schedule on top. Structures mirror the header field by field.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libgranite_hip.so")
+
+MAX_CODE = 64
+MAX_PROGRAMS = 4
+F32, F16, BF16 = 0, 1, 2
+OUT_POSITIONS, OUT_STATUS, OUT_TOTAL_CURVATURE, OUT_LENGTHS = 0, 1, 2, 3
+OUT_CURVATURE, OUT_ABORT_STEP, OUT_EXPR0, OUT_VOLUME0 = 4, 5, 8, 16
+EXPR_ALLOW_ABS = 1
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = [
+    "gr_abi_version", "gr_last_error", "gr_device_count", "gr_ctx_create", "gr_ctx_destroy",
+    "gr_ctx_synchronize", "gr_ctx_info", "gr_field_create", "gr_field_destroy",
+    "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
+    "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
+    "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
+    "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
+]
+
+
+class FieldDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+                ("ncomp", ctypes.c_uint32), ("dtype", ctypes.c_uint32),
+                ("nslots", ctypes.c_uint32), ("address", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class Program(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("code", ctypes.c_uint32 * MAX_CODE), ("imm", ctypes.c_float * MAX_CODE)]
+
+
+class ExprEnv(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_uint32), ("curvature", ctypes.c_uint32),
+                ("nconst", ctypes.c_uint32), ("nvol", ctypes.c_uint32),
+                ("const_names", ctypes.POINTER(ctypes.c_char_p)),
+                ("vol_names", ctypes.POINTER(ctypes.c_char_p)), ("flags", ctypes.c_uint32)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_uint32), ("integrator", ctypes.c_uint32),
+                ("border", ctypes.c_uint32), ("curvature", ctypes.c_uint32),
+                ("abort_mode", ctypes.c_uint32), ("uplift_mode", ctypes.c_uint32),
+                ("windfield_mode", ctypes.c_uint32), ("constants_mode", ctypes.c_uint32),
+                ("volume_mode", ctypes.c_uint32), ("reverse", ctypes.c_uint32),
+                ("weights", ctypes.c_uint32), ("time64", ctypes.c_uint32),
+                ("kahan", ctypes.c_uint32), ("local_coords", ctypes.c_uint32),
+                ("nvol", ctypes.c_uint32), ("nconst", ctypes.c_uint32),
+                ("dt", ctypes.c_float), ("dtd", ctypes.c_float),
+                ("sphere_radius", ctypes.c_float), ("max_length", ctypes.c_float),
+                ("grid_scale", ctypes.c_float * 3), ("domain", ctypes.c_float * 3),
+                ("offset", ctypes.c_double * 3), ("save_interval", ctypes.c_uint64),
+                ("nprog", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("prog", Program * MAX_PROGRAMS)]
+
+
+class Window(ctypes.Structure):
+    _fields_ = [("wind", ctypes.c_void_p), ("wind_back", ctypes.c_uint32),
+                ("wind_front", ctypes.c_uint32), ("volumes", ctypes.c_void_p),
+                ("vol_back", ctypes.c_uint32), ("vol_front", ctypes.c_uint32),
+                ("topography", ctypes.c_void_p), ("uplift_back", ctypes.c_uint32),
+                ("uplift_front", ctypes.c_uint32), ("const_back", ctypes.c_uint32),
+                ("const_front", ctypes.c_uint32), ("time_back", ctypes.c_float),
+                ("time_front", ctypes.c_float), ("time_back64", ctypes.c_double),
+                ("time_front64", ctypes.c_double), ("time_start", ctypes.c_float),
+                ("time_start64", ctypes.c_double), ("counter_start", ctypes.c_uint64),
+                ("nsteps", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libgranite_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY_PATH):
+        raise ImportError(f"{LIBRARY_PATH} missing: run `python -m pygranite_amd.build`")
+    L = ctypes.CDLL(LIBRARY_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "gr_abi_version": ([], i32),
+        "gr_last_error": ([], ctypes.c_char_p),
+        "gr_device_count": ([P(i32)], i32),
+        "gr_ctx_create": ([i32, P(vp)], i32),
+        "gr_ctx_destroy": ([vp], i32),
+        "gr_ctx_synchronize": ([vp], i32),
+        "gr_ctx_info": ([vp, ctypes.c_char_p, ctypes.c_size_t, P(u64), P(i32)], i32),
+        "gr_field_create": ([vp, P(FieldDesc), P(vp)], i32),
+        "gr_field_destroy": ([vp], i32),
+        "gr_field_upload": ([vp, u32, P(vp), i32], i32),
+        "gr_field_upload_device": ([vp, u32, P(vp)], i32),
+        "gr_field_wait": ([vp, u32], i32),
+        "gr_field_bytes": ([vp, P(u64)], i32),
+        "gr_expr_compile": ([ctypes.c_char_p, P(ExprEnv), P(Program)], i32),
+        "gr_expr_eval_host": ([P(Program), P(ctypes.c_float), P(ctypes.c_float)], i32),
+        "gr_job_create": ([vp, P(JobDesc), vp, u64, u32, P(vp)], i32),
+        "gr_job_destroy": ([vp], i32),
+        "gr_job_set_particle_data": ([vp, u32, u32, vp], i32),
+        "gr_job_advance": ([vp, P(Window)], i32),
+        "gr_job_read": ([vp, u32, vp, u64], i32),
+        "gr_job_read_saved": ([vp, u32, u64, u64, vp], i32),
+        "gr_job_count_alive": ([vp, P(u64)], i32),
+        "gr_job_saved_columns": ([vp, P(u64)], i32),
+        "gr_job_last_advance_ms": ([vp, P(ctypes.c_float)], i32),
+        "gr_job_set_launch": ([vp, u32], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+class GraniteError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise GraniteError(f"granite error {rc}: {lib().gr_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().gr_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def compile_expr(src: str, dim: int = 3, curvature: int = 0, constants=(), volumes=(),
+                 allow_abs: bool = False) -> Program:
+    env = ExprEnv()
+    env.dim = dim
+    env.curvature = curvature
+    cn = (ctypes.c_char_p * max(1, len(constants)))(*[c.encode() for c in constants])
+    vn = (ctypes.c_char_p * max(1, len(volumes)))(*[v.encode() for v in volumes])
+    env.nconst = len(constants)
+    env.nvol = len(volumes)
+    env.const_names = cn
+    env.vol_names = vn
+    env.flags = EXPR_ALLOW_ABS if allow_abs else 0
+    prog = Program()
+    rc = lib().gr_expr_compile(src.encode(), ctypes.byref(env), ctypes.byref(prog))
+    if rc != 0:
+        raise GraniteError(lib().gr_last_error().decode())
+    return prog
+
+
+def eval_expr_host(prog: Program, ref) -> float:
+    r = (ctypes.c_float * 19)(*[float(x) for x in ref])
+    out = ctypes.c_float()
+    _check(lib().gr_expr_eval_host(ctypes.byref(prog), r, ctypes.byref(out)))
+    return out.value
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().gr_ctx_create(device, ctypes.byref(self.h)))
+
+    def info(self):
+        name = ctypes.create_string_buffer(256)
+        hbm = ctypes.c_uint64()
+        cus = ctypes.c_int()
+        _check(lib().gr_ctx_info(self.h, name, 256, ctypes.byref(hbm), ctypes.byref(cus)))
+        return {"name": name.value.decode(), "hbm_bytes": hbm.value, "compute_units": cus.value}
+
+    def synchronize(self):
+        _check(lib().gr_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().gr_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Field:
+    def __init__(self, ctx: Context, nx, ny, nz, ncomp, dtype=F32, nslots=3, address=0):
+        self.ctx = ctx
+        d = FieldDesc(nx, ny, nz, ncomp, dtype, nslots, address, 0)
+        self.desc = d
+        self.h = ctypes.c_void_p()
+        _check(lib().gr_field_create(ctx.h, ctypes.byref(d), ctypes.byref(self.h)))
+
+    def upload(self, slot: int, comps, async_: bool = False):
+        arrs = [np.ascontiguousarray(c, dtype=np.float32) for c in comps]
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        _check(lib().gr_field_upload(self.h, slot, ptrs, 1 if async_ else 0))
+        self._keep = arrs
+
+    def wait(self, slot: int):
+        _check(lib().gr_field_wait(self.h, slot))
+
+    def bytes_per_slot(self) -> int:
+        b = ctypes.c_uint64()
+        _check(lib().gr_field_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if self.h:
+            lib().gr_field_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Job:
+    def __init__(self, ctx: Context, desc: JobDesc, seeds: np.ndarray, save_capacity: int = 1):
+        self.ctx = ctx
+        self.desc = desc
+        self.dim = desc.dim
+        seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+        self.n = seeds.shape[0]
+        self.h = ctypes.c_void_p()
+        _check(lib().gr_job_create(ctx.h, ctypes.byref(desc), seeds.ctypes.data, self.n,
+                                   save_capacity, ctypes.byref(self.h)))
+
+    def set_launch(self, block: int):
+        _check(lib().gr_job_set_launch(self.h, block))
+
+    def set_particle_data(self, kind: int, slot: int, data: np.ndarray):
+        a = np.ascontiguousarray(data, dtype=np.float32)
+        _check(lib().gr_job_set_particle_data(self.h, kind, slot, a.ctypes.data))
+
+    def advance(self, w: Window):
+        _check(lib().gr_job_advance(self.h, ctypes.byref(w)))
+
+    def read(self, kind: int) -> np.ndarray:
+        if kind == OUT_POSITIONS:
+            out = np.empty((self.n, self.dim), np.float32)
+        elif kind in (OUT_STATUS, OUT_ABORT_STEP):
+            out = np.empty(self.n, np.int32)
+        else:
+            out = np.empty(self.n, np.float32)
+        _check(lib().gr_job_read(self.h, kind, out.ctypes.data, out.nbytes))
+        return out
+
+    def read_saved(self, kind: int, first: int, count: int) -> np.ndarray:
+        if kind == OUT_POSITIONS:
+            out = np.empty((count, self.n, self.dim), np.float32)
+        else:
+            out = np.empty((count, self.n), np.float32)
+        _check(lib().gr_job_read_saved(self.h, kind, first, count, out.ctypes.data))
+        return out
+
+    def saved_columns(self) -> int:
+        c = ctypes.c_uint64()
+        _check(lib().gr_job_saved_columns(self.h, ctypes.byref(c)))
+        return c.value
+
+    def count_alive(self) -> int:
+        c = ctypes.c_uint64()
+        _check(lib().gr_job_count_alive(self.h, ctypes.byref(c)))
+        return c.value
+
+    def last_advance_ms(self) -> float:
+        ms = ctypes.c_float()
+        _check(lib().gr_job_last_advance_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().gr_job_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

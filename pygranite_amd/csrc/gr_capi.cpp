@@ -1,0 +1,849 @@
+// Host engine behind the C ABI (include/granite_hip.h): contexts (device + streams), fields
+// (resident time slices in node-interleaved FP32/FP16/BF16), and jobs (resident particle state
+// advanced window by window by the gfx950 step kernel).
+//
+// Replaces the reference's host data plane and driver: texture/array management and
+// synchronous uploads (source/granite/TrajectoryIntegrator.cpp:146-360, 444-609), buffer
+// setup (TrajectoryIntegrator.tpp:467-715), the per-step launch + cudaDeviceSynchronize loop
+// (tpp:973-994) and the per-save D2H (tpp:997-1049). Uploads run on a copy stream and are
+// ordered against kernels with events, so the next slice streams in while a window computes.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gr_internal.h"
+
+namespace
+{
+thread_local std::string g_err;
+
+int set_err(int code, const char * fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define GR_HIP(call)                                                                              \
+    do                                                                                            \
+    {                                                                                             \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return set_err(GR_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                           __FILE__, __LINE__);                                                   \
+    } while (0)
+
+template <class T> int dalloc(T ** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return GR_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+    if (e != hipSuccess)
+    {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return set_err(GR_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return GR_OK;
+}
+
+void dfree(void * p)
+{
+    if (p) (void)hipFree(p);
+}
+
+size_t dtype_size(uint32_t t) { return t == GR_F32 ? 4 : 2; }
+} // namespace
+
+extern "C" void gr_set_error(const char * msg) { g_err = msg ? msg : ""; }
+
+struct gr_ctx
+{
+    int device{0};
+    hipStream_t compute{nullptr};
+    hipStream_t copy{nullptr};
+    unsigned long long * dcount{nullptr};
+};
+
+struct gr_field
+{
+    gr_ctx * ctx{nullptr};
+    gr_field_desc d{};
+    uint32_t cstride{0};
+    uint64_t nodes{0};
+    uint64_t slot_bytes{0};
+    std::vector<void *> slots;
+    float * dstage{nullptr};   // FP32 component-major staging on the device
+    float * hstage{nullptr};   // pinned host staging
+    hipEvent_t stage_done{nullptr};
+    bool stage_pending{false};
+    std::vector<hipEvent_t> ready, inuse;
+    std::vector<char> has_ready, has_inuse;
+};
+
+struct gr_job
+{
+    gr_ctx * ctx{nullptr};
+    gr_job_desc d{};
+    uint64_t n{0};
+    uint32_t full{0};
+    uint32_t block{0};
+    float * p[3]{};
+    float * q[3]{};
+    float * c[3]{};
+    float * ld[3]{};
+    int32_t * status{nullptr};
+    int32_t * abort_step{nullptr};
+    float * total{nullptr};
+    float * curv{nullptr};
+    float * len{nullptr};
+    float * expr[GR_MAX_PROGRAMS]{};
+    float * vol[GR_MAX_VOLUMES]{};
+    uint32_t cap{0};
+    float * ring_pos{nullptr};
+    float * ring_curv{nullptr};
+    float * ring_expr{nullptr};
+    float * ring_vol{nullptr};
+    float * scratch_aos{nullptr};
+    float * uplift[3]{};
+    float * cst[3][GR_MAX_CONSTANTS]{};
+    uint64_t saved{0};
+    hipEvent_t ev0{nullptr}, ev1{nullptr};
+    bool timed{false};
+};
+
+// --------------------------------------------------------------------------------------------
+// library / context
+// --------------------------------------------------------------------------------------------
+extern "C" int gr_abi_version(void) { return GR_ABI_VERSION; }
+
+extern "C" const char * gr_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gr_device_count(int * count)
+{
+    if (!count) return set_err(GR_E_INVALID, "gr_device_count: null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_create(int device, gr_ctx ** out)
+{
+    if (!out) return set_err(GR_E_INVALID, "gr_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    {
+        (void)hipGetLastError();
+        return set_err(GR_E_HIP, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return set_err(GR_E_INVALID, "device %d out of range", device);
+    GR_HIP(hipSetDevice(device));
+    gr_ctx * c = new (std::nothrow) gr_ctx;
+    if (!c) return set_err(GR_E_NOMEM, "gr_ctx_create: out of host memory");
+    c->device = device;
+    // default-priority streams only
+    if (hipStreamCreateWithFlags(&c->compute, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&c->dcount), sizeof(unsigned long long)) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        if (c->compute) (void)hipStreamDestroy(c->compute);
+        if (c->copy) (void)hipStreamDestroy(c->copy);
+        delete c;
+        return set_err(GR_E_HIP, "gr_ctx_create: stream / buffer creation failed");
+    }
+    *out = c;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_destroy(gr_ctx * c)
+{
+    if (!c) return GR_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->compute);
+    (void)hipStreamSynchronize(c->copy);
+    dfree(c->dcount);
+    (void)hipStreamDestroy(c->compute);
+    (void)hipStreamDestroy(c->copy);
+    delete c;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_synchronize(gr_ctx * c)
+{
+    if (!c) return set_err(GR_E_INVALID, "null ctx");
+    GR_HIP(hipSetDevice(c->device));
+    GR_HIP(hipStreamSynchronize(c->copy));
+    GR_HIP(hipStreamSynchronize(c->compute));
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_info(gr_ctx * c, char * name, size_t name_len, uint64_t * hbm_bytes,
+                           int * compute_units)
+{
+    if (!c) return set_err(GR_E_INVALID, "null ctx");
+    hipDeviceProp_t prop;
+    GR_HIP(hipGetDeviceProperties(&prop, c->device));
+    if (name && name_len)
+    {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return GR_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// fields
+// --------------------------------------------------------------------------------------------
+static void field_free(gr_field * f)
+{
+    if (!f) return;
+    for (void * s : f->slots) dfree(s);
+    dfree(f->dstage);
+    if (f->hstage) (void)hipHostFree(f->hstage);
+    if (f->stage_done) (void)hipEventDestroy(f->stage_done);
+    for (hipEvent_t e : f->ready)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : f->inuse)
+        if (e) (void)hipEventDestroy(e);
+    delete f;
+}
+
+extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_field ** out)
+{
+    if (!ctx || !desc || !out) return set_err(GR_E_INVALID, "gr_field_create: null argument");
+    *out = nullptr;
+    const gr_field_desc & d = *desc;
+    if (d.nx == 0 || d.ny == 0 || d.nz == 0 || d.ncomp == 0 || d.ncomp > GR_MAX_VOLUMES ||
+        d.nslots == 0 || d.nslots > 8 || d.dtype > GR_BF16 || d.address > 1)
+        return set_err(GR_E_INVALID, "gr_field_create: invalid descriptor");
+    const uint64_t nodes = uint64_t(d.nx) * d.ny * d.nz;
+    if (nodes >= (1ull << 32)) return set_err(GR_E_INVALID, "gr_field_create: grid too large");
+    GR_HIP(hipSetDevice(ctx->device));
+    gr_field * f = new (std::nothrow) gr_field;
+    if (!f) return set_err(GR_E_NOMEM, "gr_field_create: out of host memory");
+    f->ctx = ctx;
+    f->d = d;
+    f->cstride = d.ncomp == 3 ? 4 : d.ncomp;
+    f->nodes = nodes;
+    f->slot_bytes = nodes * f->cstride * dtype_size(d.dtype);
+    f->slots.assign(d.nslots, nullptr);
+    f->ready.assign(d.nslots, nullptr);
+    f->inuse.assign(d.nslots, nullptr);
+    f->has_ready.assign(d.nslots, 0);
+    f->has_inuse.assign(d.nslots, 0);
+    int rc = GR_OK;
+    for (uint32_t s = 0; s < d.nslots && rc == GR_OK; ++s)
+    {
+        rc = dalloc(&f->slots[s], f->slot_bytes);
+        if (rc == GR_OK && hipMemset(f->slots[s], 0, f->slot_bytes) != hipSuccess)
+            rc = set_err(GR_E_HIP, "gr_field_create: memset failed");
+        if (rc == GR_OK && (hipEventCreateWithFlags(&f->ready[s], hipEventDisableTiming) != hipSuccess ||
+                            hipEventCreateWithFlags(&f->inuse[s], hipEventDisableTiming) != hipSuccess))
+            rc = set_err(GR_E_HIP, "gr_field_create: event creation failed");
+    }
+    if (rc == GR_OK) rc = dalloc(&f->dstage, nodes * d.ncomp * sizeof(float));
+    if (rc == GR_OK && hipHostMalloc(reinterpret_cast<void **>(&f->hstage),
+                                     nodes * d.ncomp * sizeof(float), hipHostMallocDefault) != hipSuccess)
+    {
+        f->hstage = nullptr;
+        rc = set_err(GR_E_NOMEM, "gr_field_create: pinned staging allocation failed");
+    }
+    if (rc == GR_OK && hipEventCreateWithFlags(&f->stage_done, hipEventDisableTiming) != hipSuccess)
+        rc = set_err(GR_E_HIP, "gr_field_create: event creation failed");
+    if (rc != GR_OK)
+    {
+        (void)hipGetLastError();
+        field_free(f);
+        return rc;
+    }
+    *out = f;
+    return GR_OK;
+}
+
+extern "C" int gr_field_destroy(gr_field * f)
+{
+    if (!f) return GR_OK;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->copy);
+    (void)hipStreamSynchronize(f->ctx->compute);
+    field_free(f);
+    return GR_OK;
+}
+
+static int field_pack(gr_field * f, uint32_t slot)
+{
+    gr_ctx * c = f->ctx;
+    if (gr::launch_pack(f->dstage, f->slots[slot], f->nodes, f->d.ncomp, f->cstride, f->d.dtype,
+                        c->copy) != 0)
+        return set_err(GR_E_HIP, "pack kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    GR_HIP(hipEventRecord(f->ready[slot], c->copy));
+    f->has_ready[slot] = 1;
+    return GR_OK;
+}
+
+extern "C" int gr_field_upload(gr_field * f, uint32_t slot, const float * const * comps, int async)
+{
+    if (!f || !comps) return set_err(GR_E_INVALID, "gr_field_upload: null argument");
+    if (slot >= f->d.nslots) return set_err(GR_E_INVALID, "gr_field_upload: slot %u out of range", slot);
+    for (uint32_t k = 0; k < f->d.ncomp; ++k)
+        if (!comps[k]) return set_err(GR_E_INVALID, "gr_field_upload: component %u is null", k);
+    gr_ctx * c = f->ctx;
+    GR_HIP(hipSetDevice(c->device));
+    // the pinned staging may still feed the previous H2D copy
+    if (f->stage_pending) GR_HIP(hipEventSynchronize(f->stage_done));
+    const size_t bytes = f->nodes * sizeof(float);
+    for (uint32_t k = 0; k < f->d.ncomp; ++k) std::memcpy(f->hstage + k * f->nodes, comps[k], bytes);
+    // do not overwrite a slot a queued kernel still reads
+    if (f->has_inuse[slot]) GR_HIP(hipStreamWaitEvent(c->copy, f->inuse[slot], 0));
+    GR_HIP(hipMemcpyAsync(f->dstage, f->hstage, bytes * f->d.ncomp, hipMemcpyHostToDevice, c->copy));
+    GR_HIP(hipEventRecord(f->stage_done, c->copy));
+    f->stage_pending = true;
+    int rc = field_pack(f, slot);
+    if (rc != GR_OK) return rc;
+    if (!async) GR_HIP(hipStreamSynchronize(c->copy));
+    return GR_OK;
+}
+
+extern "C" int gr_field_upload_device(gr_field * f, uint32_t slot, const float * const * dcomps)
+{
+    if (!f || !dcomps) return set_err(GR_E_INVALID, "gr_field_upload_device: null argument");
+    if (slot >= f->d.nslots) return set_err(GR_E_INVALID, "slot %u out of range", slot);
+    gr_ctx * c = f->ctx;
+    GR_HIP(hipSetDevice(c->device));
+    if (f->has_inuse[slot]) GR_HIP(hipStreamWaitEvent(c->copy, f->inuse[slot], 0));
+    const size_t bytes = f->nodes * sizeof(float);
+    for (uint32_t k = 0; k < f->d.ncomp; ++k)
+        GR_HIP(hipMemcpyAsync(f->dstage + k * f->nodes, dcomps[k], bytes, hipMemcpyDeviceToDevice, c->copy));
+    return field_pack(f, slot);
+}
+
+extern "C" int gr_field_wait(gr_field * f, uint32_t slot)
+{
+    if (!f || slot >= f->d.nslots) return set_err(GR_E_INVALID, "gr_field_wait: bad argument");
+    GR_HIP(hipSetDevice(f->ctx->device));
+    if (f->has_ready[slot]) GR_HIP(hipEventSynchronize(f->ready[slot]));
+    return GR_OK;
+}
+
+extern "C" int gr_field_bytes(const gr_field * f, uint64_t * bytes)
+{
+    if (!f || !bytes) return set_err(GR_E_INVALID, "gr_field_bytes: null argument");
+    *bytes = f->slot_bytes;
+    return GR_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// jobs
+// --------------------------------------------------------------------------------------------
+static void job_free(gr_job * j)
+{
+    if (!j) return;
+    for (int d = 0; d < 3; ++d)
+    {
+        dfree(j->p[d]);
+        dfree(j->q[d]);
+        dfree(j->c[d]);
+        dfree(j->ld[d]);
+        dfree(j->uplift[d]);
+        for (int k = 0; k < GR_MAX_CONSTANTS; ++k) dfree(j->cst[d][k]);
+    }
+    dfree(j->status);
+    dfree(j->abort_step);
+    dfree(j->total);
+    dfree(j->curv);
+    dfree(j->len);
+    for (float * e : j->expr) dfree(e);
+    for (float * v : j->vol) dfree(v);
+    dfree(j->ring_pos);
+    dfree(j->ring_curv);
+    dfree(j->ring_expr);
+    dfree(j->ring_vol);
+    dfree(j->scratch_aos);
+    if (j->ev0) (void)hipEventDestroy(j->ev0);
+    if (j->ev1) (void)hipEventDestroy(j->ev1);
+    delete j;
+}
+
+static int zalloc(float ** p, size_t n)
+{
+    int rc = dalloc(p, n * sizeof(float));
+    if (rc == GR_OK && *p && hipMemset(*p, 0, n * sizeof(float)) != hipSuccess)
+        return set_err(GR_E_HIP, "memset failed");
+    return rc;
+}
+
+extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds,
+                             uint64_t n, uint32_t save_capacity, gr_job ** out)
+{
+    if (!ctx || !desc || !out || (n && !seeds)) return set_err(GR_E_INVALID, "gr_job_create: null argument");
+    *out = nullptr;
+    const gr_job_desc & d = *desc;
+    if (d.dim != 2 && d.dim != 3) return set_err(GR_E_INVALID, "dim must be 2 or 3");
+    if (d.integrator > GR_RK4 || d.border > GR_LOOP_XY || d.curvature > GR_CURV_INDIVIDUAL_AND_TOTAL ||
+        d.abort_mode > GR_ABORT_FIT_LENGTH || d.uplift_mode > 2 || d.windfield_mode > 1 ||
+        d.constants_mode > 2 || d.volume_mode > 2 || d.weights > GR_WEIGHTS_Q8_TRUNC)
+        return set_err(GR_E_INVALID, "gr_job_create: enum out of range");
+    if (d.nprog > GR_MAX_PROGRAMS || d.nvol > GR_MAX_VOLUMES || d.nconst > GR_MAX_CONSTANTS)
+        return set_err(GR_E_INVALID, "gr_job_create: too many programs / volumes / constants");
+    if (n >= (1ull << 31)) return set_err(GR_E_INVALID, "gr_job_create: too many particles");
+    if (!(d.dt == d.dt)) return set_err(GR_E_INVALID, "gr_job_create: DeltaT is NaN");
+    for (uint32_t e = 0; e < d.nprog; ++e)
+    {
+        const gr_program & pr = d.prog[e];
+        if (pr.n == 0 || pr.n > GR_MAX_CODE)
+            return set_err(GR_E_INVALID, "gr_job_create: program %u is empty or too long", e);
+        // the device interpreter trusts the bytecode: check opcodes, operands and stack bounds
+        int sp = 0;
+        for (uint32_t i = 0; i < pr.n; ++i)
+        {
+            const uint32_t op = pr.code[i] & 0xffu, arg = pr.code[i] >> 8;
+            int pops, pushes = 1;
+            if (op == gr::OP_REF || op == gr::OP_CONST)
+            {
+                pops = 0;
+                if ((op == gr::OP_REF && arg >= 19) || (op == gr::OP_CONST && arg >= GR_MAX_CODE))
+                    return set_err(GR_E_INVALID, "program %u: operand out of range at %u", e, i);
+            }
+            else if (op == gr::OP_ADD || op == gr::OP_SUB || op == gr::OP_MUL || op == gr::OP_DIV ||
+                     op == gr::OP_ATAN2)
+                pops = 2;
+            else if (op >= gr::OP_SQRT && op <= gr::OP_ABS)
+                pops = 1;
+            else
+                return set_err(GR_E_INVALID, "program %u: bad opcode %u at %u", e, op, i);
+            if (sp < pops) return set_err(GR_E_INVALID, "program %u: stack underflow at %u", e, i);
+            sp += pushes - pops;
+            if (sp > GR_MAX_STACK) return set_err(GR_E_INVALID, "program %u: stack overflow at %u", e, i);
+        }
+        if (sp != 1) return set_err(GR_E_INVALID, "program %u leaves %d values on the stack", e, sp);
+    }
+    GR_HIP(hipSetDevice(ctx->device));
+    gr_job * j = new (std::nothrow) gr_job;
+    if (!j) return set_err(GR_E_NOMEM, "gr_job_create: out of host memory");
+    j->ctx = ctx;
+    j->d = d;
+    j->n = n;
+    j->cap = save_capacity ? save_capacity : 1;
+    j->full = (d.dim == 2 && (d.border != GR_BLOCK || d.sphere_radius > 0.f)) ||
+              d.abort_mode != GR_ABORT_TIME || d.uplift_mode != GR_MODE_OFF ||
+              d.constants_mode != GR_MODE_OFF || d.volume_mode != GR_MODE_OFF || d.nvol > 0 ||
+              d.nconst > 0;
+    const uint32_t D = d.dim;
+    int rc = GR_OK;
+    const size_t N = size_t(n ? n : 1);
+    for (uint32_t k = 0; k < D && rc == GR_OK; ++k) rc = zalloc(&j->p[k], N);
+    const bool curv_pos = d.curvature == GR_CURV_TOTAL || d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
+    for (uint32_t k = 0; k < D && rc == GR_OK && curv_pos; ++k) rc = zalloc(&j->q[k], N);
+    for (uint32_t k = 0; k < D && rc == GR_OK && d.kahan; ++k) rc = zalloc(&j->c[k], N);
+    for (uint32_t k = 0; k < D && rc == GR_OK && d.curvature == GR_CURV_FAST_TOTAL; ++k)
+        rc = zalloc(&j->ld[k], N);
+    if (rc == GR_OK) rc = dalloc(&j->status, N * sizeof(int32_t));
+    if (rc == GR_OK && hipMemset(j->status, 0, N * sizeof(int32_t)) != hipSuccess) rc = GR_E_HIP;
+    if (rc == GR_OK) rc = dalloc(&j->abort_step, N * sizeof(int32_t));
+    if (rc == GR_OK && hipMemset(j->abort_step, 0xff, N * sizeof(int32_t)) != hipSuccess) rc = GR_E_HIP;
+    if (rc == GR_OK && d.curvature != GR_CURV_OFF) rc = zalloc(&j->total, N);
+    if (rc == GR_OK && d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) rc = zalloc(&j->curv, N);
+    if (rc == GR_OK && d.abort_mode != GR_ABORT_TIME) rc = zalloc(&j->len, N);
+    for (uint32_t e = 0; e < d.nprog && rc == GR_OK; ++e) rc = zalloc(&j->expr[e], N);
+    for (uint32_t k = 0; k < d.nvol && rc == GR_OK; ++k) rc = zalloc(&j->vol[k], N);
+    if (rc == GR_OK && d.save_interval != 0)
+    {
+        rc = zalloc(&j->ring_pos, size_t(j->cap) * N * D);
+        if (rc == GR_OK && d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) rc = zalloc(&j->ring_curv, size_t(j->cap) * N);
+        if (rc == GR_OK && d.nprog) rc = zalloc(&j->ring_expr, size_t(j->cap) * N * d.nprog);
+        if (rc == GR_OK && d.nvol) rc = zalloc(&j->ring_vol, size_t(j->cap) * N * d.nvol);
+    }
+    if (rc == GR_OK) rc = dalloc(&j->scratch_aos, N * D * sizeof(float));
+    if (rc == GR_OK && (hipEventCreate(&j->ev0) != hipSuccess || hipEventCreate(&j->ev1) != hipSuccess))
+        rc = set_err(GR_E_HIP, "event creation failed");
+    if (rc == GR_OK && n)
+    {
+        // seeds: host AoS -> device AoS staging -> SoA state (optionally relative to the origin)
+        if (hipMemcpyAsync(j->scratch_aos, seeds, n * D * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->compute) != hipSuccess)
+            rc = set_err(GR_E_HIP, "seed upload failed");
+        float * soa[3] = {j->p[0], j->p[1], D == 3 ? j->p[2] : nullptr};
+        if (rc == GR_OK && gr::launch_seed(j->scratch_aos, soa, n, D, d.offset, d.local_coords, ctx->compute) != 0)
+            rc = set_err(GR_E_HIP, "seed kernel failed");
+        for (uint32_t k = 0; k < D && rc == GR_OK && curv_pos; ++k)
+            if (hipMemcpyAsync(j->q[k], j->p[k], n * sizeof(float), hipMemcpyDeviceToDevice, ctx->compute) != hipSuccess)
+                rc = set_err(GR_E_HIP, "state init failed");
+        if (rc == GR_OK && hipStreamSynchronize(ctx->compute) != hipSuccess)
+            rc = set_err(GR_E_HIP, "gr_job_create: synchronize failed");
+    }
+    if (rc != GR_OK)
+    {
+        if (g_err.empty()) set_err(rc, "gr_job_create failed");
+        (void)hipGetLastError();
+        job_free(j);
+        return rc;
+    }
+    *out = j;
+    return GR_OK;
+}
+
+extern "C" int gr_job_destroy(gr_job * j)
+{
+    if (!j) return GR_OK;
+    (void)hipSetDevice(j->ctx->device);
+    (void)hipStreamSynchronize(j->ctx->compute);
+    job_free(j);
+    return GR_OK;
+}
+
+extern "C" int gr_job_set_launch(gr_job * j, uint32_t block)
+{
+    if (!j) return set_err(GR_E_INVALID, "null job");
+    if (block != 0 && (block % 64 != 0 || block > 256))
+        return set_err(GR_E_INVALID, "block must be 0 or a multiple of 64 up to 256");
+    j->block = block;
+    return GR_OK;
+}
+
+extern "C" int gr_job_set_particle_data(gr_job * j, uint32_t kind, uint32_t slot, const float * host)
+{
+    if (!j || !host) return set_err(GR_E_INVALID, "gr_job_set_particle_data: null argument");
+    if (slot >= 3) return set_err(GR_E_INVALID, "particle-data slot %u out of range", slot);
+    GR_HIP(hipSetDevice(j->ctx->device));
+    float ** dst;
+    size_t count;
+    if (kind == 0)
+    {
+        dst = &j->uplift[slot];
+        count = size_t(j->n) * j->d.dim;
+    }
+    else if (kind >= 1 && kind <= GR_MAX_CONSTANTS)
+    {
+        dst = &j->cst[slot][kind - 1];
+        count = size_t(j->n);
+    }
+    else
+        return set_err(GR_E_INVALID, "particle-data kind %u out of range", kind);
+    if (!*dst)
+    {
+        int rc = zalloc(dst, count ? count : 1);
+        if (rc != GR_OK) return rc;
+    }
+    if (count)
+    {
+        // ordered behind queued kernels on the compute stream; pageable source -> blocking copy
+        GR_HIP(hipStreamSynchronize(j->ctx->compute));
+        GR_HIP(hipMemcpy(*dst, host, count * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return GR_OK;
+}
+
+static void view(gr::FieldView & v, const gr_field * f, uint32_t back, uint32_t front)
+{
+    std::memset(&v, 0, sizeof(v));
+    if (!f) return;
+    v.back = f->slots[back];
+    v.front = f->slots[front];
+    v.nx = f->d.nx;
+    v.ny = f->d.ny;
+    v.nz = f->d.nz;
+    v.cstride = f->cstride;
+    v.ncomp = f->d.ncomp;
+    v.address = f->d.address;
+    v.dtype = f->d.dtype;
+}
+
+extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
+{
+    if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
+    const gr_job_desc & d = j->d;
+    gr_field * wf = w->wind;
+    if (wf->d.ncomp != d.dim) return set_err(GR_E_INVALID, "wind field has %u components, job is %uD", wf->d.ncomp, d.dim);
+    if (d.dim == 2 && wf->d.nz != 1) return set_err(GR_E_INVALID, "2D job needs nz = 1");
+    if (w->wind_back >= wf->d.nslots || w->wind_front >= wf->d.nslots)
+        return set_err(GR_E_INVALID, "wind slot out of range");
+    if (d.nvol && (!w->volumes || w->volumes->d.ncomp < d.nvol || w->vol_back >= w->volumes->d.nslots ||
+                   w->vol_front >= w->volumes->d.nslots))
+        return set_err(GR_E_INVALID, "volume field missing or too small");
+    if (w->topography && (w->topography->d.nx != wf->d.nx || w->topography->d.ny != wf->d.ny))
+        return set_err(GR_E_INVALID, "topography grid does not match the wind grid");
+    if (w->nsteps == 0) return GR_OK;
+    GR_HIP(hipSetDevice(j->ctx->device));
+
+    gr::StepParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.n = uint32_t(j->n);
+    P.nsteps = w->nsteps;
+    uint32_t fl = 0;
+    const bool dyn = d.windfield_mode == 1 || d.uplift_mode == 2 || d.constants_mode == 2 || d.volume_mode == 2;
+    if (dyn) fl |= gr::F_DYNAMIC_ANY;
+    if (d.windfield_mode == 1) fl |= gr::F_WIND_DYNAMIC;
+    if (d.reverse) fl |= gr::F_REVERSE;
+    if (d.time64) fl |= gr::F_TIME64;
+    if (d.kahan) fl |= gr::F_KAHAN;
+    if (d.local_coords) fl |= gr::F_LOCAL;
+    if (d.dim == 2 && d.sphere_radius > 0.f) fl |= gr::F_SPHERE;
+    if (d.dim == 3 && w->topography) fl |= gr::F_TOPO;
+    if (d.dim == 3 && d.uplift_mode == 1) fl |= gr::F_UPLIFT_CONST;
+    if (d.dim == 3 && d.uplift_mode == 2) fl |= gr::F_UPLIFT_DYN;
+    if (d.constants_mode == 1) fl |= gr::F_CONST_CONST;
+    if (d.constants_mode == 2) fl |= gr::F_CONST_DYN;
+    if (d.nvol && d.volume_mode == 1) fl |= gr::F_VOL_CONST;
+    if (d.nvol && d.volume_mode == 2) fl |= gr::F_VOL_DYN;
+    P.flags = fl;
+    P.border = d.dim == 2 ? d.border : GR_BLOCK;
+    P.curvature = d.curvature;
+    P.abort_mode = d.abort_mode;
+    P.weights = d.weights;
+    P.nprog = d.nprog;
+    P.nvol = d.nvol;
+    P.nconst = d.nconst;
+    P.dt = d.dt;
+    P.sphere_radius = d.sphere_radius;
+    P.max_length = d.max_length;
+    for (int k = 0; k < 3; ++k)
+    {
+        P.h[k] = d.grid_scale[k];
+        P.inv_h[k] = 1.f / d.grid_scale[k];
+        P.off[k] = d.local_coords ? 0.f : float(d.offset[k]);
+        P.dom[k] = d.domain[k];
+        P.off64[k] = d.offset[k];
+    }
+    P.tb = w->time_back;
+    P.tf = w->time_front;
+    P.t_start = w->time_start;
+    P.tb64 = w->time_back64;
+    P.tf64 = w->time_front64;
+    P.t_start64 = w->time_start64;
+    P.counter_start = w->counter_start;
+    P.save_interval = d.save_interval;
+    P.save_first = ~0ull;
+    uint64_t nsaves = 0;
+    if (d.save_interval)
+    {
+        const uint64_t r = w->counter_start % d.save_interval;
+        const uint64_t first = r ? d.save_interval - r : 0;
+        if (first < w->nsteps)
+        {
+            P.save_first = first;
+            P.save_g0 = (w->counter_start + first) / d.save_interval;
+            nsaves = (w->nsteps - 1 - first) / d.save_interval + 1;
+            if (nsaves > j->cap)
+                return set_err(GR_E_STATE, "window saves %llu columns but the ring holds %u",
+                               (unsigned long long)nsaves, j->cap);
+        }
+    }
+    P.ring_cap = j->cap;
+    for (int k = 0; k < 3; ++k)
+    {
+        P.p[k] = j->p[k];
+        P.q[k] = j->q[k];
+        P.c[k] = j->c[k];
+        P.ld[k] = j->ld[k];
+    }
+    P.status = j->status;
+    P.abort_step = j->abort_step;
+    P.total = j->total;
+    P.curv = j->curv;
+    P.len = j->len;
+    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) P.expr[e] = j->expr[e];
+    for (int k = 0; k < GR_MAX_VOLUMES; ++k) P.vol[k] = j->vol[k];
+    P.ring_pos = j->ring_pos;
+    P.ring_curv = j->ring_curv;
+    P.ring_expr = j->ring_expr;
+    P.ring_vol = j->ring_vol;
+    view(P.wind, wf, w->wind_back, d.windfield_mode == 1 ? w->wind_front : w->wind_back);
+    if (d.nvol) view(P.vols, w->volumes, w->vol_back, w->vol_front);
+    if (w->topography) view(P.topo, w->topography, 0, 0);
+    if (d.dim == 3 && d.uplift_mode != GR_MODE_OFF)
+    {
+        const uint32_t b = d.uplift_mode == 1 ? 0 : w->uplift_back;
+        const uint32_t f = d.uplift_mode == 1 ? 0 : w->uplift_front;
+        if (b >= 3 || f >= 3 || !j->uplift[b] || !j->uplift[f])
+            return set_err(GR_E_STATE, "uplift data for slots %u/%u not set", b, f);
+        P.lift_b = j->uplift[b];
+        P.lift_f = j->uplift[f];
+    }
+    for (uint32_t k = 0; k < d.nconst; ++k)
+    {
+        const uint32_t b = d.constants_mode == 2 ? w->const_back : 0;
+        const uint32_t f = d.constants_mode == 2 ? w->const_front : 0;
+        if (b >= 3 || f >= 3 || !j->cst[b][k] || !j->cst[f][k])
+            return set_err(GR_E_STATE, "constant %u for slots %u/%u not set", k, b, f);
+        P.cst_b[k] = j->cst[b][k];
+        P.cst_f[k] = j->cst[f][k];
+    }
+    for (uint32_t e = 0; e < d.nprog; ++e)
+    {
+        P.prog[e].n = d.prog[e].n;
+        std::memcpy(P.prog[e].code, d.prog[e].code, sizeof(P.prog[e].code));
+        std::memcpy(P.prog[e].imm, d.prog[e].imm, sizeof(P.prog[e].imm));
+    }
+
+    hipStream_t s = j->ctx->compute;
+    // wait for the slices this window reads
+    auto wait_ready = [&](gr_field * f, uint32_t slot) -> int {
+        if (f && f->has_ready[slot]) GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
+        return GR_OK;
+    };
+    auto mark_inuse = [&](gr_field * f, uint32_t slot) -> int {
+        if (f)
+        {
+            GR_HIP(hipEventRecord(f->inuse[slot], s));
+            f->has_inuse[slot] = 1;
+        }
+        return GR_OK;
+    };
+    int rc;
+    if ((rc = wait_ready(wf, w->wind_back)) || (rc = wait_ready(wf, w->wind_front))) return rc;
+    if (d.nvol && ((rc = wait_ready(w->volumes, w->vol_back)) || (rc = wait_ready(w->volumes, w->vol_front))))
+        return rc;
+    if (w->topography && (rc = wait_ready(w->topography, 0))) return rc;
+
+    GR_HIP(hipEventRecord(j->ev0, s));
+    if (gr::launch_advance(P, d.dim, d.integrator, j->full, j->block, s) != 0)
+        return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    GR_HIP(hipEventRecord(j->ev1, s));
+    j->timed = true;
+    if ((rc = mark_inuse(wf, w->wind_back)) || (rc = mark_inuse(wf, w->wind_front))) return rc;
+    if (d.nvol && ((rc = mark_inuse(w->volumes, w->vol_back)) || (rc = mark_inuse(w->volumes, w->vol_front))))
+        return rc;
+    if (w->topography && (rc = mark_inuse(w->topography, 0))) return rc;
+    j->saved += nsaves;
+    return GR_OK;
+}
+
+extern "C" int gr_job_read(gr_job * j, uint32_t kind, void * dst, uint64_t bytes)
+{
+    if (!j || !dst) return set_err(GR_E_INVALID, "gr_job_read: null argument");
+    GR_HIP(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->compute;
+    const uint64_t n = j->n;
+    const void * src = nullptr;
+    uint64_t need = n * 4;
+    if (kind == GR_OUT_POSITIONS)
+    {
+        need = n * j->d.dim * 4;
+        if (bytes < need) return set_err(GR_E_INVALID, "gr_job_read: buffer too small");
+        if (!n) return GR_OK;
+        float * soa[3] = {j->p[0], j->p[1], j->d.dim == 3 ? j->p[2] : j->p[1]};
+        if (gr::launch_gather_positions(soa, j->scratch_aos, n, j->d.dim, j->d.offset, j->d.local_coords, s) != 0)
+            return set_err(GR_E_HIP, "gather kernel failed");
+        src = j->scratch_aos;
+    }
+    else if (kind == GR_OUT_STATUS)
+        src = j->status;
+    else if (kind == GR_OUT_ABORT_STEP)
+        src = j->abort_step;
+    else if (kind == GR_OUT_TOTAL_CURVATURE)
+        src = j->total;
+    else if (kind == GR_OUT_LENGTHS)
+        src = j->len;
+    else if (kind == GR_OUT_CURVATURE)
+        src = j->curv;
+    else if (kind >= GR_OUT_EXPR0 && kind < GR_OUT_EXPR0 + GR_MAX_PROGRAMS)
+        src = j->expr[kind - GR_OUT_EXPR0];
+    else if (kind >= GR_OUT_VOLUME0 && kind < GR_OUT_VOLUME0 + GR_MAX_VOLUMES)
+        src = j->vol[kind - GR_OUT_VOLUME0];
+    else
+        return set_err(GR_E_INVALID, "gr_job_read: unknown kind %u", kind);
+    if (!src) return set_err(GR_E_INVALID, "gr_job_read: output %u is not produced by this job", kind);
+    if (bytes < need) return set_err(GR_E_INVALID, "gr_job_read: buffer too small");
+    if (!n) return GR_OK;
+    GR_HIP(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, s));
+    GR_HIP(hipStreamSynchronize(s));
+    return GR_OK;
+}
+
+extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint64_t count, void * dst)
+{
+    if (!j || (count && !dst)) return set_err(GR_E_INVALID, "gr_job_read_saved: null argument");
+    if (count == 0) return GR_OK;
+    if (first + count > j->saved) return set_err(GR_E_STATE, "columns [%llu, %llu) not written yet",
+                                                 (unsigned long long)first, (unsigned long long)(first + count));
+    if (j->saved - first > j->cap) return set_err(GR_E_STATE, "column %llu was overwritten", (unsigned long long)first);
+    GR_HIP(hipSetDevice(j->ctx->device));
+    const uint64_t n = j->n;
+    const float * base;
+    uint64_t colsz;
+    if (kind == GR_OUT_POSITIONS)
+    {
+        base = j->ring_pos;
+        colsz = n * j->d.dim;
+    }
+    else if (kind == GR_OUT_CURVATURE)
+    {
+        base = j->ring_curv;
+        colsz = n;
+    }
+    else if (kind >= GR_OUT_EXPR0 && kind < GR_OUT_EXPR0 + j->d.nprog)
+    {
+        base = j->ring_expr ? j->ring_expr + uint64_t(kind - GR_OUT_EXPR0) * j->cap * n : nullptr;
+        colsz = n;
+    }
+    else if (kind >= GR_OUT_VOLUME0 && kind < GR_OUT_VOLUME0 + j->d.nvol)
+    {
+        base = j->ring_vol ? j->ring_vol + uint64_t(kind - GR_OUT_VOLUME0) * j->cap * n : nullptr;
+        colsz = n;
+    }
+    else
+        return set_err(GR_E_INVALID, "gr_job_read_saved: kind %u has no saved columns", kind);
+    if (!base) return set_err(GR_E_INVALID, "gr_job_read_saved: output %u is not saved by this job", kind);
+    hipStream_t s = j->ctx->compute;
+    float * out = static_cast<float *>(dst);
+    uint64_t done = 0;
+    while (done < count)
+    {
+        const uint64_t slot = (first + done) % j->cap;
+        const uint64_t run = std::min<uint64_t>(count - done, j->cap - slot);
+        GR_HIP(hipMemcpyAsync(out + done * colsz, base + slot * colsz, run * colsz * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+        done += run;
+    }
+    GR_HIP(hipStreamSynchronize(s));
+    return GR_OK;
+}
+
+extern "C" int gr_job_count_alive(gr_job * j, uint64_t * alive)
+{
+    if (!j || !alive) return set_err(GR_E_INVALID, "gr_job_count_alive: null argument");
+    GR_HIP(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->compute;
+    GR_HIP(hipMemsetAsync(j->ctx->dcount, 0, sizeof(unsigned long long), s));
+    if (gr::launch_count_alive(j->status, j->n, j->ctx->dcount, s) != 0)
+        return set_err(GR_E_HIP, "count kernel failed");
+    unsigned long long h = 0;
+    GR_HIP(hipMemcpyAsync(&h, j->ctx->dcount, sizeof(h), hipMemcpyDeviceToHost, s));
+    GR_HIP(hipStreamSynchronize(s));
+    *alive = h;
+    return GR_OK;
+}
+
+extern "C" int gr_job_saved_columns(gr_job * j, uint64_t * cols)
+{
+    if (!j || !cols) return set_err(GR_E_INVALID, "null argument");
+    *cols = j->saved;
+    return GR_OK;
+}
+
+extern "C" int gr_job_last_advance_ms(gr_job * j, float * ms)
+{
+    if (!j || !ms) return set_err(GR_E_INVALID, "null argument");
+    if (!j->timed) return set_err(GR_E_STATE, "no advance recorded");
+    GR_HIP(hipSetDevice(j->ctx->device));
+    GR_HIP(hipEventSynchronize(j->ev1));
+    GR_HIP(hipEventElapsedTime(ms, j->ev0, j->ev1));
+    return GR_OK;
+}

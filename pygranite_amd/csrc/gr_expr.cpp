@@ -1,0 +1,312 @@
+// Expression compiler: AdditionalCompute source -> postfix bytecode for the step kernel's
+// wavefront-uniform interpreter.
+//
+// Language (source/granite/TrajectoryIntegrator.cpp:10-26): binary + - (lowest) and * /, all
+// left-associative; functions sqrt exp ln sin cos tan asin acos atan atan2(a, b); registers
+// x y z cv u v w; loader names bound to F0..F7 (volumes) and C0..C3 (constants); constants
+// -?[0-9]+(.[0-9]+)?. The old register names f0..f7, c0..c3 are rejected
+// (cpp:110-114), unknown identifiers are rejected (cpp:744-768), and the 2D / curvature checks
+// of TrajectoryIntegrator.tpp:661-696 apply. The choice order of the grammar's
+// PrimaryExpression is kept: a register letter is taken as the register even when letters
+// follow, which makes e.g. `velocity` a syntax error exactly as under the PEG parser.
+//
+// Code is emitted directly while parsing (operands first, then the operator), so the
+// bytecode is the postfix form of the same tree the reference flattens in pre-order
+// (cpp:770-798).
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "gr_internal.h"
+
+namespace
+{
+
+struct Env
+{
+    const gr_expr_env * e;
+};
+
+class Compiler
+{
+public:
+    Compiler(const char * src, const gr_expr_env & env, gr_program & out)
+        : src_(src), len_(std::strlen(src)), env_(env), out_(out)
+    {}
+
+    bool run(std::string & err)
+    {
+        out_.n = 0;
+        out_.depth = 0;
+        depth_ = 0;
+        nimm_ = 0;
+        if (!expr(1)) return fail_out(err);
+        skip_ws();
+        if (pos_ != len_)
+        {
+            msg_ = std::string("syntax error, unexpected '") + src_[pos_] + "'";
+            return fail_out(err);
+        }
+        return true;
+    }
+
+private:
+    const char * src_;
+    size_t len_;
+    size_t pos_{0};
+    const gr_expr_env & env_;
+    gr_program & out_;
+    uint32_t depth_{0};
+    uint32_t nimm_{0};
+    std::string msg_;
+
+    bool fail_out(std::string & err)
+    {
+        err = msg_.empty() ? "syntax error" : msg_;
+        return false;
+    }
+
+    bool fail(const std::string & m)
+    {
+        if (msg_.empty()) msg_ = m;
+        return false;
+    }
+
+    void skip_ws()
+    {
+        while (pos_ < len_ && (src_[pos_] == ' ' || src_[pos_] == '\t')) ++pos_;
+    }
+
+    bool emit(uint32_t op, uint32_t arg, int stack_delta)
+    {
+        if (out_.n >= GR_MAX_CODE) return fail("expression too long");
+        out_.code[out_.n++] = op | (arg << 8);
+        int d = int(depth_) + stack_delta;
+        depth_ = uint32_t(d);
+        if (depth_ > out_.depth) out_.depth = depth_;
+        if (out_.depth > GR_MAX_STACK) return fail("expression nests too deeply");
+        return true;
+    }
+
+    bool push_ref(uint32_t slot) { return emit(gr::OP_REF, slot, +1); }
+
+    bool push_const(float v)
+    {
+        if (nimm_ >= GR_MAX_CODE) return fail("expression too long");
+        out_.imm[nimm_] = v;
+        return emit(gr::OP_CONST, nimm_++, +1);
+    }
+
+    bool expr(int min_level)
+    {
+        if (!primary()) return false;
+        for (;;)
+        {
+            skip_ws();
+            if (pos_ >= len_) return true;
+            const char c = src_[pos_];
+            int level;
+            uint32_t op;
+            switch (c)
+            {
+                case '+': level = 1; op = gr::OP_ADD; break;
+                case '-': level = 1; op = gr::OP_SUB; break;
+                case '*': level = 2; op = gr::OP_MUL; break;
+                case '/': level = 2; op = gr::OP_DIV; break;
+                default: return true;
+            }
+            if (level < min_level) return true;
+            ++pos_;
+            if (!expr(level + 1)) return false;
+            if (!emit(op, 0, -1)) return false;
+        }
+    }
+
+    bool starts(const char * w) const
+    {
+        const size_t n = std::strlen(w);
+        return pos_ + n <= len_ && std::strncmp(src_ + pos_, w, n) == 0;
+    }
+
+    // 'name' '(' Expression (',' Expression)? ')' in the grammar's choice order
+    int function()
+    {
+        struct F
+        {
+            const char * name;
+            uint32_t op;
+            bool two;
+        };
+        static const F fns[] = {{"sqrt", gr::OP_SQRT, false}, {"exp", gr::OP_EXP, false},
+                                {"ln", gr::OP_LN, false},     {"sin", gr::OP_SIN, false},
+                                {"cos", gr::OP_COS, false},   {"tan", gr::OP_TAN, false},
+                                {"asin", gr::OP_ASIN, false}, {"acos", gr::OP_ACOS, false},
+                                {"atan", gr::OP_ATAN, false}, {"atan2", gr::OP_ATAN2, true},
+                                {"abs", gr::OP_ABS, false}};
+        const size_t start = pos_;
+        for (const F & f : fns)
+        {
+            if (f.op == gr::OP_ABS && !(env_.flags & GR_EXPR_ALLOW_ABS)) continue;
+            pos_ = start;
+            if (!starts(f.name)) continue;
+            pos_ += std::strlen(f.name);
+            skip_ws();
+            if (pos_ >= len_ || src_[pos_] != '(') continue;
+            ++pos_;
+            if (!expr(1)) return -1;
+            skip_ws();
+            if (f.two)
+            {
+                if (pos_ >= len_ || src_[pos_] != ',') return fail("syntax error, expecting ','") ? 1 : -1;
+                ++pos_;
+                if (!expr(1)) return -1;
+                skip_ws();
+            }
+            if (pos_ >= len_ || src_[pos_] != ')') return fail("syntax error, expecting ')'") ? 1 : -1;
+            ++pos_;
+            return emit(f.op, 0, f.two ? -1 : 0) ? 1 : -1;
+        }
+        pos_ = start;
+        return 0;
+    }
+
+    bool primary()
+    {
+        skip_ws();
+        if (pos_ >= len_) return fail("syntax error, unexpected end of input");
+        const char c = src_[pos_];
+        if (c == '(')
+        {
+            ++pos_;
+            if (!expr(1)) return false;
+            skip_ws();
+            if (pos_ >= len_ || src_[pos_] != ')') return fail("syntax error, expecting ')'");
+            ++pos_;
+            return true;
+        }
+        const int f = function();
+        if (f < 0) return false;
+        if (f > 0) return true;
+
+        // registers (ref slots of TrajectoryIntegrator.tpp:381-439)
+        if (starts("cv"))
+        {
+            if (env_.curvature != GR_CURV_INDIVIDUAL_AND_TOTAL)
+                return fail("Compute code contains undefinded token 'cv'");
+            pos_ += 2;
+            return push_ref(3);
+        }
+        switch (c)
+        {
+            case 'x': ++pos_; return push_ref(0);
+            case 'y': ++pos_; return push_ref(1);
+            case 'z':
+                if (env_.dim == 2) return fail("Compute code contains undefinded token 'z'");
+                ++pos_;
+                return push_ref(2);
+            case 'u': ++pos_; return push_ref(16);
+            case 'v': ++pos_; return push_ref(17);
+            case 'w':
+                if (env_.dim == 2) return fail("Compute code contains undefinded token 'w'");
+                ++pos_;
+                return push_ref(18);
+            default: break;
+        }
+        if (pos_ + 1 < len_ && ((c == 'f' && src_[pos_ + 1] >= '0' && src_[pos_ + 1] <= '7') ||
+                                (c == 'c' && src_[pos_ + 1] >= '0' && src_[pos_ + 1] <= '3')))
+            return fail(std::string("Identifier '") + c + src_[pos_ + 1] +
+                        "' is no longer allowed, see documentation for more information.");
+        if (c == '_' || std::isalpha(static_cast<unsigned char>(c)))
+        {
+            const size_t b = pos_;
+            while (pos_ < len_ &&
+                   (src_[pos_] == '_' || std::isalnum(static_cast<unsigned char>(src_[pos_]))))
+                ++pos_;
+            const std::string id(src_ + b, pos_ - b);
+            for (uint32_t k = 0; k < env_.nvol; ++k)
+                if (env_.vol_names && env_.vol_names[k] && id == env_.vol_names[k]) return push_ref(4 + k);
+            for (uint32_t k = 0; k < env_.nconst; ++k)
+                if (env_.const_names && env_.const_names[k] && id == env_.const_names[k])
+                    return push_ref(12 + k);
+            return fail("Invalid identifier '" + id + "'.");
+        }
+        // Constant <- [-]?[0-9]+[.][0-9]+ / [-]?[0-9]+
+        size_t q = pos_;
+        if (q < len_ && src_[q] == '-') ++q;
+        const size_t d0 = q;
+        while (q < len_ && std::isdigit(static_cast<unsigned char>(src_[q]))) ++q;
+        if (q > d0)
+        {
+            if (q + 1 < len_ && src_[q] == '.' && std::isdigit(static_cast<unsigned char>(src_[q + 1])))
+            {
+                ++q;
+                while (q < len_ && std::isdigit(static_cast<unsigned char>(src_[q]))) ++q;
+            }
+            const std::string tok(src_ + pos_, q - pos_);
+            pos_ = q;
+            return push_const(std::strtof(tok.c_str(), nullptr));
+        }
+        return fail(std::string("syntax error, unexpected '") + c + "'");
+    }
+};
+
+} // namespace
+
+extern "C" void gr_set_error(const char * msg); // gr_capi.cpp
+
+extern "C" int gr_expr_compile(const char * src, const gr_expr_env * env, gr_program * out)
+{
+    if (!src || !env || !out)
+    {
+        gr_set_error("gr_expr_compile: null argument");
+        return GR_E_INVALID;
+    }
+    std::memset(out, 0, sizeof(*out));
+    Compiler c(src, *env, *out);
+    std::string err;
+    if (!c.run(err))
+    {
+        std::string m = err + " (in '" + src + "')";
+        gr_set_error(m.c_str());
+        std::memset(out, 0, sizeof(*out));
+        return GR_E_PARSE;
+    }
+    return GR_OK;
+}
+
+extern "C" int gr_expr_eval_host(const gr_program * pr, const float * ref, float * out)
+{
+    if (!pr || !ref || !out) return GR_E_INVALID;
+    float st[GR_MAX_STACK];
+    int sp = 0;
+    for (uint32_t i = 0; i < pr->n; ++i)
+    {
+        const uint32_t w = pr->code[i], op = w & 0xffu, arg = w >> 8;
+        switch (op)
+        {
+            case gr::OP_REF: st[sp++] = ref[arg]; break;
+            case gr::OP_CONST: st[sp++] = pr->imm[arg]; break;
+            case gr::OP_ADD: sp--; st[sp - 1] = st[sp - 1] + st[sp]; break;
+            case gr::OP_SUB: sp--; st[sp - 1] = st[sp - 1] - st[sp]; break;
+            case gr::OP_MUL: sp--; st[sp - 1] = st[sp - 1] * st[sp]; break;
+            case gr::OP_DIV: sp--; st[sp - 1] = st[sp - 1] / st[sp]; break;
+            case gr::OP_SQRT: st[sp - 1] = std::sqrt(st[sp - 1]); break;
+            case gr::OP_EXP: st[sp - 1] = std::exp(st[sp - 1]); break;
+            case gr::OP_LN: st[sp - 1] = std::log(st[sp - 1]); break;
+            case gr::OP_SIN: st[sp - 1] = std::sin(st[sp - 1]); break;
+            case gr::OP_COS: st[sp - 1] = std::cos(st[sp - 1]); break;
+            case gr::OP_TAN: st[sp - 1] = std::tan(st[sp - 1]); break;
+            case gr::OP_ASIN: st[sp - 1] = std::asin(st[sp - 1]); break;
+            case gr::OP_ACOS: st[sp - 1] = std::acos(st[sp - 1]); break;
+            case gr::OP_ATAN: st[sp - 1] = std::atan(st[sp - 1]); break;
+            case gr::OP_ATAN2: sp--; st[sp - 1] = std::atan2(st[sp - 1], st[sp]); break;
+            case gr::OP_ABS: st[sp - 1] = std::fabs(st[sp - 1]); break;
+            default: return GR_E_INVALID;
+        }
+    }
+    *out = sp > 0 ? st[0] : 0.f;
+    return GR_OK;
+}

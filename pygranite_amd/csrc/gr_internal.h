@@ -1,0 +1,145 @@
+// Internal data contract between the host engine (gr_capi.cpp) and the gfx950 kernels
+// (gr_kernels.hip). Replaces the by-value SimulationData kernel argument of the reference
+// (source/granite/SimulationData.hpp:21-70) with a POD of raw device pointers and uniform
+// scalars; fields are node-interleaved arrays instead of texture objects.
+#pragma once
+
+#include <stdint.h>
+
+#include "granite_hip.h"
+
+namespace gr
+{
+
+// bytecode opcodes (low 8 bits of gr_program::code; arg in the high 24 bits)
+enum Op : uint32_t
+{
+    OP_REF = 1,   // push ref[arg]
+    OP_CONST = 2, // push imm[arg]
+    OP_ADD = 3,
+    OP_SUB = 4,
+    OP_MUL = 5,
+    OP_DIV = 6,
+    OP_SQRT = 7,
+    OP_EXP = 8,
+    OP_LN = 9,
+    OP_SIN = 10,
+    OP_COS = 11,
+    OP_TAN = 12,
+    OP_ASIN = 13,
+    OP_ACOS = 14,
+    OP_ATAN = 15,
+    OP_ATAN2 = 16,
+    OP_ABS = 17
+};
+
+// Slot views of one field: node (i, j, k) of a slot starts at element
+// ((k * ny + j) * nx + i) * cstride of that slot, components consecutive.
+struct FieldView
+{
+    const void * back;
+    const void * front;
+    uint32_t nx, ny, nz;
+    uint32_t cstride;  // elements per node (4 for 3 components, else ncomp)
+    uint32_t ncomp;
+    uint32_t address;  // 0 border (zero outside), 1 clamp
+    uint32_t dtype;    // gr_dtype
+    uint32_t pad;
+};
+
+// job flags (uniform)
+enum : uint32_t
+{
+    F_DYNAMIC_ANY = 1u << 0,  // any Dynamic mode -> time weight t is computed (tpp:127-133)
+    F_WIND_DYNAMIC = 1u << 1, // lerp between back and front wind slices
+    F_REVERSE = 1u << 2,
+    F_TIME64 = 1u << 3,
+    F_KAHAN = 1u << 4,
+    F_LOCAL = 1u << 5,        // positions are relative to off64
+    F_SPHERE = 1u << 6,
+    F_TOPO = 1u << 7,
+    F_UPLIFT_CONST = 1u << 8,
+    F_UPLIFT_DYN = 1u << 9,
+    F_CONST_CONST = 1u << 10,
+    F_CONST_DYN = 1u << 11,
+    F_VOL_CONST = 1u << 12,
+    F_VOL_DYN = 1u << 13,
+};
+
+struct Program
+{
+    uint32_t n;
+    uint32_t code[GR_MAX_CODE];
+    float imm[GR_MAX_CODE];
+};
+
+struct StepParams
+{
+    uint32_t n;          // particles
+    uint32_t nsteps;     // steps in this launch
+    uint32_t flags;
+    uint32_t border;     // gr_border
+    uint32_t curvature;  // gr_curvature
+    uint32_t abort_mode; // gr_abort_mode
+    uint32_t weights;    // gr_weights
+    uint32_t nprog, nvol, nconst;
+
+    float dt, sphere_radius, max_length;
+    float h[3], inv_h[3], off[3], dom[3];
+    double off64[3];
+
+    // time of this window
+    float tb, tf, t_start;
+    double tb64, tf64, t_start64;
+    uint64_t counter_start;
+    uint64_t save_interval; // 0 = no in-kernel saves
+    uint64_t save_first;    // step index (within this launch) of the first save, or ~0
+    uint64_t save_g0;       // global save index of that first save
+    uint32_t ring_cap;      // columns in the device ring
+
+    // particle state (SoA)
+    float * p[3];
+    float * q[3];    // previous position (curvature)
+    float * c[3];    // Kahan compensation
+    float * ld[3];   // last direction (fast total curvature)
+    int32_t * status;
+    int32_t * abort_step;
+    float * total;
+    float * curv;
+    float * len;
+    float * expr[GR_MAX_PROGRAMS];
+    float * vol[GR_MAX_VOLUMES];
+
+    // saved-column ring
+    float * ring_pos;  // [cap][n][dim]
+    float * ring_curv; // [cap][n]
+    float * ring_expr; // [nprog][cap][n]
+    float * ring_vol;  // [nvol][cap][n]
+
+    // fields
+    FieldView wind;
+    FieldView vols;
+    FieldView topo;
+
+    // per-particle data
+    const float * lift_b;
+    const float * lift_f;
+    const float * cst_b[GR_MAX_CONSTANTS];
+    const float * cst_f[GR_MAX_CONSTANTS];
+
+    Program prog[GR_MAX_PROGRAMS];
+};
+
+// launchers (gr_kernels.hip)
+int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t full,
+                   uint32_t block, void * stream);
+int launch_pack(const float * src, void * dst, uint64_t nodes, uint32_t ncomp, uint32_t cstride,
+                uint32_t dtype, void * stream);
+int launch_seed(const float * aos, float * const * soa, uint64_t n, uint32_t dim,
+                const double * off64, uint32_t local, void * stream);
+int launch_count_alive(const int32_t * status, uint64_t n, unsigned long long * out,
+                       void * stream);
+int launch_gather_positions(float * const * soa, float * aos, uint64_t n, uint32_t dim,
+                            const double * off64, uint32_t local, void * stream);
+
+} // namespace gr

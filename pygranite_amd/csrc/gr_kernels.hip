@@ -1,0 +1,847 @@
+// gfx950 kernels of the trajectory engine.
+//
+// k_advance runs `nsteps` integration steps of one window for every particle, keeping the
+// particle state in registers across steps and writing saved columns to a device ring. It
+// replaces the one-launch-per-step `granite::integrate` kernel of the reference
+// (source/granite/TrajectoryIntegrator.tpp:108-457) together with the host step loop that
+// launched it (tpp:973-994). Sampling is done in software from node-interleaved FP32 / FP16 /
+// BF16 slices (one vector load per stencil corner and slice) instead of texture fetches
+// (TrajectoryIntegrator.icc:22-129).
+//
+// Arithmetic follows the reference's operation order and is compiled without FP contraction,
+// so the FP32 form performs exactly the IEEE operations of the step (reference semantics);
+// only the sampler weights differ from the texture unit (exact here, optionally quantised to
+// 1/256 for emulation).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "gr_internal.h"
+
+namespace gr
+{
+
+// --------------------------------------------------------------------------------------------
+// storage types
+// --------------------------------------------------------------------------------------------
+struct bf16_t
+{
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(__half v) { return __half2float(v); }
+__device__ __forceinline__ float to_f(bf16_t v) { return __uint_as_float(uint32_t(v.bits) << 16); }
+
+__device__ __forceinline__ float h_lo(uint32_t w) { return __half2float(__ushort_as_half(uint16_t(w & 0xffffu))); }
+__device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort_as_half(uint16_t(w >> 16))); }
+__device__ __forceinline__ float b_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float b_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Loads the NC components of node `node` (cstride = 4 for NC = 3, else NC).
+template <class T, int NC> struct NodeLoad;
+
+template <> struct NodeLoad<float, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
+    {
+        const float4 x = reinterpret_cast<const float4 *>(b)[node];
+        v[0] = x.x; v[1] = x.y; v[2] = x.z;
+    }
+};
+template <> struct NodeLoad<float, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
+    {
+        const float2 x = reinterpret_cast<const float2 *>(b)[node];
+        v[0] = x.x; v[1] = x.y;
+    }
+};
+template <> struct NodeLoad<__half, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
+    {
+        const uint2 x = reinterpret_cast<const uint2 *>(b)[node];
+        v[0] = h_lo(x.x); v[1] = h_hi(x.x); v[2] = h_lo(x.y);
+    }
+};
+template <> struct NodeLoad<__half, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
+    {
+        const uint32_t x = reinterpret_cast<const uint32_t *>(b)[node];
+        v[0] = h_lo(x); v[1] = h_hi(x);
+    }
+};
+template <> struct NodeLoad<bf16_t, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
+    {
+        const uint2 x = reinterpret_cast<const uint2 *>(b)[node];
+        v[0] = b_lo(x.x); v[1] = b_hi(x.x); v[2] = b_lo(x.y);
+    }
+};
+template <> struct NodeLoad<bf16_t, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
+    {
+        const uint32_t x = reinterpret_cast<const uint32_t *>(b)[node];
+        v[0] = b_lo(x); v[1] = b_hi(x);
+    }
+};
+
+// generic scalar component load (volumes, topography)
+__device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint64_t idx)
+{
+    if (dtype == GR_F16) return __half2float(reinterpret_cast<const __half *>(b)[idx]);
+    if (dtype == GR_BF16) return __uint_as_float(uint32_t(reinterpret_cast<const uint16_t *>(b)[idx]) << 16);
+    return reinterpret_cast<const float *>(b)[idx];
+}
+
+// --------------------------------------------------------------------------------------------
+// sampling helpers
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ float lerp(float a, float b, float t) { return a + t * (b - a); }
+
+// texel coordinate -> (index, weight). Clamping to [-1, n] leaves every tap outside the grid
+// and keeps the conversion defined (NaN -> -1).
+__device__ __forceinline__ void split(float a, uint32_t n, uint32_t wmode, int & i, float & w)
+{
+    a = fminf(fmaxf(a, -1.f), float(n));
+    const float f = floorf(a);
+    i = int(f);
+    w = a - f;
+    if (wmode == GR_WEIGHTS_Q8_ROUND) w = floorf(w * 256.f + 0.5f) / 256.f;
+    else if (wmode == GR_WEIGHTS_Q8_TRUNC) w = floorf(w * 256.f) / 256.f;
+}
+
+struct Taps
+{
+    uint32_t node[8]; // corner nodes (x fastest: bit0 = +x, bit1 = +y, bit2 = +z)
+    uint32_t valid;   // bit per corner (border addressing)
+    float wx, wy, wz;
+};
+
+// Stencil for 3D (DIM = 3) or 2D (DIM = 2, 4 corners). Border addressing zeroes taps outside
+// [0, n-1]; clamp addressing clamps the index.
+template <int DIM>
+__device__ __forceinline__ void taps(const float * arg, uint32_t nx, uint32_t ny, uint32_t nz,
+                                     uint32_t address, uint32_t wmode, Taps & T)
+{
+    int i, j, k = 0;
+    split(arg[0], nx, wmode, i, T.wx);
+    split(arg[1], ny, wmode, j, T.wy);
+    T.wz = 0.f;
+    if (DIM == 3) split(arg[2], nz, wmode, k, T.wz);
+    const int xi[2] = {i, i + 1}, yj[2] = {j, j + 1}, zk[2] = {k, k + 1};
+    T.valid = 0;
+#pragma unroll
+    for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+    {
+        const int a = xi[c & 1], b = yj[(c >> 1) & 1], d = zk[(c >> 2) & 1];
+        const bool ok = a >= 0 && a < int(nx) && b >= 0 && b < int(ny) && (DIM == 2 || (d >= 0 && d < int(nz)));
+        const int ca = min(max(a, 0), int(nx) - 1);
+        const int cb = min(max(b, 0), int(ny) - 1);
+        const int cd = DIM == 3 ? min(max(d, 0), int(nz) - 1) : 0;
+        T.node[c] = (uint32_t(cd) * ny + uint32_t(cb)) * nx + uint32_t(ca);
+        if (ok || address == 1) T.valid |= 1u << c;
+    }
+}
+
+// Nested lerps x -> y -> z of the corner values (same order as the oracle's restatement).
+template <int DIM, int NC>
+__device__ __forceinline__ void combine(const float (&v)[8][NC], const Taps & T, float (&out)[NC])
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+    {
+        const float c00 = lerp(v[0][c], v[1][c], T.wx);
+        const float c10 = lerp(v[2][c], v[3][c], T.wx);
+        if (DIM == 3)
+        {
+            const float c01 = lerp(v[4][c], v[5][c], T.wx);
+            const float c11 = lerp(v[6][c], v[7][c], T.wx);
+            const float c0 = lerp(c00, c10, T.wy);
+            const float c1 = lerp(c01, c11, T.wy);
+            out[c] = lerp(c0, c1, T.wz);
+        }
+        else
+            out[c] = lerp(c00, c10, T.wy);
+    }
+}
+
+template <int DIM, class T, int NC>
+__device__ __forceinline__ void sample_slot(const void * base, const Taps & S, float (&out)[NC])
+{
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    float v[8][NC];
+#pragma unroll
+    for (int c = 0; c < NCOR; ++c)
+    {
+        float x[NC];
+        NodeLoad<T, NC>::get(base, S.node[c], x);
+        const bool ok = (S.valid >> c) & 1u;
+#pragma unroll
+        for (int m = 0; m < NC; ++m) v[c][m] = ok ? x[m] : 0.f;
+    }
+    combine<DIM, NC>(v, S, out);
+}
+
+// generic scalar field sample (component `comp` of a field view), used for volumes and
+// topography; bilinear/trilinear with the view's addressing
+template <int DIM>
+__device__ float sample_scalar(const void * base, const FieldView & f, uint32_t comp,
+                               const float * arg, uint32_t wmode)
+{
+    Taps S;
+    taps<DIM>(arg, f.nx, f.ny, f.nz, f.address, wmode, S);
+    float v[8][1];
+#pragma unroll
+    for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+    {
+        const float x = load_comp(base, f.dtype, uint64_t(S.node[c]) * f.cstride + comp);
+        v[c][0] = ((S.valid >> c) & 1u) ? x : 0.f;
+    }
+    float o[1];
+    combine<DIM, 1>(v, S, o);
+    return o[0];
+}
+
+// --------------------------------------------------------------------------------------------
+// expression interpreter (wavefront-uniform: the program lives in the kernel arguments, every
+// lane executes the same instruction stream)
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ float run_program(const Program & pr, const float (&ref)[19])
+{
+    float st[GR_MAX_STACK];
+    int sp = 0;
+    for (uint32_t i = 0; i < pr.n; ++i)
+    {
+        const uint32_t w = pr.code[i];
+        const uint32_t op = w & 0xffu, arg = w >> 8;
+        switch (op)
+        {
+            case OP_REF: st[sp++] = ref[arg]; break;
+            case OP_CONST: st[sp++] = pr.imm[arg]; break;
+            case OP_ADD: sp--; st[sp - 1] = st[sp - 1] + st[sp]; break;
+            case OP_SUB: sp--; st[sp - 1] = st[sp - 1] - st[sp]; break;
+            case OP_MUL: sp--; st[sp - 1] = st[sp - 1] * st[sp]; break;
+            case OP_DIV: sp--; st[sp - 1] = st[sp - 1] / st[sp]; break;
+            case OP_SQRT: st[sp - 1] = sqrtf(st[sp - 1]); break;
+            case OP_EXP: st[sp - 1] = expf(st[sp - 1]); break;
+            case OP_LN: st[sp - 1] = logf(st[sp - 1]); break;
+            case OP_SIN: st[sp - 1] = sinf(st[sp - 1]); break;
+            case OP_COS: st[sp - 1] = cosf(st[sp - 1]); break;
+            case OP_TAN: st[sp - 1] = tanf(st[sp - 1]); break;
+            case OP_ASIN: st[sp - 1] = asinf(st[sp - 1]); break;
+            case OP_ACOS: st[sp - 1] = acosf(st[sp - 1]); break;
+            case OP_ATAN: st[sp - 1] = atanf(st[sp - 1]); break;
+            case OP_ATAN2: sp--; st[sp - 1] = atan2f(st[sp - 1], st[sp]); break;
+            case OP_ABS: st[sp - 1] = fabsf(st[sp - 1]); break;
+            default: break;
+        }
+    }
+    return sp > 0 ? st[0] : 0.f;
+}
+
+// --------------------------------------------------------------------------------------------
+// the step kernel
+// --------------------------------------------------------------------------------------------
+struct Ctx
+{
+    const StepParams & P;
+};
+
+template <int DIM, class T>
+__device__ __forceinline__ void velocity(const StepParams & P, const float * x, float t,
+                                         float * v)
+{
+    float arg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) arg[d] = (x[d] - P.off[d]) * P.inv_h[d];
+    Taps S;
+    taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
+    float b[DIM];
+    sample_slot<DIM, T, DIM>(P.wind.back, S, b);
+    if (P.flags & F_WIND_DYNAMIC)
+    {
+        float f[DIM];
+        sample_slot<DIM, T, DIM>(P.wind.front, S, f);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
+    }
+    else
+    {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = b[d];
+    }
+}
+
+// sphere_coordinates, TrajectoryIntegrator.tpp:98-104 (2D only)
+__device__ __forceinline__ void sphere(const StepParams & P, float py, const float * v, float * o)
+{
+    const float PI = 3.14159265358979323846f;
+    const float PI_2 = PI * 2.f;
+    o[0] = (v[0] * 180.f) / (P.sphere_radius * cosf(py * PI_2 / 360.f) * PI);
+    o[1] = (v[1] * 180.f) / (PI * P.sphere_radius);
+}
+
+__device__ __forceinline__ float unit_t(const StepParams & P, float time32, double time64, float add)
+{
+    if (P.flags & F_TIME64)
+    {
+        const double x = (time64 + double(add) - P.tb64) / (P.tf64 - P.tb64);
+        return float(x);
+    }
+    const float x = time32 + add;
+    return (x - P.tb) / (P.tf - P.tb);
+}
+
+// 2D loop borders (TrajectoryIntegrator.tpp:20-96); returns true on Domain abort.
+__device__ bool loop_border(const StepParams & P, const float * o, float * n)
+{
+    const float ox = P.off[0], oy = P.off[1], DX = P.dom[0], DY = P.dom[1];
+    const bool sph = (P.flags & F_SPHERE) != 0;
+    if (P.border == GR_LOOP_X)
+    {
+        n[0] = fmodf((n[0] - ox + DX), DX) + ox;
+        return n[1] - oy > DY || n[1] - oy < 0.f;
+    }
+    if (P.border == GR_LOOP_Y)
+    {
+        if (sph)
+            return n[0] - ox > DX || n[0] - ox < 0.f || n[1] - oy > DY || n[1] - oy < 0.f;
+        n[1] = fmodf((n[1] - oy + DY), DY) + oy;
+        return n[0] - ox > DX || n[0] - ox < 0.f;
+    }
+    if (sph)
+    {
+        const float nx_ = n[0] - ox, ny_ = n[1] - oy;
+        if (ny_ < 0.f)
+        {
+            const float ox_ = o[0] - ox, oy_ = o[1] - oy;
+            const float tx = (-oy_) / (ny_ - oy_) * (nx_ - ox_) + ox_;
+            n[1] = -(ny_) + oy;
+            n[0] = fmodf(DX + 180.0f - nx_ + 2.f * tx, DX) + ox;
+        }
+        else if (ny_ >= DY)
+        {
+            const float ox_ = o[0] - ox, oy_ = o[1] - oy;
+            const float tx = (DY - oy_) / (ny_ - oy_) * (nx_ - ox_) + ox_;
+            n[1] = (ny_ - DY) + oy;
+            n[0] = fmodf(DX + 180.0f - nx_ + 2.f * tx, DX) + ox;
+        }
+    }
+    else
+        n[1] = fmodf((n[1] - oy + DY), DY) + oy;
+    n[0] = fmodf((n[0] - ox + DX), DX) + ox;
+    return false;
+}
+
+__device__ __forceinline__ float global_coord(const StepParams & P, float x, int d)
+{
+    return (P.flags & F_LOCAL) ? float(P.off64[d] + double(x)) : x;
+}
+
+template <int DIM, int INTEG, class T, bool FULL>
+__global__ void __launch_bounds__(256) k_advance(const StepParams P)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+
+    float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
+    float ld[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) p[d] = P.p[d][pid];
+    const bool curv_on = P.curvature != GR_CURV_OFF;
+    const bool curv_pos = P.curvature == GR_CURV_TOTAL || P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
+    if (curv_pos)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) q[d] = P.q[d][pid];
+    if (P.curvature == GR_CURV_FAST_TOTAL)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) ld[d] = P.ld[d][pid];
+    if (P.flags & F_KAHAN)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) cmp[d] = P.c[d][pid];
+    int32_t status = P.status[pid];
+    int32_t abort_step = P.abort_step[pid];
+    float total = curv_on ? P.total[pid] : 0.f;
+    float curv = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? P.curv[pid] : 0.f;
+    float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? P.len[pid] : 0.f;
+    float ex[GR_MAX_PROGRAMS];
+#pragma unroll
+    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) ex[e] = e < int(P.nprog) ? P.expr[e][pid] : 0.f;
+    float vo[GR_MAX_VOLUMES];
+#pragma unroll
+    for (int k = 0; k < GR_MAX_VOLUMES; ++k) vo[k] = (FULL && k < int(P.nvol)) ? P.vol[k][pid] : 0.f;
+
+    const float dt = P.dt;
+    const float half_dt = 0.5f * dt;
+    float time32 = P.t_start;
+    double time64 = P.t_start64;
+    uint64_t next_save = P.save_first;
+    uint64_t g = P.save_g0;
+
+    for (uint32_t s = 0; s < P.nsteps; ++s)
+    {
+        if (status == GR_STATUS_TIME)
+        {
+            float t = 0.f;
+            if (P.flags & F_DYNAMIC_ANY) t = unit_t(P, time32, time64, 0.f);
+            const bool sph = FULL && DIM == 2 && (P.flags & F_SPHERE);
+            float v[3] = {0.f, 0.f, 0.f}, vc[3] = {0.f, 0.f, 0.f};
+            if (INTEG == GR_RK4)
+            {
+                float t1 = t, t2 = t;
+                if (P.flags & F_WIND_DYNAMIC)
+                {
+                    t1 = unit_t(P, time32, time64, half_dt);
+                    t2 = unit_t(P, time32, time64, dt);
+                }
+                float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
+                float x[3] = {0.f, 0.f, 0.f};
+                velocity<DIM, T>(P, p, t, k1);
+                float k1c[3] = {k1[0], k1[1], k1[2]};
+                if (sph) sphere(P, p[1], k1c, k1);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
+                velocity<DIM, T>(P, x, t1, k2);
+                float k2c[3] = {k2[0], k2[1], k2[2]};
+                if (sph) sphere(P, x[1], k2c, k2);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
+                velocity<DIM, T>(P, x, t1, k3);
+                float k3c[3] = {k3[0], k3[1], k3[2]};
+                if (sph) sphere(P, x[1], k3c, k3);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
+                velocity<DIM, T>(P, x, t2, k4);
+                float k4c[3] = {k4[0], k4[1], k4[2]};
+                if (sph) sphere(P, x[1], k4c, k4);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                {
+                    v[d] = (((k1[d] + k2[d] * 2.f) + k3[d] * 2.f) + k4[d]) * (1.0f / 6);
+                    vc[d] = sph ? (((k1c[d] + k2c[d] * 2.f) + k3c[d] * 2.f) + k4c[d]) * (1.0f / 6) : v[d];
+                }
+            }
+            else
+            {
+                velocity<DIM, T>(P, p, t, vc);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) v[d] = vc[d];
+                if (sph) sphere(P, p[1], vc, v);
+            }
+
+            // dp (tpp:233-250)
+            float dp[3] = {0.f, 0.f, 0.f};
+            const float sgn = (P.flags & F_REVERSE) ? -1.f : 1.f;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dp[d] = v[d] * dt * sgn;
+            if (FULL && DIM == 3 && (P.flags & (F_UPLIFT_CONST | F_UPLIFT_DYN)))
+            {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                {
+                    const float lb = P.lift_b[uint64_t(pid) * DIM + d];
+                    const float l = (P.flags & F_UPLIFT_DYN) ? lerp(lb, P.lift_f[uint64_t(pid) * DIM + d], t) : lb;
+                    dp[d] = dp[d] + l;
+                }
+            }
+
+            int32_t abort = GR_STATUS_TIME;
+            if (FULL && P.abort_mode != GR_ABORT_TIME) // tpp:253-280
+            {
+                float l2 = 0.f;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) l2 += dp[d] * dp[d];
+                const float dp_l = sqrtf(l2);
+                const float c_l = len;
+                const float n_l = c_l + dp_l;
+                if (P.abort_mode == GR_ABORT_FIT_LENGTH)
+                {
+                    if (n_l > P.max_length)
+                    {
+                        const float sc = (P.max_length - c_l) / dp_l;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) dp[d] = sc * dp[d];
+                        len = P.max_length;
+                        abort = GR_STATUS_LENGTH;
+                    }
+                    else
+                        len = n_l;
+                }
+                else
+                {
+                    if (n_l > P.max_length) abort = GR_STATUS_LENGTH;
+                    len = n_l;
+                }
+            }
+
+            float np[3] = {p[0], p[1], p[2]};
+            float y[3] = {0.f, 0.f, 0.f};
+            if (P.flags & F_KAHAN)
+            {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                {
+                    y[d] = ((P.flags & F_REVERSE) ? -dp[d] : dp[d]) - cmp[d];
+                    np[d] = p[d] + y[d];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) np[d] = (P.flags & F_REVERSE) ? p[d] - dp[d] : p[d] + dp[d];
+            }
+
+            float dp2 = 0.f;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) dp2 += dp[d] * dp[d];
+            if (abort == GR_STATUS_TIME) // tpp:284-323
+            {
+                if (FULL && DIM == 2 && P.border != GR_BLOCK)
+                {
+                    if (dp2 < GR_ABORT_VALUE)
+                        abort = GR_STATUS_WIND;
+                    else if (loop_border(P, p, np))
+                        abort = GR_STATUS_DOMAIN;
+                }
+                else
+                {
+                    bool topo = false;
+                    if (FULL && DIM == 3 && (P.flags & F_TOPO))
+                    {
+                        float ta[3] = {global_coord(P, np[0], 0) / P.h[0],
+                                       global_coord(P, np[1], 1) / P.h[1], 0.f};
+                        topo = sample_scalar<2>(P.topo.back, P.topo, 0, ta, P.weights) >
+                               global_coord(P, np[2], 2);
+                    }
+                    if (topo)
+                        abort = GR_STATUS_TOPOGRAPHY;
+                    else if (dp2 < GR_ABORT_VALUE)
+                        abort = GR_STATUS_WIND;
+                    else
+                    {
+                        bool out = false;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d)
+                            out = out || (np[d] - P.off[d] > P.dom[d] || np[d] - P.off[d] < 0.f);
+                        if (out) abort = GR_STATUS_DOMAIN;
+                    }
+                }
+            }
+            status = abort;
+            if (abort != GR_STATUS_TIME) abort_step = int32_t(P.counter_start + s);
+
+            if (abort == GR_STATUS_WIND || abort == GR_STATUS_TOPOGRAPHY || abort == GR_STATUS_DOMAIN)
+            {
+                if (curv_on) total = -1.f; // tpp:327-332
+            }
+            else
+            {
+                const bool tpos = (P.flags & F_TIME64) ? (time64 > 0.0) : (time32 > 0.f);
+                if (P.curvature == GR_CURV_FAST_TOTAL) // tpp:337-346
+                {
+                    float l2 = 0.f;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) l2 += v[d] * v[d];
+                    const float inv = 1.f / sqrtf(l2);
+                    float vn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) vn[d] = v[d] * inv;
+                    if (tpos)
+                    {
+                        float dd = 0.f;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) dd += ld[d] * vn[d];
+                        const float k = acosf(dd);
+                        if (!isnan(k)) total += k;
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) ld[d] = vn[d];
+                }
+                else if (curv_pos && tpos) // tpp:347-360, kappa icc:145-159
+                {
+                    float k = 0.f;
+                    if (DIM == 3)
+                    {
+                        float d1[3], d2[3];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            d1[d] = (np[d] - q[d]) / (2.f * dt);
+                            d2[d] = ((q[d] + np[d]) - 2.f * p[d]) / (dt * dt);
+                        }
+                        const float ldp = sqrtf((d1[0] * d1[0] + d1[1] * d1[1]) + d1[2] * d1[2]);
+                        const float ldp3 = (ldp * ldp) * ldp;
+                        const float cx = d1[1] * d2[2] - d1[2] * d2[1];
+                        const float cy = d1[2] * d2[0] - d1[0] * d2[2];
+                        const float cz = d1[0] * d2[1] - d1[1] * d2[0];
+                        k = fabsf(sqrtf((cx * cx + cy * cy) + cz * cz) / ldp3);
+                    }
+                    float a2 = 0.f, b2 = 0.f;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d)
+                    {
+                        a2 += (p[d] - q[d]) * (p[d] - q[d]);
+                        b2 += (np[d] - p[d]) * (np[d] - p[d]);
+                    }
+                    const float ds = 0.5f * (sqrtf(a2) + sqrtf(b2));
+                    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) curv = k;
+                    if (!isnan(k)) total += k * ds;
+                }
+
+                if (FULL && (P.flags & (F_VOL_CONST | F_VOL_DYN))) // tpp:363-377
+                {
+                    float arg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) arg[d] = (np[d] - P.off[d]) * P.inv_h[d];
+                    for (uint32_t k = 0; k < P.nvol; ++k)
+                    {
+                        float val = sample_scalar<DIM>(P.vols.back, P.vols, k, arg, P.weights);
+                        if (P.flags & F_VOL_DYN)
+                            val = lerp(val, sample_scalar<DIM>(P.vols.front, P.vols, k, arg, P.weights), t);
+                        vo[k] = val;
+                    }
+                }
+
+                if (P.nprog > 0) // tpp:379-447
+                {
+                    float ref[19];
+#pragma unroll
+                    for (int r = 0; r < 19; ++r) ref[r] = 0.f;
+                    ref[0] = global_coord(P, np[0], 0);
+                    ref[1] = global_coord(P, np[1], 1);
+                    ref[2] = DIM == 3 ? global_coord(P, np[2], 2) : 0.f;
+                    ref[3] = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f;
+                    if (FULL)
+                    {
+#pragma unroll
+                        for (int k = 0; k < GR_MAX_VOLUMES; ++k) ref[4 + k] = k < int(P.nvol) ? vo[k] : 0.f;
+#pragma unroll
+                        for (int k = 0; k < GR_MAX_CONSTANTS; ++k)
+                        {
+                            float cv = 0.f;
+                            if (k < int(P.nconst))
+                            {
+                                cv = P.cst_b[k][pid];
+                                if (P.flags & F_CONST_DYN) cv = lerp(cv, P.cst_f[k][pid], t);
+                            }
+                            ref[12 + k] = cv;
+                        }
+                    }
+                    ref[16] = vc[0];
+                    ref[17] = vc[1];
+                    ref[18] = DIM == 3 ? vc[2] : 0.f;
+                    for (uint32_t e = 0; e < P.nprog; ++e) ex[e] = run_program(P.prog[e], ref);
+                }
+
+                if (P.flags & F_KAHAN)
+                {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) cmp[d] = (np[d] - p[d]) - y[d];
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                {
+                    q[d] = p[d];
+                    p[d] = np[d];
+                }
+            }
+        }
+
+        if (s == next_save) // copyParticleInfo, tpp:997-1049
+        {
+            const uint64_t col = g % P.ring_cap;
+            float * rp = P.ring_pos + (col * P.n + pid) * DIM;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) rp[d] = global_coord(P, p[d], d);
+            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) P.ring_curv[col * P.n + pid] = curv;
+            for (uint32_t e = 0; e < P.nprog; ++e)
+                P.ring_expr[(uint64_t(e) * P.ring_cap + col) * P.n + pid] = ex[e];
+            if (FULL)
+                for (uint32_t k = 0; k < P.nvol; ++k)
+                    P.ring_vol[(uint64_t(k) * P.ring_cap + col) * P.n + pid] = vo[k];
+            g++;
+            next_save += P.save_interval;
+        }
+        time32 += dt;
+        time64 += double(dt);
+    }
+
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) P.p[d][pid] = p[d];
+    if (curv_pos)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P.q[d][pid] = q[d];
+    if (P.curvature == GR_CURV_FAST_TOTAL)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P.ld[d][pid] = ld[d];
+    if (P.flags & F_KAHAN)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P.c[d][pid] = cmp[d];
+    P.status[pid] = status;
+    P.abort_step[pid] = abort_step;
+    if (curv_on) P.total[pid] = total;
+    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) P.curv[pid] = curv;
+    if (FULL && P.abort_mode != GR_ABORT_TIME) P.len[pid] = len;
+    for (uint32_t e = 0; e < P.nprog; ++e) P.expr[e][pid] = ex[e];
+    if (FULL)
+        for (uint32_t k = 0; k < P.nvol; ++k) P.vol[k][pid] = vo[k];
+}
+
+// --------------------------------------------------------------------------------------------
+// support kernels
+// --------------------------------------------------------------------------------------------
+
+// FP32 component-major staging -> node-interleaved storage of the slot
+template <class T>
+__global__ void k_pack(const float * __restrict__ src, T * __restrict__ dst, uint64_t nodes,
+                       uint32_t ncomp, uint32_t cstride)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nodes) return;
+    for (uint32_t c = 0; c < cstride; ++c)
+    {
+        const float v = c < ncomp ? src[uint64_t(c) * nodes + i] : 0.f;
+        if constexpr (sizeof(T) == 4)
+            reinterpret_cast<float *>(dst)[i * cstride + c] = v;
+        else if constexpr (std::is_same<T, __half>::value)
+            dst[i * cstride + c] = __float2half_rn(v);
+        else
+        {
+            // round to nearest even to bfloat16
+            uint32_t u = __float_as_uint(v);
+            uint16_t b;
+            if ((u & 0x7fffffffu) > 0x7f800000u)
+                b = uint16_t((u >> 16) | 0x0040u);
+            else
+                b = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+            reinterpret_cast<uint16_t *>(dst)[i * cstride + c] = b;
+        }
+    }
+}
+
+__global__ void k_seed(const float * __restrict__ aos, float * x, float * y, float * z, uint64_t n,
+                       uint32_t dim, double o0, double o1, double o2, uint32_t local)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = aos[i * dim + 0], b = aos[i * dim + 1];
+    x[i] = local ? float(double(a) - o0) : a;
+    y[i] = local ? float(double(b) - o1) : b;
+    if (dim == 3)
+    {
+        const float c = aos[i * dim + 2];
+        z[i] = local ? float(double(c) - o2) : c;
+    }
+}
+
+__global__ void k_gather(const float * x, const float * y, const float * z, float * aos, uint64_t n,
+                         uint32_t dim, double o0, double o1, double o2, uint32_t local)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aos[i * dim + 0] = local ? float(o0 + double(x[i])) : x[i];
+    aos[i * dim + 1] = local ? float(o1 + double(y[i])) : y[i];
+    if (dim == 3) aos[i * dim + 2] = local ? float(o2 + double(z[i])) : z[i];
+}
+
+__global__ void k_count_alive(const int32_t * __restrict__ status, uint64_t n,
+                              unsigned long long * out)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool alive = i < n && status[i] == GR_STATUS_TIME;
+    const unsigned long long m = __ballot(alive);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
+// --------------------------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------------------------
+typedef void (*AdvanceFn)(const StepParams);
+
+template <int DIM, int INTEG, class T>
+static AdvanceFn pick_full(uint32_t full)
+{
+    return full ? k_advance<DIM, INTEG, T, true> : k_advance<DIM, INTEG, T, false>;
+}
+
+template <int DIM, int INTEG>
+static AdvanceFn pick_type(uint32_t dtype, uint32_t full)
+{
+    if (dtype == GR_F16) return pick_full<DIM, INTEG, __half>(full);
+    if (dtype == GR_BF16) return pick_full<DIM, INTEG, bf16_t>(full);
+    return pick_full<DIM, INTEG, float>(full);
+}
+
+int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t full,
+                   uint32_t block, void * stream)
+{
+    AdvanceFn fn;
+    if (dim == 3)
+        fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, full)
+                                  : pick_type<3, GR_EULER>(P.wind.dtype, full);
+    else
+        fn = integrator == GR_RK4 ? pick_type<2, GR_RK4>(P.wind.dtype, full)
+                                  : pick_type<2, GR_EULER>(P.wind.dtype, full);
+    if (block == 0 || block > 256) block = 256;
+    const uint32_t grid = uint32_t((uint64_t(P.n) + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, hipStream_t(stream), P);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pack(const float * src, void * dst, uint64_t nodes, uint32_t ncomp, uint32_t cstride,
+                uint32_t dtype, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((nodes + block - 1) / block);
+    if (grid == 0) return 0;
+    if (dtype == GR_F16)
+        hipLaunchKernelGGL(k_pack<__half>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
+                           reinterpret_cast<__half *>(dst), nodes, ncomp, cstride);
+    else if (dtype == GR_BF16)
+        hipLaunchKernelGGL(k_pack<bf16_t>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
+                           reinterpret_cast<bf16_t *>(dst), nodes, ncomp, cstride);
+    else
+        hipLaunchKernelGGL(k_pack<float>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
+                           reinterpret_cast<float *>(dst), nodes, ncomp, cstride);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_seed(const float * aos, float * const * soa, uint64_t n, uint32_t dim,
+                const double * off64, uint32_t local, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(k_seed, dim3(grid), dim3(block), 0, hipStream_t(stream), aos, soa[0],
+                       soa[1], soa[2], n, dim, off64[0], off64[1], off64[2], local);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_gather_positions(float * const * soa, float * aos, uint64_t n, uint32_t dim,
+                            const double * off64, uint32_t local, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(block), 0, hipStream_t(stream), soa[0], soa[1],
+                       soa[2], aos, n, dim, off64[0], off64[1], off64[2], local);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_count_alive(const int32_t * status, uint64_t n, unsigned long long * out, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(k_count_alive, dim3(grid), dim3(block), 0, hipStream_t(stream), status, n,
+                       out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+} // namespace gr

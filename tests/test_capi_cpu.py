@@ -1,0 +1,93 @@
+"""CPU checks of the native library: it loads, exports every symbol the header declares, and
+its expression compiler agrees with the oracle's parser (no GPU needed)."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from pygranite_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "granite_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s+(gr_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    L = capi.lib()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), f"{s} declared in granite_hip.h but not exported"
+    assert sorted(capi.EXPORTS) == syms
+
+
+def test_abi_version_and_device_count_without_gpu():
+    L = capi.lib()
+    assert L.gr_abi_version() == 1
+    n = capi.device_count()
+    assert n >= 0
+
+
+def test_ctx_create_fails_cleanly_without_gpu():
+    if capi.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(capi.GraniteError):
+        capi.Context(0)
+
+
+EXPRS = ["sqrt(u*u+v*v+w*w)", "w*1000/(abs(u)+abs(v)+1)", "1-2-3", "x*y+z/2-cv",
+         "atan2(y, x) + sin(z)*cos(x) - tan(0.25)", "exp(-1.5)*ln(2.5)+asin(0.5)-acos(0.25)",
+         "atan(x) / (1 + u*u)", "((x))", "2*-3+x", "temp*rho - c_2", "(u+v)*(u-v)/(w+10)"]
+
+
+@pytest.mark.parametrize("src", EXPRS)
+def test_expression_compiler_matches_oracle(src):
+    rng = np.random.default_rng(abs(hash(src)) % 2 ** 32)
+    consts = ["temp", "rho", "c_2"]
+    env = {k: 13 + i for i, k in enumerate(consts)}
+    prog = capi.compile_expr(src, dim=3, curvature=3, constants=consts, allow_abs=True)
+    assert prog.n > 0 and prog.depth <= 8
+    for _ in range(20):
+        ref = rng.uniform(0.1, 2.0, 19)
+        got = capi.eval_expr_host(prog, ref)
+        f32, f64 = oracle.eval_expr(src, ref, env, allow_abs=True)
+        assert got == pytest.approx(f64, rel=2e-5, abs=1e-6)
+
+
+@pytest.mark.parametrize("src,kw", [
+    ("velocity", {}), ("f0+1", {}), ("c1*2", {}), ("abs(x)", {}), ("1.", {}), (".5", {}),
+    ("x y", {}), ("", {}), ("sqrt(x", {}), ("nope", {}), ("z+1", {"dim": 2}),
+    ("w", {"dim": 2}), ("cv", {"curvature": 0}), ("(((((((((x)))))))))+" * 1 + "1", {}),
+])
+def test_expression_compiler_rejects(src, kw):
+    if src.startswith("((((("):
+        # deep parentheses alone do not deepen the stack: this one is valid
+        capi.compile_expr(src, **kw)
+        return
+    with pytest.raises(capi.GraniteError):
+        capi.compile_expr(src, **kw)
+
+
+def test_expression_stack_limit():
+    deep = "1+(" * 8 + "1" + ")" * 8
+    with pytest.raises(capi.GraniteError):
+        capi.compile_expr(deep)
+    ok = "1+(" * 6 + "1" + ")" * 6
+    capi.compile_expr(ok)
+
+
+def test_expression_postfix_layout():
+    """sqrt(u*u+v*v+w*w): operands before operators, 12 instructions like the reference's 12
+    flattened nodes."""
+    p = capi.compile_expr("sqrt(u*u+v*v+w*w)")
+    ops = [p.code[i] & 0xff for i in range(p.n)]
+    args = [p.code[i] >> 8 for i in range(p.n)]
+    assert p.n == 12
+    assert ops == [1, 1, 5, 1, 1, 5, 3, 1, 1, 5, 3, 7]
+    assert [a for o, a in zip(ops, args) if o == 1] == [16, 16, 17, 17, 18, 18]

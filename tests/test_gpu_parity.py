@@ -1,0 +1,91 @@
+"""GPU parity: the HIP path (through pygranite's API and the C ABI) against the CPU oracle.
+
+FP32 runs are compared bit for bit with the oracle's FP32 restatement (same IEEE operations,
+no contraction on either side): every saved position, every status, every abort step. FP16 /
+BF16 storage is compared bit for bit with the FP32 oracle fed the same rounded slices. Accuracy
+against the FP64 oracle is checked as domain-relative error.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_bitexact, domain_rel_error, run_product
+from pygranite_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def c0_settings(steps=1000, save=10):
+    wind, gs = synthetic.double_gyre(512, 256, 11)
+    st = dict(Space="Space2D", Integrator="ExplicitEuler", DeltaT=0.01, DataTimeDistance=1.0,
+              GridScale=gs, Offset=[0.0, 0.0], MaximumSimulationTime=steps * 0.01,
+              SaveInterval=save, BorderMode="Block")
+    return st, wind, gs
+
+
+def c1_settings(n=32, steps=400):
+    wind, gs = synthetic.abc_flow(n, 16)
+    st = dict(Space="Space3D", Integrator="ClassicRungeKutta", DeltaT=0.005,
+              DataTimeDistance=1.0, GridScale=gs, Offset=[0.0, 0.0, 0.0],
+              MaximumSimulationTime=steps * 0.005, SaveInterval=1,
+              CurvatureMode="IndividualAndTotalCurvature",
+              AdditionalCompute=["sqrt(u*u+v*v+w*w)"])
+    return st, wind, gs
+
+
+def test_c0_double_gyre_euler_bitexact(pg, gpu_available):
+    st, wind, gs = c0_settings()
+    h = np.array(gs)
+    seeds = synthetic.uniform_seeds(0, 4096, h, np.array([2.0, 1.0]) - h)
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, wind, seeds, real64=False)
+    assert gpu["trajectories"].shape == (4096, 101, 2)
+    assert_bitexact(gpu, ref, label="C0 ")
+    # the FP32 path is close to the FP64 oracle
+    ref64 = oracle.run(st, wind, seeds, real64=True)
+    mx, rms = domain_rel_error(gpu["trajectories"], ref64["trajectories"], [2.0, 1.0])
+    assert rms < 1e-5, (mx, rms)
+
+
+def test_c1_abc_rk4_curvature_expr_bitexact(pg, gpu_available):
+    st, wind, gs = c1_settings()
+    seeds = synthetic.uniform_seeds(1, 2048, [0.0] * 3, [2 * math.pi] * 3)
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, wind, seeds, real64=False)
+    assert_bitexact(gpu, ref, fields=("trajectories", "curvatures", "total_curvature"),
+                    label="C1 ")
+    # sqrt is correctly rounded on both sides -> the expression is bit-exact as well
+    np.testing.assert_array_equal(gpu["compute"][0], ref["compute"][0].astype(np.float32))
+    assert gpu["curvatures"].shape == (2048, 399)
+    assert gpu["compute"][0].shape == (2048, 400)
+
+
+@pytest.mark.parametrize("storage,rounder", [("Float16", synthetic.round_fp16),
+                                             ("BFloat16", synthetic.round_bf16)])
+def test_c1_half_storage_bitexact(pg, gpu_available, storage, rounder):
+    st, wind, gs = c1_settings(n=32, steps=200)
+    st = dict(st, WindfieldStorage=storage)
+    seeds = synthetic.uniform_seeds(2, 1024, [0.0] * 3, [2 * math.pi] * 3)
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, rounder(wind), seeds, real64=False)
+    assert_bitexact(gpu, ref, fields=("trajectories", "curvatures"), label=storage + " ")
+
+
+def test_texture_weight_emulation_bitexact(pg, gpu_available):
+    st, wind, gs = c0_settings(steps=300, save=5)
+    st = dict(st, TextureWeights="Quantized8")
+    seeds = synthetic.uniform_seeds(3, 2048, gs, np.array([2.0, 1.0]) - np.array(gs))
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, wind, seeds, real64=False, weights="q8")
+    assert_bitexact(gpu, ref, label="q8 ")
+
+
+def test_precision_form_kahan_time64_bitexact(pg, gpu_available):
+    st, wind, gs = c1_settings(n=32, steps=300)
+    st = dict(st, KahanSummation=True, TimeFP64=True, WindfieldStorage="Float16")
+    seeds = synthetic.uniform_seeds(4, 1024, [0.0] * 3, [2 * math.pi] * 3)
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, synthetic.round_fp16(wind), seeds, real64=False, time64=True, kahan=True)
+    assert_bitexact(gpu, ref, label="kahan ")
