@@ -56,7 +56,8 @@ def algorithmic_bytes(storage_bytes: int, outputs: int) -> int:
     return 4 * 2 * 3 * 8 * storage_bytes + (2 * 3 * 4 + 4) + outputs
 
 
-def run_form(capi, ctx, wind, seeds, gs, dtype, chunk, steps, warmup, exprs):
+def run_form(capi, ctx, wind, seeds, gs, dtype, chunk, steps, warmup, exprs, paired=True,
+             reorder=True):
     S = wind.shape[0]
     n = wind.shape[2]
     f = capi.Field(ctx, n, n, n, 3, dtype=dtype, nslots=S, address=0)
@@ -67,6 +68,9 @@ def run_form(capi, ctx, wind, seeds, gs, dtype, chunk, steps, warmup, exprs):
     dt = np.float32(0.005)
     desc = build_desc(capi, 3, gs, float(dt), exprs)
     job = capi.Job(ctx, desc, seeds, save_capacity=chunk)
+    job.set_option(capi.OPT_PAIRED, 1 if paired else 0)
+    if reorder:
+        job.reorder()
     ctx.synchronize()
 
     counter = 0
@@ -104,6 +108,8 @@ def run_form(capi, ctx, wind, seeds, gs, dtype, chunk, steps, warmup, exprs):
             t32 = t
             left -= k
             if done:
+                if reorder:
+                    job.reorder()  # re-sort by cell at every window edge (inside the timing)
                 window += 1
                 max_wt = np.float32(max_wt + np.float32(1.0))
                 if window + 1 >= S:
@@ -196,11 +202,15 @@ def main():
     print(f"# device {info['name']} CUs={info['compute_units']}", file=sys.stderr, flush=True)
 
     res = {}
-    for name, dtype in (("fp32", capi.F32), ("fp16", capi.F16)):
+    # fp32: the plain FP32 GPU form (FP32 slots sampled separately, caller particle order);
+    # fp32_opt: FP32 with the locality layout; fp16: FP16 storage with the locality layout
+    forms = (("fp32", capi.F32, False, False), ("fp32_opt", capi.F32, True, True),
+             ("fp16", capi.F16, True, True))
+    for name, dtype, paired, reorder in forms:
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {args.chunk} steps, "
               f"{args.particles} particles", file=sys.stderr, flush=True)
         res[name] = run_form(capi, ctx, wind, seeds, gs, dtype, args.chunk, args.steps,
-                             args.warmup, exprs)
+                             args.warmup, exprs, paired=paired, reorder=reorder)
         r = res[name]
         r["psps"] = args.particles * args.chunk * args.steps / r["wall_s"]
         r["psps_kernel"] = args.particles * args.chunk * args.steps / (r["kernel_ms"] / 1e3)
@@ -240,6 +250,7 @@ def main():
                    "particles": args.particles, "grid": args.grid,
                    "steps_per_launch": args.chunk, "storage": "fp16"},
         "plain_fp32_value": f32["psps"],
+        "fp32_locality_value": res["fp32_opt"]["psps"],
         "speedup_vs_plain_fp32": f16["psps"] / f32["psps"],
         "kernel_psps_fp16": f16["psps_kernel"],
         "kernel_psps_fp32": f32["psps_kernel"],

@@ -239,6 +239,20 @@ int gr_job_last_advance_ms(gr_job * job, float * ms);
 /* Block size used for the job's step kernel (0 = automatic). */
 int gr_job_set_launch(gr_job * job, uint32_t block);
 
+/* Job options. GR_OPT_BLOCK: as gr_job_set_launch. GR_OPT_PAIRED (default 1): sample dynamic
+ * wind from a time-paired copy of the (back, front) slots -- one vector load per stencil corner
+ * for both slices; 0 samples the two slots separately. */
+#define GR_OPT_BLOCK 1u
+#define GR_OPT_PAIRED 2u
+int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
+
+/* Reorders the resident particle state by the Morton code of each particle's current cell
+ * (stopped particles last), so that neighbouring lanes gather neighbouring stencils. Results
+ * stay addressed by the caller's particle index: gr_job_read and gr_job_read_saved return
+ * them in the original order. Saved columns written before the call must be read before it.
+ * No counterpart in the reference (its particle order is fixed). */
+int gr_job_reorder(gr_job * job);
+
 #ifdef __cplusplus
 }
 #endif

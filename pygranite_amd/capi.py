@@ -29,7 +29,9 @@ EXPORTS = [
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
+    "gr_job_set_option", "gr_job_reorder",
 ]
+OPT_BLOCK, OPT_PAIRED = 1, 2
 
 
 class FieldDesc(ctypes.Structure):
@@ -120,6 +122,8 @@ def lib():
         "gr_job_saved_columns": ([vp, P(u64)], i32),
         "gr_job_last_advance_ms": ([vp, P(ctypes.c_float)], i32),
         "gr_job_set_launch": ([vp, u32], i32),
+        "gr_job_set_option": ([vp, u32, ctypes.c_int64], i32),
+        "gr_job_reorder": ([vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -244,6 +248,12 @@ class Job:
 
     def set_launch(self, block: int):
         _check(lib().gr_job_set_launch(self.h, block))
+
+    def set_option(self, option: int, value: int):
+        _check(lib().gr_job_set_option(self.h, option, value))
+
+    def reorder(self):
+        _check(lib().gr_job_reorder(self.h))
 
     def set_particle_data(self, kind: int, slot: int, data: np.ndarray):
         a = np.ascontiguousarray(data, dtype=np.float32)

@@ -88,6 +88,16 @@ struct gr_field
     bool stage_pending{false};
     std::vector<hipEvent_t> ready, inuse;
     std::vector<char> has_ready, has_inuse;
+    // time-paired copies of (back, front) slot pairs, built on the compute stream on demand
+    struct Pair
+    {
+        void * buf{nullptr};
+        uint32_t b{0}, f{0};
+        bool valid{false};
+        uint64_t used{0};
+    };
+    Pair pairs[2];
+    uint64_t pair_clock{0};
 };
 
 struct gr_job
@@ -119,6 +129,17 @@ struct gr_job
     uint64_t saved{0};
     hipEvent_t ev0{nullptr}, ev1{nullptr};
     bool timed{false};
+    // layout / reordering
+    uint32_t paired{1};          // use the time-paired layout when the wind is dynamic
+    uint32_t grid[3]{0, 0, 0};   // cells per axis of the last wind field (Morton keys)
+    uint32_t * orig{nullptr};    // caller index of each state slot
+    uint32_t * keys[2]{};        // sort buffers
+    uint32_t * vals[2]{};
+    void * sort_tmp{nullptr};
+    size_t sort_tmp_bytes{0};
+    bool reordered{false};
+    uint64_t reorder_col0{0};    // first saved column written in the current order
+    float * scratch2{nullptr};   // n * dim floats
 };
 
 // --------------------------------------------------------------------------------------------
@@ -215,6 +236,7 @@ static void field_free(gr_field * f)
 {
     if (!f) return;
     for (void * s : f->slots) dfree(s);
+    for (auto & p : f->pairs) dfree(p.buf);
     dfree(f->dstage);
     if (f->hstage) (void)hipHostFree(f->hstage);
     if (f->stage_done) (void)hipEventDestroy(f->stage_done);
@@ -290,6 +312,8 @@ extern "C" int gr_field_destroy(gr_field * f)
 static int field_pack(gr_field * f, uint32_t slot)
 {
     gr_ctx * c = f->ctx;
+    for (auto & p : f->pairs)
+        if (p.b == slot || p.f == slot) p.valid = false; // stale once the slot changes
     if (gr::launch_pack(f->dstage, f->slots[slot], f->nodes, f->d.ncomp, f->cstride, f->d.dtype,
                         c->copy) != 0)
         return set_err(GR_E_HIP, "pack kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -349,6 +373,40 @@ extern "C" int gr_field_bytes(const gr_field * f, uint64_t * bytes)
     return GR_OK;
 }
 
+// Returns (building it on stream `s` if needed) the time-paired copy of slots (b, f). The caller
+// has already made `s` wait for both slots' uploads; the slots are marked in use after the
+// build so later uploads wait for it.
+static int field_pair(gr_field * f, uint32_t b, uint32_t fr, hipStream_t s, const void ** out)
+{
+    *out = nullptr;
+    if (f->d.ncomp != 2 && f->d.ncomp != 3) return set_err(GR_E_INVALID, "pairing needs 2 or 3 components");
+    gr_field::Pair * hit = nullptr;
+    for (auto & p : f->pairs)
+        if (p.valid && p.b == b && p.f == fr) hit = &p;
+    if (!hit)
+    {
+        // least recently used entry
+        hit = &f->pairs[0];
+        for (auto & p : f->pairs)
+            if (!p.buf || p.used < hit->used) hit = &p;
+        const uint32_t so = f->d.ncomp == 3 ? 8 : 4;
+        if (!hit->buf)
+        {
+            int rc = dalloc(&hit->buf, f->nodes * so * dtype_size(f->d.dtype));
+            if (rc != GR_OK) return rc;
+        }
+        if (gr::launch_pair(f->slots[b], f->slots[fr], hit->buf, f->nodes, f->d.ncomp, f->cstride,
+                            f->d.dtype, s) != 0)
+            return set_err(GR_E_HIP, "pair kernel launch failed");
+        hit->b = b;
+        hit->f = fr;
+        hit->valid = true;
+    }
+    hit->used = ++f->pair_clock;
+    *out = hit->buf;
+    return GR_OK;
+}
+
 // --------------------------------------------------------------------------------------------
 // jobs
 // --------------------------------------------------------------------------------------------
@@ -376,6 +434,14 @@ static void job_free(gr_job * j)
     dfree(j->ring_expr);
     dfree(j->ring_vol);
     dfree(j->scratch_aos);
+    dfree(j->scratch2);
+    dfree(j->orig);
+    for (int k = 0; k < 2; ++k)
+    {
+        dfree(j->keys[k]);
+        dfree(j->vals[k]);
+    }
+    dfree(j->sort_tmp);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     delete j;
@@ -471,6 +537,9 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
         if (rc == GR_OK && d.nvol) rc = zalloc(&j->ring_vol, size_t(j->cap) * N * d.nvol);
     }
     if (rc == GR_OK) rc = dalloc(&j->scratch_aos, N * D * sizeof(float));
+    if (rc == GR_OK) rc = dalloc(&j->scratch2, N * D * sizeof(float));
+    for (int k = 0; k < 3; ++k)
+        j->grid[k] = d.grid_scale[k] > 0.f ? uint32_t(std::lround(double(d.domain[k]) / d.grid_scale[k])) : 1;
     if (rc == GR_OK && (hipEventCreate(&j->ev0) != hipSuccess || hipEventCreate(&j->ev1) != hipSuccess))
         rc = set_err(GR_E_HIP, "event creation failed");
     if (rc == GR_OK && n)
@@ -514,6 +583,98 @@ extern "C" int gr_job_set_launch(gr_job * j, uint32_t block)
     if (block != 0 && (block % 64 != 0 || block > 256))
         return set_err(GR_E_INVALID, "block must be 0 or a multiple of 64 up to 256");
     j->block = block;
+    return GR_OK;
+}
+
+extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
+{
+    if (!j) return set_err(GR_E_INVALID, "null job");
+    switch (option)
+    {
+        case GR_OPT_BLOCK: return gr_job_set_launch(j, uint32_t(value));
+        case GR_OPT_PAIRED: j->paired = value ? 1 : 0; return GR_OK;
+        default: return set_err(GR_E_INVALID, "unknown option %u", option);
+    }
+}
+
+// the state arrays of a job (4-byte elements, n each) in a fixed order
+static std::vector<void *> state_arrays(gr_job * j)
+{
+    std::vector<void *> a;
+    for (int k = 0; k < 3; ++k)
+        for (void * p : {static_cast<void *>(j->p[k]), static_cast<void *>(j->q[k]),
+                         static_cast<void *>(j->c[k]), static_cast<void *>(j->ld[k])})
+            if (p) a.push_back(p);
+    for (void * p : {static_cast<void *>(j->status), static_cast<void *>(j->abort_step),
+                     static_cast<void *>(j->total), static_cast<void *>(j->curv),
+                     static_cast<void *>(j->len)})
+        if (p) a.push_back(p);
+    for (float * e : j->expr)
+        if (e) a.push_back(e);
+    for (float * v : j->vol)
+        if (v) a.push_back(v);
+    return a;
+}
+
+extern "C" int gr_job_reorder(gr_job * j)
+{
+    if (!j) return set_err(GR_E_INVALID, "null job");
+    if (j->n < 2) return GR_OK;
+    GR_HIP(hipSetDevice(j->ctx->device));
+    hipStream_t s = j->ctx->compute;
+    const uint32_t n = uint32_t(j->n);
+    int rc = GR_OK;
+    for (int k = 0; k < 2 && rc == GR_OK; ++k)
+    {
+        if (!j->keys[k]) rc = dalloc(&j->keys[k], size_t(n) * 4);
+        if (rc == GR_OK && !j->vals[k]) rc = dalloc(&j->vals[k], size_t(n) * 4);
+    }
+    if (rc != GR_OK) return rc;
+    if (!j->orig)
+    {
+        if ((rc = dalloc(&j->orig, size_t(n) * 4)) != GR_OK) return rc;
+        std::vector<uint32_t> iota(n);
+        for (uint32_t i = 0; i < n; ++i) iota[i] = i;
+        GR_HIP(hipMemcpyAsync(j->orig, iota.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
+        GR_HIP(hipStreamSynchronize(s));
+    }
+    if (!j->sort_tmp)
+    {
+        size_t bytes = 0;
+        if (gr::sort_pairs(nullptr, bytes, j->keys[0], j->keys[1], j->vals[0], j->vals[1], n, s) != 0)
+            return set_err(GR_E_HIP, "radix sort sizing failed");
+        if ((rc = dalloc(&j->sort_tmp, bytes ? bytes : 1)) != GR_OK) return rc;
+        j->sort_tmp_bytes = bytes;
+    }
+    gr::StepParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.n = n;
+    P.status = j->status;
+    for (int k = 0; k < 3; ++k)
+    {
+        P.p[k] = j->p[k];
+        P.off[k] = j->d.local_coords ? 0.f : float(j->d.offset[k]);
+        P.inv_h[k] = 1.f / j->d.grid_scale[k];
+    }
+    P.wind.nx = std::max(1u, j->grid[0]);
+    P.wind.ny = std::max(1u, j->grid[1]);
+    P.wind.nz = std::max(1u, j->grid[2]);
+    if (gr::launch_morton_keys(P, j->d.dim, j->keys[0], j->vals[0], s) != 0)
+        return set_err(GR_E_HIP, "morton key kernel failed");
+    size_t tb = j->sort_tmp_bytes;
+    if (gr::sort_pairs(j->sort_tmp, tb, j->keys[0], j->keys[1], j->vals[0], j->vals[1], n, s) != 0)
+        return set_err(GR_E_HIP, "radix sort failed");
+    const uint32_t * perm = j->vals[1];
+    // permute every state array through the scratch buffer, then the index map
+    for (void * a : state_arrays(j))
+    {
+        if (gr::launch_gather32(a, j->scratch2, perm, n, s) != 0) return set_err(GR_E_HIP, "gather failed");
+        GR_HIP(hipMemcpyAsync(a, j->scratch2, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+    }
+    if (gr::launch_gather32(j->orig, j->scratch2, perm, n, s) != 0) return set_err(GR_E_HIP, "gather failed");
+    GR_HIP(hipMemcpyAsync(j->orig, j->scratch2, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+    j->reordered = true;
+    j->reorder_col0 = j->saved;
     return GR_OK;
 }
 
@@ -712,8 +873,17 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
         return rc;
     if (w->topography && (rc = wait_ready(w->topography, 0))) return rc;
 
+    const uint32_t mode = j->full ? gr::MODE_FULL
+                                  : ((d.windfield_mode == 1 && j->paired) ? gr::MODE_PAIRED : gr::MODE_SEPARATE);
+    if (mode == gr::MODE_PAIRED)
+        if ((rc = field_pair(wf, w->wind_back, w->wind_front, s, &P.wind.pair)) != GR_OK) return rc;
+    P.orig = j->reordered ? j->orig : nullptr;
+    j->grid[0] = wf->d.nx;
+    j->grid[1] = wf->d.ny;
+    j->grid[2] = wf->d.nz;
+
     GR_HIP(hipEventRecord(j->ev0, s));
-    if (gr::launch_advance(P, d.dim, d.integrator, j->full, j->block, s) != 0)
+    if (gr::launch_advance(P, d.dim, d.integrator, mode, j->block, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     GR_HIP(hipEventRecord(j->ev1, s));
     j->timed = true;
@@ -762,6 +932,14 @@ extern "C" int gr_job_read(gr_job * j, uint32_t kind, void * dst, uint64_t bytes
     if (!src) return set_err(GR_E_INVALID, "gr_job_read: output %u is not produced by this job", kind);
     if (bytes < need) return set_err(GR_E_INVALID, "gr_job_read: buffer too small");
     if (!n) return GR_OK;
+    if (j->reordered)
+    {
+        // back to the caller's particle order
+        const uint32_t w = kind == GR_OUT_POSITIONS ? j->d.dim : 1;
+        if (gr::launch_scatter32(src, j->scratch2, j->orig, n, w, s) != 0)
+            return set_err(GR_E_HIP, "scatter kernel failed");
+        src = j->scratch2;
+    }
     GR_HIP(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, s));
     GR_HIP(hipStreamSynchronize(s));
     return GR_OK;
@@ -803,6 +981,23 @@ extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint
     if (!base) return set_err(GR_E_INVALID, "gr_job_read_saved: output %u is not saved by this job", kind);
     hipStream_t s = j->ctx->compute;
     float * out = static_cast<float *>(dst);
+    if (j->reordered)
+    {
+        if (first < j->reorder_col0)
+            return set_err(GR_E_STATE, "column %llu was written before the last reorder",
+                           (unsigned long long)first);
+        const uint32_t w = kind == GR_OUT_POSITIONS ? j->d.dim : 1;
+        for (uint64_t c = 0; c < count; ++c)
+        {
+            const uint64_t slot = (first + c) % j->cap;
+            if (gr::launch_scatter32(base + slot * colsz, j->scratch2, j->orig, n, w, s) != 0)
+                return set_err(GR_E_HIP, "scatter kernel failed");
+            GR_HIP(hipMemcpyAsync(out + c * colsz, j->scratch2, colsz * sizeof(float),
+                                  hipMemcpyDeviceToHost, s));
+        }
+        GR_HIP(hipStreamSynchronize(s));
+        return GR_OK;
+    }
     uint64_t done = 0;
     while (done < count)
     {

@@ -39,6 +39,7 @@ struct FieldView
 {
     const void * back;
     const void * front;
+    const void * pair;  // time-paired copy of (back, front), or null
     uint32_t nx, ny, nz;
     uint32_t cstride;  // elements per node (4 for 3 components, else ncomp)
     uint32_t ncomp;
@@ -121,6 +122,10 @@ struct StepParams
     FieldView vols;
     FieldView topo;
 
+    // original particle index of each (reordered) state slot, or null for identity; used for
+    // per-particle inputs that stay in the caller's order
+    const uint32_t * orig;
+
     // per-particle data
     const float * lift_b;
     const float * lift_f;
@@ -130,9 +135,26 @@ struct StepParams
     Program prog[GR_MAX_PROGRAMS];
 };
 
+// kernel modes (see gr_kernels.hip)
+enum : uint32_t { MODE_PAIRED = 0, MODE_SEPARATE = 1, MODE_FULL = 2 };
+
 // launchers (gr_kernels.hip)
-int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t full,
+int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
                    uint32_t block, void * stream);
+// builds the time-paired layout of (back, front) slots: elements per node out = 8 (3D) / 4 (2D)
+int launch_pair(const void * back, const void * front, void * out, uint64_t nodes, uint32_t ncomp,
+                uint32_t cstride, uint32_t dtype, void * stream);
+// Morton keys of the particles' current cells (dead particles sort last)
+int launch_morton_keys(const StepParams & P, uint32_t dim, uint32_t * keys, uint32_t * vals,
+                       void * stream);
+// radix sort of (key, value) pairs; temp storage sized by a first call with temp == null
+int sort_pairs(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
+               const uint32_t * vin, uint32_t * vout, uint32_t n, void * stream);
+// out[i] = in[perm[i]] for 4-byte elements
+int launch_gather32(const void * in, void * out, const uint32_t * perm, uint64_t n, void * stream);
+// out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
+int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
+                     void * stream);
 int launch_pack(const float * src, void * dst, uint64_t nodes, uint32_t ncomp, uint32_t cstride,
                 uint32_t dtype, void * stream);
 int launch_seed(const float * aos, float * const * soa, uint64_t n, uint32_t dim,

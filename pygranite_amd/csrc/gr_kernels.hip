@@ -14,6 +14,7 @@
 // 1/256 for emulation).
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+#include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
 
 #include "gr_internal.h"
@@ -252,7 +253,71 @@ struct Ctx
     const StepParams & P;
 };
 
-template <int DIM, class T>
+// Time-paired node layout: both slices of a node in one vector, 3D as
+// [u0 v0 w0 u1 v1 w1 0 0], 2D as [u0 v0 u1 v1] (slice 0 = back, 1 = front).
+template <class T, int NC> struct PairLoad;
+
+template <> struct PairLoad<float, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
+    {
+        const float4 a = reinterpret_cast<const float4 *>(b)[2 * uint64_t(node)];
+        const float4 c = reinterpret_cast<const float4 *>(b)[2 * uint64_t(node) + 1];
+        x[0] = a.x; x[1] = a.y; x[2] = a.z;
+        y[0] = a.w; y[1] = c.x; y[2] = c.y;
+    }
+};
+template <> struct PairLoad<float, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
+    {
+        const float4 a = reinterpret_cast<const float4 *>(b)[node];
+        x[0] = a.x; x[1] = a.y;
+        y[0] = a.z; y[1] = a.w;
+    }
+};
+template <> struct PairLoad<__half, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
+    {
+        const uint4 a = reinterpret_cast<const uint4 *>(b)[node];
+        x[0] = h_lo(a.x); x[1] = h_hi(a.x); x[2] = h_lo(a.y);
+        y[0] = h_hi(a.y); y[1] = h_lo(a.z); y[2] = h_hi(a.z);
+    }
+};
+template <> struct PairLoad<__half, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
+    {
+        const uint2 a = reinterpret_cast<const uint2 *>(b)[node];
+        x[0] = h_lo(a.x); x[1] = h_hi(a.x);
+        y[0] = h_lo(a.y); y[1] = h_hi(a.y);
+    }
+};
+template <> struct PairLoad<bf16_t, 3>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
+    {
+        const uint4 a = reinterpret_cast<const uint4 *>(b)[node];
+        x[0] = b_lo(a.x); x[1] = b_hi(a.x); x[2] = b_lo(a.y);
+        y[0] = b_hi(a.y); y[1] = b_lo(a.z); y[2] = b_hi(a.z);
+    }
+};
+template <> struct PairLoad<bf16_t, 2>
+{
+    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
+    {
+        const uint2 a = reinterpret_cast<const uint2 *>(b)[node];
+        x[0] = b_lo(a.x); x[1] = b_hi(a.x);
+        y[0] = b_lo(a.y); y[1] = b_hi(a.y);
+    }
+};
+
+// kernel modes: lean kernels on the time-paired layout or on separate slots, and the full
+// kernel (every reference feature, separate slots)
+enum : int { M_PAIRED = 0, M_SEPARATE = 1, M_FULL = 2 };
+
+template <int DIM, class T, int MODE>
 __device__ __forceinline__ void velocity(const StepParams & P, const float * x, float t,
                                          float * v)
 {
@@ -261,6 +326,30 @@ __device__ __forceinline__ void velocity(const StepParams & P, const float * x, 
     for (int d = 0; d < DIM; ++d) arg[d] = (x[d] - P.off[d]) * P.inv_h[d];
     Taps S;
     taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
+    if (MODE == M_PAIRED)
+    {
+        // one vector load per corner gives both slices
+        float vb[8][DIM], vf[8][DIM];
+#pragma unroll
+        for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+        {
+            float xb[DIM], xf[DIM];
+            PairLoad<T, DIM>::get(P.wind.pair, S.node[c], xb, xf);
+            const bool ok = (S.valid >> c) & 1u;
+#pragma unroll
+            for (int m = 0; m < DIM; ++m)
+            {
+                vb[c][m] = ok ? xb[m] : 0.f;
+                vf[c][m] = ok ? xf[m] : 0.f;
+            }
+        }
+        float b[DIM], f[DIM];
+        combine<DIM, DIM>(vb, S, b);
+        combine<DIM, DIM>(vf, S, f);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
+        return;
+    }
     float b[DIM];
     sample_slot<DIM, T, DIM>(P.wind.back, S, b);
     if (P.flags & F_WIND_DYNAMIC)
@@ -343,11 +432,13 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
     return (P.flags & F_LOCAL) ? float(P.off64[d] + double(x)) : x;
 }
 
-template <int DIM, int INTEG, class T, bool FULL>
+template <int DIM, int INTEG, class T, int MODE>
 __global__ void __launch_bounds__(256) k_advance(const StepParams P)
 {
+    constexpr bool FULL = MODE == M_FULL;
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P.n) return;
+    const uint32_t oid = (FULL && P.orig) ? P.orig[pid] : pid; // caller-order index
 
     float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
     float ld[3] = {0.f, 0.f, 0.f};
@@ -401,22 +492,22 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
                 }
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T>(P, p, t, k1);
+                velocity<DIM, T, MODE>(P, p, t, k1);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T>(P, x, t1, k2);
+                velocity<DIM, T, MODE>(P, x, t1, k2);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T>(P, x, t1, k3);
+                velocity<DIM, T, MODE>(P, x, t1, k3);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T>(P, x, t2, k4);
+                velocity<DIM, T, MODE>(P, x, t2, k4);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -428,7 +519,7 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
             }
             else
             {
-                velocity<DIM, T>(P, p, t, vc);
+                velocity<DIM, T, MODE>(P, p, t, vc);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -444,8 +535,8 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
                 {
-                    const float lb = P.lift_b[uint64_t(pid) * DIM + d];
-                    const float l = (P.flags & F_UPLIFT_DYN) ? lerp(lb, P.lift_f[uint64_t(pid) * DIM + d], t) : lb;
+                    const float lb = P.lift_b[uint64_t(oid) * DIM + d];
+                    const float l = (P.flags & F_UPLIFT_DYN) ? lerp(lb, P.lift_f[uint64_t(oid) * DIM + d], t) : lb;
                     dp[d] = dp[d] + l;
                 }
             }
@@ -626,8 +717,8 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
                             float cv = 0.f;
                             if (k < int(P.nconst))
                             {
-                                cv = P.cst_b[k][pid];
-                                if (P.flags & F_CONST_DYN) cv = lerp(cv, P.cst_f[k][pid], t);
+                                cv = P.cst_b[k][oid];
+                                if (P.flags & F_CONST_DYN) cv = lerp(cv, P.cst_f[k][oid], t);
                             }
                             ref[12 + k] = cv;
                         }
@@ -764,29 +855,31 @@ __global__ void k_count_alive(const int32_t * __restrict__ status, uint64_t n,
 typedef void (*AdvanceFn)(const StepParams);
 
 template <int DIM, int INTEG, class T>
-static AdvanceFn pick_full(uint32_t full)
+static AdvanceFn pick_mode(uint32_t mode)
 {
-    return full ? k_advance<DIM, INTEG, T, true> : k_advance<DIM, INTEG, T, false>;
+    if (mode == M_FULL) return k_advance<DIM, INTEG, T, M_FULL>;
+    if (mode == M_PAIRED) return k_advance<DIM, INTEG, T, M_PAIRED>;
+    return k_advance<DIM, INTEG, T, M_SEPARATE>;
 }
 
 template <int DIM, int INTEG>
-static AdvanceFn pick_type(uint32_t dtype, uint32_t full)
+static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
 {
-    if (dtype == GR_F16) return pick_full<DIM, INTEG, __half>(full);
-    if (dtype == GR_BF16) return pick_full<DIM, INTEG, bf16_t>(full);
-    return pick_full<DIM, INTEG, float>(full);
+    if (dtype == GR_F16) return pick_mode<DIM, INTEG, __half>(mode);
+    if (dtype == GR_BF16) return pick_mode<DIM, INTEG, bf16_t>(mode);
+    return pick_mode<DIM, INTEG, float>(mode);
 }
 
-int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t full,
+int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
                    uint32_t block, void * stream)
 {
     AdvanceFn fn;
     if (dim == 3)
-        fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, full)
-                                  : pick_type<3, GR_EULER>(P.wind.dtype, full);
+        fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, mode)
+                                  : pick_type<3, GR_EULER>(P.wind.dtype, mode);
     else
-        fn = integrator == GR_RK4 ? pick_type<2, GR_RK4>(P.wind.dtype, full)
-                                  : pick_type<2, GR_EULER>(P.wind.dtype, full);
+        fn = integrator == GR_RK4 ? pick_type<2, GR_RK4>(P.wind.dtype, mode)
+                                  : pick_type<2, GR_EULER>(P.wind.dtype, mode);
     if (block == 0 || block > 256) block = 256;
     const uint32_t grid = uint32_t((uint64_t(P.n) + block - 1) / block);
     if (grid == 0) return 0;
@@ -841,6 +934,151 @@ int launch_count_alive(const int32_t * status, uint64_t n, unsigned long long * 
     if (grid == 0) return 0;
     hipLaunchKernelGGL(k_count_alive, dim3(grid), dim3(block), 0, hipStream_t(stream), status, n,
                        out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// --------------------------------------------------------------------------------------------
+// locality: time-paired slices and Morton reordering of particles
+// --------------------------------------------------------------------------------------------
+
+// out node = [back comps, front comps, zero pad] with `so` elements per node; W is the raw
+// element word (uint32_t for FP32, uint16_t for FP16 / BF16) -- bits are copied, not converted
+template <class W>
+__global__ void k_pair(const W * __restrict__ back, const W * __restrict__ front, W * __restrict__ out,
+                       uint64_t nodes, uint32_t ncomp, uint32_t cstride, uint32_t so)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nodes) return;
+    for (uint32_t c = 0; c < so; ++c)
+    {
+        W v = 0;
+        if (c < ncomp)
+            v = back[i * cstride + c];
+        else if (c < 2 * ncomp)
+            v = front[i * cstride + (c - ncomp)];
+        out[i * so + c] = v;
+    }
+}
+
+int launch_pair(const void * back, const void * front, void * out, uint64_t nodes, uint32_t ncomp,
+                uint32_t cstride, uint32_t dtype, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((nodes + block - 1) / block);
+    if (grid == 0) return 0;
+    const uint32_t so = ncomp == 3 ? 8 : 4;
+    if (dtype == GR_F32)
+        hipLaunchKernelGGL(k_pair<uint32_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
+                           static_cast<const uint32_t *>(back), static_cast<const uint32_t *>(front),
+                           static_cast<uint32_t *>(out), nodes, ncomp, cstride, so);
+    else
+        hipLaunchKernelGGL(k_pair<uint16_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
+                           static_cast<const uint16_t *>(back), static_cast<const uint16_t *>(front),
+                           static_cast<uint16_t *>(out), nodes, ncomp, cstride, so);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x) // 10 bits -> every third bit
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t spread2(uint32_t x) // 16 bits -> every second bit
+{
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+template <int DIM>
+__global__ void k_morton(const StepParams P, uint32_t * keys, uint32_t * vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t key = 0xffffffffu; // stopped particles go last
+    if (P.status[i] == GR_STATUS_TIME)
+    {
+        const uint32_t lim = DIM == 3 ? 1023u : 65535u;
+        uint32_t c[3] = {0, 0, 0};
+        const uint32_t n[3] = {P.wind.nx, P.wind.ny, P.wind.nz};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+        {
+            float a = (P.p[d][i] - P.off[d]) * P.inv_h[d];
+            a = fminf(fmaxf(a, 0.f), float(n[d] - 1));
+            c[d] = min(uint32_t(a), lim);
+        }
+        key = DIM == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
+                       : (spread2(c[0]) | (spread2(c[1]) << 1));
+        key = min(key, 0xfffffffeu);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+int launch_morton_keys(const StepParams & P, uint32_t dim, uint32_t * keys, uint32_t * vals,
+                       void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = (P.n + block - 1) / block;
+    if (grid == 0) return 0;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_morton<3>, dim3(grid), dim3(block), 0, hipStream_t(stream), P, keys, vals);
+    else
+        hipLaunchKernelGGL(k_morton<2>, dim3(grid), dim3(block), 0, hipStream_t(stream), P, keys, vals);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sort_pairs(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
+               const uint32_t * vin, uint32_t * vout, uint32_t n, void * stream)
+{
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
+                                                            int(n), 0, 32, hipStream_t(stream));
+    return e == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_gather32(const uint32_t * __restrict__ in, uint32_t * __restrict__ out,
+                           const uint32_t * __restrict__ perm, uint64_t n)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+
+__global__ void k_scatter32(const uint32_t * __restrict__ in, uint32_t * __restrict__ out,
+                            const uint32_t * __restrict__ orig, uint64_t n, uint32_t w)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t o = orig[i];
+    for (uint32_t c = 0; c < w; ++c) out[o * w + c] = in[i * w + c];
+}
+
+int launch_gather32(const void * in, void * out, const uint32_t * perm, uint64_t n, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(k_gather32, dim3(grid), dim3(block), 0, hipStream_t(stream),
+                       static_cast<const uint32_t *>(in), static_cast<uint32_t *>(out), perm, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
+                     void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    if (grid == 0) return 0;
+    hipLaunchKernelGGL(k_scatter32, dim3(grid), dim3(block), 0, hipStream_t(stream),
+                       static_cast<const uint32_t *>(in), static_cast<uint32_t *>(out), orig, n, w);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

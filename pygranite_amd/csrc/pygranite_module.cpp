@@ -103,6 +103,7 @@ struct IntegratorSettings
     bool LocalCoordinates{false};
     bool AllowAbs{false};
     uint32_t BlockSize{0};
+    bool SortParticles{true}; // Morton-reorder particle state at window edges (results unchanged)
 };
 
 // ---- DataLoader: source/granite/DataLoader.hpp:26-144 ------------------------------------
@@ -643,13 +644,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     uint64_t cols_read = 0;
     bool abort_due_alive = false;
 
-    auto drain = [&](uint64_t upto_steps_counter_first_saved_time_zero) {
-        (void)upto_steps_counter_first_saved_time_zero;
-    };
-    (void)drain;
-
-    // reads saved columns [cols_read, saved) into the set; `zero_col` = global index of a save
-    // taken at Time == 0 (no curvature column then, tpp:1004)
+    // reads saved columns [cols_read, saved) into the set
     auto read_columns = [&]() {
         uint64_t saved = 0;
         check(gr_job_saved_columns(job.j, &saved));
@@ -730,10 +725,13 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
         w.time_back64 = double(window) * double(s_.DataTimeDistance);
         w.time_front64 = double(window + 1) * double(s_.DataTimeDistance);
 
+        // particles moved during the last window: regroup them by cell (all saved columns
+        // have been read at this point)
+        if (s_.SortParticles && n >= 4096) check(gr_job_reorder(job.j));
+
         // run the window in chunks (each fits the column ring); with MinimumAliveParticles the
         // alive count gates every single step (tpp:973-976)
         bool window_done = false;
-        bool launched = false;
         while (!window_done)
         {
             uint32_t k = 0;
@@ -763,20 +761,15 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
             w.counter_start = counter;
             w.nsteps = k;
             check(gr_job_advance(job.j, &w));
-            launched = true;
-            if (!s_.MinimumAliveParticles || SI)
-            {
-                // stream the next slice while the chunk computes, only after the window's
-                // last chunk is queued (the loader is stepped once per window, tpp:887)
-            }
             counter += k;
             time32 = t;
             time64 = t64;
             if (!window_done && !(float(counter) < max_steps_f)) window_done = true;
+            // columns of earlier chunks are read while later chunks queue; the window's last
+            // chunk is read after the loader has been stepped (overlapping the next upload)
             if (!window_done) read_columns();
             if (abort_due_alive) window_done = true;
         }
-        (void)launched;
 
         bool stepped = false;
         if (dyn)
@@ -955,7 +948,8 @@ PYBIND11_MODULE(pygranite, m)
         .def_readwrite("TimeFP64", &IntegratorSettings::TimeFP64)
         .def_readwrite("LocalCoordinates", &IntegratorSettings::LocalCoordinates)
         .def_readwrite("AllowAbs", &IntegratorSettings::AllowAbs)
-        .def_readwrite("BlockSize", &IntegratorSettings::BlockSize);
+        .def_readwrite("BlockSize", &IntegratorSettings::BlockSize)
+        .def_readwrite("SortParticles", &IntegratorSettings::SortParticles);
 
     py::class_<TrajectoryIntegrator>(m, "TrajectoryIntegrator")
         .def(py::init<IntegratorSettings &, DataLoader &, TrajectorySet &>(), py::keep_alive<1, 3>())

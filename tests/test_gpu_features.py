@@ -1,0 +1,185 @@
+"""GPU parity for the kernel features outside the benchmark configurations: 2D loop borders,
+sphere coordinates, Length / FitLength aborts, uplift, constants, additional volumes,
+topography, reverse integration, constant wind fields, MinimumAliveParticles, SaveInterval 0,
+and chunking invariance through the C ABI."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_bitexact, run_product
+from pygranite_amd import capi, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def gyre(n=64, m=32, S=4):
+    wind, gs = synthetic.double_gyre(n, m, S)
+    return wind, gs
+
+
+def base2(gs, **kw):
+    d = dict(Space="Space2D", Integrator="ClassicRungeKutta", DeltaT=0.02, DataTimeDistance=1.0,
+             GridScale=gs, Offset=[0.0, 0.0], MaximumSimulationTime=2.0, SaveInterval=2)
+    d.update(kw)
+    return d
+
+
+def abc(n=24, S=4):
+    return synthetic.abc_flow(n, S)
+
+
+def base3(gs, **kw):
+    d = dict(Space="Space3D", Integrator="ClassicRungeKutta", DeltaT=0.01, DataTimeDistance=1.0,
+             GridScale=gs, Offset=[0.0, 0.0, 0.0], MaximumSimulationTime=1.5, SaveInterval=3)
+    d.update(kw)
+    return d
+
+
+def seeds2(n, seed=7):
+    return synthetic.uniform_seeds(seed, n, [0.02, 0.02], [1.98, 0.98])
+
+
+def seeds3(n, seed=8):
+    return synthetic.uniform_seeds(seed, n, [0.05] * 3, [2 * math.pi - 0.05] * 3)
+
+
+@pytest.mark.parametrize("border", ["LoopX", "LoopY", "LoopXY"])
+def test_loop_borders_2d(pg, gpu_available, border):
+    wind, gs = gyre()
+    wind = wind.copy()
+    wind[:, 0] += 0.3  # drift so particles cross the x edges
+    wind[:, 1] += 0.2
+    st = base2(gs, BorderMode=border, Integrator="ExplicitEuler")
+    s = seeds2(1024)
+    gpu = run_product(pg, st, wind, s)
+    ref = oracle.run(st, wind, s, real64=False)
+    assert_bitexact(gpu, ref, label=border + " ")
+
+
+def test_sphere_coordinates_loopxy(pg, gpu_available):
+    """Sphere coordinates call cosf (device libm vs glibc): compared at tolerance."""
+    N, M = 72, 36
+    lon = np.arange(N) * 5.0
+    lat = np.arange(M) * 5.0 - 87.5
+    wind = np.zeros((3, 2, M, N), np.float32)
+    wind[:, 0] = 20.0
+    wind[:, 1] = 5.0 * np.cos(np.radians(lon))[None, None, :]
+    st = base2([5.0, 5.0], BorderMode="LoopXY", SphereRadius=6371000.0, DeltaT=600.0,
+               DataTimeDistance=3600.0, MaximumSimulationTime=7200.0, Offset=[0.0, -90.0],
+               SaveInterval=1)
+    s = synthetic.uniform_seeds(9, 512, [0.0, -80.0], [355.0, 80.0])
+    gpu = run_product(pg, st, wind, s)
+    ref = oracle.run(st, wind, s, real64=False)
+    assert np.array_equal(gpu["status"], ref["status"])
+    np.testing.assert_allclose(gpu["trajectories"], ref["trajectories"], rtol=0, atol=2e-3)
+
+
+@pytest.mark.parametrize("mode", ["Length", "FitLength"])
+def test_length_aborts(pg, gpu_available, mode):
+    wind, gs = abc()
+    st = base3(gs, AbortMode=mode, MaximumLength=1.5)
+    s = seeds3(1024)
+    gpu = run_product(pg, st, wind, s)
+    ref = oracle.run(st, wind, s, real64=False)
+    assert_bitexact(gpu, ref, fields=("trajectories", "lengths"), label=mode + " ")
+    assert (gpu["status"] == 1).sum() > 100
+
+
+def test_reverse_and_constant_windfield(pg, gpu_available):
+    wind, gs = abc()
+    for extra in (dict(Reverse=True), dict(WindfieldMode="Constant")):
+        st = base3(gs, **extra)
+        s = seeds3(512)
+        gpu = run_product(pg, st, wind, s)
+        ref = oracle.run(st, wind, s, real64=False)
+        assert_bitexact(gpu, ref, label=str(extra) + " ")
+
+
+def test_uplift_constants_volumes_topography(pg, gpu_available):
+    wind, gs = abc(24, 4)
+    n = 512
+    s = seeds3(n)
+    rng = np.random.default_rng(3)
+    S = wind.shape[0]
+    uplift = rng.normal(0, 1e-3, (S, n, 3)).astype(np.float32)
+    consts = {"alpha": rng.uniform(0, 1, (S, n)).astype(np.float32),
+              "beta": rng.uniform(1, 2, (S, n)).astype(np.float32)}
+    vols = {"temp": rng.uniform(250, 300, (S, 24, 24, 24)).astype(np.float32),
+            "qv": rng.uniform(0, 0.01, (S, 24, 24, 24)).astype(np.float32)}
+    topo = np.full((24, 24), 0.3, np.float32)
+    st = base3(gs, UpLiftMode="Dynamic", ConstantsMode="Dynamic", AdditionalVolumeMode="Constant",
+               AdditionalCompute=["alpha*temp + beta*qv", "x+y+z-u*v"])
+    gpu = run_product(pg, st, wind, s, uplift=uplift, constants=consts, volumes=vols,
+                      topography=topo)
+    ref = oracle.run(st, wind, s, real64=False, uplift=uplift, constants=consts, volumes=vols,
+                     topography=topo)
+    assert_bitexact(gpu, ref, label="extras ")
+    for e in range(2):
+        np.testing.assert_array_equal(gpu["compute"][e], ref["compute"][e].astype(np.float32))
+    for k in vols:
+        np.testing.assert_array_equal(gpu["volumes"][k], ref["volumes"][k].astype(np.float32))
+    assert (gpu["status"] == 3).sum() > 0  # some particles hit the topography
+
+
+def test_fast_total_curvature(pg, gpu_available):
+    wind, gs = abc()
+    st = base3(gs, CurvatureMode="FastTotalCurvature")
+    s = seeds3(512)
+    gpu = run_product(pg, st, wind, s)
+    ref = oracle.run(st, wind, s, real64=False)
+    assert_bitexact(gpu, ref, label="fast ")
+    # acosf differs between device libm and glibc by ulps
+    np.testing.assert_allclose(gpu["total_curvature"], ref["total_curvature"], rtol=1e-4, atol=1e-4)
+
+
+def test_minimum_alive_and_save_interval_zero(pg, gpu_available):
+    wind, gs = gyre()
+    st = base2(gs, MinimumAliveParticles=400, SaveInterval=0, MaximumSimulationTime=3.0)
+    s = synthetic.uniform_seeds(5, 512, [0.02, 0.02], [1.98, 0.98])
+    wind = wind.copy()
+    wind[:, 0] += 0.5  # particles leave through x = 2 over time
+    gpu = run_product(pg, st, wind, s)
+    ref = oracle.run(st, wind, s, real64=False)
+    assert_bitexact(gpu, ref, label="alive ")
+    assert gpu["trajectories"].shape[1] == 2
+
+
+def test_chunked_advances_equal_one_advance(gpu_available):
+    """Splitting a window into several launches gives identical state (C ABI)."""
+    wind, gs = abc(32, 3)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F32, nslots=3)
+    for sl in range(3):
+        f.upload(sl, [wind[sl, c] for c in range(3)])
+    s = seeds3(2048)
+    outs = []
+    for splits in ([100], [7, 13, 30, 50]):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode = 3, 1, 3, 1
+        d.dt, d.dtd = 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k] = gs[k]
+            d.domain[k] = 32 * gs[k]
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v+w*w)", curvature=3)
+        job = capi.Job(ctx, d, s, save_capacity=100)
+        counter, t = 0, np.float32(0)
+        for k in splits:
+            w = capi.Window()
+            w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+            w.time_back, w.time_front = 0.0, 1.0
+            w.time_start, w.counter_start, w.nsteps = float(t), counter, k
+            job.advance(w)
+            for _ in range(k):
+                t = np.float32(t + np.float32(0.01))
+            counter += k
+        outs.append((job.read(capi.OUT_POSITIONS), job.read_saved(capi.OUT_POSITIONS, 0, 100),
+                     job.read_saved(capi.OUT_EXPR0, 0, 100), job.read(capi.OUT_TOTAL_CURVATURE)))
+        job.close()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    f.close()
+    ctx.close()
