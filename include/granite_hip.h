@@ -253,6 +253,33 @@ int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
  * No counterpart in the reference (its particle order is fixed). */
 int gr_job_reorder(gr_job * job);
 
+/* ---- batches: many jobs in one persistent launch -------------------------------------------
+ * Each span advances one job through consecutive windows of a wind field whose slices are all
+ * resident (slot s holds slice s). Window edges follow the reference's host schedule
+ * (FP32 time, max_windfield_time += DataTimeDistance per window, TrajectoryIntegrator.tpp:
+ * 852-994); a span ends after max_steps steps, when the step counter reaches
+ * max_sim_time / dt, or when no further slice exists (the loader's step() == false). Jobs of
+ * different dimension, integrator and storage type run in the same launch: workgroups pull
+ * (job, particle block) items from a device counter, longest spans first. Jobs must use the
+ * lean feature set (no uplift, constants, volumes, topography, loop borders, sphere or length
+ * aborts). No counterpart in the reference (one job per TrajectoryIntegrator). */
+typedef struct {
+    gr_job * job;
+    gr_field * wind;
+    uint32_t first_window;  /* COUNTER of the first window */
+    uint32_t max_steps;
+    float time_start;       /* data.Time at the first step */
+    float max_sim_time;     /* MaximumSimulationTime (<= 0: no limit) */
+    double time_start64;
+    uint64_t counter_start; /* _simulation_counter at the first step */
+} gr_span;
+
+/* Runs the spans; steps_out[i] receives the steps span i ran (may be NULL). `block` = 0 picks
+ * the default (256). The call returns after queueing; *ms (may be NULL) receives the device
+ * time of the launch and makes the call wait for it. */
+int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
+                 uint32_t * steps_out, float * ms);
+
 #ifdef __cplusplus
 }
 #endif

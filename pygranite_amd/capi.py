@@ -29,7 +29,7 @@ EXPORTS = [
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
-    "gr_job_set_option", "gr_job_reorder",
+    "gr_job_set_option", "gr_job_reorder", "gr_batch_run",
 ]
 OPT_BLOCK, OPT_PAIRED = 1, 2
 
@@ -83,6 +83,13 @@ class Window(ctypes.Structure):
                 ("nsteps", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class Span(ctypes.Structure):
+    _fields_ = [("job", ctypes.c_void_p), ("wind", ctypes.c_void_p),
+                ("first_window", ctypes.c_uint32), ("max_steps", ctypes.c_uint32),
+                ("time_start", ctypes.c_float), ("max_sim_time", ctypes.c_float),
+                ("time_start64", ctypes.c_double), ("counter_start", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -124,6 +131,7 @@ def lib():
         "gr_job_set_launch": ([vp, u32], i32),
         "gr_job_set_option": ([vp, u32, ctypes.c_int64], i32),
         "gr_job_reorder": ([vp], i32),
+        "gr_batch_run": ([vp, P(Span), u32, u32, P(u32), P(ctypes.c_float)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -178,6 +186,25 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = ctypes.c_void_p()
         _check(lib().gr_ctx_create(device, ctypes.byref(self.h)))
+
+    def batch_run(self, spans, block: int = 0, timed: bool = True):
+        """Runs [(job, field, dict(first_window=, max_steps=, time_start=, max_sim_time=,
+        counter_start=))...] in one persistent launch; returns (steps per span, device ms)."""
+        arr = (Span * max(1, len(spans)))()
+        for i, (job, field, kw) in enumerate(spans):
+            s = arr[i]
+            s.job, s.wind = job.h, field.h
+            s.first_window = kw.get("first_window", 0)
+            s.max_steps = kw["max_steps"]
+            s.time_start = kw.get("time_start", 0.0)
+            s.time_start64 = kw.get("time_start", 0.0)
+            s.max_sim_time = kw.get("max_sim_time", 0.0)
+            s.counter_start = kw.get("counter_start", 0)
+        steps = (ctypes.c_uint32 * max(1, len(spans)))()
+        ms = ctypes.c_float()
+        _check(lib().gr_batch_run(self.h, arr, len(spans), block, steps,
+                                  ctypes.byref(ms) if timed else None))
+        return list(steps)[:len(spans)], ms.value
 
     def info(self):
         name = ctypes.create_string_buffer(256)

@@ -9,6 +9,8 @@
 // ordered against kernels with events, so the next slice streams in while a window computes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -72,6 +74,16 @@ struct gr_ctx
     hipStream_t compute{nullptr};
     hipStream_t copy{nullptr};
     unsigned long long * dcount{nullptr};
+    // persistent batched launches
+    std::vector<gr::StepParams> h_params;
+    std::vector<gr::BatchItem> h_items;
+    gr::StepParams * d_params{nullptr};
+    size_t params_cap{0};
+    gr::BatchItem * d_items{nullptr};
+    size_t items_cap{0};
+    uint32_t * d_counter{nullptr};
+    hipEvent_t bev0{nullptr}, bev1{nullptr};
+    int cus{0};
 };
 
 struct gr_field
@@ -96,7 +108,7 @@ struct gr_field
         bool valid{false};
         uint64_t used{0};
     };
-    Pair pairs[2];
+    std::vector<Pair> pairs; // at most max(2, nslots - 1): every window of a resident dataset
     uint64_t pair_clock{0};
 };
 
@@ -140,6 +152,12 @@ struct gr_job
     bool reordered{false};
     uint64_t reorder_col0{0};    // first saved column written in the current order
     float * scratch2{nullptr};   // n * dim floats
+    // window tables of multi-window launches (kept alive while copies are queued)
+    std::vector<const void *> h_win_ptr;
+    std::vector<uint32_t> h_win_edge;
+    const void ** d_win_ptr{nullptr};
+    uint32_t * d_win_edge{nullptr};
+    size_t win_cap{0};
 };
 
 // --------------------------------------------------------------------------------------------
@@ -199,6 +217,11 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     (void)hipStreamSynchronize(c->compute);
     (void)hipStreamSynchronize(c->copy);
     dfree(c->dcount);
+    dfree(c->d_params);
+    dfree(c->d_items);
+    dfree(c->d_counter);
+    if (c->bev0) (void)hipEventDestroy(c->bev0);
+    if (c->bev1) (void)hipEventDestroy(c->bev1);
     (void)hipStreamDestroy(c->compute);
     (void)hipStreamDestroy(c->copy);
     delete c;
@@ -385,10 +408,25 @@ static int field_pair(gr_field * f, uint32_t b, uint32_t fr, hipStream_t s, cons
         if (p.valid && p.b == b && p.f == fr) hit = &p;
     if (!hit)
     {
-        // least recently used entry
-        hit = &f->pairs[0];
-        for (auto & p : f->pairs)
-            if (!p.buf || p.used < hit->used) hit = &p;
+        const size_t max_pairs = std::max<size_t>(2, f->d.nslots > 1 ? f->d.nslots - 1 : 1);
+        if (f->pairs.size() < max_pairs)
+        {
+            // a stale entry is reused before the cache grows
+            for (auto & p : f->pairs)
+                if (!p.valid) hit = &p;
+            if (!hit)
+            {
+                f->pairs.emplace_back();
+                hit = &f->pairs.back();
+            }
+        }
+        else
+        {
+            // stale entries first, then the least recently used one
+            hit = &f->pairs[0];
+            for (auto & p : f->pairs)
+                if ((!p.valid && hit->valid) || (p.valid == hit->valid && p.used < hit->used)) hit = &p;
+        }
         const uint32_t so = f->d.ncomp == 3 ? 8 : 4;
         if (!hit->buf)
         {
@@ -442,6 +480,8 @@ static void job_free(gr_job * j)
         dfree(j->vals[k]);
     }
     dfree(j->sort_tmp);
+    dfree(j->d_win_ptr);
+    dfree(j->d_win_edge);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     delete j;
@@ -726,7 +766,10 @@ static void view(gr::FieldView & v, const gr_field * f, uint32_t back, uint32_t 
     v.dtype = f->d.dtype;
 }
 
-extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
+// Validates the window, fills the kernel parameters, makes the compute stream wait for the
+// slices the window reads and builds the paired copy when the paired layout is used.
+static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, uint32_t & mode,
+                           uint64_t & nsaves)
 {
     if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
     const gr_job_desc & d = j->d;
@@ -740,10 +783,8 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
         return set_err(GR_E_INVALID, "volume field missing or too small");
     if (w->topography && (w->topography->d.nx != wf->d.nx || w->topography->d.ny != wf->d.ny))
         return set_err(GR_E_INVALID, "topography grid does not match the wind grid");
-    if (w->nsteps == 0) return GR_OK;
     GR_HIP(hipSetDevice(j->ctx->device));
 
-    gr::StepParams P;
     std::memset(&P, 0, sizeof(P));
     P.n = uint32_t(j->n);
     P.nsteps = w->nsteps;
@@ -791,7 +832,7 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     P.counter_start = w->counter_start;
     P.save_interval = d.save_interval;
     P.save_first = ~0ull;
-    uint64_t nsaves = 0;
+    nsaves = 0;
     if (d.save_interval)
     {
         const uint64_t r = w->counter_start % d.save_interval;
@@ -859,21 +900,13 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
         if (f && f->has_ready[slot]) GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
         return GR_OK;
     };
-    auto mark_inuse = [&](gr_field * f, uint32_t slot) -> int {
-        if (f)
-        {
-            GR_HIP(hipEventRecord(f->inuse[slot], s));
-            f->has_inuse[slot] = 1;
-        }
-        return GR_OK;
-    };
     int rc;
     if ((rc = wait_ready(wf, w->wind_back)) || (rc = wait_ready(wf, w->wind_front))) return rc;
     if (d.nvol && ((rc = wait_ready(w->volumes, w->vol_back)) || (rc = wait_ready(w->volumes, w->vol_front))))
         return rc;
     if (w->topography && (rc = wait_ready(w->topography, 0))) return rc;
 
-    const uint32_t mode = j->full ? gr::MODE_FULL
+    mode = j->full ? gr::MODE_FULL
                                   : ((d.windfield_mode == 1 && j->paired) ? gr::MODE_PAIRED : gr::MODE_SEPARATE);
     if (mode == gr::MODE_PAIRED)
         if ((rc = field_pair(wf, w->wind_back, w->wind_front, s, &P.wind.pair)) != GR_OK) return rc;
@@ -882,16 +915,288 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     j->grid[1] = wf->d.ny;
     j->grid[2] = wf->d.nz;
 
+    return GR_OK;
+}
+
+static int mark_inuse(gr_field * f, uint32_t slot, hipStream_t s)
+{
+    if (f)
+    {
+        GR_HIP(hipEventRecord(f->inuse[slot], s));
+        f->has_inuse[slot] = 1;
+    }
+    return GR_OK;
+}
+
+// after the launch: later uploads into the slots read by the window wait for it
+static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves)
+{
+    hipStream_t s = j->ctx->compute;
+    int rc;
+    if ((rc = mark_inuse(w->wind, w->wind_back, s)) || (rc = mark_inuse(w->wind, w->wind_front, s))) return rc;
+    if (j->d.nvol && ((rc = mark_inuse(w->volumes, w->vol_back, s)) || (rc = mark_inuse(w->volumes, w->vol_front, s))))
+        return rc;
+    if (w->topography && (rc = mark_inuse(w->topography, 0, s))) return rc;
+    j->saved += nsaves;
+    return GR_OK;
+}
+
+extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
+{
+    if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
+    if (w->nsteps == 0) return GR_OK;
+    gr::StepParams P;
+    uint32_t mode = 0;
+    uint64_t nsaves = 0;
+    int rc = prepare_advance(j, w, P, mode, nsaves);
+    if (rc != GR_OK) return rc;
+    hipStream_t s = j->ctx->compute;
     GR_HIP(hipEventRecord(j->ev0, s));
-    if (gr::launch_advance(P, d.dim, d.integrator, mode, j->block, s) != 0)
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     GR_HIP(hipEventRecord(j->ev1, s));
     j->timed = true;
-    if ((rc = mark_inuse(wf, w->wind_back)) || (rc = mark_inuse(wf, w->wind_front))) return rc;
-    if (d.nvol && ((rc = mark_inuse(w->volumes, w->vol_back)) || (rc = mark_inuse(w->volumes, w->vol_front))))
-        return rc;
-    if (w->topography && (rc = mark_inuse(w->topography, 0))) return rc;
-    j->saved += nsaves;
+    return finish_advance(j, w, nsaves);
+}
+
+// --------------------------------------------------------------------------------------------
+// batches
+// --------------------------------------------------------------------------------------------
+
+// makes stream `s` wait for the last upload of a slot
+static int f_wait(gr_field * f, uint32_t slot, hipStream_t s)
+{
+    if (f && slot < f->d.nslots && f->has_ready[slot]) GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
+    return GR_OK;
+}
+
+// Steps per window of a span under the reference's host schedule (tpp:852-994): FP32 (or FP64)
+// time, max_windfield_time accumulated by += DataTimeDistance, stop at counter >= max_sim / dt
+// or when slice c + 1 does not exist.
+static void plan_span(const gr_job_desc & d, const gr_span & sp, uint32_t nslices,
+                      std::vector<uint32_t> & win_steps)
+{
+    win_steps.clear();
+    float mwt = d.dtd;
+    double mwt64 = d.dtd;
+    for (uint32_t c = 0; c < sp.first_window; ++c)
+    {
+        mwt += d.dtd;
+        mwt64 += double(d.dtd);
+    }
+    const bool capped = sp.max_sim_time > 0.f;
+    const float maxs_f = capped ? sp.max_sim_time / d.dt : 0.f;
+    float t = sp.time_start;
+    double t64 = sp.time_start64;
+    uint64_t counter = sp.counter_start;
+    uint64_t total = 0;
+    uint32_t c = sp.first_window;
+    while (total < sp.max_steps && c + 1 < nslices)
+    {
+        uint32_t k = 0;
+        bool window_end = false;
+        while (total + k < sp.max_steps && (!capped || float(counter + k) < maxs_f))
+        {
+            ++k;
+            t += d.dt;
+            t64 += double(d.dt);
+            if (d.time64 ? (t64 >= mwt64) : (t >= mwt))
+            {
+                window_end = true;
+                break;
+            }
+        }
+        if (k == 0) break;
+        win_steps.push_back(k);
+        total += k;
+        counter += k;
+        if (!window_end) break;
+        mwt += d.dtd;
+        mwt64 += double(d.dtd);
+        ++c;
+    }
+}
+
+extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
+                            uint32_t * steps_out, float * ms)
+{
+    if (!ctx || (nspans && !spans)) return set_err(GR_E_INVALID, "gr_batch_run: null argument");
+    if (block == 0) block = 256;
+    if (block % 64 != 0 || block > 256) return set_err(GR_E_INVALID, "block must be a multiple of 64 up to 256");
+    GR_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->compute;
+    if (!ctx->bev0)
+    {
+        GR_HIP(hipEventCreate(&ctx->bev0));
+        GR_HIP(hipEventCreate(&ctx->bev1));
+        hipDeviceProp_t prop;
+        GR_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        ctx->cus = prop.multiProcessorCount;
+    }
+    ctx->h_params.resize(nspans);
+    std::vector<uint64_t> saves(nspans, 0);
+    std::vector<uint32_t> span_steps(nspans, 0);
+    std::vector<uint32_t> span_nwin(nspans, 0);
+    std::vector<uint32_t> win_steps;
+    std::vector<gr_window> first_win(nspans);
+    int rc;
+    for (uint32_t i = 0; i < nspans; ++i)
+    {
+        const gr_span & sp = spans[i];
+        gr_job * j = sp.job;
+        if (!j || !sp.wind) return set_err(GR_E_INVALID, "span %u: null job or field", i);
+        if (j->ctx != ctx) return set_err(GR_E_INVALID, "span %u: job of another context", i);
+        if (j->full) return set_err(GR_E_INVALID, "span %u: the job uses features outside the batched kernel", i);
+        const gr_job_desc & d = j->d;
+        const uint32_t nslices = sp.wind->d.nslots;
+        if (d.windfield_mode == 1)
+            plan_span(d, sp, nslices, win_steps);
+        else
+        {
+            win_steps.clear(); // constant wind: one window of slot 0, bounded by max_sim_time
+            gr_span one = sp;
+            one.first_window = 0;
+            gr_job_desc dd = d;
+            dd.dtd = sp.max_sim_time > 0.f ? sp.max_sim_time : 3.4e38f;
+            plan_span(dd, one, 2, win_steps);
+            if (win_steps.size() > 1) win_steps.resize(1);
+        }
+        uint32_t total = 0;
+        for (uint32_t k : win_steps) total += k;
+        span_steps[i] = total;
+        span_nwin[i] = uint32_t(win_steps.size());
+        gr_window & w = first_win[i];
+        std::memset(&w, 0, sizeof(w));
+        const uint32_t c0 = d.windfield_mode == 1 ? sp.first_window : 0;
+        w.wind = sp.wind;
+        w.wind_back = c0;
+        w.wind_front = d.windfield_mode == 1 ? c0 + 1 : c0;
+        w.time_back = float(c0) * d.dtd;
+        w.time_front = float(c0 + 1) * d.dtd;
+        w.time_back64 = double(c0) * double(d.dtd);
+        w.time_front64 = double(c0 + 1) * double(d.dtd);
+        w.time_start = sp.time_start;
+        w.time_start64 = sp.time_start64;
+        w.counter_start = sp.counter_start;
+        w.nsteps = total;
+        gr::StepParams & P = ctx->h_params[i];
+        uint32_t mode = 0;
+        if (total == 0)
+        {
+            std::memset(&P, 0, sizeof(P));
+            continue;
+        }
+        if ((rc = prepare_advance(j, &w, P, mode, saves[i])) != GR_OK) return rc;
+        // the remaining windows of the span: slices c, c + 1 for c = c0 + 1 ...
+        const uint32_t nwin = uint32_t(win_steps.size());
+        P.nwin = nwin;
+        P.win_first = c0;
+        P.dtd = d.dtd;
+        if (nwin > 1)
+        {
+            j->h_win_ptr.assign(size_t(3) * nwin, nullptr);
+            j->h_win_edge.assign(nwin, 0);
+            uint32_t edge = 0;
+            for (uint32_t k = 0; k < nwin; ++k)
+            {
+                const uint32_t c = c0 + k;
+                edge += win_steps[k];
+                j->h_win_edge[k] = edge;
+                if (f_wait(sp.wind, c, s) != GR_OK || f_wait(sp.wind, c + 1, s) != GR_OK) return GR_E_HIP;
+                j->h_win_ptr[3 * k + 0] = sp.wind->slots[c];
+                j->h_win_ptr[3 * k + 1] = sp.wind->slots[c + 1];
+                if (mode == gr::MODE_PAIRED)
+                {
+                    const void * pr = nullptr;
+                    if ((rc = field_pair(sp.wind, c, c + 1, s, &pr)) != GR_OK) return rc;
+                    j->h_win_ptr[3 * k + 2] = pr;
+                }
+            }
+            if (j->win_cap < nwin)
+            {
+                dfree(j->d_win_ptr);
+                dfree(j->d_win_edge);
+                j->d_win_ptr = nullptr;
+                j->d_win_edge = nullptr;
+                j->win_cap = 0;
+                if ((rc = dalloc(&j->d_win_ptr, size_t(3) * nwin * sizeof(void *))) != GR_OK) return rc;
+                if ((rc = dalloc(&j->d_win_edge, size_t(nwin) * 4)) != GR_OK) return rc;
+                j->win_cap = nwin;
+            }
+            GR_HIP(hipMemcpyAsync(j->d_win_ptr, j->h_win_ptr.data(), size_t(3) * nwin * sizeof(void *),
+                                  hipMemcpyHostToDevice, s));
+            GR_HIP(hipMemcpyAsync(j->d_win_edge, j->h_win_edge.data(), size_t(nwin) * 4,
+                                  hipMemcpyHostToDevice, s));
+            P.win_ptr = j->d_win_ptr;
+            P.win_edge = j->d_win_edge;
+        }
+    }
+
+    // work items, longest spans first
+    std::vector<uint32_t> order(nspans);
+    for (uint32_t i = 0; i < nspans; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return span_steps[a] > span_steps[b]; });
+    ctx->h_items.clear();
+    for (uint32_t i : order)
+    {
+        if (!span_steps[i]) continue;
+        gr_job * j = spans[i].job;
+        const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
+        for (uint64_t f = 0; f < j->n; f += block)
+            ctx->h_items.push_back(gr::BatchItem{i, uint32_t(f), uint32_t(std::min<uint64_t>(block, j->n - f)), var});
+    }
+    const uint32_t nitems = uint32_t(ctx->h_items.size());
+    if (nspans > ctx->params_cap)
+    {
+        dfree(ctx->d_params);
+        ctx->d_params = nullptr;
+        ctx->params_cap = 0;
+        if ((rc = dalloc(&ctx->d_params, nspans * sizeof(gr::StepParams))) != GR_OK) return rc;
+        ctx->params_cap = nspans;
+    }
+    if (nitems > ctx->items_cap)
+    {
+        dfree(ctx->d_items);
+        ctx->d_items = nullptr;
+        ctx->items_cap = 0;
+        if ((rc = dalloc(&ctx->d_items, nitems * sizeof(gr::BatchItem))) != GR_OK) return rc;
+        ctx->items_cap = nitems;
+    }
+    if (!ctx->d_counter && (rc = dalloc(&ctx->d_counter, sizeof(uint32_t))) != GR_OK) return rc;
+    if (nspans)
+        GR_HIP(hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), nspans * sizeof(gr::StepParams),
+                              hipMemcpyHostToDevice, s));
+    if (nitems)
+        GR_HIP(hipMemcpyAsync(ctx->d_items, ctx->h_items.data(), nitems * sizeof(gr::BatchItem),
+                              hipMemcpyHostToDevice, s));
+    GR_HIP(hipMemsetAsync(ctx->d_counter, 0, sizeof(uint32_t), s));
+    int per_cu = 0;
+    if (gr::batch_occupancy(block, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nitems, uint32_t(per_cu) * uint32_t(ctx->cus)));
+    GR_HIP(hipEventRecord(ctx->bev0, s));
+    if (gr::launch_batch(ctx->d_params, ctx->d_items, nitems, ctx->d_counter, grid, block, s) != 0)
+        return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    GR_HIP(hipEventRecord(ctx->bev1, s));
+    for (uint32_t i = 0; i < nspans; ++i)
+    {
+        if (!span_steps[i]) continue;
+        gr_job * j = spans[i].job;
+        // every slice the span read is in use until the launch ends
+        const uint32_t c_end = std::min<uint32_t>(first_win[i].wind_back + span_nwin[i] + 1,
+                                                  spans[i].wind->d.nslots);
+        for (uint32_t c = first_win[i].wind_back; c < c_end; ++c)
+            if ((rc = mark_inuse(spans[i].wind, c, s)) != GR_OK) return rc;
+        if ((rc = finish_advance(j, &first_win[i], saves[i])) != GR_OK) return rc;
+        j->timed = false;
+    }
+    if (steps_out)
+        for (uint32_t i = 0; i < nspans; ++i) steps_out[i] = span_steps[i];
+    if (ms)
+    {
+        GR_HIP(hipEventSynchronize(ctx->bev1));
+        GR_HIP(hipEventElapsedTime(ms, ctx->bev0, ctx->bev1));
+    }
     return GR_OK;
 }
 

@@ -98,6 +98,15 @@ struct StepParams
     uint64_t save_g0;       // global save index of that first save
     uint32_t ring_cap;      // columns in the device ring
 
+    // launches that cross window edges: window i of the launch (i = 0 is `wind` above) reads
+    // win_ptr[3 i + {0, 1, 2}] = (back, front, pair) and starts at step win_edge[i - 1];
+    // its time stamps are (win_first + i) * dtd and (win_first + i + 1) * dtd
+    uint32_t nwin;
+    uint32_t win_first;
+    float dtd;
+    const void * const * win_ptr;
+    const uint32_t * win_edge;
+
     // particle state (SoA)
     float * p[3];
     float * q[3];    // previous position (curvature)
@@ -137,6 +146,19 @@ struct StepParams
 
 // kernel modes (see gr_kernels.hip)
 enum : uint32_t { MODE_PAIRED = 0, MODE_SEPARATE = 1, MODE_FULL = 2 };
+
+// one work item of the persistent batched kernel: `count` particles of job `job`
+struct BatchItem
+{
+    uint32_t job;
+    uint32_t first;
+    uint32_t count;
+    uint32_t variant; // batch_variant(dim, integrator, dtype)
+};
+int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype);
+int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t nitems,
+                 uint32_t * counter, uint32_t grid, uint32_t block, void * stream);
+int batch_occupancy(uint32_t block, int * blocks_per_cu);
 
 // launchers (gr_kernels.hip)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,

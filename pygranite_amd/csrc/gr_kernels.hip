@@ -317,9 +317,19 @@ template <> struct PairLoad<bf16_t, 2>
 // kernel (every reference feature, separate slots)
 enum : int { M_PAIRED = 0, M_SEPARATE = 1, M_FULL = 2 };
 
+// slices and time stamps of the window being integrated (changes at window edges)
+struct Win
+{
+    const void * back;
+    const void * front;
+    const void * pair;
+    float tb, tf;
+    double tb64, tf64;
+};
+
 template <int DIM, class T, int MODE>
-__device__ __forceinline__ void velocity(const StepParams & P, const float * x, float t,
-                                         float * v)
+__device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
+                                         float t, float * v)
 {
     float arg[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -334,7 +344,7 @@ __device__ __forceinline__ void velocity(const StepParams & P, const float * x, 
         for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
         {
             float xb[DIM], xf[DIM];
-            PairLoad<T, DIM>::get(P.wind.pair, S.node[c], xb, xf);
+            PairLoad<T, DIM>::get(W.pair, S.node[c], xb, xf);
             const bool ok = (S.valid >> c) & 1u;
 #pragma unroll
             for (int m = 0; m < DIM; ++m)
@@ -351,11 +361,11 @@ __device__ __forceinline__ void velocity(const StepParams & P, const float * x, 
         return;
     }
     float b[DIM];
-    sample_slot<DIM, T, DIM>(P.wind.back, S, b);
+    sample_slot<DIM, T, DIM>(W.back, S, b);
     if (P.flags & F_WIND_DYNAMIC)
     {
         float f[DIM];
-        sample_slot<DIM, T, DIM>(P.wind.front, S, f);
+        sample_slot<DIM, T, DIM>(W.front, S, f);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
     }
@@ -375,15 +385,17 @@ __device__ __forceinline__ void sphere(const StepParams & P, float py, const flo
     o[1] = (v[1] * 180.f) / (PI * P.sphere_radius);
 }
 
-__device__ __forceinline__ float unit_t(const StepParams & P, float time32, double time64, float add)
+// unit_interval(Time + add, TimeFront, TimeBack) (tpp:132, 141-144)
+__device__ __forceinline__ float unit_t(const StepParams & P, const Win & W, float time32,
+                                        double time64, float add)
 {
     if (P.flags & F_TIME64)
     {
-        const double x = (time64 + double(add) - P.tb64) / (P.tf64 - P.tb64);
+        const double x = (time64 + double(add) - W.tb64) / (W.tf64 - W.tb64);
         return float(x);
     }
     const float x = time32 + add;
-    return (x - P.tb) / (P.tf - P.tb);
+    return (x - W.tb) / (W.tf - W.tb);
 }
 
 // 2D loop borders (TrajectoryIntegrator.tpp:20-96); returns true on Domain abort.
@@ -433,12 +445,18 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
 }
 
 template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) k_advance(const StepParams P)
+__device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t pid);
+
+template <int DIM, int INTEG, class T, int MODE>
+__device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t pid)
 {
     constexpr bool FULL = MODE == M_FULL;
-    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= P.n) return;
     const uint32_t oid = (FULL && P.orig) ? P.orig[pid] : pid; // caller-order index
+
+    // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
+    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
+    uint32_t wi = 0;
+    uint32_t edge = P.nwin > 1 ? P.win_edge[0] : 0xffffffffu;
 
     float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
     float ld[3] = {0.f, 0.f, 0.f};
@@ -476,10 +494,23 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
 
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
+        if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
+        {
+            ++wi;
+            const uint32_t c = P.win_first + wi;
+            W.back = P.win_ptr[3 * wi + 0];
+            W.front = P.win_ptr[3 * wi + 1];
+            W.pair = P.win_ptr[3 * wi + 2];
+            W.tb = float(c) * P.dtd;
+            W.tf = float(c + 1) * P.dtd;
+            W.tb64 = double(c) * double(P.dtd);
+            W.tf64 = double(c + 1) * double(P.dtd);
+            edge = wi + 1 < P.nwin ? P.win_edge[wi] : 0xffffffffu;
+        }
         if (status == GR_STATUS_TIME)
         {
             float t = 0.f;
-            if (P.flags & F_DYNAMIC_ANY) t = unit_t(P, time32, time64, 0.f);
+            if (P.flags & F_DYNAMIC_ANY) t = unit_t(P, W, time32, time64, 0.f);
             const bool sph = FULL && DIM == 2 && (P.flags & F_SPHERE);
             float v[3] = {0.f, 0.f, 0.f}, vc[3] = {0.f, 0.f, 0.f};
             if (INTEG == GR_RK4)
@@ -487,27 +518,27 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
                 float t1 = t, t2 = t;
                 if (P.flags & F_WIND_DYNAMIC)
                 {
-                    t1 = unit_t(P, time32, time64, half_dt);
-                    t2 = unit_t(P, time32, time64, dt);
+                    t1 = unit_t(P, W, time32, time64, half_dt);
+                    t2 = unit_t(P, W, time32, time64, dt);
                 }
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE>(P, p, t, k1);
+                velocity<DIM, T, MODE>(P, W, p, t, k1);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE>(P, x, t1, k2);
+                velocity<DIM, T, MODE>(P, W, x, t1, k2);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE>(P, x, t1, k3);
+                velocity<DIM, T, MODE>(P, W, x, t1, k3);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE>(P, x, t2, k4);
+                velocity<DIM, T, MODE>(P, W, x, t2, k4);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -519,7 +550,7 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
             }
             else
             {
-                velocity<DIM, T, MODE>(P, p, t, vc);
+                velocity<DIM, T, MODE>(P, W, p, t, vc);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -781,6 +812,84 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
     for (uint32_t e = 0; e < P.nprog; ++e) P.expr[e][pid] = ex[e];
     if (FULL)
         for (uint32_t k = 0; k < P.nvol; ++k) P.vol[k][pid] = vo[k];
+}
+
+template <int DIM, int INTEG, class T, int MODE>
+__global__ void __launch_bounds__(256) k_advance(const StepParams P)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P.n) return;
+    advance_one<DIM, INTEG, T, MODE>(P, pid);
+}
+
+// Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
+// until the list is exhausted; each item runs its job's whole span of steps. Jobs of different
+// dimension, integrator and storage type share the launch; the variant is uniform per item.
+template <int DIM, int INTEG, class T>
+__device__ __forceinline__ void run_item(const StepParams & P, uint32_t pid)
+{
+    if (P.flags & F_WIND_DYNAMIC)
+        advance_one<DIM, INTEG, T, M_PAIRED>(P, pid);
+    else
+        advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
+}
+
+__global__ void __launch_bounds__(256) k_batch(const StepParams * __restrict__ jobs,
+                                               const BatchItem * __restrict__ items,
+                                               uint32_t nitems, uint32_t * counter)
+{
+    __shared__ uint32_t s_item;
+    for (;;)
+    {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= nitems) return; // every workgroup exits once the list is drained
+        const BatchItem b = items[it];
+        const uint32_t pid = b.first + threadIdx.x;
+        if (threadIdx.x < b.count)
+        {
+            const StepParams & P = jobs[b.job];
+            switch (b.variant)
+            {
+                case 0: run_item<2, GR_EULER, float>(P, pid); break;
+                case 1: run_item<2, GR_EULER, __half>(P, pid); break;
+                case 2: run_item<2, GR_EULER, bf16_t>(P, pid); break;
+                case 3: run_item<2, GR_RK4, float>(P, pid); break;
+                case 4: run_item<2, GR_RK4, __half>(P, pid); break;
+                case 5: run_item<2, GR_RK4, bf16_t>(P, pid); break;
+                case 6: run_item<3, GR_EULER, float>(P, pid); break;
+                case 7: run_item<3, GR_EULER, __half>(P, pid); break;
+                case 8: run_item<3, GR_EULER, bf16_t>(P, pid); break;
+                case 9: run_item<3, GR_RK4, float>(P, pid); break;
+                case 10: run_item<3, GR_RK4, __half>(P, pid); break;
+                default: run_item<3, GR_RK4, bf16_t>(P, pid); break;
+            }
+        }
+    }
+}
+
+int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
+{
+    return int((dim == 3 ? 6 : 0) + (integrator == GR_RK4 ? 3 : 0) + (dtype > 2 ? 0 : dtype));
+}
+
+int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t nitems,
+                 uint32_t * counter, uint32_t grid, uint32_t block, void * stream)
+{
+    if (nitems == 0) return 0;
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+                       nitems, counter);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int batch_occupancy(uint32_t block, int * blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch, int(block), 0) ==
+                   hipSuccess
+               ? 0
+               : -1;
 }
 
 // --------------------------------------------------------------------------------------------

@@ -1,0 +1,137 @@
+"""The persistent batched kernel: jobs of mixed dimension, integrator and storage type in one
+launch, each crossing several windows, compared bit for bit with the FP32 oracle (fed the same
+rounded slices) and with the same job run alone."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from pygranite_amd import capi, synthetic
+
+pytestmark = pytest.mark.gpu
+
+ROUND = {capi.F32: lambda a: a, capi.F16: synthetic.round_fp16, capi.BF16: synthetic.round_bf16}
+
+
+def make_desc(dim, integ, gs, shape, dt, curvature=0, exprs=(), save_interval=0):
+    d = capi.JobDesc()
+    d.dim, d.integrator, d.curvature, d.windfield_mode = dim, integ, curvature, 1
+    d.dt, d.dtd = dt, 1.0
+    n = [shape[-1], shape[-2], shape[-3] if dim == 3 else 1]
+    for k in range(3):
+        d.grid_scale[k] = gs[k] if k < len(gs) else gs[0]
+        d.domain[k] = float(np.float32(n[k]) * np.float32(d.grid_scale[k]))
+    d.save_interval = save_interval
+    d.nprog = len(exprs)
+    for i, e in enumerate(exprs):
+        d.prog[i] = capi.compile_expr(e, dim=dim, curvature=curvature)
+    return d
+
+
+def oracle_settings(dim, integ, gs, dt, T, curvature=0, exprs=()):
+    return dict(Space=0 if dim == 3 else 1, Integrator=integ, DeltaT=dt, DataTimeDistance=1.0,
+                GridScale=list(gs), Offset=[0.0] * dim, MaximumSimulationTime=T, SaveInterval=0,
+                CurvatureMode=curvature, AdditionalCompute=list(exprs))
+
+
+JOBS = [
+    # dim, integrator, dtype, particles, simulated time, curvature, expressions
+    (2, 0, capi.F16, 700, 2.5, 0, ()),
+    (2, 1, capi.F32, 1500, 3.2, 0, ("sqrt(u*u+v*v)",)),
+    (3, 1, capi.F16, 2048, 2.2, 3, ("sqrt(u*u+v*v+w*w)",)),
+    (3, 0, capi.BF16, 333, 1.7, 0, ()),
+    (3, 1, capi.F32, 1000, 3.6, 2, ()),
+    (2, 1, capi.BF16, 64, 0.5, 2, ()),
+]
+
+
+def test_batch_mixed_jobs_bitexact(gpu_available):
+    ctx = capi.Context(0)
+    g2, gs2 = synthetic.double_gyre(96, 48, 6)
+    g3, gs3 = synthetic.abc_flow(24, 6)
+    fields = {}
+    for dim, src in ((2, g2), (3, g3)):
+        for dt in (capi.F32, capi.F16, capi.BF16):
+            S = src.shape[0]
+            shp = src.shape[2:]
+            f = capi.Field(ctx, shp[-1], shp[-2], shp[-3] if dim == 3 else 1, dim, dt, nslots=S,
+                           address=0 if dim == 3 else 1)
+            for s in range(S):
+                f.upload(s, [src[s, c] for c in range(dim)])
+            fields[(dim, dt)] = f
+    jobs, spans, refs = [], [], []
+    for idx, (dim, integ, dt_, n, T, curv, exprs) in enumerate(JOBS):
+        src, gs = (g2, gs2) if dim == 2 else (g3, gs3)
+        step = 0.01 if dim == 2 else 0.005
+        lo = [0.02] * dim
+        hi = [1.98, 0.98] if dim == 2 else [2 * math.pi - 0.02] * 3
+        seeds = synthetic.uniform_seeds(100 + idx, n, lo, hi)
+        d = make_desc(dim, integ, gs, src.shape, step, curv, exprs)
+        job = capi.Job(ctx, d, seeds)
+        if idx % 2 == 0:
+            job.reorder()  # batched items read reordered state too
+        jobs.append(job)
+        spans.append((job, fields[(dim, dt_)], dict(max_steps=1 << 30, max_sim_time=T)))
+        st = oracle_settings(dim, integ, gs, step, T, curv, exprs)
+        refs.append(oracle.run(st, ROUND[dt_](src), seeds, real64=False))
+    steps, ms = ctx.batch_run(spans)
+    assert ms > 0
+    for idx, (job, ref) in enumerate(zip(jobs, refs)):
+        assert steps[idx] == ref["steps"], (idx, steps[idx], ref["steps"])
+        pos = job.read(capi.OUT_POSITIONS)
+        want = ref["trajectories"][:, -1, :].astype(np.float32)
+        np.testing.assert_array_equal(pos, want, err_msg=f"job {idx}")
+        np.testing.assert_array_equal(job.read(capi.OUT_STATUS), ref["status"])
+        np.testing.assert_array_equal(job.read(capi.OUT_ABORT_STEP), ref["abort_step"])
+        if JOBS[idx][6]:
+            np.testing.assert_array_equal(job.read(capi.OUT_EXPR0),
+                                          ref["compute"][0][:, -1].astype(np.float32))
+    for j in jobs:
+        j.close()
+    for f in fields.values():
+        f.close()
+    ctx.close()
+
+
+def test_batch_equals_single_job_advances(gpu_available):
+    """A span in the batch == the same job advanced window by window with gr_job_advance."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 5)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
+    for s in range(5):
+        f.upload(s, [src[s, c] for c in range(3)])
+    seeds = synthetic.uniform_seeds(7, 4096, [0.1] * 3, [6.1] * 3)
+    d = make_desc(3, 1, gs, src.shape, 0.005, 3, ("sqrt(u*u+v*v+w*w)",), save_interval=10)
+    a = capi.Job(ctx, d, seeds, save_capacity=200)
+    b = capi.Job(ctx, d, seeds, save_capacity=200)
+    steps, _ = ctx.batch_run([(a, f, dict(max_steps=700))])
+    assert steps == [700]
+    # b: window by window under the same FP32 schedule
+    t, counter, c, mwt, left = np.float32(0), 0, 0, np.float32(1.0), 700
+    while left:
+        k, tt, edge = 0, t, False
+        while k < left:
+            k += 1
+            tt = np.float32(tt + np.float32(0.005))
+            if tt >= mwt:
+                edge = True
+                break
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, c, c + 1
+        w.time_back, w.time_front = float(np.float32(c)), float(np.float32(c + 1))
+        w.time_start, w.counter_start, w.nsteps = float(t), counter, k
+        b.advance(w)
+        t, counter, left = tt, counter + k, left - k
+        if edge:
+            c += 1
+            mwt = np.float32(mwt + np.float32(1.0))
+    for kind in (capi.OUT_POSITIONS, capi.OUT_STATUS, capi.OUT_TOTAL_CURVATURE, capi.OUT_EXPR0):
+        np.testing.assert_array_equal(a.read(kind), b.read(kind))
+    assert a.saved_columns() == b.saved_columns() == 70
+    np.testing.assert_array_equal(a.read_saved(capi.OUT_POSITIONS, 0, 70),
+                                  b.read_saved(capi.OUT_POSITIONS, 0, 70))
+    a.close()
+    b.close()
+    f.close()
+    ctx.close()
