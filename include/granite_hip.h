@@ -115,6 +115,11 @@ int gr_field_upload_device(gr_field * field, uint32_t slot, const float * const 
 /* Blocks until the slot's last upload finished. */
 int gr_field_wait(gr_field * field, uint32_t slot);
 int gr_field_bytes(const gr_field * field, uint64_t * bytes_per_slot);
+/* Stretched vertical axis (3D): node k lies at height levels[k] relative to Offset.Z
+ * (strictly increasing, n = nz). Sampling then maps a height to k + (z - z_k)/(z_{k+1} - z_k).
+ * Jobs on such a field set domain[2] to the top level so that leaving [0, top] terminates the
+ * particle (Domain). No counterpart in the reference (uniform GridScale only, icc:112-120). */
+int gr_field_set_z_levels(gr_field * field, const float * levels, uint32_t n);
 
 /* ---- expressions -------------------------------------------------------------------------
  * Compiles one AdditionalCompute program (grammar of TrajectoryIntegrator.cpp:10-26,

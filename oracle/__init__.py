@@ -58,7 +58,7 @@ class _Args(ctypes.Structure):
         ("vol_names", ctypes.c_char_p * 8),
         ("wind", ctypes.c_void_p), ("topo", ctypes.c_void_p), ("uplift", ctypes.c_void_p),
         ("constants", ctypes.c_void_p), ("volumes", ctypes.c_void_p),
-        ("seeds", ctypes.c_void_p), ("n", ctypes.c_uint64),
+        ("seeds", ctypes.c_void_p), ("n", ctypes.c_uint64), ("zlev", ctypes.c_void_p),
     ]
 
 
@@ -131,7 +131,8 @@ def _enum(v, table):
 def run(settings: dict, wind: np.ndarray, seeds: np.ndarray, *, real64: bool = True,
         weights: str = "exact", time64: bool = False, kahan: bool = False,
         topography=None, uplift=None, constants: dict | None = None,
-        volumes: dict | None = None, allow_abs: bool = False, nthreads: int = 0) -> dict:
+        volumes: dict | None = None, allow_abs: bool = False, nthreads: int = 0,
+        zlevels=None) -> dict:
     """Integrates on the CPU.
 
     wind: float32 array (S, D, ny, nx) for 2D or (S, D, nz, ny, nx) for 3D -- the slices the
@@ -193,6 +194,12 @@ def run(settings: dict, wind: np.ndarray, seeds: np.ndarray, *, real64: bool = T
     a.n = n
     a.allow_abs = 1 if allow_abs else 0
     a.nthreads = nthreads if nthreads > 0 else (os.cpu_count() or 1)
+    if zlevels is not None:
+        zlevels = np.ascontiguousarray(zlevels, dtype=np.float32)
+        if zlevels.size != a.nz:
+            raise ValueError("zlevels needs one height per z node")
+        keep.append(zlevels)
+        a.zlev = zlevels.ctypes.data
     if topography is not None:
         topography = np.ascontiguousarray(topography, dtype=np.float32)
         keep.append(topography)

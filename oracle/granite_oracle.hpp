@@ -69,6 +69,8 @@ struct Settings
     std::vector<Program> exprs;
     int nconst{0}, nvol{0};
     int have_topo{0};
+    // stretched z axis (extension): node k at height zlev[k] relative to Offset.Z
+    std::vector<float> zlev;
 };
 
 // One integration step of the host schedule.
@@ -360,11 +362,42 @@ private:
         return in_.wind + (size_t(slice) * D_ + comp) * slab_;
     }
 
+    // texture_arg (icc:112-129); on a stretched z axis the height maps to
+    // k + (z - z_k) / (z_{k+1} - z_k) with z_k <= z < z_{k+1} (first / last cell continued
+    // linearly outside the levels) -- found here by binary search
+    void texel_arg(const R * p, R * arg) const
+    {
+        for (int d = 0; d < D_; ++d) arg[d] = (p[d] - off_[d]) * inv_h_[d];
+        if (D_ == 3 && !s_.zlev.empty())
+        {
+            const std::vector<float> & z = s_.zlev;
+            const int nz = int(z.size());
+            const R zr = p[2] - off_[2];
+            int k = 0;
+            if (zr >= R(z[0]))
+            {
+                // last level <= zr, at most nz - 2
+                int lo = 0, hi = nz - 2;
+                while (lo < hi)
+                {
+                    const int mid = (lo + hi + 1) / 2;
+                    if (R(z[size_t(mid)]) <= zr)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                k = lo;
+            }
+            const R z0 = R(z[size_t(k)]), z1 = R(z[size_t(k) + 1]);
+            arg[2] = R(k) + (zr - z0) / (z1 - z0);
+        }
+    }
+
     // interpolate(data, p, t) icc:42-50 / 70-77; constant mode icc:79-87 / 104-110
     void velocity(const R * p, R t, uint32_t back, uint32_t front, bool timed, R * v) const
     {
         R arg[3] = {R(0), R(0), R(0)};
-        for (int d = 0; d < D_; ++d) arg[d] = (p[d] - off_[d]) * inv_h_[d];
+        texel_arg(p, arg);
         for (int c = 0; c < D_; ++c)
         {
             if (timed)
@@ -595,7 +628,7 @@ private:
         if (s_.volume_mode != 0 && in_.volumes) // tpp:363-377 (volume sampled at new_pos)
         {
             R arg[3] = {R(0), R(0), R(0)};
-            for (int d = 0; d < D_; ++d) arg[d] = (np[d] - off_[d]) * inv_h_[d];
+            texel_arg(np, arg);
             for (int k = 0; k < s_.nvol; ++k)
             {
                 const float * vb =

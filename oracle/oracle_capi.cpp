@@ -33,6 +33,7 @@ struct OracleArgs
     const float * volumes;
     const float * seeds;
     uint64_t n;
+    const float * zlev; // stretched z axis (nz levels) or NULL
 };
 
 enum OracleOut
@@ -112,6 +113,7 @@ void * oracle_run(const OracleArgs * a, char * err, int errlen)
         s.nconst = a->constants ? a->nconst : 0;
         s.nvol = a->volumes ? a->nvol : 0;
         s.have_topo = a->topo != nullptr;
+        if (a->zlev && a->dim == 3) s.zlev.assign(a->zlev, a->zlev + s.nz);
 
         // identifier environment: volume keys -> F0.., constant keys -> C0.. (cpp:155-226)
         std::map<std::string, int> env;

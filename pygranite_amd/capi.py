@@ -26,6 +26,7 @@ EXPORTS = [
     "gr_abi_version", "gr_last_error", "gr_device_count", "gr_ctx_create", "gr_ctx_destroy",
     "gr_ctx_synchronize", "gr_ctx_info", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
+    "gr_field_set_z_levels",
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
@@ -117,6 +118,7 @@ def lib():
         "gr_field_upload_device": ([vp, u32, P(vp)], i32),
         "gr_field_wait": ([vp, u32], i32),
         "gr_field_bytes": ([vp, P(u64)], i32),
+        "gr_field_set_z_levels": ([vp, P(ctypes.c_float), u32], i32),
         "gr_expr_compile": ([ctypes.c_char_p, P(ExprEnv), P(Program)], i32),
         "gr_expr_eval_host": ([P(Program), P(ctypes.c_float), P(ctypes.c_float)], i32),
         "gr_job_create": ([vp, P(JobDesc), vp, u64, u32, P(vp)], i32),
@@ -244,6 +246,16 @@ class Field:
 
     def wait(self, slot: int):
         _check(lib().gr_field_wait(self.h, slot))
+
+    def upload_device(self, slot: int, ptrs):
+        """Uploads FP32 component arrays already on the device (e.g. torch tensors' data_ptr())."""
+        arr = (ctypes.c_void_p * len(ptrs))(*[int(p) for p in ptrs])
+        _check(lib().gr_field_upload_device(self.h, slot, arr))
+
+    def set_z_levels(self, levels):
+        z = np.ascontiguousarray(levels, dtype=np.float32)
+        _check(lib().gr_field_set_z_levels(self.h, z.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           z.size))
 
     def bytes_per_slot(self) -> int:
         b = ctypes.c_uint64()

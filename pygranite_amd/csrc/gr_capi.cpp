@@ -110,6 +110,11 @@ struct gr_field
     };
     std::vector<Pair> pairs; // at most max(2, nslots - 1): every window of a resident dataset
     uint64_t pair_clock{0};
+    // stretched z axis
+    float * zlev{nullptr};
+    int32_t * zlut{nullptr};
+    uint32_t nlut{0};
+    float zlut_z0{0.f}, zlut_scale{0.f};
 };
 
 struct gr_job
@@ -260,6 +265,8 @@ static void field_free(gr_field * f)
     if (!f) return;
     for (void * s : f->slots) dfree(s);
     for (auto & p : f->pairs) dfree(p.buf);
+    dfree(f->zlev);
+    dfree(f->zlut);
     dfree(f->dstage);
     if (f->hstage) (void)hipHostFree(f->hstage);
     if (f->stage_done) (void)hipEventDestroy(f->stage_done);
@@ -276,7 +283,7 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
     *out = nullptr;
     const gr_field_desc & d = *desc;
     if (d.nx == 0 || d.ny == 0 || d.nz == 0 || d.ncomp == 0 || d.ncomp > GR_MAX_VOLUMES ||
-        d.nslots == 0 || d.nslots > 8 || d.dtype > GR_BF16 || d.address > 1)
+        d.nslots == 0 || d.nslots > 4096 || d.dtype > GR_BF16 || d.address > 1)
         return set_err(GR_E_INVALID, "gr_field_create: invalid descriptor");
     const uint64_t nodes = uint64_t(d.nx) * d.ny * d.nz;
     if (nodes >= (1ull << 32)) return set_err(GR_E_INVALID, "gr_field_create: grid too large");
@@ -386,6 +393,45 @@ extern "C" int gr_field_wait(gr_field * f, uint32_t slot)
     if (!f || slot >= f->d.nslots) return set_err(GR_E_INVALID, "gr_field_wait: bad argument");
     GR_HIP(hipSetDevice(f->ctx->device));
     if (f->has_ready[slot]) GR_HIP(hipEventSynchronize(f->ready[slot]));
+    return GR_OK;
+}
+
+extern "C" int gr_field_set_z_levels(gr_field * f, const float * levels, uint32_t n)
+{
+    if (!f || !levels) return set_err(GR_E_INVALID, "gr_field_set_z_levels: null argument");
+    if (n != f->d.nz || n < 2) return set_err(GR_E_INVALID, "need nz = %u levels", f->d.nz);
+    double min_dz = 1e300;
+    for (uint32_t k = 0; k + 1 < n; ++k)
+    {
+        const double dz = double(levels[k + 1]) - double(levels[k]);
+        if (!(dz > 0.0)) return set_err(GR_E_INVALID, "z levels must increase strictly");
+        min_dz = std::min(min_dz, dz);
+    }
+    const double span = double(levels[n - 1]) - double(levels[0]);
+    // bins at most half the thinnest layer, so the search from the table is a step or two
+    const uint32_t nlut = uint32_t(std::min<double>(1 << 20, std::ceil(2.0 * span / min_dz) + 1));
+    const float scale = float(double(nlut) / span);
+    std::vector<int32_t> lut(nlut);
+    int32_t k = 0;
+    for (uint32_t m = 0; m < nlut; ++m)
+    {
+        const double zb = double(levels[0]) + double(m) / double(scale);
+        while (k + 2 < int32_t(n) && double(levels[k + 1]) <= zb) ++k;
+        lut[m] = k;
+    }
+    GR_HIP(hipSetDevice(f->ctx->device));
+    dfree(f->zlev);
+    dfree(f->zlut);
+    f->zlev = nullptr;
+    f->zlut = nullptr;
+    int rc;
+    if ((rc = dalloc(&f->zlev, n * sizeof(float))) != GR_OK) return rc;
+    if ((rc = dalloc(&f->zlut, nlut * sizeof(int32_t))) != GR_OK) return rc;
+    GR_HIP(hipMemcpy(f->zlev, levels, n * sizeof(float), hipMemcpyHostToDevice));
+    GR_HIP(hipMemcpy(f->zlut, lut.data(), nlut * sizeof(int32_t), hipMemcpyHostToDevice));
+    f->nlut = nlut;
+    f->zlut_z0 = levels[0];
+    f->zlut_scale = scale;
     return GR_OK;
 }
 
@@ -764,6 +810,11 @@ static void view(gr::FieldView & v, const gr_field * f, uint32_t back, uint32_t 
     v.ncomp = f->d.ncomp;
     v.address = f->d.address;
     v.dtype = f->d.dtype;
+    v.zlev = f->zlev;
+    v.zlut = f->zlut;
+    v.nlut = f->nlut;
+    v.zlut_z0 = f->zlut_z0;
+    v.zlut_scale = f->zlut_scale;
 }
 
 // Validates the window, fills the kernel parameters, makes the compute stream wait for the

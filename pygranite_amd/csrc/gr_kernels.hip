@@ -327,13 +327,43 @@ struct Win
     double tb64, tf64;
 };
 
+// Texel coordinate of height zr (relative to the origin) on a stretched z axis:
+// k + (zr - z_k) / (z_{k+1} - z_k) with z_k <= zr < z_{k+1}; linear continuation of the first /
+// last cell outside the levels. The lookup table gives a k at or below the answer, the loop
+// finishes the search.
+__device__ __forceinline__ float zcoord(const FieldView & f, float zr)
+{
+    const int nz = int(f.nz);
+    const float u = (zr - f.zlut_z0) * f.zlut_scale;
+    int k;
+    if (!(u >= 0.f)) // below the first level (or NaN)
+        k = 0;
+    else if (u >= float(f.nlut))
+        k = nz - 2;
+    else
+    {
+        k = f.zlut[int(u)];
+        while (k > 0 && zr < f.zlev[k]) --k; // guards the float rounding of u at bin edges
+        while (k + 2 < nz && zr >= f.zlev[k + 1]) ++k;
+    }
+    const float z0 = f.zlev[k], z1 = f.zlev[k + 1];
+    return float(k) + (zr - z0) / (z1 - z0);
+}
+
+template <int DIM>
+__device__ __forceinline__ void texel_arg(const StepParams & P, const float * x, float * arg)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) arg[d] = (x[d] - P.off[d]) * P.inv_h[d];
+    if (DIM == 3 && P.wind.zlev) arg[2] = zcoord(P.wind, x[2] - P.off[2]);
+}
+
 template <int DIM, class T, int MODE>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
                                          float t, float * v)
 {
     float arg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) arg[d] = (x[d] - P.off[d]) * P.inv_h[d];
+    texel_arg<DIM>(P, x, arg);
     Taps S;
     taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
     if (MODE == M_PAIRED)
@@ -718,8 +748,7 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
                 if (FULL && (P.flags & (F_VOL_CONST | F_VOL_DYN))) // tpp:363-377
                 {
                     float arg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) arg[d] = (np[d] - P.off[d]) * P.inv_h[d];
+                    texel_arg<DIM>(P, np, arg);
                     for (uint32_t k = 0; k < P.nvol; ++k)
                     {
                         float val = sample_scalar<DIM>(P.vols.back, P.vols, k, arg, P.weights);
