@@ -1,17 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: RK4 3D particle-steps/s on the time-modulated ABC flow (one MI355X).
 
-Workload (the 3D ABC configuration): 128^3 grid over [0, 2*pi)^3, 16 hourly-spaced slices
-(DataTimeDistance 1), 1,048,576 uniform seeds (SplitMix64 seed 1), RK4 with dt = 0.005,
-per-step curvature (IndividualAndTotalCurvature) and the expression sqrt(u*u+v*v+w*w), saved
-at every step into a device-resident column ring. All 16 slices are resident in HBM before the
-timed region; a bench "step" is one launch of the fused step kernel advancing every particle by
-`--chunk` integration steps (default 50, so 40 bench steps = the full 2000-step job).
+Workload (the 3D ABC configuration): 128^3 grid over [0, 2*pi)^3, 16 slices one time unit
+apart, 1,048,576 uniform seeds (SplitMix64 seed 1), RK4 with dt = 0.005, per-step curvature
+(IndividualAndTotalCurvature) and the expression sqrt(u*u+v*v+w*w), saved at every step into a
+device-resident column ring. All slices are resident in HBM before the timed region. A bench
+"step" is one launch advancing every particle by `--chunk` integration steps (default 50);
+the default 2 warm-up + 38 timed launches cover the whole 2000-step job.
 
-Two forms run on identical inputs: the plain FP32 form (FP32 slices) and the FP16 form (FP16
-slices, FP32 interpolation and positions). `value` is the FP16 form; the FP32 figure, the
-speed-up and the FP16 form's RMS / max position error against the FP64 CPU oracle (as a fraction
-of the domain extent 2*pi, on a particle subsample) are reported beside it.
+Forms, all on identical inputs:
+  plain_fp32   FP32 slices sampled slot by slot, particles in caller order, one job per launch
+               (the plain FP32 GPU form)
+  fp32_local   FP32 slices, time-paired layout, Morton-ordered particles
+  fp16         FP16 slices, time-paired layout, Morton-ordered particles
+  fp16_batched the fp16 form run by the persistent batched kernel, the particle set split into
+               16 jobs of 65,536 (value)
+The FP16 batched form's final positions are compared with the FP64 CPU oracle on a particle
+subsample (domain-relative error over particles whose termination step agrees).
+
+Also reported: a lite batched-jobs run (mixed 2D/3D, Euler/RK4 jobs in one persistent launch
+vs sequential plain FP32 launches), a lite serving run (Poisson arrivals, p50/p99 latency), and
+the CPU baseline (the oracle's FP32 restatement on the host's threads, bounded sample).
 """
 from __future__ import annotations
 
@@ -28,25 +37,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+EXPRS = ["sqrt(u*u+v*v+w*w)"]
+DT = np.float32(0.005)
 
 
-def build_desc(capi, dim, gs, dt, n_prog_src, curvature=3, save_interval=1):
+def build_desc(capi, gs, n, save_interval=1, curvature=3, exprs=EXPRS):
     d = capi.JobDesc()
-    d.dim = dim
-    d.integrator = 1
-    d.curvature = curvature
-    d.windfield_mode = 1
-    d.dt = dt
-    d.dtd = 1.0
+    d.dim, d.integrator, d.curvature, d.windfield_mode = 3, 1, curvature, 1
+    d.dt, d.dtd = float(DT), 1.0
     for k in range(3):
         d.grid_scale[k] = gs[k]
-        d.domain[k] = 128 * gs[k]
-        d.offset[k] = 0.0
+        d.domain[k] = float(np.float32(n) * np.float32(gs[k]))
     d.save_interval = save_interval
-    progs = [capi.compile_expr(s, dim=dim, curvature=curvature) for s in n_prog_src]
-    d.nprog = len(progs)
-    for i, p in enumerate(progs):
-        d.prog[i] = p
+    d.nprog = len(exprs)
+    for i, e in enumerate(exprs):
+        d.prog[i] = capi.compile_expr(e, dim=3, curvature=curvature)
     return d
 
 
@@ -56,138 +61,307 @@ def algorithmic_bytes(storage_bytes: int, outputs: int) -> int:
     return 4 * 2 * 3 * 8 * storage_bytes + (2 * 3 * 4 + 4) + outputs
 
 
-def run_form(capi, ctx, wind, seeds, gs, dtype, chunk, steps, warmup, exprs, paired=True,
-             reorder=True):
-    S = wind.shape[0]
-    n = wind.shape[2]
+class Schedule:
+    """The reference's FP32 host schedule (window edges at Time >= max_windfield_time)."""
+
+    def __init__(self):
+        self.t = np.float32(0)
+        self.counter = 0
+        self.window = 0
+        self.mwt = np.float32(1.0)
+
+    def chunk(self, k_max):
+        """Splits up to k_max steps at the next window edge: returns (k, edge_reached)."""
+        k, t = 0, self.t
+        while k < k_max:
+            k += 1
+            t = np.float32(t + DT)
+            if t >= self.mwt:
+                return k, True, t
+        return k, False, t
+
+    def advance(self, k, edge, t):
+        self.counter += k
+        self.t = t
+        if edge:
+            self.window += 1
+            self.mwt = np.float32(self.mwt + np.float32(1.0))
+
+
+def make_field(capi, ctx, wind, dtype):
+    S, n = wind.shape[0], wind.shape[2]
     f = capi.Field(ctx, n, n, n, 3, dtype=dtype, nslots=S, address=0)
     for s in range(S):
         f.upload(s, [wind[s, c] for c in range(3)])
     for s in range(S):
         f.wait(s)
-    dt = np.float32(0.005)
-    desc = build_desc(capi, 3, gs, float(dt), exprs)
-    job = capi.Job(ctx, desc, seeds, save_capacity=chunk)
+    return f
+
+
+def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder):
+    job = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
     job.set_option(capi.OPT_PAIRED, 1 if paired else 0)
     if reorder:
         job.reorder()
     ctx.synchronize()
+    sch = Schedule()
+    kms = []
 
-    counter = 0
-    t32 = np.float32(0.0)
-    window = 0
-    max_wt = np.float32(1.0)
-    kernel_ms = []
-
-    def one_step():
-        nonlocal counter, t32, window, max_wt
-        # split the chunk at window edges exactly like the host schedule (FP32 time)
+    def launch():
         left = chunk
         while left:
-            k = 0
-            t = t32
-            done = False
-            while k < left:
-                k += 1
-                t = np.float32(t + dt)
-                if t >= max_wt:
-                    done = True
-                    break
+            k, edge, t = sch.chunk(left)
             w = capi.Window()
-            w.wind = f.h
-            w.wind_back, w.wind_front = window, window + 1
-            w.time_back, w.time_front = float(np.float32(window)), float(np.float32(window + 1))
-            w.time_back64, w.time_front64 = float(window), float(window + 1)
-            w.time_start = float(t32)
-            w.time_start64 = float(t32)
-            w.counter_start = counter
-            w.nsteps = k
+            w.wind, w.wind_back, w.wind_front = f.h, sch.window, sch.window + 1
+            w.time_back, w.time_front = float(np.float32(sch.window)), float(np.float32(sch.window + 1))
+            w.time_back64, w.time_front64 = float(sch.window), float(sch.window + 1)
+            w.time_start = w.time_start64 = float(sch.t)
+            w.counter_start, w.nsteps = sch.counter, k
             job.advance(w)
-            kernel_ms.append(job.last_advance_ms())
-            counter += k
-            t32 = t
+            kms.append(job.last_advance_ms())
+            sch.advance(k, edge, t)
             left -= k
-            if done:
-                if reorder:
-                    job.reorder()  # re-sort by cell at every window edge (inside the timing)
-                window += 1
-                max_wt = np.float32(max_wt + np.float32(1.0))
-                if window + 1 >= S:
-                    raise RuntimeError("ran out of slices")
+            if edge and reorder:
+                job.reorder()  # regroup by cell at every window edge (inside the timing)
 
     for _ in range(warmup):
-        one_step()
-    kernel_ms.clear()
+        launch()
+    kms.clear()
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        one_step()
+        launch()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    pos = job.read(capi.OUT_POSITIONS)
-    status = job.read(capi.OUT_STATUS)
-    result = {
-        "wall_s": wall,
-        "kernel_ms": float(np.sum(kernel_ms)),
-        "launches": len(kernel_ms),
-        "steps_done": counter,
-        "positions": pos,
-        "status": status,
-        "field_bytes": f.bytes_per_slot() * S,
-    }
+    out = {"wall_s": wall, "kernel_ms": float(np.sum(kms)), "launches": len(kms),
+           "steps_done": sch.counter, "positions": job.read(capi.OUT_POSITIONS),
+           "status": job.read(capi.OUT_STATUS), "abort_step": job.read(capi.OUT_ABORT_STEP)}
     job.close()
-    f.close()
-    return result
+    return out
 
 
-def fp64_error(wind, seeds_sub, gs, steps_done, got_sub):
+def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
+    parts = np.array_split(np.arange(seeds.shape[0]), njobs)
+    jobs = []
+    for idx in parts:
+        j = capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk)
+        j.reorder()
+        jobs.append(j)
+    ctx.synchronize()
+    state = dict(window=0, time=0.0, time64=0.0, counter=0)
+    kms = []
+
+    def launch():
+        spans = [(j, f, dict(first_window=state["window"], max_steps=chunk,
+                             time_start=state["time"], time_start64=state["time64"],
+                             counter_start=state["counter"])) for j in jobs]
+        ends, ms = ctx.batch_run(spans, timed=True, ends=True)
+        kms.append(ms)
+        e = ends[0]
+        if e["window"] != state["window"]:
+            for j in jobs:
+                j.reorder()
+        state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
+
+    for _ in range(warmup):
+        launch()
+    kms.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        launch()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    pos = np.concatenate([j.read(capi.OUT_POSITIONS) for j in jobs])
+    status = np.concatenate([j.read(capi.OUT_STATUS) for j in jobs])
+    ab = np.concatenate([j.read(capi.OUT_ABORT_STEP) for j in jobs])
+    for j in jobs:
+        j.close()
+    return {"wall_s": wall, "kernel_ms": float(np.sum(kms)), "launches": len(kms),
+            "steps_done": state["counter"], "positions": pos, "status": status, "abort_step": ab}
+
+
+def fp64_error(wind, seeds_sub, gs, steps_done, res, sub):
+    """Final-position error of the GPU form vs the FP64 oracle over particles whose status and
+    abort step agree (chaotic / threshold cases counted separately)."""
     import oracle
-    st = dict(Space=0, Integrator=1, DeltaT=0.005, DataTimeDistance=1.0, GridScale=gs,
-              Offset=[0, 0, 0], MaximumSimulationTime=steps_done * 0.005, SaveInterval=0)
+    st = dict(Space=0, Integrator=1, DeltaT=float(DT), DataTimeDistance=1.0, GridScale=gs,
+              Offset=[0, 0, 0], MaximumSimulationTime=float(steps_done * DT) + 1e-4,
+              SaveInterval=0)
     ref = oracle.run(st, wind, seeds_sub, real64=True)
     if ref["steps"] != steps_done:
-        return None
+        return {"error": f"oracle ran {ref['steps']} steps, GPU {steps_done}"}
     p_ref = ref["trajectories"][:, -1, :]
-    both = (ref["status"] == 0)
-    d = np.abs(got_sub[both].astype(np.float64) - p_ref[both]) / (2 * math.pi)
+    same = (ref["status"] == res["status"][:sub]) & (ref["abort_step"] == res["abort_step"][:sub])
+    d = np.abs(res["positions"][:sub][same].astype(np.float64) - p_ref[same]) / (2 * math.pi)
     return {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max()),
-            "particles": int(both.sum())}
+            "particles": int(same.sum()), "termination_mismatch": int((~same).sum()),
+            "alive_at_end": int((ref["status"] == 0).sum()), "steps": int(steps_done)}
 
 
-def cpu_baseline(wind, gs, seconds_hint=15.0):
-    """Plain FP32 CPU path (the oracle's FP32 restatement, all host threads) on a bounded
-    sample of the same workload: 65,536 particles over 200 steps (one window)."""
+def cpu_baseline(wind, gs):
+    """Plain FP32 CPU path: the oracle's FP32 restatement on the host threads over a bounded
+    sample of the same workload (131,072 particles x 1,000 steps)."""
     import oracle
     from pygranite_amd import synthetic
-    n, steps = 65536, 200
+    n, steps = 131072, 1000
     seeds = synthetic.uniform_seeds(11, n, [0.0] * 3, [2 * math.pi] * 3)
-    st = dict(Space=0, Integrator=1, DeltaT=0.005, DataTimeDistance=1.0, GridScale=gs,
-              Offset=[0, 0, 0], MaximumSimulationTime=steps * 0.005, SaveInterval=1,
-              CurvatureMode=3, AdditionalCompute=["sqrt(u*u+v*v+w*w)"])
+    st = dict(Space=0, Integrator=1, DeltaT=float(DT), DataTimeDistance=1.0, GridScale=gs,
+              Offset=[0, 0, 0], MaximumSimulationTime=steps * float(DT), SaveInterval=1,
+              CurvatureMode=3, AdditionalCompute=EXPRS)
     threads = min(os.cpu_count() or 1, 16)
     t0 = time.perf_counter()
-    r = oracle.run(st, wind[:3], seeds, real64=False, nthreads=threads)
+    r = oracle.run(st, wind[:7], seeds, real64=False, nthreads=threads)
     el = time.perf_counter() - t0
     return {"value": n * r["steps"] / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n} particles x {r['steps']} RK4 steps of the same 3D ABC workload "
-                      f"(FP32 oracle restatement, {threads} threads, {el:.1f} s)"}
+            "sample": f"{n} particles x {r['steps']} RK4 steps of the same 3D ABC workload with "
+                      f"curvature + expression saved every step (FP32 oracle restatement, "
+                      f"{threads} threads, {el:.1f} s)"}
+
+
+def batched_jobs_lite(capi, ctx, seed=3, njobs=64):
+    """Mixed jobs (half 2D double gyre, half 3D ABC; Euler:RK4 = 1:3; FP16; particle counts
+    log-uniform [1e3, 1e5]; steps uniform [100, 1000]) in one persistent launch vs the same jobs
+    as sequential plain FP32 launches."""
+    from pygranite_amd import synthetic
+    u = synthetic.uniform01(seed, 4 * njobs).reshape(njobs, 4)
+    g2, gs2 = synthetic.double_gyre(512, 256, 11)
+    g3, gs3 = synthetic.abc_flow(128, 16)
+    fields = {(2, capi.F16): make_field2(capi, ctx, g2, capi.F16),
+              (2, capi.F32): make_field2(capi, ctx, g2, capi.F32),
+              (3, capi.F16): make_field(capi, ctx, g3, capi.F16),
+              (3, capi.F32): make_field(capi, ctx, g3, capi.F32)}
+    specs = []
+    for i in range(njobs):
+        dim = 2 if i % 2 == 0 else 3
+        integ = 0 if u[i, 0] < 0.25 else 1
+        n = int(round(math.exp(math.log(1e3) + u[i, 1] * (math.log(1e5) - math.log(1e3)))))
+        steps = int(100 + u[i, 2] * 900)
+        specs.append((dim, integ, n, steps))
+
+    def make_jobs(dtype, reorder):
+        jobs = []
+        for i, (dim, integ, n, steps) in enumerate(specs):
+            gs = gs2 if dim == 2 else gs3
+            if dim == 2:
+                seeds = synthetic.uniform_seeds(1000 + i, n, [0.01, 0.01], [1.99, 0.99])
+                d = capi.JobDesc()
+                d.dim, d.integrator, d.windfield_mode = 2, integ, 1
+                d.dt, d.dtd = 0.01, 1.0
+                d.grid_scale[0], d.grid_scale[1], d.grid_scale[2] = gs[0], gs[1], gs[0]
+                d.domain[0] = float(np.float32(512) * np.float32(gs[0]))
+                d.domain[1] = float(np.float32(256) * np.float32(gs[1]))
+            else:
+                seeds = synthetic.uniform_seeds(1000 + i, n, [0.0] * 3, [2 * math.pi] * 3)
+                d = capi.JobDesc()
+                d.dim, d.integrator, d.windfield_mode = 3, integ, 1
+                d.dt, d.dtd = float(DT), 1.0
+                for k in range(3):
+                    d.grid_scale[k] = gs[k]
+                    d.domain[k] = float(np.float32(128) * np.float32(gs[k]))
+            j = capi.Job(ctx, d, seeds)
+            if reorder and n >= 4096:
+                j.reorder()
+            jobs.append((j, fields[(dim, dtype)], steps, n))
+        return jobs
+
+    # batched FP16: one persistent launch
+    jobs = make_jobs(capi.F16, True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    steps_run, ms = ctx.batch_run([(j, f, dict(max_steps=s)) for j, f, s, _ in jobs])
+    wall_b = time.perf_counter() - t0
+    ps = sum(k * n for k, (_, _, _, n) in zip(steps_run, jobs))
+    for j, *_ in jobs:
+        j.close()
+    # sequential plain FP32 launches (one job per launch, slot-by-slot sampling, caller order)
+    jobs = make_jobs(capi.F32, False)
+    for j, *_ in jobs:
+        j.set_option(capi.OPT_PAIRED, 0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for (j, f, s, n) in jobs:
+        ctx.batch_run([(j, f, dict(max_steps=s))], timed=False)
+    ctx.synchronize()
+    wall_s = time.perf_counter() - t0
+    for j, *_ in jobs:
+        j.close()
+    for f in fields.values():
+        f.close()
+    return {"jobs": njobs, "particle_steps": ps, "batched_fp16_s": wall_b,
+            "sequential_plain_fp32_s": wall_s, "speedup": wall_s / wall_b,
+            "batched_psps": ps / wall_b}
+
+
+def make_field2(capi, ctx, wind, dtype):
+    S, ny, nx = wind.shape[0], wind.shape[2], wind.shape[3]
+    f = capi.Field(ctx, nx, ny, 1, 2, dtype=dtype, nslots=S, address=1)
+    for s in range(S):
+        f.upload(s, [wind[s, c] for c in range(2)])
+    for s in range(S):
+        f.wait(s)
+    return f
+
+
+def serving_lite(capi, ctx, rate=50.0, duration=8.0, seed=4):
+    """Poisson arrivals at `rate` jobs/s for `duration` s onto two resident 3D datasets
+    (ABC 128^3 x 16 slices, FP16); job sizes log-uniform [1e3, 1e5] particles, uniform
+    [100, 1000] steps, RK4:Euler = 3:1."""
+    from pygranite_amd import serving, synthetic
+    g3, gs3 = synthetic.abc_flow(128, 16)
+    ds = {}
+    for i in range(2):
+        f = make_field(capi, ctx, g3 * (1.0 + 0.2 * i), capi.F16)
+        ds[f"d{i}"] = serving.Dataset(f, gs3, [float(np.float32(128) * np.float32(g))
+                                              for g in gs3], 1.0, 16)
+    arr = serving.poisson_arrivals(seed, rate, duration)
+    u = synthetic.uniform01(seed + 1, 4 * len(arr)).reshape(len(arr), 4)
+    reqs = []
+    for i, t in enumerate(arr):
+        n = int(round(math.exp(math.log(1e3) + u[i, 0] * (math.log(1e5) - math.log(1e3)))))
+        reqs.append(serving.Request(
+            rid=i, arrival=float(t), dataset=f"d{int(u[i, 1] * 2)}",
+            seeds=synthetic.uniform_seeds(5000 + i, n, [0.0] * 3, [2 * math.pi] * 3),
+            steps=int(100 + u[i, 2] * 900), integrator=0 if u[i, 3] < 0.25 else 1,
+            dt=float(DT)))
+    srv = serving.Server(ctx, ds, hbm_budget=32 << 30, quantum=200)
+    srv.run(reqs)
+    rep = srv.report()
+    rep.update(arrival_rate_per_s=rate, duration_s=duration,
+               offered_particle_steps_per_s=sum(r.seeds.shape[0] * r.steps for r in reqs) / duration)
+    for d in ds.values():
+        d.field.close()
+    return rep
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (profiles/traffic_k_batch_abc3d.json), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_k_batch_abc3d.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=38)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--jobs", type=int, default=16)
+    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by "
-              "particles with no exchange during integration, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by "
+              "particles with no exchange during integration; see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
     from pygranite_amd import capi, synthetic
@@ -196,70 +370,88 @@ def main():
         sys.exit(1)
     ctx = capi.Context(0)
     info = ctx.info()
-    wind, gs = synthetic.abc_flow(args.grid, 16)
+    n = args.grid
+    wind, gs = synthetic.abc_flow(n, 16)
     seeds = synthetic.uniform_seeds(1, args.particles, [0.0] * 3, [2 * math.pi] * 3)
-    exprs = ["sqrt(u*u+v*v+w*w)"]
     print(f"# device {info['name']} CUs={info['compute_units']}", file=sys.stderr, flush=True)
+    f32 = make_field(capi, ctx, wind, capi.F32)
+    f16 = make_field(capi, ctx, wind, capi.F16)
 
+    forms = args.forms.split(",")
     res = {}
-    # fp32: the plain FP32 GPU form (FP32 slots sampled separately, caller particle order);
-    # fp32_opt: FP32 with the locality layout; fp16: FP16 storage with the locality layout
-    forms = (("fp32", capi.F32, False, False), ("fp32_opt", capi.F32, True, True),
-             ("fp16", capi.F16, True, True))
-    for name, dtype, paired, reorder in forms:
+    for name in forms:
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {args.chunk} steps, "
               f"{args.particles} particles", file=sys.stderr, flush=True)
-        res[name] = run_form(capi, ctx, wind, seeds, gs, dtype, args.chunk, args.steps,
-                             args.warmup, exprs, paired=paired, reorder=reorder)
-        r = res[name]
+        if name == "plain_fp32":
+            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, False, False)
+        elif name == "fp32_local":
+            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True)
+        elif name == "fp16":
+            r = run_single(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True)
+        elif name == "fp16_batched":
+            r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs)
+        else:
+            raise SystemExit(f"unknown form {name}")
         r["psps"] = args.particles * args.chunk * args.steps / r["wall_s"]
-        r["psps_kernel"] = args.particles * args.chunk * args.steps / (r["kernel_ms"] / 1e3)
-        print(f"#   {name}: {r['psps']:.3e} particle-steps/s (wall), kernel avg "
-              f"{r['kernel_ms'] / r['launches']:.3f} ms/launch", file=sys.stderr, flush=True)
+        r["avg_launch_ms"] = r["kernel_ms"] / max(1, r["launches"])
+        res[name] = r
+        print(f"#   {name}: {r['psps']:.3e} particle-steps/s (wall), device avg "
+              f"{r['avg_launch_ms']:.3f} ms/launch", file=sys.stderr, flush=True)
 
-    f16, f32 = res["fp16"], res["fp32"]
+    head = "fp16_batched" if "fp16_batched" in res else forms[-1]
+    hv = res[head]
     outputs = 3 * 4 + 4 + 4  # saved position, curvature, expression per particle-step
     B16 = algorithmic_bytes(2, outputs)
-    B32 = algorithmic_bytes(4, outputs)
-    steps_per_launch = f16["steps_done"] / max(1, f16["launches"] + 0)  # informational
-    avg_ms = f16["kernel_ms"] / f16["launches"]
-    ps_per_launch = args.particles * args.chunk * args.steps / f16["launches"]
-    achieved = ps_per_launch * B16 / (avg_ms / 1e3) / 1e9
+    ps_per_launch = args.particles * args.chunk * args.steps / hv["launches"]
+    achieved = ps_per_launch * B16 / (hv["avg_launch_ms"] / 1e3) / 1e9
 
     err = None
     if not args.no_error:
-        sub = 256
-        err = fp64_error(wind, seeds[:sub], gs, f16["steps_done"], f16["positions"][:sub])
-    cpu = None if args.no_cpu else cpu_baseline(wind, gs)
+        print("# FP64 oracle error check", file=sys.stderr, flush=True)
+        sub = 384
+        err = fp64_error(wind, seeds[:sub], gs, hv["steps_done"], hv, sub)
+    extras = {}
+    if not args.no_extras:
+        print("# lite batched-jobs run", file=sys.stderr, flush=True)
+        extras["batched_jobs"] = batched_jobs_lite(capi, ctx)
+        print("# lite serving run", file=sys.stderr, flush=True)
+        extras["serving"] = serving_lite(capi, ctx)
+    cpu = None
+    if not args.no_cpu:
+        print("# CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wind, gs)
+    traffic = load_traffic()
 
+    plain = res.get("plain_fp32", {}).get("psps")
     line = {
         "metric": "RK4 3D particle-steps/s vs plain FP32 at <=1e-4 RMS err; p50/p99 job latency",
-        "value": f16["psps"],
+        "value": hv["psps"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": f16["wall_s"] * 1e3 / args.steps,
+        "ms_per_step": hv["wall_s"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (fp16 field storage)",
+        "dtype": "f32 arithmetic, fp16 field storage",
         "data": "synthetic (analytic ABC flow, SplitMix64 seeds)",
-        "config": {"workload": "3D ABC flow RK4, 128^3 x 16 slices, 1,048,576 particles, "
-                               "dt 0.005, curvature + sqrt(u*u+v*v+w*w) saved every step",
-                   "particles": args.particles, "grid": args.grid,
-                   "steps_per_launch": args.chunk, "storage": "fp16"},
-        "plain_fp32_value": f32["psps"],
-        "fp32_locality_value": res["fp32_opt"]["psps"],
-        "speedup_vs_plain_fp32": f16["psps"] / f32["psps"],
-        "kernel_psps_fp16": f16["psps_kernel"],
-        "kernel_psps_fp32": f32["psps_kernel"],
-        "fp64_error": err,
+        "config": {"workload": "3D ABC flow RK4, 128^3 x 16 slices, 1,048,576 particles, dt "
+                               "0.005, 2000 steps, curvature + sqrt(u*u+v*v+w*w) saved every step",
+                   "form": head, "particles": args.particles, "grid": n,
+                   "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1},
+        "forms_psps": {k: v["psps"] for k, v in res.items()},
+        "plain_fp32_value": plain,
+        "speedup_vs_plain_fp32": hv["psps"] / plain if plain else None,
+        "fp64_error_domain_relative": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_advance<3,RK4,half>", "bytes_per_particle_step": B16,
-                     "avg_launch_ms": avg_ms},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "kernel": "k_batch (RK4 3D FP16 items)" if head == "fp16_batched" else "k_advance",
+                     "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
+                     "particle_steps_per_launch": ps_per_launch},
         "cpu_baseline": cpu,
+        "extras": extras,
         "device": info["name"],
     }
     print(json.dumps(line), flush=True)

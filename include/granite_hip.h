@@ -279,11 +279,22 @@ typedef struct {
     uint64_t counter_start; /* _simulation_counter at the first step */
 } gr_span;
 
-/* Runs the spans; steps_out[i] receives the steps span i ran (may be NULL). `block` = 0 picks
- * the default (256). The call returns after queueing; *ms (may be NULL) receives the device
- * time of the launch and makes the call wait for it. */
+/* Where a span stopped: the next span of the same job starts from these values. */
+typedef struct {
+    uint32_t steps;     /* steps run */
+    uint32_t window;    /* window COUNTER of the next step */
+    float time;         /* data.Time of the next step */
+    uint32_t done;      /* 1: no further step is possible (slices exhausted or the step
+                           counter reached max_sim_time / dt) */
+    double time64;
+    uint64_t counter;   /* _simulation_counter of the next step */
+} gr_span_end;
+
+/* Runs the spans; ends[i] (may be NULL) receives where span i stopped. `block` = 0 picks the
+ * default (256). The call returns after queueing; *ms (may be NULL) receives the device time of
+ * the launch and makes the call wait for it. */
 int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
-                 uint32_t * steps_out, float * ms);
+                 gr_span_end * ends, float * ms);
 
 #ifdef __cplusplus
 }

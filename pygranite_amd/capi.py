@@ -91,6 +91,12 @@ class Span(ctypes.Structure):
                 ("time_start64", ctypes.c_double), ("counter_start", ctypes.c_uint64)]
 
 
+class SpanEnd(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint32), ("window", ctypes.c_uint32),
+                ("time", ctypes.c_float), ("done", ctypes.c_uint32),
+                ("time64", ctypes.c_double), ("counter", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -133,7 +139,7 @@ def lib():
         "gr_job_set_launch": ([vp, u32], i32),
         "gr_job_set_option": ([vp, u32, ctypes.c_int64], i32),
         "gr_job_reorder": ([vp], i32),
-        "gr_batch_run": ([vp, P(Span), u32, u32, P(u32), P(ctypes.c_float)], i32),
+        "gr_batch_run": ([vp, P(Span), u32, u32, P(SpanEnd), P(ctypes.c_float)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -189,9 +195,11 @@ class Context:
         self.h = ctypes.c_void_p()
         _check(lib().gr_ctx_create(device, ctypes.byref(self.h)))
 
-    def batch_run(self, spans, block: int = 0, timed: bool = True):
-        """Runs [(job, field, dict(first_window=, max_steps=, time_start=, max_sim_time=,
-        counter_start=))...] in one persistent launch; returns (steps per span, device ms)."""
+    def batch_run(self, spans, block: int = 0, timed: bool = True, ends: bool = False):
+        """Runs [(job, field, dict(first_window=, max_steps=, time_start=, time_start64=,
+        max_sim_time=, counter_start=))...] in one persistent launch.
+
+        Returns (steps per span, device ms), or (list of end records, ms) with ends=True."""
         arr = (Span * max(1, len(spans)))()
         for i, (job, field, kw) in enumerate(spans):
             s = arr[i]
@@ -199,14 +207,18 @@ class Context:
             s.first_window = kw.get("first_window", 0)
             s.max_steps = kw["max_steps"]
             s.time_start = kw.get("time_start", 0.0)
-            s.time_start64 = kw.get("time_start", 0.0)
+            s.time_start64 = kw.get("time_start64", kw.get("time_start", 0.0))
             s.max_sim_time = kw.get("max_sim_time", 0.0)
             s.counter_start = kw.get("counter_start", 0)
-        steps = (ctypes.c_uint32 * max(1, len(spans)))()
+        out = (SpanEnd * max(1, len(spans)))()
         ms = ctypes.c_float()
-        _check(lib().gr_batch_run(self.h, arr, len(spans), block, steps,
+        _check(lib().gr_batch_run(self.h, arr, len(spans), block, out,
                                   ctypes.byref(ms) if timed else None))
-        return list(steps)[:len(spans)], ms.value
+        recs = [dict(steps=e.steps, window=e.window, time=e.time, time64=e.time64,
+                     counter=e.counter, done=bool(e.done)) for e in list(out)[:len(spans)]]
+        if ends:
+            return recs, ms.value
+        return [r["steps"] for r in recs], ms.value
 
     def info(self):
         name = ctypes.create_string_buffer(256)

@@ -1025,7 +1025,7 @@ static int f_wait(gr_field * f, uint32_t slot, hipStream_t s)
 // time, max_windfield_time accumulated by += DataTimeDistance, stop at counter >= max_sim / dt
 // or when slice c + 1 does not exist.
 static void plan_span(const gr_job_desc & d, const gr_span & sp, uint32_t nslices,
-                      std::vector<uint32_t> & win_steps)
+                      std::vector<uint32_t> & win_steps, gr_span_end * end = nullptr)
 {
     win_steps.clear();
     float mwt = d.dtd;
@@ -1066,10 +1066,19 @@ static void plan_span(const gr_job_desc & d, const gr_span & sp, uint32_t nslice
         mwt64 += double(d.dtd);
         ++c;
     }
+    if (end)
+    {
+        end->steps = uint32_t(total);
+        end->window = c;
+        end->time = t;
+        end->time64 = t64;
+        end->counter = counter;
+        end->done = (c + 1 >= nslices) || (capped && !(float(counter) < maxs_f)) ? 1u : 0u;
+    }
 }
 
 extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
-                            uint32_t * steps_out, float * ms)
+                            gr_span_end * ends, float * ms)
 {
     if (!ctx || (nspans && !spans)) return set_err(GR_E_INVALID, "gr_batch_run: null argument");
     if (block == 0) block = 256;
@@ -1100,8 +1109,9 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if (j->full) return set_err(GR_E_INVALID, "span %u: the job uses features outside the batched kernel", i);
         const gr_job_desc & d = j->d;
         const uint32_t nslices = sp.wind->d.nslots;
+        gr_span_end end_i{};
         if (d.windfield_mode == 1)
-            plan_span(d, sp, nslices, win_steps);
+            plan_span(d, sp, nslices, win_steps, &end_i);
         else
         {
             win_steps.clear(); // constant wind: one window of slot 0, bounded by max_sim_time
@@ -1109,9 +1119,11 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             one.first_window = 0;
             gr_job_desc dd = d;
             dd.dtd = sp.max_sim_time > 0.f ? sp.max_sim_time : 3.4e38f;
-            plan_span(dd, one, 2, win_steps);
+            plan_span(dd, one, 2, win_steps, &end_i);
             if (win_steps.size() > 1) win_steps.resize(1);
+            end_i.window = 0;
         }
+        if (ends) ends[i] = end_i;
         uint32_t total = 0;
         for (uint32_t k : win_steps) total += k;
         span_steps[i] = total;
@@ -1241,8 +1253,6 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if ((rc = finish_advance(j, &first_win[i], saves[i])) != GR_OK) return rc;
         j->timed = false;
     }
-    if (steps_out)
-        for (uint32_t i = 0; i < nspans; ++i) steps_out[i] = span_steps[i];
     if (ms)
     {
         GR_HIP(hipEventSynchronize(ctx->bev1));

@@ -857,7 +857,8 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P)
 template <int DIM, int INTEG, class T>
 __device__ __forceinline__ void run_item(const StepParams & P, uint32_t pid)
 {
-    if (P.flags & F_WIND_DYNAMIC)
+    // the host builds a paired copy only for jobs that use it (dynamic wind, paired option)
+    if (P.wind.pair)
         advance_one<DIM, INTEG, T, M_PAIRED>(P, pid);
     else
         advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);

@@ -1,0 +1,218 @@
+"""Continuous-batching trajectory server on one GPU.
+
+Requests (a seed set, a resident dataset, a step budget, an integrator) arrive over time. The
+server keeps every dataset's slices resident in HBM, admits queued requests in arrival order
+while their particle state fits the HBM budget, and advances all admitted jobs together in
+rounds: each round is one launch of the persistent batched kernel (`gr_batch_run`) in which
+every active job runs up to `quantum` steps. Jobs that finish (step budget reached, slices
+exhausted, or MaximumSimulationTime reached) are retired -- final positions and statuses are
+copied to the host -- and their memory is released, which lets the next requests in. Particle
+state stays on the device between rounds, and a job's results do not depend on which other
+jobs shared its rounds (each particle's arithmetic is independent of the batch).
+
+Latency of a request = completion time - arrival time (host clock), including queueing,
+admission (state allocation + seed upload), all rounds and the final copy.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from . import capi
+
+
+@dataclasses.dataclass
+class Dataset:
+    field: capi.Field
+    grid_scale: list
+    domain: list
+    dtd: float
+    nslices: int
+    zlevels: object = None
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                  # seconds on the server clock
+    dataset: str
+    seeds: np.ndarray               # (n, 3) float32
+    steps: int                      # step budget
+    integrator: int = 1             # 0 Euler, 1 RK4
+    dt: float = 60.0
+    curvature: int = 0
+    exprs: tuple = ()
+    allow_abs: bool = False
+    # filled by the server
+    admitted: float = float("nan")
+    finished: float = float("nan")
+    steps_done: int = 0
+    positions: np.ndarray = None
+    status: np.ndarray = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class _Active:
+    __slots__ = ("req", "job", "window", "time", "time64", "counter", "bytes")
+
+    def __init__(self, req, job, nbytes):
+        self.req = req
+        self.job = job
+        self.window = 0
+        self.time = 0.0
+        self.time64 = 0.0
+        self.counter = 0
+        self.bytes = nbytes
+
+
+def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0) -> int:
+    """Device bytes of a job's state (positions, scratch, status, abort step, sort buffers,
+    curvature and expression state); no saved columns."""
+    per = 4 * (dim * 3 + 2 + 5)
+    if curvature in (1, 3):
+        per += 4 * (dim + 2)
+    elif curvature == 2:
+        per += 4 * (dim + 1)
+    per += 4 * nprog
+    return int(n) * per + (1 << 16)
+
+
+class Server:
+    def __init__(self, ctx: capi.Context, datasets: dict, hbm_budget: int = 64 << 30,
+                 quantum: int = 250, block: int = 0, reorder: bool = True):
+        self.ctx = ctx
+        self.datasets = datasets
+        self.hbm_budget = int(hbm_budget)
+        self.quantum = int(quantum)
+        self.block = block
+        self.reorder = reorder
+        self.queue = collections.deque()
+        self.active: list[_Active] = []
+        self.done: list[Request] = []
+        self.in_use = 0
+        self.rounds = 0
+        self.particle_steps = 0
+        self.device_ms = 0.0
+        self.t0 = None
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, req: Request):
+        self.queue.append(req)
+
+    # -- admission ------------------------------------------------------------------------
+    def _make_job(self, req: Request):
+        ds = self.datasets[req.dataset]
+        d = capi.JobDesc()
+        d.dim = 3
+        d.integrator = req.integrator
+        d.curvature = req.curvature
+        d.windfield_mode = 1
+        d.dt = req.dt
+        d.dtd = ds.dtd
+        for k in range(3):
+            d.grid_scale[k] = ds.grid_scale[k]
+            d.domain[k] = ds.domain[k]
+        d.save_interval = 0
+        d.nprog = len(req.exprs)
+        for i, e in enumerate(req.exprs):
+            d.prog[i] = capi.compile_expr(e, dim=3, curvature=req.curvature,
+                                          allow_abs=req.allow_abs)
+        job = capi.Job(self.ctx, d, req.seeds)
+        if self.reorder and req.seeds.shape[0] >= 4096:
+            job.reorder()
+        return job
+
+    def admit(self):
+        while self.queue and self.queue[0].arrival <= self.now():
+            req = self.queue[0]
+            nb = job_bytes(req.seeds.shape[0], 3, req.curvature, len(req.exprs))
+            if self.in_use + nb > self.hbm_budget and self.active:
+                break  # wait for memory (FIFO: no overtaking)
+            self.queue.popleft()
+            job = self._make_job(req)
+            req.admitted = self.now()
+            self.active.append(_Active(req, job, nb))
+            self.in_use += nb
+
+    # -- one round ------------------------------------------------------------------------
+    def round(self):
+        spans = []
+        for a in self.active:
+            left = a.req.steps - a.req.steps_done
+            spans.append((a.job, self.datasets[a.req.dataset].field,
+                          dict(first_window=a.window, max_steps=min(left, self.quantum),
+                               time_start=a.time, time_start64=a.time64,
+                               counter_start=a.counter)))
+        ends, ms = self.ctx.batch_run(spans, block=self.block, timed=True, ends=True)
+        self.rounds += 1
+        self.device_ms += ms
+        keep = []
+        for a, e in zip(self.active, ends):
+            a.req.steps_done += e["steps"]
+            self.particle_steps += e["steps"] * a.req.seeds.shape[0]
+            a.window, a.time, a.time64, a.counter = e["window"], e["time"], e["time64"], e["counter"]
+            if e["done"] or e["steps"] == 0 or a.req.steps_done >= a.req.steps:
+                self._retire(a)
+            else:
+                keep.append(a)
+        self.active = keep
+
+    def _retire(self, a: _Active):
+        req = a.req
+        req.positions = a.job.read(capi.OUT_POSITIONS)
+        req.status = a.job.read(capi.OUT_STATUS)
+        a.job.close()
+        self.in_use -= a.bytes
+        req.finished = self.now()
+        self.done.append(req)
+
+    # -- driver ---------------------------------------------------------------------------
+    def run(self, requests, deadline: float | None = None):
+        """Serves `requests` (sorted by arrival) in real time; returns the finished ones."""
+        for r in sorted(requests, key=lambda r: r.arrival):
+            self.submit(r)
+        self.t0 = time.perf_counter()
+        while self.queue or self.active:
+            if deadline is not None and self.now() > deadline:
+                break
+            self.admit()
+            if self.active:
+                self.round()
+            elif self.queue:
+                wait = self.queue[0].arrival - self.now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+        return self.done
+
+    def report(self) -> dict:
+        lat = np.array([r.latency for r in self.done]) if self.done else np.array([np.nan])
+        wall = self.now() if self.t0 is not None else float("nan")
+        return {
+            "jobs": len(self.done),
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(np.mean(lat)),
+            "particle_steps": int(self.particle_steps),
+            "particle_steps_per_s": self.particle_steps / wall if wall > 0 else float("nan"),
+            "rounds": self.rounds,
+            "device_busy_frac": (self.device_ms / 1e3) / wall if wall > 0 else float("nan"),
+            "wall_s": wall,
+        }
+
+
+def poisson_arrivals(seed: int, rate: float, duration: float) -> np.ndarray:
+    """Arrival times of a Poisson process (exponential gaps from SplitMix64 uniforms)."""
+    from .synthetic import uniform01
+    n_max = int(rate * duration * 3 + 100)
+    u = uniform01(seed, n_max)
+    gaps = -np.log1p(-u) / rate
+    t = np.cumsum(gaps)
+    return t[t < duration]

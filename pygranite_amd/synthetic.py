@@ -106,6 +106,111 @@ def abc_flow(n=128, nslices=16, dtd=1.0):
 
 
 # ----------------------------------------------------------------------------------------------
+# Atmosphere-like field: Gaussian random field (k^-5/3 horizontal energy spectrum) + zonal jet
+# ----------------------------------------------------------------------------------------------
+def stretched_levels(nz=64, dz0=20.0, ratio=1.05):
+    """Heights z_k = dz0 (ratio^k - 1) / (ratio - 1), k = 0..nz-1 (z_0 = 0)."""
+    k = np.arange(nz, dtype=np.float64)
+    return (dz0 * (ratio ** k - 1.0) / (ratio - 1.0)).astype(np.float32)
+
+
+def _grf_amplitude(xp, nz, ny, nx, dx, levels_dz):
+    """Spectral amplitude |u_hat| ~ k_h^(-4/3) (E(k) ~ k^-5/3 in 2D), smooth in z."""
+    ky = xp.fft.fftfreq(ny, d=dx)
+    kx = xp.fft.rfftfreq(nx, d=dx)
+    kz = xp.fft.fftfreq(nz, d=1.0)
+    kh = xp.sqrt(ky[:, None] ** 2 + kx[None, :] ** 2)
+    kmin = 1.0 / (max(nx, ny) * dx)
+    amp_h = xp.where(kh > 0, (xp.maximum(kh, kmin) / kmin) ** (-4.0 / 3.0), 0.0)
+    amp_z = xp.exp(-(kz / 0.08) ** 2)  # vertical correlation over ~ a dozen levels
+    return amp_z[:, None, None] * amp_h[None, :, :]
+
+
+def _grf(xp, rng_normal, amp, shape, rms):
+    nz, ny, nx = shape
+    spec = (rng_normal((nz, ny, nx // 2 + 1)) + 1j * rng_normal((nz, ny, nx // 2 + 1))) * amp
+    f = xp.fft.irfftn(spec, s=(nz, ny, nx), axes=(0, 1, 2))
+    f = f - f.mean()
+    return f * (rms / xp.sqrt(xp.mean(f * f)))
+
+
+def atmosphere(nx=512, ny=512, nz=64, nslices=25, seed=1, dx=3000.0, dz0=20.0, ratio=1.05,
+               device=None):
+    """Wind slices (S, 3, nz, ny, nx) float32 of an atmosphere-like flow.
+
+    Recipe: for u, v two independent GRFs A, B with horizontal spectrum E(k) ~ k^-5/3 (RMS
+    10 m/s, correlated over ~12 levels in z); slice s (hourly) is cos(th) A + sin(th) B with
+    th = 2 pi s / 48, which keeps the RMS and the spectrum and decorrelates over a day. A zonal
+    jet 20 cos^2(pi (y - yc) / Ly) m/s is added to u. w is built the same way with RMS
+    0.05 m/s. Vertical nodes sit on `stretched_levels(nz, dz0, ratio)`. With `device` set (a
+    torch device), the field is generated there with torch and returned as a torch tensor;
+    otherwise numpy on the host.
+
+    Returns (wind, grid_scale [dx, dx, mean dz], z_levels).
+    """
+    levels = stretched_levels(nz, dz0, ratio)
+    if device is not None:
+        import torch
+        xp = torch
+        g = torch.Generator(device=device)
+        g.manual_seed(int(seed))
+
+        def normal(shape):
+            return torch.randn(shape, generator=g, device=device, dtype=torch.float64)
+
+        ky = torch.fft.fftfreq(ny, d=dx, device=device, dtype=torch.float64)
+        kx = torch.fft.rfftfreq(nx, d=dx, device=device, dtype=torch.float64)
+        kz = torch.fft.fftfreq(nz, d=1.0, device=device, dtype=torch.float64)
+        kh = torch.sqrt(ky[:, None] ** 2 + kx[None, :] ** 2)
+        kmin = 1.0 / (max(nx, ny) * dx)
+        amp_h = torch.where(kh > 0, (torch.clamp(kh, min=kmin) / kmin) ** (-4.0 / 3.0),
+                            torch.zeros_like(kh))
+        amp = torch.exp(-(kz / 0.08) ** 2)[:, None, None] * amp_h[None, :, :]
+
+        def grf(rms):
+            spec = torch.complex(normal((nz, ny, nx // 2 + 1)), normal((nz, ny, nx // 2 + 1))) * amp
+            f = torch.fft.irfftn(spec, s=(nz, ny, nx), dim=(0, 1, 2))
+            f = f - f.mean()
+            return (f * (rms / torch.sqrt(torch.mean(f * f)))).to(torch.float32)
+
+        y = torch.arange(ny, device=device, dtype=torch.float32) * dx
+        Ly = ny * dx
+        jet = (20.0 * torch.cos(math.pi * (y - Ly / 2) / Ly) ** 2)[None, :, None]
+        base = [(grf(10.0), grf(10.0)), (grf(10.0), grf(10.0)), (grf(0.05), grf(0.05))]
+        out = torch.empty((nslices, 3, nz, ny, nx), device=device, dtype=torch.float32)
+        for s in range(nslices):
+            th = 2 * math.pi * s / 48.0
+            for c in range(3):
+                a, b = base[c]
+                out[s, c] = math.cos(th) * a + math.sin(th) * b
+            out[s, 0] += jet
+        return out, [dx, dx, float(levels[-1]) / (nz - 1)], levels
+
+    rng = np.random.default_rng(seed)
+    amp = _grf_amplitude(np, nz, ny, nx, dx, None)
+    normal = rng.standard_normal
+    base = [(_grf(np, normal, amp, (nz, ny, nx), 10.0), _grf(np, normal, amp, (nz, ny, nx), 10.0)),
+            (_grf(np, normal, amp, (nz, ny, nx), 10.0), _grf(np, normal, amp, (nz, ny, nx), 10.0)),
+            (_grf(np, normal, amp, (nz, ny, nx), 0.05), _grf(np, normal, amp, (nz, ny, nx), 0.05))]
+    y = np.arange(ny, dtype=np.float64) * dx
+    Ly = ny * dx
+    jet = (20.0 * np.cos(math.pi * (y - Ly / 2) / Ly) ** 2)[None, :, None]
+    out = np.empty((nslices, 3, nz, ny, nx), np.float32)
+    for s in range(nslices):
+        th = 2 * math.pi * s / 48.0
+        for c in range(3):
+            a, b = base[c]
+            out[s, c] = math.cos(th) * a + math.sin(th) * b
+        out[s, 0] += jet
+    return out, [dx, dx, float(levels[-1]) / (nz - 1)], levels
+
+
+def loguniform_heights(seed, n, lo, hi):
+    u = uniform01(seed, n)
+    return (lo * np.exp(u * math.log(hi / lo))).astype(np.float32)
+
+
+# ----------------------------------------------------------------------------------------------
 # Storage rounding helpers (what FP16 / BF16 slices hold once converted on the device)
 # ----------------------------------------------------------------------------------------------
 def round_fp16(a: np.ndarray) -> np.ndarray:

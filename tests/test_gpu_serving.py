@@ -1,0 +1,85 @@
+"""Serving: continuous batching gives every request the same result alone and in the mix,
+and matches the oracle; admission respects the HBM budget."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from pygranite_amd import capi, serving, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def make_datasets(ctx, n=24, S=6, storage=capi.F16):
+    out, raw = {}, {}
+    for i in range(2):
+        wind, gs = synthetic.abc_flow(n, S)
+        wind = wind * (1.0 + 0.25 * i)  # two different datasets
+        f = capi.Field(ctx, n, n, n, 3, storage, nslots=S)
+        for s in range(S):
+            f.upload(s, [wind[s, c] for c in range(3)])
+        dom = [float(np.float32(n) * np.float32(g)) for g in gs]
+        out[f"ds{i}"] = serving.Dataset(f, gs, dom, 1.0, S)
+        raw[f"ds{i}"] = (wind, gs)
+    return out, raw
+
+
+def requests(seed0=0, count=8):
+    reqs = []
+    for i in range(count):
+        n = [300, 5000, 64, 20000, 1200, 777, 9000, 128][i % 8]
+        seeds = synthetic.uniform_seeds(seed0 + i, n, [0.05] * 3, [2 * math.pi - 0.05] * 3)
+        reqs.append(serving.Request(rid=i, arrival=0.01 * i, dataset=f"ds{i % 2}", seeds=seeds,
+                                    steps=[150, 420, 90, 700, 333, 260, 500, 610][i % 8],
+                                    integrator=1 if i % 4 else 0, dt=0.005,
+                                    exprs=("sqrt(u*u+v*v+w*w)",) if i % 3 == 0 else ()))
+    return reqs
+
+
+def test_alone_equals_in_the_mix(gpu_available):
+    ctx = capi.Context(0)
+    ds, raw = make_datasets(ctx)
+    # the mix: all requests with overlapping arrivals and small rounds
+    mix = serving.Server(ctx, ds, quantum=64)
+    done_mix = {r.rid: r for r in mix.run(requests())}
+    assert len(done_mix) == 8
+    rep = mix.report()
+    assert rep["rounds"] > 3 and rep["particle_steps"] > 0
+    # each request alone, with a different quantum
+    for r in requests():
+        alone = serving.Server(ctx, ds, quantum=1000).run([r])[0]
+        m = done_mix[r.rid]
+        assert alone.steps_done == m.steps_done == r.steps
+        np.testing.assert_array_equal(alone.positions, m.positions)
+        np.testing.assert_array_equal(alone.status, m.status)
+    # and one request against the FP32 oracle on the same FP16-rounded slices
+    r = requests()[3]
+    wind, gs = raw[r.dataset]
+    st = dict(Space=0, Integrator=r.integrator, DeltaT=r.dt, DataTimeDistance=1.0, GridScale=gs,
+              Offset=[0, 0, 0], MaximumSimulationTime=r.steps * r.dt, SaveInterval=0)
+    ref = oracle.run(st, synthetic.round_fp16(wind), r.seeds, real64=False)
+    if ref["steps"] == r.steps:
+        np.testing.assert_array_equal(done_mix[r.rid].positions,
+                                      ref["trajectories"][:, -1].astype(np.float32))
+    ctx.close()
+
+
+def test_admission_respects_budget(gpu_available):
+    ctx = capi.Context(0)
+    ds, _ = make_datasets(ctx)
+    reqs = requests()
+    biggest = max(serving.job_bytes(r.seeds.shape[0]) for r in reqs)
+    srv = serving.Server(ctx, ds, hbm_budget=biggest + 1, quantum=128)
+    peak = [0]
+    orig_round = srv.round
+
+    def watched():
+        peak[0] = max(peak[0], srv.in_use)
+        orig_round()
+
+    srv.round = watched
+    srv.run(reqs)
+    assert len(srv.done) == len(reqs)
+    assert peak[0] <= biggest + 1
+    ctx.close()
