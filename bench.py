@@ -335,6 +335,98 @@ def serving_lite(capi, ctx, rate=50.0, duration=8.0, seed=4):
     return rep
 
 
+def atmosphere_workload(capi, ctx, args):
+    """Precision form on the atmosphere-like field: 512x512x64 (stretched z, ratio 1.05),
+    25 hourly slices (24 windows = 1440 steps of 60 s), 16,777,216 seeds (uniform x-y,
+    log-uniform height), RK4, Kahan positions, FP64 time, the expression
+    w*1000/(abs(u)+abs(v)+1) saved every step. Storage FP32 / FP16 / BF16; error of each
+    against the FP64 oracle on a subsample. Prints one JSON line."""
+    import torch
+    from pygranite_amd import synthetic
+    nx = ny = args.atmo_nx
+    nz, S = 64, 25
+    dev = torch.device("cuda")
+    t0 = time.perf_counter()
+    wind_t, gs, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=1, device=dev)
+    torch.cuda.synchronize()
+    print(f"# atmosphere field generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    dom = [float(np.float32(nx) * np.float32(gs[0])), float(np.float32(ny) * np.float32(gs[1])),
+           float(zlev[-1])]
+    n = args.atmo_particles
+    xy = synthetic.uniform_seeds(21, n, [0.0, 0.0], [dom[0], dom[1]])
+    z = synthetic.loguniform_heights(22, n, float(zlev[1]), float(zlev[-1]))
+    seeds = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
+    expr = "w*1000/(abs(u)+abs(v)+1)"
+    dt, chunk = np.float32(60.0), args.chunk
+    total_steps = 1440
+    out = {"workload": f"atmosphere {nx}x{ny}x{nz} stretched z, {S} slices, {n} particles, RK4 dt 60 s, "
+                       f"{total_steps} steps, Kahan + FP64 time, expression saved every step"}
+    finals = {}
+    for name, dtype in (("fp32", capi.F32), ("fp16", capi.F16), ("bf16", capi.BF16)):
+        f = capi.Field(ctx, nx, ny, nz, 3, dtype=dtype, nslots=S, address=0)
+        f.set_z_levels(zlev)
+        for s in range(S):
+            f.upload_device(s, [wind_t[s, c].data_ptr() for c in range(3)])
+        ctx.synchronize()
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.windfield_mode = 3, 1, 1
+        d.dt, d.dtd = float(dt), 3600.0
+        d.kahan, d.time64 = 1, 1
+        for k in range(3):
+            d.grid_scale[k] = gs[k]
+            d.domain[k] = dom[k]
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr(expr, dim=3, allow_abs=True)
+        job = capi.Job(ctx, d, seeds, save_capacity=chunk)
+        job.reorder()
+        ctx.synchronize()
+        st = dict(window=0, time=0.0, time64=0.0, counter=0)
+        t0 = time.perf_counter()
+        kms = []
+        while st["counter"] < total_steps:
+            ends, ms = ctx.batch_run([(job, f, dict(first_window=st["window"],
+                                                    max_steps=min(chunk, total_steps - st["counter"]),
+                                                    time_start=st["time"], time_start64=st["time64"],
+                                                    counter_start=st["counter"]))], ends=True)
+            kms.append(ms)
+            e = ends[0]
+            if e["window"] != st["window"]:
+                job.reorder()
+            st.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
+            if e["done"] or e["steps"] == 0:
+                break
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        out[name] = {"particle_steps_per_s": n * st["counter"] / wall, "wall_s": wall,
+                     "device_s": float(np.sum(kms)) / 1e3, "steps": st["counter"],
+                     "field_bytes_per_slice": f.bytes_per_slot()}
+        print(f"#   atmosphere {name}: {out[name]['particle_steps_per_s']:.3e} particle-steps/s",
+              file=sys.stderr, flush=True)
+        finals[name] = (job.read(capi.OUT_POSITIONS), job.read(capi.OUT_STATUS),
+                        job.read(capi.OUT_ABORT_STEP), st["counter"])
+        job.close()
+        f.close()
+    if not args.no_error:
+        import oracle
+        sub = 256
+        wind_h = wind_t.cpu().numpy()
+        del wind_t
+        st = dict(Space=0, Integrator=1, DeltaT=float(dt), DataTimeDistance=3600.0,
+                  GridScale=list(gs), Offset=[0, 0, 0], SaveInterval=0,
+                  MaximumSimulationTime=float(total_steps) * float(dt) + 1.0)
+        ref = oracle.run(st, wind_h, seeds[:sub], real64=True, zlevels=zlev, time64=True)
+        ext = np.array(dom, np.float64)
+        for name, (pos, status, ab, steps) in finals.items():
+            same = (ref["status"] == status[:sub]) & (ref["abort_step"] == ab[:sub])
+            d = np.abs(pos[:sub][same].astype(np.float64) - ref["trajectories"][same, -1, :]) / ext
+            out[name]["fp64_error"] = {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max()),
+                                       "particles": int(same.sum()),
+                                       "termination_mismatch": int((~same).sum())}
+    out["speedup_fp16_vs_fp32"] = out["fp16"]["particle_steps_per_s"] / out["fp32"]["particle_steps_per_s"]
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     (profiles/traffic_k_batch_abc3d.json), or None."""
@@ -358,6 +450,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
+    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere"])
+    ap.add_argument("--atmo-nx", type=int, default=512)
+    ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by "
@@ -370,6 +465,9 @@ def main():
         sys.exit(1)
     ctx = capi.Context(0)
     info = ctx.info()
+    if args.workload == "atmosphere":
+        atmosphere_workload(capi, ctx, args)
+        return
     n = args.grid
     wind, gs = synthetic.abc_flow(n, 16)
     seeds = synthetic.uniform_seeds(1, args.particles, [0.0] * 3, [2 * math.pi] * 3)

@@ -7,6 +7,7 @@ pygranite host schedule on top. Structures mirror the header field by field.
 from __future__ import annotations
 
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -191,9 +192,16 @@ def eval_expr_host(prog: Program, ref) -> float:
 
 
 class Context:
+    """A device context. Fields and jobs made from it hold a reference to it; closing the
+    context closes them first (the library refuses to destroy a context with live children)."""
+
     def __init__(self, device: int = 0):
         self.h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()
         _check(lib().gr_ctx_create(device, ctypes.byref(self.h)))
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def batch_run(self, spans, block: int = 0, timed: bool = True, ends: bool = False):
         """Runs [(job, field, dict(first_window=, max_steps=, time_start=, time_start64=,
@@ -232,7 +240,9 @@ class Context:
 
     def close(self):
         if self.h:
-            lib().gr_ctx_destroy(self.h)
+            for child in list(self._children):
+                child.close()
+            _check(lib().gr_ctx_destroy(self.h))
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -249,6 +259,7 @@ class Field:
         self.desc = d
         self.h = ctypes.c_void_p()
         _check(lib().gr_field_create(ctx.h, ctypes.byref(d), ctypes.byref(self.h)))
+        ctx._adopt(self)
 
     def upload(self, slot: int, comps, async_: bool = False):
         arrs = [np.ascontiguousarray(c, dtype=np.float32) for c in comps]
@@ -296,6 +307,7 @@ class Job:
         self.h = ctypes.c_void_p()
         _check(lib().gr_job_create(ctx.h, ctypes.byref(desc), seeds.ctypes.data, self.n,
                                    save_capacity, ctypes.byref(self.h)))
+        ctx._adopt(self)
 
     def set_launch(self, block: int):
         _check(lib().gr_job_set_launch(self.h, block))

@@ -84,6 +84,8 @@ struct gr_ctx
     uint32_t * d_counter{nullptr};
     hipEvent_t bev0{nullptr}, bev1{nullptr};
     int cus{0};
+    // live fields and jobs: the context outlives them (gr_ctx_destroy refuses otherwise)
+    int children{0};
 };
 
 struct gr_field
@@ -218,6 +220,9 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
 extern "C" int gr_ctx_destroy(gr_ctx * c)
 {
     if (!c) return GR_OK;
+    if (c->children > 0)
+        return set_err(GR_E_STATE, "gr_ctx_destroy: %d fields/jobs of this context are still alive",
+                       c->children);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->compute);
     (void)hipStreamSynchronize(c->copy);
@@ -325,6 +330,7 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
         field_free(f);
         return rc;
     }
+    ctx->children++;
     *out = f;
     return GR_OK;
 }
@@ -332,10 +338,12 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
 extern "C" int gr_field_destroy(gr_field * f)
 {
     if (!f) return GR_OK;
-    (void)hipSetDevice(f->ctx->device);
-    (void)hipStreamSynchronize(f->ctx->copy);
-    (void)hipStreamSynchronize(f->ctx->compute);
+    gr_ctx * c = f->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->copy);
+    (void)hipStreamSynchronize(c->compute);
     field_free(f);
+    c->children--;
     return GR_OK;
 }
 
@@ -650,6 +658,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
         job_free(j);
         return rc;
     }
+    ctx->children++;
     *out = j;
     return GR_OK;
 }
@@ -657,9 +666,11 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
 extern "C" int gr_job_destroy(gr_job * j)
 {
     if (!j) return GR_OK;
-    (void)hipSetDevice(j->ctx->device);
-    (void)hipStreamSynchronize(j->ctx->compute);
+    gr_ctx * c = j->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->compute);
     job_free(j);
+    c->children--;
     return GR_OK;
 }
 
