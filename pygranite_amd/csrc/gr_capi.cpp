@@ -482,13 +482,15 @@ static int field_pair(gr_field * f, uint32_t b, uint32_t fr, hipStream_t s, cons
                 if ((!p.valid && hit->valid) || (p.valid == hit->valid && p.used < hit->used)) hit = &p;
         }
         const uint32_t so = f->d.ncomp == 3 ? 8 : 4;
+        const uint32_t dim = f->d.ncomp == 3 ? 3 : 2;
         if (!hit->buf)
         {
-            int rc = dalloc(&hit->buf, f->nodes * so * dtype_size(f->d.dtype));
+            const uint64_t pn = gr::pair_nodes(f->d.nx, f->d.ny, f->d.nz, dim);
+            int rc = dalloc(&hit->buf, pn * so * dtype_size(f->d.dtype));
             if (rc != GR_OK) return rc;
         }
-        if (gr::launch_pair(f->slots[b], f->slots[fr], hit->buf, f->nodes, f->d.ncomp, f->cstride,
-                            f->d.dtype, s) != 0)
+        if (gr::launch_pair(f->slots[b], f->slots[fr], hit->buf, f->d.nx, f->d.ny, f->d.nz, dim,
+                            f->d.address, f->d.ncomp, f->cstride, f->d.dtype, s) != 0)
             return set_err(GR_E_HIP, "pair kernel launch failed");
         hit->b = b;
         hit->f = fr;
@@ -954,6 +956,8 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
         P.prog[e].n = d.prog[e].n;
         std::memcpy(P.prog[e].code, d.prog[e].code, sizeof(P.prog[e].code));
         std::memcpy(P.prog[e].imm, d.prog[e].imm, sizeof(P.prog[e].imm));
+        for (uint32_t i = 0; i < d.prog[e].n; ++i)
+            if ((d.prog[e].code[i] & 0xffu) == gr::OP_REF) P.refmask |= 1u << (d.prog[e].code[i] >> 8);
     }
 
     hipStream_t s = j->ctx->compute;

@@ -89,6 +89,7 @@ struct StepParams
     uint32_t abort_mode; // gr_abort_mode
     uint32_t weights;    // gr_weights
     uint32_t nprog, nvol, nconst;
+    uint32_t refmask; // reference-vector slots read by the programs (bit per slot)
 
     float dt, sphere_radius, max_length;
     float h[3], inv_h[3], off[3], dom[3];
@@ -168,9 +169,12 @@ int batch_occupancy(uint32_t block, int * blocks_per_cu);
 // launchers (gr_kernels.hip)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
                    uint32_t block, void * stream);
-// builds the time-paired layout of (back, front) slots: elements per node out = 8 (3D) / 4 (2D)
-int launch_pair(const void * back, const void * front, void * out, uint64_t nodes, uint32_t ncomp,
-                uint32_t cstride, uint32_t dtype, void * stream);
+// builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
+// plane), elements per node 8 (3D) / 4 (2D)
+uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);
+int launch_pair(const void * back, const void * front, void * out, uint32_t nx, uint32_t ny,
+                uint32_t nz, uint32_t dim, uint32_t address, uint32_t ncomp, uint32_t cstride,
+                uint32_t dtype, void * stream);
 // Morton keys of the particles' current cells (dead particles sort last)
 int launch_morton_keys(const StepParams & P, uint32_t dim, uint32_t * keys, uint32_t * vals,
                        void * stream);

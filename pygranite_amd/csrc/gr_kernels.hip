@@ -39,6 +39,14 @@ __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort
 __device__ __forceinline__ float b_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float b_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
+// Global-address-space views of field memory: the slot pointers travel through structs, so
+// without the cast the compiler emits flat loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GR_LOAD(T, b, i) (((const __attribute__((address_space(1))) T *)(b))[i])
+#else
+#define GR_LOAD(T, b, i) (reinterpret_cast<const T *>(b)[i])
+#endif
+
 // Loads the NC components of node `node` (cstride = 4 for NC = 3, else NC).
 template <class T, int NC> struct NodeLoad;
 
@@ -46,7 +54,7 @@ template <> struct NodeLoad<float, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
     {
-        const float4 x = reinterpret_cast<const float4 *>(b)[node];
+        const float4 x = GR_LOAD(float4, b, node);
         v[0] = x.x; v[1] = x.y; v[2] = x.z;
     }
 };
@@ -54,7 +62,7 @@ template <> struct NodeLoad<float, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
     {
-        const float2 x = reinterpret_cast<const float2 *>(b)[node];
+        const float2 x = GR_LOAD(float2, b, node);
         v[0] = x.x; v[1] = x.y;
     }
 };
@@ -62,7 +70,7 @@ template <> struct NodeLoad<__half, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
     {
-        const uint2 x = reinterpret_cast<const uint2 *>(b)[node];
+        const uint2 x = GR_LOAD(uint2, b, node);
         v[0] = h_lo(x.x); v[1] = h_hi(x.x); v[2] = h_lo(x.y);
     }
 };
@@ -70,7 +78,7 @@ template <> struct NodeLoad<__half, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
     {
-        const uint32_t x = reinterpret_cast<const uint32_t *>(b)[node];
+        const uint32_t x = GR_LOAD(uint32_t, b, node);
         v[0] = h_lo(x); v[1] = h_hi(x);
     }
 };
@@ -78,7 +86,7 @@ template <> struct NodeLoad<bf16_t, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[3])
     {
-        const uint2 x = reinterpret_cast<const uint2 *>(b)[node];
+        const uint2 x = GR_LOAD(uint2, b, node);
         v[0] = b_lo(x.x); v[1] = b_hi(x.x); v[2] = b_lo(x.y);
     }
 };
@@ -86,7 +94,7 @@ template <> struct NodeLoad<bf16_t, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&v)[2])
     {
-        const uint32_t x = reinterpret_cast<const uint32_t *>(b)[node];
+        const uint32_t x = GR_LOAD(uint32_t, b, node);
         v[0] = b_lo(x); v[1] = b_hi(x);
     }
 };
@@ -212,37 +220,63 @@ __device__ float sample_scalar(const void * base, const FieldView & f, uint32_t 
 // expression interpreter (wavefront-uniform: the program lives in the kernel arguments, every
 // lane executes the same instruction stream)
 // --------------------------------------------------------------------------------------------
-__device__ __forceinline__ float run_program(const Program & pr, const float (&ref)[19])
+// The top of the stack lives in a register; deeper entries live in LDS, one column per lane
+// (entry d of lane l at stk[d * blockDim.x + l]), so the wave-uniform stack depth turns into a
+// fixed LDS offset instead of dynamically indexed registers. The reference vector is staged in
+// LDS the same way (only the slots the programs read, `refmask`), so OP_REF is one LDS read.
+typedef __attribute__((address_space(3))) float lds_float;
+
+// One LDS area per kernel (a non-template function, so every inlined step variant of the
+// batched kernel shares it): 19 reference slots + (stack depth - 1) stack entries per lane.
+__device__ __forceinline__ lds_float * expr_lds()
 {
-    float st[GR_MAX_STACK];
-    int sp = 0;
+    __shared__ float buf[(19 + GR_MAX_STACK - 1) * 256];
+    return (lds_float *)buf;
+}
+
+__device__ __forceinline__ float run_program(const Program & pr, const lds_float * refs,
+                                             lds_float * stk, uint32_t lane, uint32_t bs)
+{
+    float acc = 0.f;
+    uint32_t depth = 0; // values on the stack, acc included (uniform)
     for (uint32_t i = 0; i < pr.n; ++i)
     {
         const uint32_t w = pr.code[i];
         const uint32_t op = w & 0xffu, arg = w >> 8;
-        switch (op)
+        if (op == OP_REF || op == OP_CONST)
         {
-            case OP_REF: st[sp++] = ref[arg]; break;
-            case OP_CONST: st[sp++] = pr.imm[arg]; break;
-            case OP_ADD: sp--; st[sp - 1] = st[sp - 1] + st[sp]; break;
-            case OP_SUB: sp--; st[sp - 1] = st[sp - 1] - st[sp]; break;
-            case OP_MUL: sp--; st[sp - 1] = st[sp - 1] * st[sp]; break;
-            case OP_DIV: sp--; st[sp - 1] = st[sp - 1] / st[sp]; break;
-            case OP_SQRT: st[sp - 1] = sqrtf(st[sp - 1]); break;
-            case OP_EXP: st[sp - 1] = expf(st[sp - 1]); break;
-            case OP_LN: st[sp - 1] = logf(st[sp - 1]); break;
-            case OP_SIN: st[sp - 1] = sinf(st[sp - 1]); break;
-            case OP_COS: st[sp - 1] = cosf(st[sp - 1]); break;
-            case OP_TAN: st[sp - 1] = tanf(st[sp - 1]); break;
-            case OP_ASIN: st[sp - 1] = asinf(st[sp - 1]); break;
-            case OP_ACOS: st[sp - 1] = acosf(st[sp - 1]); break;
-            case OP_ATAN: st[sp - 1] = atanf(st[sp - 1]); break;
-            case OP_ATAN2: sp--; st[sp - 1] = atan2f(st[sp - 1], st[sp]); break;
-            case OP_ABS: st[sp - 1] = fabsf(st[sp - 1]); break;
-            default: break;
+            if (depth) stk[(depth - 1) * bs + lane] = acc;
+            acc = op == OP_REF ? refs[arg * bs + lane] : pr.imm[arg];
+            ++depth;
+        }
+        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_ATAN2)
+        {
+            --depth;
+            const float a = stk[(depth - 1) * bs + lane];
+            if (op == OP_ADD) acc = a + acc;
+            else if (op == OP_SUB) acc = a - acc;
+            else if (op == OP_MUL) acc = a * acc;
+            else if (op == OP_DIV) acc = a / acc;
+            else acc = atan2f(a, acc);
+        }
+        else
+        {
+            switch (op)
+            {
+                case OP_SQRT: acc = sqrtf(acc); break;
+                case OP_EXP: acc = expf(acc); break;
+                case OP_LN: acc = logf(acc); break;
+                case OP_SIN: acc = sinf(acc); break;
+                case OP_COS: acc = cosf(acc); break;
+                case OP_TAN: acc = tanf(acc); break;
+                case OP_ASIN: acc = asinf(acc); break;
+                case OP_ACOS: acc = acosf(acc); break;
+                case OP_ATAN: acc = atanf(acc); break;
+                default: acc = fabsf(acc); break; // OP_ABS
+            }
         }
     }
-    return sp > 0 ? st[0] : 0.f;
+    return acc;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -261,8 +295,8 @@ template <> struct PairLoad<float, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
     {
-        const float4 a = reinterpret_cast<const float4 *>(b)[2 * uint64_t(node)];
-        const float4 c = reinterpret_cast<const float4 *>(b)[2 * uint64_t(node) + 1];
+        const float4 a = GR_LOAD(float4, b, 2 * uint64_t(node));
+        const float4 c = GR_LOAD(float4, b, 2 * uint64_t(node) + 1);
         x[0] = a.x; x[1] = a.y; x[2] = a.z;
         y[0] = a.w; y[1] = c.x; y[2] = c.y;
     }
@@ -271,7 +305,7 @@ template <> struct PairLoad<float, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
     {
-        const float4 a = reinterpret_cast<const float4 *>(b)[node];
+        const float4 a = GR_LOAD(float4, b, node);
         x[0] = a.x; x[1] = a.y;
         y[0] = a.z; y[1] = a.w;
     }
@@ -280,7 +314,7 @@ template <> struct PairLoad<__half, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
     {
-        const uint4 a = reinterpret_cast<const uint4 *>(b)[node];
+        const uint4 a = GR_LOAD(uint4, b, node);
         x[0] = h_lo(a.x); x[1] = h_hi(a.x); x[2] = h_lo(a.y);
         y[0] = h_hi(a.y); y[1] = h_lo(a.z); y[2] = h_hi(a.z);
     }
@@ -289,7 +323,7 @@ template <> struct PairLoad<__half, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
     {
-        const uint2 a = reinterpret_cast<const uint2 *>(b)[node];
+        const uint2 a = GR_LOAD(uint2, b, node);
         x[0] = h_lo(a.x); x[1] = h_hi(a.x);
         y[0] = h_lo(a.y); y[1] = h_hi(a.y);
     }
@@ -298,7 +332,7 @@ template <> struct PairLoad<bf16_t, 3>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
     {
-        const uint4 a = reinterpret_cast<const uint4 *>(b)[node];
+        const uint4 a = GR_LOAD(uint4, b, node);
         x[0] = b_lo(a.x); x[1] = b_hi(a.x); x[2] = b_lo(a.y);
         y[0] = b_hi(a.y); y[1] = b_lo(a.z); y[2] = b_hi(a.z);
     }
@@ -307,7 +341,7 @@ template <> struct PairLoad<bf16_t, 2>
 {
     static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
     {
-        const uint2 a = reinterpret_cast<const uint2 *>(b)[node];
+        const uint2 a = GR_LOAD(uint2, b, node);
         x[0] = b_lo(a.x); x[1] = b_hi(a.x);
         y[0] = b_lo(a.y); y[1] = b_hi(a.y);
     }
@@ -364,24 +398,26 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
 {
     float arg[3] = {0.f, 0.f, 0.f};
     texel_arg<DIM>(P, x, arg);
-    Taps S;
-    taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
     if (MODE == M_PAIRED)
     {
-        // one vector load per corner gives both slices
+        // Padded paired copy: node (i, j, k), i in [-1, n + 1], at
+        // ((k + 1) * PY + (j + 1)) * PX + (i + 1) with PX = nx + 3, PY = ny + 3; the halo holds
+        // zeros (border addressing) or edge copies (clamp addressing), so the 8 corners are
+        // fixed offsets from the base node and need no per-corner clamp or select.
+        Taps S;
+        int i, j, k = -1;
+        split(arg[0], P.wind.nx, P.weights, i, S.wx);
+        split(arg[1], P.wind.ny, P.weights, j, S.wy);
+        S.wz = 0.f;
+        if (DIM == 3) split(arg[2], P.wind.nz, P.weights, k, S.wz);
+        const uint32_t PX = P.wind.nx + 3, PXY = PX * (P.wind.ny + 3);
+        const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
         float vb[8][DIM], vf[8][DIM];
 #pragma unroll
         for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
         {
-            float xb[DIM], xf[DIM];
-            PairLoad<T, DIM>::get(W.pair, S.node[c], xb, xf);
-            const bool ok = (S.valid >> c) & 1u;
-#pragma unroll
-            for (int m = 0; m < DIM; ++m)
-            {
-                vb[c][m] = ok ? xb[m] : 0.f;
-                vf[c][m] = ok ? xf[m] : 0.f;
-            }
+            const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
+            PairLoad<T, DIM>::get(W.pair, node, vb[c], vf[c]);
         }
         float b[DIM], f[DIM];
         combine<DIM, DIM>(vb, S, b);
@@ -390,6 +426,8 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
         return;
     }
+    Taps S;
+    taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
     float b[DIM];
     sample_slot<DIM, T, DIM>(W.back, S, b);
     if (P.flags & F_WIND_DYNAMIC)
@@ -760,33 +798,38 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
 
                 if (P.nprog > 0) // tpp:379-447
                 {
-                    float ref[19];
-#pragma unroll
-                    for (int r = 0; r < 19; ++r) ref[r] = 0.f;
-                    ref[0] = global_coord(P, np[0], 0);
-                    ref[1] = global_coord(P, np[1], 1);
-                    ref[2] = DIM == 3 ? global_coord(P, np[2], 2) : 0.f;
-                    ref[3] = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f;
+                    // reference vector slots (tpp:381-439) staged in LDS, only those read
+                    lds_float * s_refs = expr_lds();
+                    lds_float * s_stk = s_refs + 19 * 256;
+                    const uint32_t lane = threadIdx.x, bs = blockDim.x, m = P.refmask;
+                    lds_float * R = s_refs + lane;
+                    if (m & (1u << 0)) R[0 * bs] = global_coord(P, np[0], 0);
+                    if (m & (1u << 1)) R[1 * bs] = global_coord(P, np[1], 1);
+                    if (m & (1u << 2)) R[2 * bs] = DIM == 3 ? global_coord(P, np[2], 2) : 0.f;
+                    if (m & (1u << 3)) R[3 * bs] = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f;
                     if (FULL)
                     {
 #pragma unroll
-                        for (int k = 0; k < GR_MAX_VOLUMES; ++k) ref[4 + k] = k < int(P.nvol) ? vo[k] : 0.f;
+                        for (int k = 0; k < GR_MAX_VOLUMES; ++k)
+                            if (m & (1u << (4 + k))) R[(4 + k) * bs] = k < int(P.nvol) ? vo[k] : 0.f;
 #pragma unroll
                         for (int k = 0; k < GR_MAX_CONSTANTS; ++k)
                         {
+                            if (!(m & (1u << (12 + k)))) continue;
                             float cv = 0.f;
                             if (k < int(P.nconst))
                             {
                                 cv = P.cst_b[k][oid];
                                 if (P.flags & F_CONST_DYN) cv = lerp(cv, P.cst_f[k][oid], t);
                             }
-                            ref[12 + k] = cv;
+                            R[(12 + k) * bs] = cv;
                         }
                     }
-                    ref[16] = vc[0];
-                    ref[17] = vc[1];
-                    ref[18] = DIM == 3 ? vc[2] : 0.f;
-                    for (uint32_t e = 0; e < P.nprog; ++e) ex[e] = run_program(P.prog[e], ref);
+                    if (m & (1u << 16)) R[16 * bs] = vc[0];
+                    if (m & (1u << 17)) R[17 * bs] = vc[1];
+                    if (m & (1u << 18)) R[18 * bs] = DIM == 3 ? vc[2] : 0.f;
+                    for (uint32_t e = 0; e < P.nprog; ++e)
+                        ex[e] = run_program(P.prog[e], s_refs, s_stk, lane, bs);
                 }
 
                 if (P.flags & F_KAHAN)
@@ -1080,28 +1123,51 @@ int launch_count_alive(const int32_t * status, uint64_t n, unsigned long long * 
 // locality: time-paired slices and Morton reordering of particles
 // --------------------------------------------------------------------------------------------
 
-// out node = [back comps, front comps, zero pad] with `so` elements per node; W is the raw
-// element word (uint32_t for FP32, uint16_t for FP16 / BF16) -- bits are copied, not converted
+// Padded paired copy. Output node (pi, pj, pk) of the (nx + 3) x (ny + 3) x (nz + 3) grid
+// (2D: one plane) stands for grid node (pi - 1, pj - 1, pk - 1) and holds
+// [back comps, front comps, zero pad] with `so` elements; outside the grid it holds zeros
+// (border addressing) or the nearest edge node (clamp addressing). W is the raw element word
+// (uint32_t for FP32, uint16_t for FP16 / BF16): bits are copied, not converted.
 template <class W>
 __global__ void k_pair(const W * __restrict__ back, const W * __restrict__ front, W * __restrict__ out,
-                       uint64_t nodes, uint32_t ncomp, uint32_t cstride, uint32_t so)
+                       uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim, uint32_t address,
+                       uint32_t ncomp, uint32_t cstride, uint32_t so)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nodes) return;
+    const uint32_t PX = nx + 3, PY = ny + 3, PZ = dim == 3 ? nz + 3 : 1;
+    const uint64_t o = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (o >= uint64_t(PX) * PY * PZ) return;
+    const int pi = int(o % PX), pj = int((o / PX) % PY), pk = int(o / (uint64_t(PX) * PY));
+    int i = pi - 1, j = pj - 1, k = dim == 3 ? pk - 1 : 0;
+    const bool inside = i >= 0 && i < int(nx) && j >= 0 && j < int(ny) && k >= 0 && k < int(nz);
+    if (address == 1)
+    {
+        i = min(max(i, 0), int(nx) - 1);
+        j = min(max(j, 0), int(ny) - 1);
+        k = min(max(k, 0), int(nz) - 1);
+    }
+    const bool copy = inside || address == 1;
+    const uint64_t src = copy ? (uint64_t(k) * ny + uint64_t(j)) * nx + uint64_t(i) : 0;
     for (uint32_t c = 0; c < so; ++c)
     {
         W v = 0;
-        if (c < ncomp)
-            v = back[i * cstride + c];
-        else if (c < 2 * ncomp)
-            v = front[i * cstride + (c - ncomp)];
-        out[i * so + c] = v;
+        if (copy && c < ncomp)
+            v = back[src * cstride + c];
+        else if (copy && c < 2 * ncomp)
+            v = front[src * cstride + (c - ncomp)];
+        out[o * so + c] = v;
     }
 }
 
-int launch_pair(const void * back, const void * front, void * out, uint64_t nodes, uint32_t ncomp,
-                uint32_t cstride, uint32_t dtype, void * stream)
+uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim)
 {
+    return uint64_t(nx + 3) * (ny + 3) * (dim == 3 ? nz + 3 : 1);
+}
+
+int launch_pair(const void * back, const void * front, void * out, uint32_t nx, uint32_t ny,
+                uint32_t nz, uint32_t dim, uint32_t address, uint32_t ncomp, uint32_t cstride,
+                uint32_t dtype, void * stream)
+{
+    const uint64_t nodes = pair_nodes(nx, ny, nz, dim);
     const uint32_t block = 256;
     const uint32_t grid = uint32_t((nodes + block - 1) / block);
     if (grid == 0) return 0;
@@ -1109,11 +1175,11 @@ int launch_pair(const void * back, const void * front, void * out, uint64_t node
     if (dtype == GR_F32)
         hipLaunchKernelGGL(k_pair<uint32_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
                            static_cast<const uint32_t *>(back), static_cast<const uint32_t *>(front),
-                           static_cast<uint32_t *>(out), nodes, ncomp, cstride, so);
+                           static_cast<uint32_t *>(out), nx, ny, nz, dim, address, ncomp, cstride, so);
     else
         hipLaunchKernelGGL(k_pair<uint16_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
                            static_cast<const uint16_t *>(back), static_cast<const uint16_t *>(front),
-                           static_cast<uint16_t *>(out), nodes, ncomp, cstride, so);
+                           static_cast<uint16_t *>(out), nx, ny, nz, dim, address, ncomp, cstride, so);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
