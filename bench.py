@@ -5,8 +5,13 @@ Workload (the 3D ABC configuration): 128^3 grid over [0, 2*pi)^3, 16 slices one 
 apart, 1,048,576 uniform seeds (SplitMix64 seed 1), RK4 with dt = 0.005, per-step curvature
 (IndividualAndTotalCurvature) and the expression sqrt(u*u+v*v+w*w), saved at every step into a
 device-resident column ring. All slices are resident in HBM before the timed region. A bench
-"step" is one launch advancing every particle by `--chunk` integration steps (default 50);
-the default 2 warm-up + 38 timed launches cover the whole 2000-step job.
+"step" is one launch advancing every particle by `--chunk` integration steps (default 50); the
+timed region is the job from its first step: the default 40 timed launches are the whole
+2000-step job. The warm-up launches run on a separate copy of the job before it.
+
+Every seed of this workload leaves the domain (3D border aborts) within about 1000 steps, so
+the headline counts integrated particle-steps: steps a particle began alive, counted on the
+device by the step kernels. The nominal count (particles x steps) is reported beside it.
 
 Forms, all on identical inputs:
   plain_fp32   FP32 slices sampled slot by slot, particles in caller order, one job per launch
@@ -99,13 +104,15 @@ def make_field(capi, ctx, wind, dtype):
 
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder):
-    job = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
-    job.set_option(capi.OPT_PAIRED, 1 if paired else 0)
-    if reorder:
-        job.reorder()
-    ctx.synchronize()
+    def new_job():
+        j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
+        j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
+        if reorder:
+            j.reorder()
+        return j
+
+    job = new_job()
     sch = Schedule()
-    kms = []
 
     def launch():
         left = chunk
@@ -118,7 +125,6 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
             w.time_start = w.time_start64 = float(sch.t)
             w.counter_start, w.nsteps = sch.counter, k
             job.advance(w)
-            kms.append(job.last_advance_ms())
             sch.advance(k, edge, t)
             left -= k
             if edge and reorder:
@@ -126,15 +132,22 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
 
     for _ in range(warmup):
         launch()
-    kms.clear()
+    # the timed job starts from step 0 on fresh state
+    job.close()
+    job = new_job()
+    sch = Schedule()
     ctx.synchronize()
+    ctx.take_work()
+    ctx.take_kernel_ms()
     t0 = time.perf_counter()
     for _ in range(steps):
         launch()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    out = {"wall_s": wall, "kernel_ms": float(np.sum(kms)), "launches": len(kms),
-           "steps_done": sch.counter, "positions": job.read(capi.OUT_POSITIONS),
+    work = ctx.take_work()
+    kernel_ms, launches = ctx.take_kernel_ms()  # HIP events around every step-kernel launch
+    out = {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
+           "steps_done": sch.counter, "work": work, "positions": job.read(capi.OUT_POSITIONS),
            "status": job.read(capi.OUT_STATUS), "abort_step": job.read(capi.OUT_ABORT_STEP)}
     job.close()
     return out
@@ -142,21 +155,23 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
 
 def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
-    jobs = []
-    for idx in parts:
-        j = capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk)
-        j.reorder()
-        jobs.append(j)
-    ctx.synchronize()
+
+    def new_jobs():
+        out = []
+        for idx in parts:
+            j = capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk)
+            j.reorder()
+            out.append(j)
+        return out
+
+    jobs = new_jobs()
     state = dict(window=0, time=0.0, time64=0.0, counter=0)
-    kms = []
 
     def launch():
         spans = [(j, f, dict(first_window=state["window"], max_steps=chunk,
                              time_start=state["time"], time_start64=state["time64"],
                              counter_start=state["counter"])) for j in jobs]
-        ends, ms = ctx.batch_run(spans, timed=True, ends=True)
-        kms.append(ms)
+        ends, _ = ctx.batch_run(spans, timed=False, ends=True)
         e = ends[0]
         if e["window"] != state["window"]:
             for j in jobs:
@@ -165,28 +180,38 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
 
     for _ in range(warmup):
         launch()
-    kms.clear()
+    for j in jobs:
+        j.close()
+    jobs[:] = new_jobs()
+    state.update(window=0, time=0.0, time64=0.0, counter=0)
     ctx.synchronize()
+    ctx.take_work()
+    ctx.take_kernel_ms()
     t0 = time.perf_counter()
     for _ in range(steps):
         launch()
     ctx.synchronize()
     wall = time.perf_counter() - t0
+    work = ctx.take_work()
+    kernel_ms, launches = ctx.take_kernel_ms()
     pos = np.concatenate([j.read(capi.OUT_POSITIONS) for j in jobs])
     status = np.concatenate([j.read(capi.OUT_STATUS) for j in jobs])
     ab = np.concatenate([j.read(capi.OUT_ABORT_STEP) for j in jobs])
     for j in jobs:
         j.close()
-    return {"wall_s": wall, "kernel_ms": float(np.sum(kms)), "launches": len(kms),
-            "steps_done": state["counter"], "positions": pos, "status": status, "abort_step": ab}
+    return {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
+            "steps_done": state["counter"], "work": work, "positions": pos, "status": status,
+            "abort_step": ab}
 
 
 def fp64_error(wind, seeds_sub, gs, steps_done, res, sub):
     """Final-position error of the GPU form vs the FP64 oracle over particles whose status and
     abort step agree (chaotic / threshold cases counted separately)."""
     import oracle
+    # the oracle steps while counter < MaximumSimulationTime / dt (FP32): half a step below
+    # steps_done * dt it runs exactly steps_done steps
     st = dict(Space=0, Integrator=1, DeltaT=float(DT), DataTimeDistance=1.0, GridScale=gs,
-              Offset=[0, 0, 0], MaximumSimulationTime=float(steps_done * DT) + 1e-4,
+              Offset=[0, 0, 0], MaximumSimulationTime=(steps_done - 0.5) * float(DT),
               SaveInterval=0)
     ref = oracle.run(st, wind, seeds_sub, real64=True)
     if ref["steps"] != steps_done:
@@ -213,11 +238,14 @@ def cpu_baseline(wind, gs):
     t0 = time.perf_counter()
     r = oracle.run(st, wind[:7], seeds, real64=False, nthreads=threads)
     el = time.perf_counter() - t0
-    return {"value": n * r["steps"] / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
+    # integrated particle-steps, as on the GPU: steps begun alive, the aborting step included
+    alive = r["status"] == 0
+    work = int(np.where(alive, r["steps"], r["abort_step"].astype(np.int64) + 1).sum())
+    return {"value": work / el, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "nominal_value": n * r["steps"] / el,
             "sample": f"{n} particles x {r['steps']} RK4 steps of the same 3D ABC workload with "
                       f"curvature + expression saved every step (FP32 oracle restatement, "
-                      f"{threads} threads, {el:.1f} s)"}
+                      f"{threads} threads, {el:.1f} s; {work} integrated particle-steps)"}
 
 
 def batched_jobs_lite(capi, ctx, seed=3, njobs=64):
@@ -269,9 +297,11 @@ def batched_jobs_lite(capi, ctx, seed=3, njobs=64):
     # batched FP16: one persistent launch
     jobs = make_jobs(capi.F16, True)
     ctx.synchronize()
+    ctx.take_work()
     t0 = time.perf_counter()
     steps_run, ms = ctx.batch_run([(j, f, dict(max_steps=s)) for j, f, s, _ in jobs])
     wall_b = time.perf_counter() - t0
+    work_b = ctx.take_work()
     ps = sum(k * n for k, (_, _, _, n) in zip(steps_run, jobs))
     for j, *_ in jobs:
         j.close()
@@ -285,13 +315,15 @@ def batched_jobs_lite(capi, ctx, seed=3, njobs=64):
         ctx.batch_run([(j, f, dict(max_steps=s))], timed=False)
     ctx.synchronize()
     wall_s = time.perf_counter() - t0
+    work_s = ctx.take_work()
     for j, *_ in jobs:
         j.close()
     for f in fields.values():
         f.close()
-    return {"jobs": njobs, "particle_steps": ps, "batched_fp16_s": wall_b,
+    return {"jobs": njobs, "nominal_particle_steps": ps, "integrated_particle_steps": work_b,
+            "integrated_particle_steps_plain": work_s, "batched_fp16_s": wall_b,
             "sequential_plain_fp32_s": wall_s, "speedup": wall_s / wall_b,
-            "batched_psps": ps / wall_b}
+            "batched_psps": work_b / wall_b}
 
 
 def make_field2(capi, ctx, wind, dtype):
@@ -326,8 +358,11 @@ def serving_lite(capi, ctx, rate=50.0, duration=8.0, seed=4):
             steps=int(100 + u[i, 2] * 900), integrator=0 if u[i, 3] < 0.25 else 1,
             dt=float(DT)))
     srv = serving.Server(ctx, ds, hbm_budget=32 << 30, quantum=200)
+    ctx.take_work()
     srv.run(reqs)
     rep = srv.report()
+    rep["integrated_particle_steps"] = ctx.take_work()
+    rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]
     rep.update(arrival_rate_per_s=rate, duration_s=duration,
                offered_particle_steps_per_s=sum(r.seeds.shape[0] * r.steps for r in reqs) / duration)
     for d in ds.values():
@@ -427,6 +462,19 @@ def atmosphere_workload(capi, ctx, args):
     print(json.dumps(out), flush=True)
 
 
+def copy_calibration():
+    """Three copies of a 1 GiB device buffer: the reference for scaling HBM byte counters."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copies": 3, "bytes_read_each": a.numel() * 4, "bytes_written_each": a.numel() * 4}))
+
+
 def load_traffic():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     (profiles/traffic_k_batch_abc3d.json), or None."""
@@ -440,7 +488,7 @@ def load_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=38)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--particles", type=int, default=1 << 20)
@@ -450,21 +498,32 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
-    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere"])
+    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere", "copy"],
+                    help="copy: a plain 1 GiB device copy (x3), the calibration run for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--atmo-nx", type=int, default=512)
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
+    ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
+    ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by "
               "particles with no exchange during integration; see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy":
+        copy_calibration()
+        return
     from pygranite_amd import capi, synthetic
     if capi.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     ctx = capi.Context(0)
     info = ctx.info()
+    if args.batch_block:
+        ctx.set_option(capi.CTX_BATCH_BLOCK, args.batch_block)
+    if args.batch_waves >= 0:
+        ctx.set_option(capi.CTX_BATCH_WAVES, args.batch_waves)
     if args.workload == "atmosphere":
         atmosphere_workload(capi, ctx, args)
         return
@@ -490,17 +549,20 @@ def main():
             r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs)
         else:
             raise SystemExit(f"unknown form {name}")
-        r["psps"] = args.particles * args.chunk * args.steps / r["wall_s"]
+        r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
+        r["psps"] = r["work"] / r["wall_s"]  # integrated particle-steps per second
         r["avg_launch_ms"] = r["kernel_ms"] / max(1, r["launches"])
         res[name] = r
-        print(f"#   {name}: {r['psps']:.3e} particle-steps/s (wall), device avg "
-              f"{r['avg_launch_ms']:.3f} ms/launch", file=sys.stderr, flush=True)
+        print(f"#   {name}: {r['psps']:.3e} integrated particle-steps/s (wall; "
+              f"{r['work'] / (args.particles * max(1, r['steps_done'])):.3f} of nominal), device "
+              f"{r['kernel_ms']:.1f} ms in {r['launches']} launches", file=sys.stderr, flush=True)
 
     head = "fp16_batched" if "fp16_batched" in res else forms[-1]
     hv = res[head]
     outputs = 3 * 4 + 4 + 4  # saved position, curvature, expression per particle-step
     B16 = algorithmic_bytes(2, outputs)
-    ps_per_launch = args.particles * args.chunk * args.steps / hv["launches"]
+    # algorithmic bytes of the integrated particle-steps over the kernel's device time
+    ps_per_launch = hv["work"] / max(1, hv["launches"])
     achieved = ps_per_launch * B16 / (hv["avg_launch_ms"] / 1e3) / 1e9
 
     err = None
@@ -524,6 +586,8 @@ def main():
     line = {
         "metric": "RK4 3D particle-steps/s vs plain FP32 at <=1e-4 RMS err; p50/p99 job latency",
         "value": hv["psps"],
+        "nominal_particle_steps_per_s": hv["nominal_psps"],
+        "integrated_fraction": hv["work"] / (args.particles * max(1, hv["steps_done"])),
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -533,12 +597,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 arithmetic, fp16 field storage",
-        "data": "synthetic (analytic ABC flow, SplitMix64 seeds)",
+        "data": "synthetic (analytic ABC flow, SplitMix64 seeds); value counts integrated "
+                "particle-steps (steps begun alive), counted on the device",
         "config": {"workload": "3D ABC flow RK4, 128^3 x 16 slices, 1,048,576 particles, dt "
                                "0.005, 2000 steps, curvature + sqrt(u*u+v*v+w*w) saved every step",
                    "form": head, "particles": args.particles, "grid": n,
-                   "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1},
+                   "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
+                   "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
+                   "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES)},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
+        "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},
         "plain_fp32_value": plain,
         "speedup_vs_plain_fp32": hv["psps"] / plain if plain else None,
         "fp64_error_domain_relative": err,
@@ -547,7 +615,8 @@ def main():
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": "k_batch (RK4 3D FP16 items)" if head == "fp16_batched" else "k_advance",
                      "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
-                     "particle_steps_per_launch": ps_per_launch},
+                     "integrated_particle_steps_per_launch": ps_per_launch,
+                     "algorithmic_bytes_per_launch": ps_per_launch * B16},
         "cpu_baseline": cpu,
         "extras": extras,
         "device": info["name"],

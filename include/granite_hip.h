@@ -84,6 +84,15 @@ int gr_device_count(int * count);
 int gr_ctx_create(int device, gr_ctx ** out);
 int gr_ctx_destroy(gr_ctx * ctx);
 int gr_ctx_synchronize(gr_ctx * ctx);
+/* Integrated particle-steps (steps a particle began alive, its aborting step included) of
+ * every launch on this context since the previous call, then resets the count. Synchronizes
+ * the compute stream. No reference counterpart: it is the work measure of throughput figures
+ * (dead particles cost no sampling). */
+int gr_ctx_take_work(gr_ctx * ctx, uint64_t * particle_steps);
+/* Summed device time (HIP events on the compute stream) and number of the step-kernel launches
+ * (gr_job_advance, gr_batch_run) since the previous call, then resets both. Waits for the last
+ * launch; launches are not serialised by the timing. No reference counterpart. */
+int gr_ctx_take_kernel_ms(gr_ctx * ctx, double * ms, uint64_t * launches);
 /* device name and HBM bytes of the context's GPU */
 int gr_ctx_info(gr_ctx * ctx, char * name, size_t name_len, uint64_t * hbm_bytes,
                 int * compute_units);
@@ -264,8 +273,10 @@ int gr_job_reorder(gr_job * job);
  * (FP32 time, max_windfield_time += DataTimeDistance per window, TrajectoryIntegrator.tpp:
  * 852-994); a span ends after max_steps steps, when the step counter reaches
  * max_sim_time / dt, or when no further slice exists (the loader's step() == false). Jobs of
- * different dimension, integrator and storage type run in the same launch: workgroups pull
- * (job, particle block) items from a device counter, longest spans first. Jobs must use the
+ * different dimension, integrator and storage type run in the same launch: every wavefront
+ * pulls (job, 64 particles) items from one queue per XCD (items split by their position in the
+ * job's particle order, longest spans first inside a queue) and moves on to the other queues
+ * once its own is drained. Jobs must use the
  * lean feature set (no uplift, constants, volumes, topography, loop borders, sphere or length
  * aborts). No counterpart in the reference (one job per TrajectoryIntegrator). */
 typedef struct {
@@ -291,10 +302,21 @@ typedef struct {
 } gr_span_end;
 
 /* Runs the spans; ends[i] (may be NULL) receives where span i stopped. `block` = 0 picks the
- * default (256). The call returns after queueing; *ms (may be NULL) receives the device time of
- * the launch and makes the call wait for it. */
+ * context's batch block size (GR_CTX_BATCH_BLOCK). The call returns after queueing; *ms (may be
+ * NULL) receives the device time of the launch and makes the call wait for it. */
 int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
                  gr_span_end * ends, float * ms);
+
+/* Context options of the batched kernel (no counterpart in the reference). Results do not
+ * depend on them, only speed.
+ * GR_CTX_BATCH_BLOCK: workgroup size when gr_batch_run gets block = 0 (64, 128, 192 or 256;
+ *   default 256).
+ * GR_CTX_BATCH_WAVES: register budget of the batched kernel: 0 = the compiler's allocation
+ *   (default), 4 = at most 128 VGPRs so that 4 wavefronts fit per SIMD. */
+#define GR_CTX_BATCH_BLOCK 1u
+#define GR_CTX_BATCH_WAVES 2u
+int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
+int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,8 @@ EXPR_ALLOW_ABS = 1
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
     "gr_abi_version", "gr_last_error", "gr_device_count", "gr_ctx_create", "gr_ctx_destroy",
-    "gr_ctx_synchronize", "gr_ctx_info", "gr_field_create", "gr_field_destroy",
+    "gr_ctx_synchronize", "gr_ctx_info", "gr_ctx_take_work", "gr_ctx_take_kernel_ms", "gr_ctx_set_option",
+    "gr_ctx_get_option", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
@@ -34,6 +35,7 @@ EXPORTS = [
     "gr_job_set_option", "gr_job_reorder", "gr_batch_run",
 ]
 OPT_BLOCK, OPT_PAIRED = 1, 2
+CTX_BATCH_BLOCK, CTX_BATCH_WAVES = 1, 2
 
 
 class FieldDesc(ctypes.Structure):
@@ -119,6 +121,10 @@ def lib():
         "gr_ctx_destroy": ([vp], i32),
         "gr_ctx_synchronize": ([vp], i32),
         "gr_ctx_info": ([vp, ctypes.c_char_p, ctypes.c_size_t, P(u64), P(i32)], i32),
+        "gr_ctx_take_work": ([vp, P(u64)], i32),
+        "gr_ctx_take_kernel_ms": ([vp, P(ctypes.c_double), P(u64)], i32),
+        "gr_ctx_set_option": ([vp, u32, ctypes.c_int64], i32),
+        "gr_ctx_get_option": ([vp, u32, P(ctypes.c_int64)], i32),
         "gr_field_create": ([vp, P(FieldDesc), P(vp)], i32),
         "gr_field_destroy": ([vp], i32),
         "gr_field_upload": ([vp, u32, P(vp), i32], i32),
@@ -237,6 +243,26 @@ class Context:
 
     def synchronize(self):
         _check(lib().gr_ctx_synchronize(self.h))
+
+    def set_option(self, option: int, value: int):
+        _check(lib().gr_ctx_set_option(self.h, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64(0)
+        _check(lib().gr_ctx_get_option(self.h, option, ctypes.byref(v)))
+        return int(v.value)
+
+    def take_kernel_ms(self):
+        """(summed device ms, launches) of the step kernels since the previous call (then reset)."""
+        ms, n = ctypes.c_double(0), ctypes.c_uint64(0)
+        _check(lib().gr_ctx_take_kernel_ms(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return float(ms.value), int(n.value)
+
+    def take_work(self) -> int:
+        """Integrated particle-steps of every launch since the previous call (then reset)."""
+        v = ctypes.c_uint64(0)
+        _check(lib().gr_ctx_take_work(self.h, ctypes.byref(v)))
+        return int(v.value)
 
     def close(self):
         if self.h:

@@ -73,7 +73,7 @@ struct gr_ctx
     int device{0};
     hipStream_t compute{nullptr};
     hipStream_t copy{nullptr};
-    unsigned long long * dcount{nullptr};
+    unsigned long long * dcount{nullptr}; // [0] alive count scratch, [1] integrated particle-steps
     // persistent batched launches
     std::vector<gr::StepParams> h_params;
     std::vector<gr::BatchItem> h_items;
@@ -81,9 +81,18 @@ struct gr_ctx
     size_t params_cap{0};
     gr::BatchItem * d_items{nullptr};
     size_t items_cap{0};
-    uint32_t * d_counter{nullptr};
+    uint32_t * d_queues{nullptr}; // MAX_QUEUES counters, then MAX_QUEUES + 1 queue starts
+    std::vector<uint32_t> h_queues;
     hipEvent_t bev0{nullptr}, bev1{nullptr};
     int cus{0};
+    uint32_t nxcd{1}; // XCDs of the device
+    uint32_t batch_block{256};
+    uint32_t batch_waves{0};
+    // device time of every step-kernel launch (event pairs, summed by gr_ctx_take_kernel_ms)
+    std::vector<hipEvent_t> kev;
+    size_t kpending{0};
+    double kms_acc{0.0};
+    uint64_t klaunch_acc{0};
     // live fields and jobs: the context outlives them (gr_ctx_destroy refuses otherwise)
     int children{0};
 };
@@ -205,7 +214,8 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
     // default-priority streams only
     if (hipStreamCreateWithFlags(&c->compute, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&c->dcount), sizeof(unsigned long long)) != hipSuccess)
+        hipMalloc(reinterpret_cast<void **>(&c->dcount), 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(c->dcount, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
     {
         (void)hipGetLastError();
         if (c->compute) (void)hipStreamDestroy(c->compute);
@@ -213,6 +223,12 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
         delete c;
         return set_err(GR_E_HIP, "gr_ctx_create: stream / buffer creation failed");
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    int nx = 1;
+    if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && nx >= 1)
+        c->nxcd = uint32_t(std::min<int>(nx, int(gr::MAX_QUEUES)));
+    (void)hipGetLastError();
     *out = c;
     return GR_OK;
 }
@@ -226,10 +242,11 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->compute);
     (void)hipStreamSynchronize(c->copy);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     dfree(c->dcount);
     dfree(c->d_params);
     dfree(c->d_items);
-    dfree(c->d_counter);
+    dfree(c->d_queues);
     if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->bev1) (void)hipEventDestroy(c->bev1);
     (void)hipStreamDestroy(c->compute);
@@ -244,6 +261,96 @@ extern "C" int gr_ctx_synchronize(gr_ctx * c)
     GR_HIP(hipSetDevice(c->device));
     GR_HIP(hipStreamSynchronize(c->copy));
     GR_HIP(hipStreamSynchronize(c->compute));
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
+{
+    if (!c) return set_err(GR_E_INVALID, "null ctx");
+    switch (option)
+    {
+        case GR_CTX_BATCH_BLOCK:
+            if (value <= 0 || value % 64 != 0 || value > 256)
+                return set_err(GR_E_INVALID, "batch block must be a multiple of 64 up to 256");
+            c->batch_block = uint32_t(value);
+            return GR_OK;
+        case GR_CTX_BATCH_WAVES:
+            if (value != 0 && value != 4) return set_err(GR_E_INVALID, "batch waves must be 0 or 4");
+            c->batch_waves = uint32_t(value);
+            return GR_OK;
+        default: return set_err(GR_E_INVALID, "unknown context option %u", option);
+    }
+}
+
+extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
+{
+    if (!c || !value) return set_err(GR_E_INVALID, "gr_ctx_get_option: null argument");
+    switch (option)
+    {
+        case GR_CTX_BATCH_BLOCK: *value = c->batch_block; return GR_OK;
+        case GR_CTX_BATCH_WAVES: *value = c->batch_waves; return GR_OK;
+        default: return set_err(GR_E_INVALID, "unknown context option %u", option);
+    }
+}
+
+// sums the recorded launch times into the accumulator (waits for the last launch)
+static int kt_fold(gr_ctx * c)
+{
+    if (!c->kpending) return GR_OK;
+    GR_HIP(hipEventSynchronize(c->kev[2 * c->kpending - 1]));
+    for (size_t i = 0; i < c->kpending; ++i)
+    {
+        float ms = 0.f;
+        GR_HIP(hipEventElapsedTime(&ms, c->kev[2 * i], c->kev[2 * i + 1]));
+        c->kms_acc += ms;
+    }
+    c->klaunch_acc += c->kpending;
+    c->kpending = 0;
+    return GR_OK;
+}
+
+// records the start (end = false) or end of a step-kernel launch on stream s
+static int kt_mark(gr_ctx * c, hipStream_t s, bool end)
+{
+    if (!end && c->kpending == 1024)
+    {
+        int rc = kt_fold(c);
+        if (rc != GR_OK) return rc;
+    }
+    const size_t i = 2 * c->kpending + (end ? 1 : 0);
+    while (c->kev.size() <= i)
+    {
+        hipEvent_t e;
+        GR_HIP(hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    GR_HIP(hipEventRecord(c->kev[i], s));
+    if (end) ++c->kpending;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_take_kernel_ms(gr_ctx * c, double * ms, uint64_t * launches)
+{
+    if (!c || !ms) return set_err(GR_E_INVALID, "gr_ctx_take_kernel_ms: null argument");
+    GR_HIP(hipSetDevice(c->device));
+    int rc = kt_fold(c);
+    if (rc != GR_OK) return rc;
+    *ms = c->kms_acc;
+    if (launches) *launches = c->klaunch_acc;
+    c->kms_acc = 0.0;
+    c->klaunch_acc = 0;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_take_work(gr_ctx * c, uint64_t * particle_steps)
+{
+    if (!c || !particle_steps) return set_err(GR_E_INVALID, "gr_ctx_take_work: null argument");
+    GR_HIP(hipSetDevice(c->device));
+    unsigned long long h = 0;
+    GR_HIP(hipMemcpyAsync(&h, c->dcount + 1, sizeof(h), hipMemcpyDeviceToHost, c->compute));
+    GR_HIP(hipMemsetAsync(c->dcount + 1, 0, sizeof(h), c->compute));
+    GR_HIP(hipStreamSynchronize(c->compute));
+    *particle_steps = h;
     return GR_OK;
 }
 
@@ -851,6 +958,7 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
 
     std::memset(&P, 0, sizeof(P));
     P.n = uint32_t(j->n);
+    P.work = j->ctx->dcount + 1;
     P.nsteps = w->nsteps;
     uint32_t fl = 0;
     const bool dyn = d.windfield_mode == 1 || d.uplift_mode == 2 || d.constants_mode == 2 || d.volume_mode == 2;
@@ -1018,8 +1126,10 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if (rc != GR_OK) return rc;
     hipStream_t s = j->ctx->compute;
     GR_HIP(hipEventRecord(j->ev0, s));
-    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, s) != 0)
+    if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if ((rc = kt_mark(j->ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev1, s));
     j->timed = true;
     return finish_advance(j, w, nsaves);
@@ -1096,7 +1206,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
                             gr_span_end * ends, float * ms)
 {
     if (!ctx || (nspans && !spans)) return set_err(GR_E_INVALID, "gr_batch_run: null argument");
-    if (block == 0) block = 256;
+    if (block == 0) block = ctx->batch_block;
     if (block % 64 != 0 || block > 256) return set_err(GR_E_INVALID, "block must be a multiple of 64 up to 256");
     GR_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->compute;
@@ -1104,9 +1214,6 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     {
         GR_HIP(hipEventCreate(&ctx->bev0));
         GR_HIP(hipEventCreate(&ctx->bev1));
-        hipDeviceProp_t prop;
-        GR_HIP(hipGetDeviceProperties(&prop, ctx->device));
-        ctx->cus = prop.multiProcessorCount;
     }
     ctx->h_params.resize(nspans);
     std::vector<uint64_t> saves(nspans, 0);
@@ -1210,20 +1317,50 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         }
     }
 
-    // work items, longest spans first
-    std::vector<uint32_t> order(nspans);
-    for (uint32_t i = 0; i < nspans; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return span_steps[a] > span_steps[b]; });
-    ctx->h_items.clear();
-    for (uint32_t i : order)
+    // Work items of one wavefront of particles. Items go to one queue per XCD by their position
+    // in their job's particle range (Morton-ordered after gr_job_reorder), and inside a queue
+    // they are ordered by span length class (longest first), field, then position: the
+    // wavefronts in flight on one XCD then sample one compact region of one field, and long
+    // spans start early.
+    const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>(ctx->nxcd, gr::MAX_QUEUES));
+    struct ItemKey
+    {
+        uint32_t q, cls;
+        uintptr_t field;
+        float pos;
+        gr::BatchItem item;
+    };
+    std::vector<ItemKey> keyed;
+    for (uint32_t i = 0; i < nspans; ++i)
     {
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
         const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
-        for (uint64_t f = 0; f < j->n; f += block)
-            ctx->h_items.push_back(gr::BatchItem{i, uint32_t(f), uint32_t(std::min<uint64_t>(block, j->n - f)), var});
+        uint32_t cls = 0;
+        for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls;
+        for (uint64_t f = 0; f < j->n; f += gr::BATCH_ITEM)
+        {
+            const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, j->n - f));
+            const float pos = float((double(f) + 0.5 * cnt) / double(j->n));
+            const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(pos * float(nq)));
+            keyed.push_back(ItemKey{q, cls, uintptr_t(spans[i].wind), pos, gr::BatchItem{i, uint32_t(f), cnt, var}});
+        }
     }
+    std::stable_sort(keyed.begin(), keyed.end(), [](const ItemKey & a, const ItemKey & b) {
+        if (a.q != b.q) return a.q < b.q;
+        if (a.cls != b.cls) return a.cls > b.cls;
+        if (a.field != b.field) return a.field < b.field;
+        return a.pos < b.pos;
+    });
+    ctx->h_items.resize(keyed.size());
+    ctx->h_queues.assign(2 * gr::MAX_QUEUES + 1, 0u); // counters (zero), then queue starts
+    uint32_t * qb = ctx->h_queues.data() + gr::MAX_QUEUES;
+    for (size_t k = 0; k < keyed.size(); ++k)
+    {
+        ctx->h_items[k] = keyed[k].item;
+        ++qb[keyed[k].q + 1];
+    }
+    for (uint32_t q = 0; q < nq; ++q) qb[q + 1] += qb[q];
     const uint32_t nitems = uint32_t(ctx->h_items.size());
     if (nspans > ctx->params_cap)
     {
@@ -1241,20 +1378,29 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if ((rc = dalloc(&ctx->d_items, nitems * sizeof(gr::BatchItem))) != GR_OK) return rc;
         ctx->items_cap = nitems;
     }
-    if (!ctx->d_counter && (rc = dalloc(&ctx->d_counter, sizeof(uint32_t))) != GR_OK) return rc;
+    if (!ctx->d_queues && (rc = dalloc(&ctx->d_queues, ctx->h_queues.size() * sizeof(uint32_t))) != GR_OK)
+        return rc;
     if (nspans)
         GR_HIP(hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), nspans * sizeof(gr::StepParams),
                               hipMemcpyHostToDevice, s));
     if (nitems)
         GR_HIP(hipMemcpyAsync(ctx->d_items, ctx->h_items.data(), nitems * sizeof(gr::BatchItem),
                               hipMemcpyHostToDevice, s));
-    GR_HIP(hipMemsetAsync(ctx->d_counter, 0, sizeof(uint32_t), s));
+    // zeroed counters and the queue starts in one copy
+    GR_HIP(hipMemcpyAsync(ctx->d_queues, ctx->h_queues.data(), ctx->h_queues.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
     int per_cu = 0;
-    if (gr::batch_occupancy(block, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(nitems, uint32_t(per_cu) * uint32_t(ctx->cus)));
+    if (gr::batch_occupancy(block, ctx->batch_waves, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
+    // enough workgroups to fill the device, no more than one wavefront per item
+    const uint32_t wpb = block / 64u;
+    const uint32_t grid = std::max<uint32_t>(
+        1, std::min<uint32_t>((nitems + wpb - 1) / wpb, uint32_t(per_cu) * uint32_t(std::max(1, ctx->cus))));
     GR_HIP(hipEventRecord(ctx->bev0, s));
-    if (gr::launch_batch(ctx->d_params, ctx->d_items, nitems, ctx->d_counter, grid, block, s) != 0)
+    if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
+    if (nitems && gr::launch_batch(ctx->d_params, ctx->d_items, ctx->d_queues + gr::MAX_QUEUES, nq,
+                                   ctx->d_queues, grid, block, ctx->batch_waves, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (nitems && (rc = kt_mark(ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(ctx->bev1, s));
     for (uint32_t i = 0; i < nspans; ++i)
     {
@@ -1362,6 +1508,29 @@ extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint
     if (!base) return set_err(GR_E_INVALID, "gr_job_read_saved: output %u is not saved by this job", kind);
     hipStream_t s = j->ctx->compute;
     float * out = static_cast<float *>(dst);
+    // the step kernels save only particles alive at the saving step; complete the columns with
+    // the final state of the particles stopped before them
+    {
+        const float * src[3] = {nullptr, nullptr, nullptr};
+        uint32_t w = 1, local = 0;
+        if (kind == GR_OUT_POSITIONS)
+        {
+            w = j->d.dim;
+            for (uint32_t d = 0; d < w; ++d) src[d] = j->p[d];
+            local = j->d.local_coords ? 1u : 0u;
+        }
+        else if (kind == GR_OUT_CURVATURE)
+            src[0] = j->curv;
+        else if (kind >= GR_OUT_VOLUME0)
+            src[0] = j->vol[kind - GR_OUT_VOLUME0];
+        else
+            src[0] = j->expr[kind - GR_OUT_EXPR0];
+        for (uint64_t c = 0; c < count; c += 65536)
+            if (gr::launch_fill_frozen(const_cast<float *>(base), j->cap, first + c,
+                                       uint32_t(std::min<uint64_t>(65536, count - c)), n, w, j->status,
+                                       j->abort_step, src, j->d.save_interval, local, j->d.offset, s) != 0)
+                return set_err(GR_E_HIP, "fill kernel failed");
+    }
     if (j->reordered)
     {
         if (first < j->reorder_col0)

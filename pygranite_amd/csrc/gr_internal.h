@@ -148,12 +148,16 @@ struct StepParams
     const float * cst_f[GR_MAX_CONSTANTS];
 
     Program prog[GR_MAX_PROGRAMS];
+
+    // integrated particle-steps (steps begun alive) are added here, or null
+    unsigned long long * work;
 };
 
 // kernel modes (see gr_kernels.hip)
 enum : uint32_t { MODE_PAIRED = 0, MODE_SEPARATE = 1, MODE_FULL = 2 };
 
-// one work item of the persistent batched kernel: `count` particles of job `job`
+// one work item of the persistent batched kernel: `count` (<= 64, one wavefront) particles of
+// job `job`
 struct BatchItem
 {
     uint32_t job;
@@ -161,14 +165,20 @@ struct BatchItem
     uint32_t count;
     uint32_t variant; // batch_variant(dim, integrator, dtype)
 };
+enum : uint32_t { BATCH_ITEM = 64, MAX_QUEUES = 16 };
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype);
-int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t nitems,
-                 uint32_t * counter, uint32_t grid, uint32_t block, void * stream);
-int batch_occupancy(uint32_t block, int * blocks_per_cu);
+// items are split into nq queues, queue q = items[qbegin[q] .. qbegin[q + 1]); counters[q] must
+// be zero at launch
+// waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD)
+int launch_batch(const StepParams * jobs, const BatchItem * items, const uint32_t * qbegin,
+                 uint32_t nq, uint32_t * counters, uint32_t grid, uint32_t block, uint32_t waves,
+                 void * stream);
+int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
 
 // launchers (gr_kernels.hip)
+// nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, void * stream);
+                   uint32_t block, uint32_t nxcd, void * stream);
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);
@@ -186,6 +196,13 @@ int launch_gather32(const void * in, void * out, const uint32_t * perm, uint64_t
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
                      void * stream);
+// fills the saved-column entries of stopped particles (columns saved after their stop step)
+// with their final state: ring holds `cap` columns of n x w floats, columns [first, first +
+// count) are completed; src[0..w-1] are the final values; off64 is added when local != 0
+int launch_fill_frozen(float * ring, uint32_t cap, uint64_t first, uint32_t count, uint64_t n,
+                       uint32_t w, const int32_t * status, const int32_t * abort_step,
+                       const float * const * src, uint64_t save_interval, uint32_t local,
+                       const double * off64, void * stream);
 int launch_pack(const float * src, void * dst, uint64_t nodes, uint32_t ncomp, uint32_t cstride,
                 uint32_t dtype, void * stream);
 int launch_seed(const float * aos, float * const * soa, uint64_t n, uint32_t dim,

@@ -47,6 +47,17 @@ __device__ __forceinline__ float b_hi(uint32_t w) { return __uint_as_float(w & 0
 #define GR_LOAD(T, b, i) (reinterpret_cast<const T *>(b)[i])
 #endif
 
+// the same for state and output arrays reached through structs in global memory (the batched
+// kernel's job table): G(p)[i] is a global-address-space access
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T * G(T * p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+#else
+template <class T> __host__ __device__ T * G(T * p) { return p; }
+#endif
+
 // Loads the NC components of node `node` (cstride = 4 for NC = 3, else NC).
 template <class T, int NC> struct NodeLoad;
 
@@ -102,9 +113,9 @@ template <> struct NodeLoad<bf16_t, 2>
 // generic scalar component load (volumes, topography)
 __device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint64_t idx)
 {
-    if (dtype == GR_F16) return __half2float(reinterpret_cast<const __half *>(b)[idx]);
-    if (dtype == GR_BF16) return __uint_as_float(uint32_t(reinterpret_cast<const uint16_t *>(b)[idx]) << 16);
-    return reinterpret_cast<const float *>(b)[idx];
+    if (dtype == GR_F16) return __half2float(GR_LOAD(__half, b, idx));
+    if (dtype == GR_BF16) return __uint_as_float(uint32_t(GR_LOAD(uint16_t, b, idx)) << 16);
+    return GR_LOAD(float, b, idx);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -376,11 +387,11 @@ __device__ __forceinline__ float zcoord(const FieldView & f, float zr)
         k = nz - 2;
     else
     {
-        k = f.zlut[int(u)];
-        while (k > 0 && zr < f.zlev[k]) --k; // guards the float rounding of u at bin edges
-        while (k + 2 < nz && zr >= f.zlev[k + 1]) ++k;
+        k = GR_LOAD(int32_t, f.zlut, int(u));
+        while (k > 0 && zr < GR_LOAD(float, f.zlev, k)) --k; // guards the float rounding of u at bin edges
+        while (k + 2 < nz && zr >= GR_LOAD(float, f.zlev, k + 1)) ++k;
     }
-    const float z0 = f.zlev[k], z1 = f.zlev[k + 1];
+    const float z0 = GR_LOAD(float, f.zlev, k), z1 = GR_LOAD(float, f.zlev, k + 1);
     return float(k) + (zr - z0) / (z1 - z0);
 }
 
@@ -512,46 +523,50 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
     return (P.flags & F_LOCAL) ? float(P.off64[d] + double(x)) : x;
 }
 
+// Advances one particle by the launch's steps; returns the number of steps it integrated
+// (steps begun alive, the aborting step included).
 template <int DIM, int INTEG, class T, int MODE>
-__device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t pid);
-
-template <int DIM, int INTEG, class T, int MODE>
-__device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t pid)
+__device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint32_t pid)
 {
     constexpr bool FULL = MODE == M_FULL;
-    const uint32_t oid = (FULL && P.orig) ? P.orig[pid] : pid; // caller-order index
+    const uint32_t oid = (FULL && P.orig) ? GR_LOAD(uint32_t, P.orig, pid) : pid; // caller-order index
 
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
     Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
     uint32_t wi = 0;
-    uint32_t edge = P.nwin > 1 ? P.win_edge[0] : 0xffffffffu;
+    uint32_t edge = P.nwin > 1 ? G(P.win_edge)[0] : 0xffffffffu;
+
+    // A stopped particle's state never changes again (tpp:452-455 copies it unchanged) and its
+    // saved columns after the stop are left to the reader, which fills them from the final
+    // state: nothing to do.
+    int32_t status = G(P.status)[pid];
+    if (status != GR_STATUS_TIME) return 0;
 
     float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
     float ld[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) p[d] = P.p[d][pid];
+    for (int d = 0; d < DIM; ++d) p[d] = G(P.p[d])[pid];
     const bool curv_on = P.curvature != GR_CURV_OFF;
     const bool curv_pos = P.curvature == GR_CURV_TOTAL || P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
     if (curv_pos)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) q[d] = P.q[d][pid];
+        for (int d = 0; d < DIM; ++d) q[d] = G(P.q[d])[pid];
     if (P.curvature == GR_CURV_FAST_TOTAL)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) ld[d] = P.ld[d][pid];
+        for (int d = 0; d < DIM; ++d) ld[d] = G(P.ld[d])[pid];
     if (P.flags & F_KAHAN)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) cmp[d] = P.c[d][pid];
-    int32_t status = P.status[pid];
-    int32_t abort_step = P.abort_step[pid];
-    float total = curv_on ? P.total[pid] : 0.f;
-    float curv = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? P.curv[pid] : 0.f;
-    float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? P.len[pid] : 0.f;
+        for (int d = 0; d < DIM; ++d) cmp[d] = G(P.c[d])[pid];
+    int32_t abort_step = G(P.abort_step)[pid];
+    float total = curv_on ? G(P.total)[pid] : 0.f;
+    float curv = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? G(P.curv)[pid] : 0.f;
+    float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? G(P.len)[pid] : 0.f;
     float ex[GR_MAX_PROGRAMS];
 #pragma unroll
-    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) ex[e] = e < int(P.nprog) ? P.expr[e][pid] : 0.f;
+    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) ex[e] = e < int(P.nprog) ? G(P.expr[e])[pid] : 0.f;
     float vo[GR_MAX_VOLUMES];
 #pragma unroll
-    for (int k = 0; k < GR_MAX_VOLUMES; ++k) vo[k] = (FULL && k < int(P.nvol)) ? P.vol[k][pid] : 0.f;
+    for (int k = 0; k < GR_MAX_VOLUMES; ++k) vo[k] = (FULL && k < int(P.nvol)) ? G(P.vol[k])[pid] : 0.f;
 
     const float dt = P.dt;
     const float half_dt = 0.5f * dt;
@@ -560,23 +575,26 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
     uint64_t next_save = P.save_first;
     uint64_t g = P.save_g0;
 
+    uint32_t integrated = 0;
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
         if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
         {
             ++wi;
             const uint32_t c = P.win_first + wi;
-            W.back = P.win_ptr[3 * wi + 0];
-            W.front = P.win_ptr[3 * wi + 1];
-            W.pair = P.win_ptr[3 * wi + 2];
+            W.back = G(P.win_ptr)[3 * wi + 0];
+            W.front = G(P.win_ptr)[3 * wi + 1];
+            W.pair = G(P.win_ptr)[3 * wi + 2];
             W.tb = float(c) * P.dtd;
             W.tf = float(c + 1) * P.dtd;
             W.tb64 = double(c) * double(P.dtd);
             W.tf64 = double(c + 1) * double(P.dtd);
-            edge = wi + 1 < P.nwin ? P.win_edge[wi] : 0xffffffffu;
+            edge = wi + 1 < P.nwin ? G(P.win_edge)[wi] : 0xffffffffu;
         }
-        if (status == GR_STATUS_TIME)
+        const bool live = status == GR_STATUS_TIME; // alive at the start of step s
+        if (live)
         {
+            ++integrated;
             float t = 0.f;
             if (P.flags & F_DYNAMIC_ANY) t = unit_t(P, W, time32, time64, 0.f);
             const bool sph = FULL && DIM == 2 && (P.flags & F_SPHERE);
@@ -634,8 +652,8 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
                 {
-                    const float lb = P.lift_b[uint64_t(oid) * DIM + d];
-                    const float l = (P.flags & F_UPLIFT_DYN) ? lerp(lb, P.lift_f[uint64_t(oid) * DIM + d], t) : lb;
+                    const float lb = G(P.lift_b)[uint64_t(oid) * DIM + d];
+                    const float l = (P.flags & F_UPLIFT_DYN) ? lerp(lb, G(P.lift_f)[uint64_t(oid) * DIM + d], t) : lb;
                     dp[d] = dp[d] + l;
                 }
             }
@@ -819,8 +837,8 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
                             float cv = 0.f;
                             if (k < int(P.nconst))
                             {
-                                cv = P.cst_b[k][oid];
-                                if (P.flags & F_CONST_DYN) cv = lerp(cv, P.cst_f[k][oid], t);
+                                cv = G(P.cst_b[k])[oid];
+                                if (P.flags & F_CONST_DYN) cv = lerp(cv, G(P.cst_f[k])[oid], t);
                             }
                             R[(12 + k) * bs] = cv;
                         }
@@ -846,99 +864,149 @@ __device__ __forceinline__ void advance_one(const StepParams & P, const uint32_t
             }
         }
 
-        if (s == next_save) // copyParticleInfo, tpp:997-1049
+        // copyParticleInfo, tpp:997-1049. Columns after a particle's stop step hold its final
+        // state and are filled when read (k_fill_frozen).
+        if (s == next_save && live)
         {
             const uint64_t col = g % P.ring_cap;
-            float * rp = P.ring_pos + (col * P.n + pid) * DIM;
+            auto rp = G(P.ring_pos) + (col * P.n + pid) * DIM;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) rp[d] = global_coord(P, p[d], d);
-            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) P.ring_curv[col * P.n + pid] = curv;
+            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.ring_curv)[col * P.n + pid] = curv;
             for (uint32_t e = 0; e < P.nprog; ++e)
-                P.ring_expr[(uint64_t(e) * P.ring_cap + col) * P.n + pid] = ex[e];
+                G(P.ring_expr)[(uint64_t(e) * P.ring_cap + col) * P.n + pid] = ex[e];
             if (FULL)
                 for (uint32_t k = 0; k < P.nvol; ++k)
-                    P.ring_vol[(uint64_t(k) * P.ring_cap + col) * P.n + pid] = vo[k];
+                    G(P.ring_vol)[(uint64_t(k) * P.ring_cap + col) * P.n + pid] = vo[k];
+        }
+        if (s == next_save)
+        {
             g++;
             next_save += P.save_interval;
         }
         time32 += dt;
         time64 += double(dt);
+        if (status != GR_STATUS_TIME) break; // stopped: nothing more to compute or save
     }
 
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) P.p[d][pid] = p[d];
+    for (int d = 0; d < DIM; ++d) G(P.p[d])[pid] = p[d];
     if (curv_pos)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) P.q[d][pid] = q[d];
+        for (int d = 0; d < DIM; ++d) G(P.q[d])[pid] = q[d];
     if (P.curvature == GR_CURV_FAST_TOTAL)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) P.ld[d][pid] = ld[d];
+        for (int d = 0; d < DIM; ++d) G(P.ld[d])[pid] = ld[d];
     if (P.flags & F_KAHAN)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) P.c[d][pid] = cmp[d];
-    P.status[pid] = status;
-    P.abort_step[pid] = abort_step;
-    if (curv_on) P.total[pid] = total;
-    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) P.curv[pid] = curv;
-    if (FULL && P.abort_mode != GR_ABORT_TIME) P.len[pid] = len;
-    for (uint32_t e = 0; e < P.nprog; ++e) P.expr[e][pid] = ex[e];
+        for (int d = 0; d < DIM; ++d) G(P.c[d])[pid] = cmp[d];
+    G(P.status)[pid] = status;
+    G(P.abort_step)[pid] = abort_step;
+    if (curv_on) G(P.total)[pid] = total;
+    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.curv)[pid] = curv;
+    if (FULL && P.abort_mode != GR_ABORT_TIME) G(P.len)[pid] = len;
+    for (uint32_t e = 0; e < P.nprog; ++e) G(P.expr[e])[pid] = ex[e];
     if (FULL)
-        for (uint32_t k = 0; k < P.nvol; ++k) P.vol[k][pid] = vo[k];
+        for (uint32_t k = 0; k < P.nvol; ++k) G(P.vol[k])[pid] = vo[k];
+    return integrated;
+}
+
+// Adds the wavefront's integrated particle-steps to *work (all 64 lanes must be active). The
+// sum is formed from 16-bit halves so that no 32-bit partial sum can overflow.
+__device__ __forceinline__ void wave_add_work(unsigned long long * work, uint32_t v)
+{
+    uint32_t lo = v & 0xffffu, hi = v >> 16;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        lo += __shfl_xor(lo, o, 64);
+        hi += __shfl_xor(hi, o, 64);
+    }
+    if ((threadIdx.x & 63u) == 0 && (lo | hi))
+        atomicAdd(work, (unsigned long long)lo + ((unsigned long long)hi << 16));
+}
+
+// Workgroups are handed to the XCDs round-robin (workgroup b runs on XCD b % nxcd). Renumbering
+// them so that each XCD gets one contiguous run of workgroups keeps the particles in flight on
+// an XCD (Morton-ordered) in one compact region, whose field tiles then stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint32_t nxcd)
+{
+    if (nxcd <= 1) return b;
+    const uint32_t x = b % nxcd, local = b / nxcd;
+    const uint32_t per = nblocks / nxcd, rem = nblocks % nxcd;
+    // XCD x owns per + (x < rem) consecutive logical blocks
+    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
 template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) k_advance(const StepParams P)
+__global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nxcd)
 {
-    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= P.n) return;
-    advance_one<DIM, INTEG, T, MODE>(P, pid);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
+    const uint32_t pid = b * blockDim.x + threadIdx.x;
+    uint32_t w = 0;
+    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE>(P, pid);
+    if (P.work) wave_add_work(P.work, w); // block sizes are multiples of 64
 }
 
 // Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
 // until the list is exhausted; each item runs its job's whole span of steps. Jobs of different
 // dimension, integrator and storage type share the launch; the variant is uniform per item.
 template <int DIM, int INTEG, class T>
-__device__ __forceinline__ void run_item(const StepParams & P, uint32_t pid)
+__device__ __forceinline__ uint32_t run_item(const StepParams & P, uint32_t pid)
 {
     // the host builds a paired copy only for jobs that use it (dynamic wind, paired option)
-    if (P.wind.pair)
-        advance_one<DIM, INTEG, T, M_PAIRED>(P, pid);
-    else
-        advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
+    if (P.wind.pair) return advance_one<DIM, INTEG, T, M_PAIRED>(P, pid);
+    return advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
 }
 
-__global__ void __launch_bounds__(256) k_batch(const StepParams * __restrict__ jobs,
-                                               const BatchItem * __restrict__ items,
-                                               uint32_t nitems, uint32_t * counter)
+// Items are one wavefront of particles and every wavefront takes its own: a wavefront whose
+// particles are all dead moves on at once instead of waiting at a workgroup barrier. The item
+// list is split into one queue per XCD (queue q = items [qbegin[q], qbegin[q + 1])), each a
+// compact region of the Morton-ordered particle sets; wavefronts drain their own XCD's queue
+// first, then the following ones. WAVES is the minimum number of wavefronts per SIMD the
+// register allocation must allow (1: no limit; 4: at most 128 VGPRs, spilling if needed).
+template <int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ items,
+        const uint32_t * __restrict__ qbegin, uint32_t nq, uint32_t * counters)
 {
-    __shared__ uint32_t s_item;
-    for (;;)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t home = blockIdx.x % nq;
+    for (uint32_t k = 0; k < nq; ++k)
     {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t it = s_item;
-        __syncthreads();
-        if (it >= nitems) return; // every workgroup exits once the list is drained
-        const BatchItem b = items[it];
-        const uint32_t pid = b.first + threadIdx.x;
-        if (threadIdx.x < b.count)
+        const uint32_t q = home + k < nq ? home + k : home + k - nq;
+        const uint32_t q0 = qbegin[q], q1 = qbegin[q + 1];
+        for (;;)
         {
+            uint32_t v = 0;
+            if (lane == 0) v = atomicAdd(&counters[q], 1u);
+            // every lane is active here, so the first active lane is lane 0; the result is
+            // wavefront-uniform (a scalar register), which keeps the item and job loads scalar
+            const uint32_t it = q0 + __builtin_amdgcn_readfirstlane(v);
+            if (it >= q1) break; // this queue is drained (the counter only grows)
+            const BatchItem b = items[it];
+            const uint32_t pid = b.first + lane;
             const StepParams & P = jobs[b.job];
-            switch (b.variant)
+            uint32_t w = 0;
+            if (lane < b.count)
             {
-                case 0: run_item<2, GR_EULER, float>(P, pid); break;
-                case 1: run_item<2, GR_EULER, __half>(P, pid); break;
-                case 2: run_item<2, GR_EULER, bf16_t>(P, pid); break;
-                case 3: run_item<2, GR_RK4, float>(P, pid); break;
-                case 4: run_item<2, GR_RK4, __half>(P, pid); break;
-                case 5: run_item<2, GR_RK4, bf16_t>(P, pid); break;
-                case 6: run_item<3, GR_EULER, float>(P, pid); break;
-                case 7: run_item<3, GR_EULER, __half>(P, pid); break;
-                case 8: run_item<3, GR_EULER, bf16_t>(P, pid); break;
-                case 9: run_item<3, GR_RK4, float>(P, pid); break;
-                case 10: run_item<3, GR_RK4, __half>(P, pid); break;
-                default: run_item<3, GR_RK4, bf16_t>(P, pid); break;
+                switch (b.variant)
+                {
+                    case 0: w = run_item<2, GR_EULER, float>(P, pid); break;
+                    case 1: w = run_item<2, GR_EULER, __half>(P, pid); break;
+                    case 2: w = run_item<2, GR_EULER, bf16_t>(P, pid); break;
+                    case 3: w = run_item<2, GR_RK4, float>(P, pid); break;
+                    case 4: w = run_item<2, GR_RK4, __half>(P, pid); break;
+                    case 5: w = run_item<2, GR_RK4, bf16_t>(P, pid); break;
+                    case 6: w = run_item<3, GR_EULER, float>(P, pid); break;
+                    case 7: w = run_item<3, GR_EULER, __half>(P, pid); break;
+                    case 8: w = run_item<3, GR_EULER, bf16_t>(P, pid); break;
+                    case 9: w = run_item<3, GR_RK4, float>(P, pid); break;
+                    case 10: w = run_item<3, GR_RK4, __half>(P, pid); break;
+                    default: w = run_item<3, GR_RK4, bf16_t>(P, pid); break;
+                }
             }
+            if (P.work) wave_add_work(P.work, w);
         }
     }
 }
@@ -948,21 +1016,26 @@ int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
     return int((dim == 3 ? 6 : 0) + (integrator == GR_RK4 ? 3 : 0) + (dtype > 2 ? 0 : dtype));
 }
 
-int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t nitems,
-                 uint32_t * counter, uint32_t grid, uint32_t block, void * stream)
+int launch_batch(const StepParams * jobs, const BatchItem * items, const uint32_t * qbegin,
+                 uint32_t nq, uint32_t * counters, uint32_t grid, uint32_t block, uint32_t waves,
+                 void * stream)
 {
-    if (nitems == 0) return 0;
-    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                       nitems, counter);
+    if (nq == 0 || grid == 0) return 0;
+    if (waves >= 4)
+        hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+                           qbegin, nq, counters);
+    else
+        hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+                           qbegin, nq, counters);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int batch_occupancy(uint32_t block, int * blocks_per_cu)
+int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch, int(block), 0) ==
-                   hipSuccess
-               ? 0
-               : -1;
+    const hipError_t e =
+        waves >= 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<4>, int(block), 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<1>, int(block), 0);
+    return e == hipSuccess ? 0 : -1;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -1031,10 +1104,37 @@ __global__ void k_count_alive(const int32_t * __restrict__ status, uint64_t n,
     if ((threadIdx.x & (warpSize - 1)) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
 }
 
+// Completes saved columns: a particle that stopped before column c's step keeps its final
+// state, which the step kernels do not write into the ring. ring: `count` columns starting at
+// global column `first` of a ring of `cap` columns of n x w floats; src[d]: final values (w
+// arrays); column c was saved at step c * save_interval.
+__global__ void k_fill_frozen(float * __restrict__ ring, uint32_t cap, uint64_t first, uint32_t count,
+                              uint64_t n, uint32_t w, const int32_t * __restrict__ status,
+                              const int32_t * __restrict__ abort_step, const float * src0,
+                              const float * src1, const float * src2, uint64_t save_interval,
+                              uint32_t local, double o0, double o1, double o2)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || status[i] == GR_STATUS_TIME) return;
+    const int64_t stop = abort_step[i];
+    const float * src[3] = {src0, src1, src2};
+    const double off[3] = {o0, o1, o2};
+    float v[3];
+    for (uint32_t d = 0; d < w; ++d)
+        v[d] = local ? float(off[d] + double(src[d][i])) : src[d][i];
+    for (uint32_t c = 0; c < count; ++c)
+    {
+        const uint64_t g = first + c;
+        if (int64_t(g * save_interval) <= stop) continue; // written by the step kernel
+        float * col = ring + (g % cap) * n * w;
+        for (uint32_t d = 0; d < w; ++d) col[i * w + d] = v[d];
+    }
+}
+
 // --------------------------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------------------------
-typedef void (*AdvanceFn)(const StepParams);
+typedef void (*AdvanceFn)(const StepParams, uint32_t);
 
 template <int DIM, int INTEG, class T>
 static AdvanceFn pick_mode(uint32_t mode)
@@ -1053,7 +1153,7 @@ static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
 }
 
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, void * stream)
+                   uint32_t block, uint32_t nxcd, void * stream)
 {
     AdvanceFn fn;
     if (dim == 3)
@@ -1065,7 +1165,21 @@ int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint
     if (block == 0 || block > 256) block = 256;
     const uint32_t grid = uint32_t((uint64_t(P.n) + block - 1) / block);
     if (grid == 0) return 0;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, hipStream_t(stream), P);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, hipStream_t(stream), P, nxcd);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fill_frozen(float * ring, uint32_t cap, uint64_t first, uint32_t count, uint64_t n,
+                       uint32_t w, const int32_t * status, const int32_t * abort_step,
+                       const float * const * src, uint64_t save_interval, uint32_t local,
+                       const double * off64, void * stream)
+{
+    if (n == 0 || count == 0) return 0;
+    const uint32_t block = 256;
+    const uint32_t grid = uint32_t((n + block - 1) / block);
+    hipLaunchKernelGGL(k_fill_frozen, dim3(grid), dim3(block), 0, hipStream_t(stream), ring, cap,
+                       first, count, n, w, status, abort_step, src[0], w > 1 ? src[1] : src[0],
+                       w > 2 ? src[2] : src[0], save_interval, local, off64[0], off64[1], off64[2]);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -135,3 +135,84 @@ def test_batch_equals_single_job_advances(gpu_available):
     b.close()
     f.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("block,waves", [(64, 0), (128, 4), (256, 4), (192, 0)])
+def test_batch_launch_options_change_nothing(gpu_available, block, waves):
+    """Workgroup size and register budget of the batched kernel leave every result unchanged,
+    and the device work counter equals the integrated particle-steps implied by the final
+    status and abort step of every particle."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 5)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
+    for s in range(5):
+        f.upload(s, [src[s, c] for c in range(3)])
+    g2, gs2 = synthetic.double_gyre(96, 48, 6)
+    f2 = capi.Field(ctx, 96, 48, 1, 2, capi.F32, nslots=6, address=1)
+    for s in range(6):
+        f2.upload(s, [g2[s, c] for c in range(2)])
+    # seeds over the whole domain: most leave it (3D border) at different steps
+    s3 = synthetic.uniform_seeds(17, 5000, [0.0] * 3, [2 * math.pi] * 3)
+    s2 = synthetic.uniform_seeds(18, 3001, [0.02, 0.02], [1.98, 0.98])
+    d3 = make_desc(3, 1, gs, src.shape, 0.005, 3, ("sqrt(u*u+v*v+w*w)",))
+    d2 = make_desc(2, 0, gs2, g2.shape, 0.01)
+
+    def run(opts):
+        if opts:
+            ctx.set_option(capi.CTX_BATCH_BLOCK, opts[0])
+            ctx.set_option(capi.CTX_BATCH_WAVES, opts[1])
+        a = capi.Job(ctx, d3, s3)
+        a.reorder()
+        b = capi.Job(ctx, d2, s2)
+        ctx.take_work()
+        steps, _ = ctx.batch_run([(a, f, dict(max_steps=640)), (b, f2, dict(max_steps=333))])
+        work = ctx.take_work()
+        out = [(j.read(capi.OUT_POSITIONS), j.read(capi.OUT_STATUS), j.read(capi.OUT_ABORT_STEP))
+               for j in (a, b)]
+        if len(out[0]) == 3:
+            out[0] = out[0] + (a.read(capi.OUT_EXPR0), a.read(capi.OUT_TOTAL_CURVATURE))
+        a.close()
+        b.close()
+        return steps, work, out
+
+    steps0, work0, ref = run(None)
+    assert ctx.get_option(capi.CTX_BATCH_BLOCK) == 256 and ctx.get_option(capi.CTX_BATCH_WAVES) == 0
+    steps1, work1, got = run((block, waves))
+    assert steps0 == steps1 == [640, 333]
+    for r, g in zip(ref, got):
+        for x, y in zip(r, g):
+            np.testing.assert_array_equal(x, y)
+    expect = 0
+    for (pos, status, ab), k in zip([r[:3] for r in ref], steps0):
+        expect += int(np.where(status == 0, k, ab.astype(np.int64) + 1).sum())
+    assert work0 == work1 == expect
+    assert (ref[0][1] != 0).sum() > 100  # the 3D job really lost particles on the way
+    with pytest.raises(capi.GraniteError):
+        ctx.set_option(capi.CTX_BATCH_BLOCK, 96)
+    with pytest.raises(capi.GraniteError):
+        ctx.set_option(capi.CTX_BATCH_WAVES, 3)
+    f.close()
+    f2.close()
+    ctx.close()
+
+
+def test_single_job_work_counter(gpu_available):
+    """gr_job_advance adds the same integrated particle-steps to the context's work counter."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(24, 3)
+    f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=3)
+    for s in range(3):
+        f.upload(s, [src[s, c] for c in range(3)])
+    seeds = synthetic.uniform_seeds(19, 777, [0.0] * 3, [2 * math.pi] * 3)
+    j = capi.Job(ctx, make_desc(3, 1, gs, src.shape, 0.005), seeds)
+    w = capi.Window()
+    w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+    w.time_back, w.time_front, w.time_start, w.counter_start, w.nsteps = 0.0, 1.0, 0.0, 0, 150
+    ctx.take_work()
+    j.advance(w)
+    status, ab = j.read(capi.OUT_STATUS), j.read(capi.OUT_ABORT_STEP)
+    assert ctx.take_work() == int(np.where(status == 0, 150, ab.astype(np.int64) + 1).sum())
+    assert ctx.take_work() == 0
+    j.close()
+    f.close()
+    ctx.close()

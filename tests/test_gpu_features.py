@@ -183,3 +183,55 @@ def test_chunked_advances_equal_one_advance(gpu_available):
         np.testing.assert_array_equal(a, b)
     f.close()
     ctx.close()
+
+
+def test_saved_columns_of_stopped_particles(gpu_available):
+    """The step kernels save only particles alive at the saving step; reading completes the
+    columns with each stopped particle's final state. Checked against the FP32 oracle on a
+    reordered job whose particles leave the domain at many different steps, with partial and
+    repeated reads, and on the local-coordinate form (FP64 offset added on output)."""
+    wind, gs = abc(24, 3)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=3)
+    for sl in range(3):
+        f.upload(sl, [wind[sl, c] for c in range(3)])
+    s = synthetic.uniform_seeds(31, 5000, [0.0] * 3, [2 * math.pi] * 3)
+    steps, si = 96, 3  # inside the first window (t < 1)
+    st = base3(gs, DeltaT=0.01, MaximumSimulationTime=steps * 0.01, SaveInterval=si,
+               CurvatureMode="IndividualAndTotalCurvature", AdditionalCompute=["sqrt(u*u+v*v+w*w)"])
+    ref = oracle.run(st, wind, s, real64=False)
+    assert (ref["status"] != 0).sum() > 1000  # many stop on the way, at different steps
+    ncols = steps // si
+    for local in (0, 1):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode = 3, 1, 3, 1
+        d.dt, d.dtd = 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k] = gs[k]
+            d.domain[k] = float(np.float32(24) * np.float32(gs[k]))
+        d.save_interval = si
+        d.local_coords = local
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v+w*w)", curvature=3)
+        job = capi.Job(ctx, d, s, save_capacity=ncols)
+        job.reorder()
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+        w.time_back, w.time_front = 0.0, 1.0
+        w.time_start, w.counter_start, w.nsteps = 0.0, 0, steps
+        job.advance(w)
+        assert job.saved_columns() == ncols
+        # partial reads, then the whole range again (the completion is idempotent)
+        a = job.read_saved(capi.OUT_POSITIONS, 0, 20)
+        b = job.read_saved(capi.OUT_POSITIONS, 20, ncols - 20)
+        pos = job.read_saved(capi.OUT_POSITIONS, 0, ncols)
+        np.testing.assert_array_equal(np.concatenate([a, b]), pos)
+        want = ref["trajectories"][:, 1:ncols + 1, :].astype(np.float32)  # column k = step 3k
+        got = np.transpose(pos.reshape(ncols, 5000, 3), (1, 0, 2))
+        np.testing.assert_array_equal(got, want)
+        ex = job.read_saved(capi.OUT_EXPR0, 0, ncols).reshape(ncols, 5000).T
+        np.testing.assert_array_equal(ex, ref["compute"][0][:, :ncols].astype(np.float32))
+        np.testing.assert_array_equal(job.read(capi.OUT_STATUS), ref["status"])
+        job.close()
+    f.close()
+    ctx.close()
