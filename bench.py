@@ -157,11 +157,8 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
-        out = []
-        for idx in parts:
-            j = capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk)
-            j.reorder()
-            out.append(j)
+        out = [capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk) for idx in parts]
+        ctx.reorder(out)
         return out
 
     jobs = new_jobs()
@@ -174,8 +171,7 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
         ends, _ = ctx.batch_run(spans, timed=False, ends=True)
         e = ends[0]
         if e["window"] != state["window"]:
-            for j in jobs:
-                j.reorder()
+            ctx.reorder(jobs)  # every job regrouped by cell at the window edge, in one pass
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
 
     for _ in range(warmup):
@@ -505,6 +501,8 @@ def main():
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the batched-kernel autotuner (default: tune before the batched form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by "
@@ -535,6 +533,14 @@ def main():
     f16 = make_field(capi, ctx, wind, capi.F16)
 
     forms = args.forms.split(",")
+    tune = None
+    if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0):
+        # launch configuration of the batched kernel from the autotuner (a 65,536-particle probe
+        # of the same job description over the same FP16 field, 50 steps, before any timing)
+        tune = ctx.autotune(f16, build_desc(capi, gs, n), n=65536, steps=args.chunk, trials=2)
+        print("# autotune: " + ", ".join(f"b{t['block']}/w{t['waves']} {t['ms']:.3f} ms" for t in tune)
+              + f" -> block {ctx.get_option(capi.CTX_BATCH_BLOCK)}, waves "
+              f"{ctx.get_option(capi.CTX_BATCH_WAVES)}", file=sys.stderr, flush=True)
     res = {}
     for name in forms:
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {args.chunk} steps, "
@@ -607,6 +613,7 @@ def main():
                    "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES)},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
         "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},
+        "autotune": tune,
         "plain_fp32_value": plain,
         "speedup_vs_plain_fp32": hv["psps"] / plain if plain else None,
         "fp64_error_domain_relative": err,

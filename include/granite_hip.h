@@ -266,6 +266,9 @@ int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
  * them in the original order. Saved columns written before the call must be read before it.
  * No counterpart in the reference (its particle order is fixed). */
 int gr_job_reorder(gr_job * job);
+/* gr_job_reorder for several jobs of one context in one pass (one key kernel, one radix sort,
+ * one gather launch for all their state arrays). */
+int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs);
 
 /* ---- batches: many jobs in one persistent launch -------------------------------------------
  * Each span advances one job through consecutive windows of a wind field whose slices are all
@@ -317,6 +320,22 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
 #define GR_CTX_BATCH_WAVES 2u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
+
+/* Autotuner of the batched kernel (no counterpart in the reference). Times every candidate
+ * launch configuration (workgroup size 64 / 128 / 256 x register budget 0 / 4) on a probe job
+ * of `desc` with n seeds spread over the domain, advanced `steps` steps through `wind` from
+ * window 0 (best of `trials` runs each, all from identical state), and sets the fastest as the
+ * context's GR_CTX_BATCH_BLOCK / GR_CTX_BATCH_WAVES. results[0 .. *nresults) receive every
+ * candidate's device time (may be NULL). The context's work counter and kernel-time sums are
+ * unchanged. */
+typedef struct {
+    uint32_t block;
+    uint32_t waves;
+    float ms;
+} gr_tune_entry;
+int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
+                      uint32_t steps, uint32_t trials, gr_tune_entry * results,
+                      uint32_t max_results, uint32_t * nresults);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ EXPORTS = [
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
-    "gr_job_set_option", "gr_job_reorder", "gr_batch_run",
+    "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
 OPT_BLOCK, OPT_PAIRED = 1, 2
 CTX_BATCH_BLOCK, CTX_BATCH_WAVES = 1, 2
@@ -100,6 +100,10 @@ class SpanEnd(ctypes.Structure):
                 ("time64", ctypes.c_double), ("counter", ctypes.c_uint64)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
 _lib = None
 
 
@@ -146,7 +150,9 @@ def lib():
         "gr_job_set_launch": ([vp, u32], i32),
         "gr_job_set_option": ([vp, u32, ctypes.c_int64], i32),
         "gr_job_reorder": ([vp], i32),
+        "gr_jobs_reorder": ([P(vp), u32], i32),
         "gr_batch_run": ([vp, P(Span), u32, u32, P(SpanEnd), P(ctypes.c_float)], i32),
+        "gr_batch_autotune": ([vp, vp, P(JobDesc), u32, u32, u32, P(TuneEntry), u32, P(u32)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -251,6 +257,20 @@ class Context:
         v = ctypes.c_int64(0)
         _check(lib().gr_ctx_get_option(self.h, option, ctypes.byref(v)))
         return int(v.value)
+
+    def autotune(self, field, desc, n: int = 65536, steps: int = 50, trials: int = 2):
+        """Runs the batched-kernel autotuner on a probe job of `desc` over `field`; sets the
+        fastest configuration on this context and returns every candidate's device time."""
+        res = (TuneEntry * 16)()
+        cnt = ctypes.c_uint32(0)
+        _check(lib().gr_batch_autotune(self.h, field.h, ctypes.byref(desc), n, steps, trials,
+                                       res, 16, ctypes.byref(cnt)))
+        return [{"block": r.block, "waves": r.waves, "ms": r.ms} for r in res[:cnt.value]]
+
+    def reorder(self, jobs):
+        """Reorders several jobs of this context in one pass (gr_jobs_reorder)."""
+        arr = (ctypes.c_void_p * max(1, len(jobs)))(*[j.h for j in jobs])
+        _check(lib().gr_jobs_reorder(arr, len(jobs)))
 
     def take_kernel_ms(self):
         """(summed device ms, launches) of the step kernels since the previous call (then reset)."""

@@ -75,14 +75,34 @@ struct gr_ctx
     hipStream_t copy{nullptr};
     unsigned long long * dcount{nullptr}; // [0] alive count scratch, [1] integrated particle-steps
     // persistent batched launches
-    std::vector<gr::StepParams> h_params;
-    std::vector<gr::BatchItem> h_items;
-    gr::StepParams * d_params{nullptr};
-    size_t params_cap{0};
+    std::vector<gr::BatchItem> h_items;  // the last item list (reused while the spans match)
+    std::vector<uint32_t> qstart;         // its queue starts
+    std::vector<uint64_t> items_key;
+    bool items_valid{false};
     gr::BatchItem * d_items{nullptr};
     size_t items_cap{0};
-    uint32_t * d_queues{nullptr}; // MAX_QUEUES counters, then MAX_QUEUES + 1 queue starts
-    std::vector<uint32_t> h_queues;
+    struct Stage
+    {
+        void * h{nullptr}; // pinned
+        size_t cap{0};
+        hipEvent_t done{nullptr};
+        bool pending{false};
+    };
+    Stage stage[2];
+    int stage_i{0};
+    char * d_stage{nullptr}; // device copy of the staging block (queues, params, window tables)
+    size_t d_stage_cap{0};
+    // reorders: keys / values of all particles of the jobs being reordered, sort temp storage,
+    // job and gather tables
+    uint64_t * skeys[2]{};
+    uint32_t * svals[2]{};
+    size_t scap{0};
+    void * stmp{nullptr};
+    size_t stmp_bytes{0};
+    gr::SortJob * d_sjobs{nullptr};
+    size_t sjobs_cap{0};
+    gr::GatherEntry * d_gtab{nullptr};
+    size_t gtab_cap{0};
     hipEvent_t bev0{nullptr}, bev1{nullptr};
     int cus{0};
     uint32_t nxcd{1}; // XCDs of the device
@@ -111,6 +131,7 @@ struct gr_field
     bool stage_pending{false};
     std::vector<hipEvent_t> ready, inuse;
     std::vector<char> has_ready, has_inuse;
+    std::vector<char> waited; // the compute stream already waits for the slot's current ready event
     // time-paired copies of (back, front) slot pairs, built on the compute stream on demand
     struct Pair
     {
@@ -161,19 +182,11 @@ struct gr_job
     uint32_t paired{1};          // use the time-paired layout when the wind is dynamic
     uint32_t grid[3]{0, 0, 0};   // cells per axis of the last wind field (Morton keys)
     uint32_t * orig{nullptr};    // caller index of each state slot
-    uint32_t * keys[2]{};        // sort buffers
-    uint32_t * vals[2]{};
-    void * sort_tmp{nullptr};
-    size_t sort_tmp_bytes{0};
+    std::vector<void *> shadow;  // second copy of each state array (reorders gather into it)
     bool reordered{false};
     uint64_t reorder_col0{0};    // first saved column written in the current order
     float * scratch2{nullptr};   // n * dim floats
     // window tables of multi-window launches (kept alive while copies are queued)
-    std::vector<const void *> h_win_ptr;
-    std::vector<uint32_t> h_win_edge;
-    const void ** d_win_ptr{nullptr};
-    uint32_t * d_win_edge{nullptr};
-    size_t win_cap{0};
 };
 
 // --------------------------------------------------------------------------------------------
@@ -244,9 +257,21 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     (void)hipStreamSynchronize(c->copy);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     dfree(c->dcount);
-    dfree(c->d_params);
+    dfree(c->d_stage);
     dfree(c->d_items);
-    dfree(c->d_queues);
+    for (auto & st : c->stage)
+    {
+        if (st.h) (void)hipHostFree(st.h);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
+    for (int k = 0; k < 2; ++k)
+    {
+        dfree(c->skeys[k]);
+        dfree(c->svals[k]);
+    }
+    dfree(c->stmp);
+    dfree(c->d_sjobs);
+    dfree(c->d_gtab);
     if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->bev1) (void)hipEventDestroy(c->bev1);
     (void)hipStreamDestroy(c->compute);
@@ -412,6 +437,7 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
     f->inuse.assign(d.nslots, nullptr);
     f->has_ready.assign(d.nslots, 0);
     f->has_inuse.assign(d.nslots, 0);
+    f->waited.assign(d.nslots, 0);
     int rc = GR_OK;
     for (uint32_t s = 0; s < d.nslots && rc == GR_OK; ++s)
     {
@@ -464,6 +490,7 @@ static int field_pack(gr_field * f, uint32_t slot)
         return set_err(GR_E_HIP, "pack kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     GR_HIP(hipEventRecord(f->ready[slot], c->copy));
     f->has_ready[slot] = 1;
+    f->waited[slot] = 0;
     return GR_OK;
 }
 
@@ -637,14 +664,7 @@ static void job_free(gr_job * j)
     dfree(j->scratch_aos);
     dfree(j->scratch2);
     dfree(j->orig);
-    for (int k = 0; k < 2; ++k)
-    {
-        dfree(j->keys[k]);
-        dfree(j->vals[k]);
-    }
-    dfree(j->sort_tmp);
-    dfree(j->d_win_ptr);
-    dfree(j->d_win_edge);
+    for (void * a : j->shadow) dfree(a);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
     delete j;
@@ -803,85 +823,173 @@ extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
     }
 }
 
-// the state arrays of a job (4-byte elements, n each) in a fixed order
-static std::vector<void *> state_arrays(gr_job * j)
+// the state arrays of a job (4-byte elements, n each) in a fixed order, as slots so that a
+// reorder can swap each with its shadow copy; the index map comes last
+static std::vector<void **> state_slots(gr_job * j)
 {
-    std::vector<void *> a;
+    std::vector<void **> a;
+    auto add = [&](auto *& p) {
+        if (p) a.push_back(reinterpret_cast<void **>(&p));
+    };
     for (int k = 0; k < 3; ++k)
-        for (void * p : {static_cast<void *>(j->p[k]), static_cast<void *>(j->q[k]),
-                         static_cast<void *>(j->c[k]), static_cast<void *>(j->ld[k])})
-            if (p) a.push_back(p);
-    for (void * p : {static_cast<void *>(j->status), static_cast<void *>(j->abort_step),
-                     static_cast<void *>(j->total), static_cast<void *>(j->curv),
-                     static_cast<void *>(j->len)})
-        if (p) a.push_back(p);
-    for (float * e : j->expr)
-        if (e) a.push_back(e);
-    for (float * v : j->vol)
-        if (v) a.push_back(v);
+    {
+        add(j->p[k]);
+        add(j->q[k]);
+        add(j->c[k]);
+        add(j->ld[k]);
+    }
+    add(j->status);
+    add(j->abort_step);
+    add(j->total);
+    add(j->curv);
+    add(j->len);
+    for (float *& e : j->expr) add(e);
+    for (float *& v : j->vol) add(v);
+    add(j->orig);
     return a;
+}
+
+template <class T> static int grow(T *& buf, size_t & cap, size_t need)
+{
+    if (need <= cap) return GR_OK;
+    dfree(buf);
+    buf = nullptr;
+    cap = 0;
+    int rc = dalloc(&buf, need * sizeof(T));
+    if (rc == GR_OK) cap = need;
+    return rc;
+}
+
+// Reorders several jobs of one context by the Morton code of each particle's current cell
+// (stopped particles last): one key kernel, one radix sort over all particles (the job index in
+// the high key bits keeps every job in its own range) and one gather launch for every state
+// array of every job, each gathered into its shadow copy, which then takes the array's place.
+extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
+{
+    if (njobs && !jobs) return set_err(GR_E_INVALID, "gr_jobs_reorder: null argument");
+    std::vector<gr_job *> js;
+    gr_ctx * ctx = nullptr;
+    for (uint32_t i = 0; i < njobs; ++i)
+    {
+        gr_job * j = jobs[i];
+        if (!j) return set_err(GR_E_INVALID, "gr_jobs_reorder: null job %u", i);
+        if (ctx && j->ctx != ctx) return set_err(GR_E_INVALID, "gr_jobs_reorder: jobs of different contexts");
+        ctx = j->ctx;
+        if (j->n >= 2) js.push_back(j);
+    }
+    if (js.empty()) return GR_OK;
+    if (js.size() > 0xffffu) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many jobs");
+    GR_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->compute;
+    int rc;
+    // index maps and shadow arrays on first use
+    for (gr_job * j : js)
+    {
+        if (!j->orig)
+        {
+            if ((rc = dalloc(&j->orig, size_t(j->n) * 4)) != GR_OK) return rc;
+            std::vector<uint32_t> iota(j->n);
+            for (uint64_t i = 0; i < j->n; ++i) iota[i] = uint32_t(i);
+            GR_HIP(hipMemcpyAsync(j->orig, iota.data(), size_t(j->n) * 4, hipMemcpyHostToDevice, s));
+            GR_HIP(hipStreamSynchronize(s));
+        }
+        const size_t na = state_slots(j).size();
+        while (j->shadow.size() < na)
+        {
+            void * b = nullptr;
+            if ((rc = dalloc(&b, size_t(j->n) * 4)) != GR_OK) return rc;
+            j->shadow.push_back(b);
+        }
+    }
+    // job table
+    std::vector<gr::SortJob> tab(js.size());
+    uint64_t total = 0;
+    for (size_t k = 0; k < js.size(); ++k)
+    {
+        gr_job * j = js[k];
+        gr::SortJob & t = tab[k];
+        std::memset(&t, 0, sizeof(t));
+        for (int d = 0; d < 3; ++d)
+        {
+            t.p[d] = j->p[d];
+            t.off[d] = j->d.local_coords ? 0.f : float(j->d.offset[d]);
+            t.inv_h[d] = j->d.grid_scale[d] > 0.f ? 1.f / j->d.grid_scale[d] : 1.f;
+            t.grid[d] = std::max(1u, j->grid[d]);
+        }
+        t.status = j->status;
+        t.base = total;
+        t.n = uint32_t(j->n);
+        t.dim = j->d.dim;
+        total += j->n;
+    }
+    if (total > ctx->scap)
+    {
+        for (int k = 0; k < 2; ++k)
+        {
+            dfree(ctx->skeys[k]);
+            dfree(ctx->svals[k]);
+            ctx->skeys[k] = nullptr;
+            ctx->svals[k] = nullptr;
+        }
+        ctx->scap = 0;
+        for (int k = 0; k < 2; ++k)
+            if ((rc = dalloc(&ctx->skeys[k], total * sizeof(uint64_t))) != GR_OK ||
+                (rc = dalloc(&ctx->svals[k], total * sizeof(uint32_t))) != GR_OK)
+                return rc;
+        ctx->scap = total;
+    }
+    uint32_t jb = 0;
+    while ((size_t(1) << jb) < js.size()) ++jb;
+    const uint32_t end_bit = 32 + jb;
+    size_t need = 0;
+    if (gr::sort_pairs64(nullptr, need, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total, end_bit, s) != 0)
+        return set_err(GR_E_HIP, "radix sort sizing failed");
+    if (need > ctx->stmp_bytes)
+    {
+        dfree(ctx->stmp);
+        ctx->stmp = nullptr;
+        ctx->stmp_bytes = 0;
+        if ((rc = dalloc(&ctx->stmp, need)) != GR_OK) return rc;
+        ctx->stmp_bytes = need;
+    }
+    if ((rc = grow(ctx->d_sjobs, ctx->sjobs_cap, tab.size())) != GR_OK) return rc;
+    GR_HIP(hipMemcpyAsync(ctx->d_sjobs, tab.data(), tab.size() * sizeof(gr::SortJob), hipMemcpyHostToDevice, s));
+    if (gr::launch_morton_multi(ctx->d_sjobs, uint32_t(tab.size()), total, ctx->skeys[0], ctx->svals[0], s) != 0)
+        return set_err(GR_E_HIP, "morton key kernel failed");
+    size_t tb = ctx->stmp_bytes;
+    if (gr::sort_pairs64(ctx->stmp, tb, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total, end_bit, s) != 0)
+        return set_err(GR_E_HIP, "radix sort failed");
+    // gather every state array (and the index map) into its shadow, then swap
+    std::vector<gr::GatherEntry> g;
+    uint32_t maxn = 0;
+    for (size_t k = 0; k < js.size(); ++k)
+    {
+        gr_job * j = js[k];
+        const std::vector<void **> slots = state_slots(j);
+        for (size_t a = 0; a < slots.size(); ++a)
+            g.push_back(gr::GatherEntry{static_cast<const uint32_t *>(*slots[a]), static_cast<uint32_t *>(j->shadow[a]),
+                                        ctx->svals[1] + tab[k].base, uint32_t(j->n), 0u});
+        maxn = std::max(maxn, uint32_t(j->n));
+    }
+    if (g.size() > 65535) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many arrays in one call");
+    if ((rc = grow(ctx->d_gtab, ctx->gtab_cap, g.size())) != GR_OK) return rc;
+    GR_HIP(hipMemcpyAsync(ctx->d_gtab, g.data(), g.size() * sizeof(gr::GatherEntry), hipMemcpyHostToDevice, s));
+    if (gr::launch_gather_multi(ctx->d_gtab, uint32_t(g.size()), maxn, s) != 0)
+        return set_err(GR_E_HIP, "gather kernel failed");
+    for (gr_job * j : js)
+    {
+        const std::vector<void **> slots = state_slots(j);
+        for (size_t a = 0; a < slots.size(); ++a) std::swap(*slots[a], j->shadow[a]);
+        j->reordered = true;
+        j->reorder_col0 = j->saved;
+    }
+    return GR_OK;
 }
 
 extern "C" int gr_job_reorder(gr_job * j)
 {
     if (!j) return set_err(GR_E_INVALID, "null job");
-    if (j->n < 2) return GR_OK;
-    GR_HIP(hipSetDevice(j->ctx->device));
-    hipStream_t s = j->ctx->compute;
-    const uint32_t n = uint32_t(j->n);
-    int rc = GR_OK;
-    for (int k = 0; k < 2 && rc == GR_OK; ++k)
-    {
-        if (!j->keys[k]) rc = dalloc(&j->keys[k], size_t(n) * 4);
-        if (rc == GR_OK && !j->vals[k]) rc = dalloc(&j->vals[k], size_t(n) * 4);
-    }
-    if (rc != GR_OK) return rc;
-    if (!j->orig)
-    {
-        if ((rc = dalloc(&j->orig, size_t(n) * 4)) != GR_OK) return rc;
-        std::vector<uint32_t> iota(n);
-        for (uint32_t i = 0; i < n; ++i) iota[i] = i;
-        GR_HIP(hipMemcpyAsync(j->orig, iota.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
-        GR_HIP(hipStreamSynchronize(s));
-    }
-    if (!j->sort_tmp)
-    {
-        size_t bytes = 0;
-        if (gr::sort_pairs(nullptr, bytes, j->keys[0], j->keys[1], j->vals[0], j->vals[1], n, s) != 0)
-            return set_err(GR_E_HIP, "radix sort sizing failed");
-        if ((rc = dalloc(&j->sort_tmp, bytes ? bytes : 1)) != GR_OK) return rc;
-        j->sort_tmp_bytes = bytes;
-    }
-    gr::StepParams P;
-    std::memset(&P, 0, sizeof(P));
-    P.n = n;
-    P.status = j->status;
-    for (int k = 0; k < 3; ++k)
-    {
-        P.p[k] = j->p[k];
-        P.off[k] = j->d.local_coords ? 0.f : float(j->d.offset[k]);
-        P.inv_h[k] = 1.f / j->d.grid_scale[k];
-    }
-    P.wind.nx = std::max(1u, j->grid[0]);
-    P.wind.ny = std::max(1u, j->grid[1]);
-    P.wind.nz = std::max(1u, j->grid[2]);
-    if (gr::launch_morton_keys(P, j->d.dim, j->keys[0], j->vals[0], s) != 0)
-        return set_err(GR_E_HIP, "morton key kernel failed");
-    size_t tb = j->sort_tmp_bytes;
-    if (gr::sort_pairs(j->sort_tmp, tb, j->keys[0], j->keys[1], j->vals[0], j->vals[1], n, s) != 0)
-        return set_err(GR_E_HIP, "radix sort failed");
-    const uint32_t * perm = j->vals[1];
-    // permute every state array through the scratch buffer, then the index map
-    for (void * a : state_arrays(j))
-    {
-        if (gr::launch_gather32(a, j->scratch2, perm, n, s) != 0) return set_err(GR_E_HIP, "gather failed");
-        GR_HIP(hipMemcpyAsync(a, j->scratch2, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
-    }
-    if (gr::launch_gather32(j->orig, j->scratch2, perm, n, s) != 0) return set_err(GR_E_HIP, "gather failed");
-    GR_HIP(hipMemcpyAsync(j->orig, j->scratch2, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
-    j->reordered = true;
-    j->reorder_col0 = j->saved;
-    return GR_OK;
+    return gr_jobs_reorder(&j, 1);
 }
 
 extern "C" int gr_job_set_particle_data(gr_job * j, uint32_t kind, uint32_t slot, const float * host)
@@ -939,6 +1047,18 @@ static void view(gr::FieldView & v, const gr_field * f, uint32_t back, uint32_t 
 
 // Validates the window, fills the kernel parameters, makes the compute stream wait for the
 // slices the window reads and builds the paired copy when the paired layout is used.
+// makes the compute stream `s` wait for the last upload of a slot (once per upload: later work on
+// the stream is ordered behind the first wait)
+static int f_wait(gr_field * f, uint32_t slot, hipStream_t s)
+{
+    if (f && slot < f->d.nslots && f->has_ready[slot] && !f->waited[slot])
+    {
+        GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
+        f->waited[slot] = 1;
+    }
+    return GR_OK;
+}
+
 static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, uint32_t & mode,
                            uint64_t & nsaves)
 {
@@ -1070,10 +1190,7 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
 
     hipStream_t s = j->ctx->compute;
     // wait for the slices this window reads
-    auto wait_ready = [&](gr_field * f, uint32_t slot) -> int {
-        if (f && f->has_ready[slot]) GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
-        return GR_OK;
-    };
+    auto wait_ready = [&](gr_field * f, uint32_t slot) -> int { return f_wait(f, slot, s); };
     int rc;
     if ((rc = wait_ready(wf, w->wind_back)) || (rc = wait_ready(wf, w->wind_front))) return rc;
     if (d.nvol && ((rc = wait_ready(w->volumes, w->vol_back)) || (rc = wait_ready(w->volumes, w->vol_front))))
@@ -1102,11 +1219,17 @@ static int mark_inuse(gr_field * f, uint32_t slot, hipStream_t s)
     return GR_OK;
 }
 
-// after the launch: later uploads into the slots read by the window wait for it
-static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves)
+// after the launch: later uploads into the slots read by the window wait for it (the batched
+// path marks each slot once per launch itself: mark = false)
+static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, bool mark = true)
 {
     hipStream_t s = j->ctx->compute;
     int rc;
+    if (!mark)
+    {
+        j->saved += nsaves;
+        return GR_OK;
+    }
     if ((rc = mark_inuse(w->wind, w->wind_back, s)) || (rc = mark_inuse(w->wind, w->wind_front, s))) return rc;
     if (j->d.nvol && ((rc = mark_inuse(w->volumes, w->vol_back, s)) || (rc = mark_inuse(w->volumes, w->vol_front, s))))
         return rc;
@@ -1139,12 +1262,6 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
 // batches
 // --------------------------------------------------------------------------------------------
 
-// makes stream `s` wait for the last upload of a slot
-static int f_wait(gr_field * f, uint32_t slot, hipStream_t s)
-{
-    if (f && slot < f->d.nslots && f->has_ready[slot]) GR_HIP(hipStreamWaitEvent(s, f->ready[slot], 0));
-    return GR_OK;
-}
 
 // Steps per window of a span under the reference's host schedule (tpp:852-994): FP32 (or FP64)
 // time, max_windfield_time accumulated by += DataTimeDistance, stop at counter >= max_sim / dt
@@ -1202,6 +1319,33 @@ static void plan_span(const gr_job_desc & d, const gr_span & sp, uint32_t nslice
     }
 }
 
+// pinned host staging of a batch launch (double-buffered so the host can fill the next launch's
+// tables while the previous copy is still queued)
+static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
+{
+    gr_ctx::Stage & st = ctx->stage[k];
+    if (st.pending)
+    {
+        GR_HIP(hipEventSynchronize(st.done));
+        st.pending = false;
+    }
+    if (bytes <= st.cap) return GR_OK;
+    if (st.h) (void)hipHostFree(st.h);
+    st.h = nullptr;
+    st.cap = 0;
+    const size_t want = std::max<size_t>(bytes, 64 << 10);
+    if (hipHostMalloc(&st.h, want, hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return set_err(GR_E_NOMEM, "pinned staging of %zu bytes failed", want);
+    }
+    st.cap = want;
+    if (!st.done) GR_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    return GR_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
                             gr_span_end * ends, float * ms)
 {
@@ -1215,12 +1359,17 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         GR_HIP(hipEventCreate(&ctx->bev0));
         GR_HIP(hipEventCreate(&ctx->bev1));
     }
-    ctx->h_params.resize(nspans);
+    std::vector<gr::StepParams> params(nspans);
     std::vector<uint64_t> saves(nspans, 0);
     std::vector<uint32_t> span_steps(nspans, 0);
     std::vector<uint32_t> span_nwin(nspans, 0);
     std::vector<uint32_t> win_steps;
     std::vector<gr_window> first_win(nspans);
+    // window tables of the spans that cross window edges: (back, front, pair) per window and the
+    // step at which each window ends; placed in the staging block, addresses patched below
+    std::vector<const void *> wptr;
+    std::vector<uint32_t> wedge;
+    std::vector<size_t> wofs(nspans, 0);
     int rc;
     for (uint32_t i = 0; i < nspans; ++i)
     {
@@ -1264,7 +1413,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         w.time_start64 = sp.time_start64;
         w.counter_start = sp.counter_start;
         w.nsteps = total;
-        gr::StepParams & P = ctx->h_params[i];
+        gr::StepParams & P = params[i];
         uint32_t mode = 0;
         if (total == 0)
         {
@@ -1279,41 +1428,20 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         P.dtd = d.dtd;
         if (nwin > 1)
         {
-            j->h_win_ptr.assign(size_t(3) * nwin, nullptr);
-            j->h_win_edge.assign(nwin, 0);
+            wofs[i] = wedge.size();
             uint32_t edge = 0;
             for (uint32_t k = 0; k < nwin; ++k)
             {
                 const uint32_t c = c0 + k;
                 edge += win_steps[k];
-                j->h_win_edge[k] = edge;
+                wedge.push_back(edge);
                 if (f_wait(sp.wind, c, s) != GR_OK || f_wait(sp.wind, c + 1, s) != GR_OK) return GR_E_HIP;
-                j->h_win_ptr[3 * k + 0] = sp.wind->slots[c];
-                j->h_win_ptr[3 * k + 1] = sp.wind->slots[c + 1];
-                if (mode == gr::MODE_PAIRED)
-                {
-                    const void * pr = nullptr;
-                    if ((rc = field_pair(sp.wind, c, c + 1, s, &pr)) != GR_OK) return rc;
-                    j->h_win_ptr[3 * k + 2] = pr;
-                }
+                const void * pr = nullptr;
+                if (mode == gr::MODE_PAIRED && (rc = field_pair(sp.wind, c, c + 1, s, &pr)) != GR_OK) return rc;
+                wptr.push_back(sp.wind->slots[c]);
+                wptr.push_back(sp.wind->slots[c + 1]);
+                wptr.push_back(pr);
             }
-            if (j->win_cap < nwin)
-            {
-                dfree(j->d_win_ptr);
-                dfree(j->d_win_edge);
-                j->d_win_ptr = nullptr;
-                j->d_win_edge = nullptr;
-                j->win_cap = 0;
-                if ((rc = dalloc(&j->d_win_ptr, size_t(3) * nwin * sizeof(void *))) != GR_OK) return rc;
-                if ((rc = dalloc(&j->d_win_edge, size_t(nwin) * 4)) != GR_OK) return rc;
-                j->win_cap = nwin;
-            }
-            GR_HIP(hipMemcpyAsync(j->d_win_ptr, j->h_win_ptr.data(), size_t(3) * nwin * sizeof(void *),
-                                  hipMemcpyHostToDevice, s));
-            GR_HIP(hipMemcpyAsync(j->d_win_edge, j->h_win_edge.data(), size_t(nwin) * 4,
-                                  hipMemcpyHostToDevice, s));
-            P.win_ptr = j->d_win_ptr;
-            P.win_edge = j->d_win_edge;
         }
     }
 
@@ -1321,99 +1449,131 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     // in their job's particle range (Morton-ordered after gr_job_reorder), and inside a queue
     // they are ordered by span length class (longest first), field, then position: the
     // wavefronts in flight on one XCD then sample one compact region of one field, and long
-    // spans start early.
+    // spans start early. The list depends only on the spans' jobs, sizes, variants, fields and
+    // length classes, so a launch with the same ones reuses the previous list on the device.
     const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>(ctx->nxcd, gr::MAX_QUEUES));
-    struct ItemKey
-    {
-        uint32_t q, cls;
-        uintptr_t field;
-        float pos;
-        gr::BatchItem item;
-    };
-    std::vector<ItemKey> keyed;
+    std::vector<uint64_t> key;
+    key.push_back(nq);
+    std::vector<uint32_t> cls_of(nspans, 0);
     for (uint32_t i = 0; i < nspans; ++i)
     {
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
-        const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
-        uint32_t cls = 0;
-        for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls;
-        for (uint64_t f = 0; f < j->n; f += gr::BATCH_ITEM)
+        for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls_of[i];
+        key.insert(key.end(), {i, uint64_t(uintptr_t(j)), j->n, uint64_t(uintptr_t(spans[i].wind)),
+                               uint64_t(spans[i].wind->d.dtype), cls_of[i]});
+    }
+    const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
+    if (items_dirty)
+    {
+        struct ItemKey
         {
-            const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, j->n - f));
-            const float pos = float((double(f) + 0.5 * cnt) / double(j->n));
-            const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(pos * float(nq)));
-            keyed.push_back(ItemKey{q, cls, uintptr_t(spans[i].wind), pos, gr::BatchItem{i, uint32_t(f), cnt, var}});
+            uint32_t q, cls;
+            uintptr_t field;
+            float pos;
+            gr::BatchItem item;
+        };
+        std::vector<ItemKey> keyed;
+        for (uint32_t i = 0; i < nspans; ++i)
+        {
+            if (!span_steps[i]) continue;
+            gr_job * j = spans[i].job;
+            const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
+            for (uint64_t f = 0; f < j->n; f += gr::BATCH_ITEM)
+            {
+                const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, j->n - f));
+                const float pos = float((double(f) + 0.5 * cnt) / double(j->n));
+                const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(pos * float(nq)));
+                keyed.push_back(ItemKey{q, cls_of[i], uintptr_t(spans[i].wind), pos, gr::BatchItem{i, uint32_t(f), cnt, var}});
+            }
         }
+        std::stable_sort(keyed.begin(), keyed.end(), [](const ItemKey & a, const ItemKey & b) {
+            if (a.q != b.q) return a.q < b.q;
+            if (a.cls != b.cls) return a.cls > b.cls;
+            if (a.field != b.field) return a.field < b.field;
+            return a.pos < b.pos;
+        });
+        ctx->h_items.resize(keyed.size());
+        ctx->qstart.assign(gr::MAX_QUEUES + 1, 0u);
+        for (size_t k = 0; k < keyed.size(); ++k)
+        {
+            ctx->h_items[k] = keyed[k].item;
+            ++ctx->qstart[keyed[k].q + 1];
+        }
+        for (uint32_t q = 0; q < nq; ++q) ctx->qstart[q + 1] += ctx->qstart[q];
+        ctx->items_key = key;
+        ctx->items_valid = false; // until the copy below is queued
     }
-    std::stable_sort(keyed.begin(), keyed.end(), [](const ItemKey & a, const ItemKey & b) {
-        if (a.q != b.q) return a.q < b.q;
-        if (a.cls != b.cls) return a.cls > b.cls;
-        if (a.field != b.field) return a.field < b.field;
-        return a.pos < b.pos;
-    });
-    ctx->h_items.resize(keyed.size());
-    ctx->h_queues.assign(2 * gr::MAX_QUEUES + 1, 0u); // counters (zero), then queue starts
-    uint32_t * qb = ctx->h_queues.data() + gr::MAX_QUEUES;
-    for (size_t k = 0; k < keyed.size(); ++k)
-    {
-        ctx->h_items[k] = keyed[k].item;
-        ++qb[keyed[k].q + 1];
-    }
-    for (uint32_t q = 0; q < nq; ++q) qb[q + 1] += qb[q];
     const uint32_t nitems = uint32_t(ctx->h_items.size());
-    if (nspans > ctx->params_cap)
-    {
-        dfree(ctx->d_params);
-        ctx->d_params = nullptr;
-        ctx->params_cap = 0;
-        if ((rc = dalloc(&ctx->d_params, nspans * sizeof(gr::StepParams))) != GR_OK) return rc;
-        ctx->params_cap = nspans;
-    }
-    if (nitems > ctx->items_cap)
-    {
-        dfree(ctx->d_items);
-        ctx->d_items = nullptr;
-        ctx->items_cap = 0;
-        if ((rc = dalloc(&ctx->d_items, nitems * sizeof(gr::BatchItem))) != GR_OK) return rc;
-        ctx->items_cap = nitems;
-    }
-    if (!ctx->d_queues && (rc = dalloc(&ctx->d_queues, ctx->h_queues.size() * sizeof(uint32_t))) != GR_OK)
-        return rc;
-    if (nspans)
-        GR_HIP(hipMemcpyAsync(ctx->d_params, ctx->h_params.data(), nspans * sizeof(gr::StepParams),
-                              hipMemcpyHostToDevice, s));
-    if (nitems)
-        GR_HIP(hipMemcpyAsync(ctx->d_items, ctx->h_items.data(), nitems * sizeof(gr::BatchItem),
-                              hipMemcpyHostToDevice, s));
-    // zeroed counters and the queue starts in one copy
-    GR_HIP(hipMemcpyAsync(ctx->d_queues, ctx->h_queues.data(), ctx->h_queues.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, s));
+
+    // staging block: queue counters (zero) + queue starts | params | window pointers | window
+    // edges | items (only when rebuilt); one copy to the device block, one to the item table
+    const size_t off_p = 256;
+    const size_t off_w = align256(off_p + size_t(nspans) * sizeof(gr::StepParams));
+    const size_t off_e = off_w + wptr.size() * sizeof(void *);
+    const size_t off_i = align256(off_e + wedge.size() * sizeof(uint32_t));
+    const size_t dev_bytes = off_i;
+    const size_t host_bytes = off_i + (items_dirty ? size_t(nitems) * sizeof(gr::BatchItem) : 0);
+    const int k = ctx->stage_i;
+    if ((rc = stage_reserve(ctx, k, host_bytes)) != GR_OK) return rc;
+    if ((rc = grow(ctx->d_stage, ctx->d_stage_cap, dev_bytes)) != GR_OK) return rc;
+    if (items_dirty && (rc = grow(ctx->d_items, ctx->items_cap, std::max<size_t>(1, nitems))) != GR_OK) return rc;
+    char * h = static_cast<char *>(ctx->stage[k].h);
+    char * dbase = reinterpret_cast<char *>(ctx->d_stage);
+    uint32_t * hq = reinterpret_cast<uint32_t *>(h);
+    std::memset(hq, 0, gr::MAX_QUEUES * sizeof(uint32_t));
+    std::memcpy(hq + gr::MAX_QUEUES, ctx->qstart.data(), (gr::MAX_QUEUES + 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < nspans; ++i)
+        if (span_nwin[i] > 1 && span_steps[i])
+        {
+            params[i].win_ptr = reinterpret_cast<const void * const *>(dbase + off_w + 3 * wofs[i] * sizeof(void *));
+            params[i].win_edge = reinterpret_cast<const uint32_t *>(dbase + off_e + wofs[i] * sizeof(uint32_t));
+        }
+    if (nspans) std::memcpy(h + off_p, params.data(), size_t(nspans) * sizeof(gr::StepParams));
+    if (!wptr.empty()) std::memcpy(h + off_w, wptr.data(), wptr.size() * sizeof(void *));
+    if (!wedge.empty()) std::memcpy(h + off_e, wedge.data(), wedge.size() * sizeof(uint32_t));
+    if (items_dirty && nitems) std::memcpy(h + off_i, ctx->h_items.data(), size_t(nitems) * sizeof(gr::BatchItem));
+    GR_HIP(hipMemcpyAsync(ctx->d_stage, h, dev_bytes, hipMemcpyHostToDevice, s));
+    if (items_dirty && nitems)
+        GR_HIP(hipMemcpyAsync(ctx->d_items, h + off_i, size_t(nitems) * sizeof(gr::BatchItem), hipMemcpyHostToDevice, s));
+    GR_HIP(hipEventRecord(ctx->stage[k].done, s));
+    ctx->stage[k].pending = true;
+    ctx->stage_i = 1 - k;
+    ctx->items_valid = true;
+
     int per_cu = 0;
     if (gr::batch_occupancy(block, ctx->batch_waves, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
     // enough workgroups to fill the device, no more than one wavefront per item
     const uint32_t wpb = block / 64u;
     const uint32_t grid = std::max<uint32_t>(
         1, std::min<uint32_t>((nitems + wpb - 1) / wpb, uint32_t(per_cu) * uint32_t(std::max(1, ctx->cus))));
+    const gr::StepParams * dparams = reinterpret_cast<const gr::StepParams *>(dbase + off_p);
+    uint32_t * dq = reinterpret_cast<uint32_t *>(dbase);
     GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
-    if (nitems && gr::launch_batch(ctx->d_params, ctx->d_items, ctx->d_queues + gr::MAX_QUEUES, nq,
-                                   ctx->d_queues, grid, block, ctx->batch_waves, s) != 0)
+    if (nitems && gr::launch_batch(dparams, ctx->d_items, dq + gr::MAX_QUEUES, nq, dq, grid, block,
+                                   ctx->batch_waves, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if (nitems && (rc = kt_mark(ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(ctx->bev1, s));
+    std::vector<std::pair<gr_field *, uint32_t>> used;
     for (uint32_t i = 0; i < nspans; ++i)
     {
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
-        // every slice the span read is in use until the launch ends
+        // every slice the span read is in use until the launch ends (one event record per slot)
         const uint32_t c_end = std::min<uint32_t>(first_win[i].wind_back + span_nwin[i] + 1,
                                                   spans[i].wind->d.nslots);
         for (uint32_t c = first_win[i].wind_back; c < c_end; ++c)
-            if ((rc = mark_inuse(spans[i].wind, c, s)) != GR_OK) return rc;
-        if ((rc = finish_advance(j, &first_win[i], saves[i])) != GR_OK) return rc;
+        {
+            const std::pair<gr_field *, uint32_t> u(spans[i].wind, c);
+            if (std::find(used.begin(), used.end(), u) == used.end()) used.push_back(u);
+        }
+        if ((rc = finish_advance(j, &first_win[i], saves[i], false)) != GR_OK) return rc;
         j->timed = false;
     }
+    for (const auto & u : used)
+        if ((rc = mark_inuse(u.first, u.second, s)) != GR_OK) return rc;
     if (ms)
     {
         GR_HIP(hipEventSynchronize(ctx->bev1));
@@ -1591,4 +1751,103 @@ extern "C" int gr_job_last_advance_ms(gr_job * j, float * ms)
     GR_HIP(hipEventSynchronize(j->ev1));
     GR_HIP(hipEventElapsedTime(ms, j->ev0, j->ev1));
     return GR_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// autotuner of the batched kernel
+// --------------------------------------------------------------------------------------------
+
+// Runs every candidate launch configuration of the batched kernel (workgroup size x register
+// budget) on the same probe workload -- a job of the caller's description with n seeds spread
+// uniformly over the domain, Morton-ordered, advanced `steps` steps from window 0 -- and keeps
+// the fastest as the context's batch options. Every candidate starts from identical state and
+// computes identical results, so device times compare directly. The context's work counter and
+// kernel-time pool are left as they were.
+extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
+                                 uint32_t steps, uint32_t trials, gr_tune_entry * results,
+                                 uint32_t max_results, uint32_t * nresults)
+{
+    if (!ctx || !wind || !desc) return set_err(GR_E_INVALID, "gr_batch_autotune: null argument");
+    if (n == 0 || steps == 0) return set_err(GR_E_INVALID, "gr_batch_autotune: n and steps must be > 0");
+    if (trials == 0) trials = 2;
+    GR_HIP(hipSetDevice(ctx->device));
+    int rc;
+    // park the context's measurement state
+    if ((rc = kt_fold(ctx)) != GR_OK) return rc;
+    const double kms_saved = ctx->kms_acc;
+    const uint64_t kl_saved = ctx->klaunch_acc;
+    unsigned long long work_saved = 0;
+    GR_HIP(hipMemcpyAsync(&work_saved, ctx->dcount + 1, sizeof(work_saved), hipMemcpyDeviceToHost, ctx->compute));
+    GR_HIP(hipStreamSynchronize(ctx->compute));
+    const uint32_t block0 = ctx->batch_block, waves0 = ctx->batch_waves;
+
+    // probe seeds: a deterministic low-discrepancy spread over the domain (inset by 1e-3)
+    const uint32_t dim = desc->dim;
+    std::vector<float> seeds(size_t(n) * dim);
+    const double alpha[3] = {0.7548776662466927, 0.5698402909980532, 0.4142135623730950};
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t d = 0; d < dim; ++d)
+        {
+            double u = std::fmod(0.5 + alpha[d] * double(i + 1), 1.0);
+            u = 1e-3 + u * (1.0 - 2e-3);
+            seeds[size_t(i) * dim + d] = float(desc->offset[d] + u * double(desc->domain[d]));
+        }
+    gr_job_desc d = *desc;
+    d.save_interval = desc->save_interval;
+
+    struct Cand { uint32_t block, waves; };
+    const Cand cands[] = {{64, 0}, {128, 0}, {256, 0}, {64, 4}, {128, 4}, {256, 4}};
+    const uint32_t ncand = sizeof(cands) / sizeof(cands[0]);
+    float best = 3.4e38f;
+    uint32_t best_i = 0;
+    uint32_t nres = 0;
+    for (uint32_t ci = 0; ci < ncand; ++ci)
+    {
+        float tmin = 3.4e38f;
+        for (uint32_t t = 0; t < trials; ++t)
+        {
+            gr_job * j = nullptr;
+            if ((rc = gr_job_create(ctx, &d, seeds.data(), n, steps, &j)) != GR_OK) break;
+            if (n >= 4096 && (rc = gr_job_reorder(j)) != GR_OK)
+            {
+                gr_job_destroy(j);
+                break;
+            }
+            ctx->batch_block = cands[ci].block;
+            ctx->batch_waves = cands[ci].waves;
+            gr_span sp{};
+            sp.job = j;
+            sp.wind = wind;
+            sp.max_steps = steps;
+            float ms = 0.f;
+            rc = gr_batch_run(ctx, &sp, 1, 0, nullptr, &ms);
+            gr_job_destroy(j);
+            if (rc != GR_OK) break;
+            tmin = std::min(tmin, ms);
+        }
+        if (rc != GR_OK) break;
+        if (results && nres < max_results) results[nres++] = gr_tune_entry{cands[ci].block, cands[ci].waves, tmin};
+        if (tmin < best)
+        {
+            best = tmin;
+            best_i = ci;
+        }
+    }
+    // restore the measurement state; keep the winner (or the old options on failure)
+    int rc2 = kt_fold(ctx);
+    ctx->kms_acc = kms_saved;
+    ctx->klaunch_acc = kl_saved;
+    if (hipMemcpyAsync(ctx->dcount + 1, &work_saved, sizeof(work_saved), hipMemcpyHostToDevice, ctx->compute) != hipSuccess ||
+        hipStreamSynchronize(ctx->compute) != hipSuccess)
+        rc2 = set_err(GR_E_HIP, "gr_batch_autotune: restoring the work counter failed");
+    if (rc != GR_OK)
+    {
+        ctx->batch_block = block0;
+        ctx->batch_waves = waves0;
+        return rc;
+    }
+    ctx->batch_block = cands[best_i].block;
+    ctx->batch_waves = cands[best_i].waves;
+    if (nresults) *nresults = nres;
+    return rc2;
 }

@@ -185,14 +185,34 @@ uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);
 int launch_pair(const void * back, const void * front, void * out, uint32_t nx, uint32_t ny,
                 uint32_t nz, uint32_t dim, uint32_t address, uint32_t ncomp, uint32_t cstride,
                 uint32_t dtype, void * stream);
-// Morton keys of the particles' current cells (dead particles sort last)
-int launch_morton_keys(const StepParams & P, uint32_t dim, uint32_t * keys, uint32_t * vals,
-                       void * stream);
-// radix sort of (key, value) pairs; temp storage sized by a first call with temp == null
-int sort_pairs(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
-               const uint32_t * vin, uint32_t * vout, uint32_t n, void * stream);
-// out[i] = in[perm[i]] for 4-byte elements
-int launch_gather32(const void * in, void * out, const uint32_t * perm, uint64_t n, void * stream);
+// one job of a multi-job reorder: its particles occupy keys [base, base + n)
+struct SortJob
+{
+    const float * p[3];
+    const int32_t * status;
+    uint64_t base;
+    uint32_t n, dim;
+    float off[3], inv_h[3];
+    uint32_t grid[3];
+};
+// dst[i] = src[perm[i]], i < n (4-byte elements)
+struct GatherEntry
+{
+    const uint32_t * src;
+    uint32_t * dst;
+    const uint32_t * perm;
+    uint32_t n;
+    uint32_t pad;
+};
+// keys (job << 32 | Morton code of the current cell, stopped particles last) and in-job indices
+int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, uint64_t * keys,
+                        uint32_t * vals, void * stream);
+// radix sort of (key, value) pairs on key bits [0, end_bit); temp storage sized by a first call
+// with temp == null
+int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
+                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
+// all entries in one launch (maxn: largest entry n)
+int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
                      void * stream);

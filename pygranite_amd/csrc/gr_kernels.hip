@@ -17,6 +17,8 @@
 #include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "gr_internal.h"
 
 namespace gr
@@ -188,6 +190,64 @@ __device__ __forceinline__ void combine(const float (&v)[8][NC], const Taps & T,
         else
             out[c] = lerp(c00, c10, T.wy);
     }
+}
+
+// FP16 corner values enter the first (x) lerp without separate conversions: with f16 operands
+// fma(b, 1, -a) is b - a and fma(a, 1, x) is a + x, each rounded once after the exact f16 -> f32
+// conversion, i.e. the same IEEE results as lerp((float)a, (float)b, t); the compiler forms
+// v_fma_mix_f32 from them. `one` is 1.0 hidden from constant folding (which would turn the fma
+// back into an add of converted values).
+typedef _Float16 f16_t;
+__device__ __forceinline__ f16_t f16_lo(uint32_t w) { return __builtin_bit_cast(f16_t, uint16_t(w & 0xffffu)); }
+__device__ __forceinline__ f16_t f16_hi(uint32_t w) { return __builtin_bit_cast(f16_t, uint16_t(w >> 16)); }
+__device__ __forceinline__ float lerp_h(f16_t a, f16_t b, float t, f16_t one)
+{
+    const float d = __builtin_fmaf(float(b), float(one), -float(a));
+    return __builtin_fmaf(float(a), float(one), t * d);
+}
+
+// Component c of slice s (0 back, 1 front) of a time-paired FP16 node: 3D words
+// [u0 v0 | w0 u1 | v1 w1 | 0 0], 2D words [u0 v0 | u1 v1].
+template <int DIM>
+__device__ __forceinline__ f16_t pair_h(const uint4 & a, int s, int c)
+{
+    if (DIM == 3)
+    {
+        const int e = 3 * s + c; // element 0..5
+        const uint32_t w = e < 2 ? a.x : (e < 4 ? a.y : a.z);
+        return (e & 1) ? f16_hi(w) : f16_lo(w);
+    }
+    const uint32_t w = s ? a.y : a.x;
+    return c ? f16_hi(w) : f16_lo(w);
+}
+
+// Trilinear (bilinear) sample of both slices of a paired FP16 stencil, same operation order as
+// combine(): x lerps, then y, then z.
+template <int DIM>
+__device__ __forceinline__ void combine_pair_h(const uint4 (&a)[8], const Taps & T, f16_t one,
+                                               float (&b)[DIM], float (&f)[DIM])
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+        {
+            const float c00 = lerp_h(pair_h<DIM>(a[0], s, c), pair_h<DIM>(a[1], s, c), T.wx, one);
+            const float c10 = lerp_h(pair_h<DIM>(a[2], s, c), pair_h<DIM>(a[3], s, c), T.wx, one);
+            float r;
+            if (DIM == 3)
+            {
+                const float c01 = lerp_h(pair_h<DIM>(a[4], s, c), pair_h<DIM>(a[5], s, c), T.wx, one);
+                const float c11 = lerp_h(pair_h<DIM>(a[6], s, c), pair_h<DIM>(a[7], s, c), T.wx, one);
+                const float c0 = lerp(c00, c10, T.wy);
+                const float c1 = lerp(c01, c11, T.wy);
+                r = lerp(c0, c1, T.wz);
+            }
+            else
+                r = lerp(c00, c10, T.wy);
+            if (s == 0) b[c] = r;
+            else f[c] = r;
+        }
 }
 
 template <int DIM, class T, int NC>
@@ -405,7 +465,7 @@ __device__ __forceinline__ void texel_arg(const StepParams & P, const float * x,
 
 template <int DIM, class T, int MODE>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
-                                         float t, float * v)
+                                         float t, float * v, f16_t one)
 {
     float arg[3] = {0.f, 0.f, 0.f};
     texel_arg<DIM>(P, x, arg);
@@ -423,6 +483,27 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         if (DIM == 3) split(arg[2], P.wind.nz, P.weights, k, S.wz);
         const uint32_t PX = P.wind.nx + 3, PXY = PX * (P.wind.ny + 3);
         const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
+        float b[DIM], f[DIM];
+        if constexpr (std::is_same<T, __half>::value)
+        {
+            uint4 raw[8];
+#pragma unroll
+            for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+            {
+                const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
+                if (DIM == 3)
+                    raw[c] = GR_LOAD(uint4, W.pair, node);
+                else
+                {
+                    const uint2 h = GR_LOAD(uint2, W.pair, node);
+                    raw[c] = make_uint4(h.x, h.y, 0u, 0u);
+                }
+            }
+            combine_pair_h<DIM>(raw, S, one, b, f);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
+            return;
+        }
         float vb[8][DIM], vf[8][DIM];
 #pragma unroll
         for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
@@ -430,7 +511,6 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
             const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
             PairLoad<T, DIM>::get(W.pair, node, vb[c], vf[c]);
         }
-        float b[DIM], f[DIM];
         combine<DIM, DIM>(vb, S, b);
         combine<DIM, DIM>(vf, S, f);
 #pragma unroll
@@ -541,6 +621,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     // state: nothing to do.
     int32_t status = G(P.status)[pid];
     if (status != GR_STATUS_TIME) return 0;
+    f16_t one = f16_t(1.0f);
+    asm volatile("" : "+v"(one)); // opaque 1.0 for the mixed-precision lerps (lerp_h)
 
     float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
     float ld[3] = {0.f, 0.f, 0.f};
@@ -609,22 +691,22 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 }
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE>(P, W, p, t, k1);
+                velocity<DIM, T, MODE>(P, W, p, t, k1, one);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE>(P, W, x, t1, k2);
+                velocity<DIM, T, MODE>(P, W, x, t1, k2, one);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE>(P, W, x, t1, k3);
+                velocity<DIM, T, MODE>(P, W, x, t1, k3, one);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE>(P, W, x, t2, k4);
+                velocity<DIM, T, MODE>(P, W, x, t2, k4, one);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -636,7 +718,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                velocity<DIM, T, MODE>(P, W, p, t, vc);
+                velocity<DIM, T, MODE>(P, W, p, t, vc, one);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -1317,58 +1399,77 @@ __device__ __forceinline__ uint32_t spread2(uint32_t x) // 16 bits -> every seco
     return x;
 }
 
-template <int DIM>
-__global__ void k_morton(const StepParams P, uint32_t * keys, uint32_t * vals)
+// Keys of a reorder of several jobs at once: key = (job << 32) | Morton code of the particle's
+// cell (stopped particles 0xffffffff, i.e. last), value = the particle's index in its job. One
+// radix sort over all keys then orders every job by cell inside its own key range.
+__global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs, uint64_t total,
+                               uint64_t * __restrict__ keys, uint32_t * __restrict__ vals)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    uint32_t key = 0xffffffffu; // stopped particles go last
-    if (P.status[i] == GR_STATUS_TIME)
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    uint32_t lo = 0, hi = njobs; // last job with base <= g
+    while (hi - lo > 1)
     {
-        const uint32_t lim = DIM == 3 ? 1023u : 65535u;
+        const uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].base <= g) lo = mid;
+        else hi = mid;
+    }
+    const SortJob & J = jobs[lo];
+    const uint32_t i = uint32_t(g - J.base);
+    uint32_t code = 0xffffffffu;
+    if (J.status[i] == GR_STATUS_TIME)
+    {
+        const uint32_t lim = J.dim == 3 ? 1023u : 65535u;
         uint32_t c[3] = {0, 0, 0};
-        const uint32_t n[3] = {P.wind.nx, P.wind.ny, P.wind.nz};
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
+        for (uint32_t d = 0; d < J.dim; ++d)
         {
-            float a = (P.p[d][i] - P.off[d]) * P.inv_h[d];
-            a = fminf(fmaxf(a, 0.f), float(n[d] - 1));
+            float a = (J.p[d][i] - J.off[d]) * J.inv_h[d];
+            a = fminf(fmaxf(a, 0.f), float(J.grid[d] - 1));
             c[d] = min(uint32_t(a), lim);
         }
-        key = DIM == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
-                       : (spread2(c[0]) | (spread2(c[1]) << 1));
-        key = min(key, 0xfffffffeu);
+        code = J.dim == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
+                          : (spread2(c[0]) | (spread2(c[1]) << 1));
+        code = min(code, 0xfffffffeu);
     }
-    keys[i] = key;
-    vals[i] = i;
+    keys[g] = (uint64_t(lo) << 32) | code;
+    vals[g] = i;
 }
 
-int launch_morton_keys(const StepParams & P, uint32_t dim, uint32_t * keys, uint32_t * vals,
-                       void * stream)
+int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, uint64_t * keys,
+                        uint32_t * vals, void * stream)
 {
     const uint32_t block = 256;
-    const uint32_t grid = (P.n + block - 1) / block;
+    const uint64_t grid = (total + block - 1) / block;
     if (grid == 0) return 0;
-    if (dim == 3)
-        hipLaunchKernelGGL(k_morton<3>, dim3(grid), dim3(block), 0, hipStream_t(stream), P, keys, vals);
-    else
-        hipLaunchKernelGGL(k_morton<2>, dim3(grid), dim3(block), 0, hipStream_t(stream), P, keys, vals);
+    hipLaunchKernelGGL(k_morton_multi, dim3(uint32_t(grid)), dim3(block), 0, hipStream_t(stream),
+                       jobs, njobs, total, keys, vals);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int sort_pairs(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
-               const uint32_t * vin, uint32_t * vout, uint32_t n, void * stream)
+int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
+                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream)
 {
     const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
-                                                            int(n), 0, 32, hipStream_t(stream));
+                                                            int(n), 0, int(end_bit), hipStream_t(stream));
     return e == hipSuccess ? 0 : -1;
 }
 
-__global__ void k_gather32(const uint32_t * __restrict__ in, uint32_t * __restrict__ out,
-                           const uint32_t * __restrict__ perm, uint64_t n)
+// Every (array, job) entry of a multi-job reorder in one launch: blockIdx.y picks the entry,
+// dst[i] = src[perm[i]] for its n elements.
+__global__ void k_gather_multi(const GatherEntry * __restrict__ tab)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[perm[i]];
+    const GatherEntry & E = tab[blockIdx.y];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < E.n) E.dst[i] = E.src[E.perm[i]];
+}
+
+int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream)
+{
+    const uint32_t block = 256;
+    const uint32_t grid = (maxn + block - 1) / block;
+    if (grid == 0 || nent == 0) return 0;
+    hipLaunchKernelGGL(k_gather_multi, dim3(grid, nent), dim3(block), 0, hipStream_t(stream), tab);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 __global__ void k_scatter32(const uint32_t * __restrict__ in, uint32_t * __restrict__ out,
@@ -1378,16 +1479,6 @@ __global__ void k_scatter32(const uint32_t * __restrict__ in, uint32_t * __restr
     if (i >= n) return;
     const uint64_t o = orig[i];
     for (uint32_t c = 0; c < w; ++c) out[o * w + c] = in[i * w + c];
-}
-
-int launch_gather32(const void * in, void * out, const uint32_t * perm, uint64_t n, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_gather32, dim3(grid), dim3(block), 0, hipStream_t(stream),
-                       static_cast<const uint32_t *>(in), static_cast<uint32_t *>(out), perm, n);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,

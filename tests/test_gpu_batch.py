@@ -216,3 +216,86 @@ def test_single_job_work_counter(gpu_available):
     j.close()
     f.close()
     ctx.close()
+
+
+def test_autotune_picks_a_candidate_and_leaves_counters(gpu_available):
+    """The autotuner times every candidate, sets the fastest on the context, and leaves the
+    context's work counter and kernel-time sums as they were; later batches give the same
+    results as with the default configuration."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 5)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
+    for s in range(5):
+        f.upload(s, [src[s, c] for c in range(3)])
+    d = make_desc(3, 1, gs, src.shape, 0.005, 3, ("sqrt(u*u+v*v+w*w)",))
+    seeds = synthetic.uniform_seeds(23, 3000, [0.0] * 3, [2 * math.pi] * 3)
+    a = capi.Job(ctx, d, seeds)
+    ctx.batch_run([(a, f, dict(max_steps=100))])
+    ctx.synchronize()
+    res = ctx.autotune(f, d, n=8192, steps=40, trials=1)
+    assert len(res) == 6 and all(r["ms"] > 0 for r in res)
+    best = min(res, key=lambda r: r["ms"])
+    assert ctx.get_option(capi.CTX_BATCH_BLOCK) == best["block"]
+    assert ctx.get_option(capi.CTX_BATCH_WAVES) == best["waves"]
+    # the probe runs did not count: only job a's 100 steps are in the counters
+    status, ab = a.read(capi.OUT_STATUS), a.read(capi.OUT_ABORT_STEP)
+    assert ctx.take_work() == int(np.where(status == 0, 100, ab.astype(np.int64) + 1).sum())
+    ms, launches = ctx.take_kernel_ms()
+    assert launches == 1 and ms > 0
+    # same results under the tuned configuration as under the default one
+    b = capi.Job(ctx, d, seeds)
+    ctx.batch_run([(b, f, dict(max_steps=100))])
+    np.testing.assert_array_equal(a.read(capi.OUT_POSITIONS), b.read(capi.OUT_POSITIONS))
+    a.close()
+    b.close()
+    f.close()
+    ctx.close()
+
+
+def test_joint_reorder_of_mixed_jobs(gpu_available):
+    """Reordering 2D and 3D jobs together (one key kernel, one sort, one gather) changes no
+    result: positions, status, abort steps and saved columns equal those of unreordered jobs,
+    also after a second joint reorder half way."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 5)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
+    for s in range(5):
+        f.upload(s, [src[s, c] for c in range(3)])
+    g2, gs2 = synthetic.double_gyre(96, 48, 6)
+    f2 = capi.Field(ctx, 96, 48, 1, 2, capi.F32, nslots=6, address=1)
+    for s in range(6):
+        f2.upload(s, [g2[s, c] for c in range(2)])
+    d3 = make_desc(3, 1, gs, src.shape, 0.005, 3, ("sqrt(u*u+v*v+w*w)",), save_interval=7)
+    d2 = make_desc(2, 1, gs2, g2.shape, 0.01, 0, (), save_interval=7)
+    seeds = [synthetic.uniform_seeds(40, 9000, [0.0] * 3, [2 * math.pi] * 3),
+             synthetic.uniform_seeds(41, 5000, [0.02, 0.02], [1.98, 0.98]),
+             synthetic.uniform_seeds(42, 777, [0.0] * 3, [2 * math.pi] * 3)]
+    descs, fields = [d3, d2, d3], [f, f2, f]
+
+    def run(joint):
+        jobs = [capi.Job(ctx, d, sd, save_capacity=40) for d, sd in zip(descs, seeds)]
+        if joint:
+            ctx.reorder(jobs)
+        ends, _ = ctx.batch_run([(j, fl, dict(max_steps=140)) for j, fl in zip(jobs, fields)], ends=True)
+        cols = [j.read_saved(capi.OUT_POSITIONS, 0, 20) for j in jobs]
+        if joint:
+            ctx.reorder(jobs)
+        ctx.batch_run([(j, fl, dict(max_steps=140, first_window=e["window"], time_start=e["time"],
+                                    time_start64=e["time64"], counter_start=e["counter"]))
+                       for j, fl, e in zip(jobs, fields, ends)])
+        out = [(j.read(capi.OUT_POSITIONS), j.read(capi.OUT_STATUS), j.read(capi.OUT_ABORT_STEP),
+                j.read_saved(capi.OUT_POSITIONS, 20, 20)) for j in jobs]
+        for j in jobs:
+            j.close()
+        return cols, out
+
+    c0, o0 = run(False)
+    c1, o1 = run(True)
+    for a, b in zip(c0, c1):
+        np.testing.assert_array_equal(a, b)
+    for x, y in zip(o0, o1):
+        for a, b in zip(x, y):
+            np.testing.assert_array_equal(a, b)
+    f.close()
+    f2.close()
+    ctx.close()
