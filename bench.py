@@ -535,12 +535,15 @@ def main():
     forms = args.forms.split(",")
     tune = None
     if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0):
-        # launch configuration of the batched kernel from the autotuner (a 65,536-particle probe
-        # of the same job description over the same FP16 field, 50 steps, before any timing)
-        tune = ctx.autotune(f16, build_desc(capi, gs, n), n=65536, steps=args.chunk, trials=2)
-        print("# autotune: " + ", ".join(f"b{t['block']}/w{t['waves']} {t['ms']:.3f} ms" for t in tune)
+        # launch configuration of the batched kernel from the autotuner (probe jobs of the same
+        # count, size and description over the same FP16 field, one launch, before any timing)
+        tune = ctx.autotune(f16, build_desc(capi, gs, n), n=max(1, args.particles // args.jobs),
+                            njobs=args.jobs, steps=args.chunk, trials=2)
+        print("# autotune: " + ", ".join(f"b{t['block']}/w{t['waves']}/g{t['group']} {t['ms']:.3f} ms"
+                                         for t in tune)
               + f" -> block {ctx.get_option(capi.CTX_BATCH_BLOCK)}, waves "
-              f"{ctx.get_option(capi.CTX_BATCH_WAVES)}", file=sys.stderr, flush=True)
+              f"{ctx.get_option(capi.CTX_BATCH_WAVES)}, group {ctx.get_option(capi.CTX_BATCH_GROUP)}",
+              file=sys.stderr, flush=True)
     res = {}
     for name in forms:
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {args.chunk} steps, "
@@ -610,7 +613,8 @@ def main():
                    "form": head, "particles": args.particles, "grid": n,
                    "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
                    "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
-                   "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES)},
+                   "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
+                   "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP)},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
         "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},
         "autotune": tune,

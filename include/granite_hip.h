@@ -315,26 +315,30 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
  * GR_CTX_BATCH_BLOCK: workgroup size when gr_batch_run gets block = 0 (64, 128, 192 or 256;
  *   default 256).
  * GR_CTX_BATCH_WAVES: register budget of the batched kernel: 0 = the compiler's allocation
- *   (default), 4 = at most 128 VGPRs so that 4 wavefronts fit per SIMD. */
+ *   (default), 4 = at most 128 VGPRs so that 4 wavefronts fit per SIMD.
+ * GR_CTX_BATCH_GROUP: compute units that share one work queue (1 .. number of CUs; default:
+ *   the CUs of one XCD). Each queue holds one compact region of the particle sets. */
 #define GR_CTX_BATCH_BLOCK 1u
 #define GR_CTX_BATCH_WAVES 2u
+#define GR_CTX_BATCH_GROUP 3u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 
 /* Autotuner of the batched kernel (no counterpart in the reference). Times every candidate
- * launch configuration (workgroup size 64 / 128 / 256 x register budget 0 / 4) on a probe job
- * of `desc` with n seeds spread over the domain, advanced `steps` steps through `wind` from
- * window 0 (best of `trials` runs each, all from identical state), and sets the fastest as the
- * context's GR_CTX_BATCH_BLOCK / GR_CTX_BATCH_WAVES. results[0 .. *nresults) receive every
- * candidate's device time (may be NULL). The context's work counter and kernel-time sums are
- * unchanged. */
+ * launch configuration (workgroup size 64 / 128 / 256 x register budget 0 / 4 x compute units
+ * per queue: one XCD, 8, 1) on njobs probe jobs of `desc` with n seeds each spread over the
+ * domain, advanced `steps` steps through `wind` from window 0 (best of `trials` runs each, all
+ * from identical state), and sets the fastest as the context's batch options. results[0 ..
+ * *nresults) receive every candidate's device time (may be NULL). The context's work counter
+ * and kernel-time sums are unchanged. */
 typedef struct {
     uint32_t block;
     uint32_t waves;
+    uint32_t group;
     float ms;
 } gr_tune_entry;
 int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
-                      uint32_t steps, uint32_t trials, gr_tune_entry * results,
+                      uint32_t njobs, uint32_t steps, uint32_t trials, gr_tune_entry * results,
                       uint32_t max_results, uint32_t * nresults);
 
 #ifdef __cplusplus

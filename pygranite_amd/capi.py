@@ -35,7 +35,7 @@ EXPORTS = [
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
 OPT_BLOCK, OPT_PAIRED = 1, 2
-CTX_BATCH_BLOCK, CTX_BATCH_WAVES = 1, 2
+CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP = 1, 2, 3
 
 
 class FieldDesc(ctypes.Structure):
@@ -101,7 +101,8 @@ class SpanEnd(ctypes.Structure):
 
 
 class TuneEntry(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("ms", ctypes.c_float)]
+    _fields_ = [("block", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("group", ctypes.c_uint32),
+                ("ms", ctypes.c_float)]
 
 
 _lib = None
@@ -152,7 +153,7 @@ def lib():
         "gr_job_reorder": ([vp], i32),
         "gr_jobs_reorder": ([P(vp), u32], i32),
         "gr_batch_run": ([vp, P(Span), u32, u32, P(SpanEnd), P(ctypes.c_float)], i32),
-        "gr_batch_autotune": ([vp, vp, P(JobDesc), u32, u32, u32, P(TuneEntry), u32, P(u32)], i32),
+        "gr_batch_autotune": ([vp, vp, P(JobDesc), u32, u32, u32, u32, P(TuneEntry), u32, P(u32)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -258,14 +259,16 @@ class Context:
         _check(lib().gr_ctx_get_option(self.h, option, ctypes.byref(v)))
         return int(v.value)
 
-    def autotune(self, field, desc, n: int = 65536, steps: int = 50, trials: int = 2):
-        """Runs the batched-kernel autotuner on a probe job of `desc` over `field`; sets the
-        fastest configuration on this context and returns every candidate's device time."""
-        res = (TuneEntry * 16)()
+    def autotune(self, field, desc, n: int = 65536, njobs: int = 1, steps: int = 50, trials: int = 2):
+        """Runs the batched-kernel autotuner on njobs probe jobs of n particles of `desc` over
+        `field`; sets the fastest configuration on this context and returns every candidate's
+        device time."""
+        res = (TuneEntry * 64)()
         cnt = ctypes.c_uint32(0)
-        _check(lib().gr_batch_autotune(self.h, field.h, ctypes.byref(desc), n, steps, trials,
-                                       res, 16, ctypes.byref(cnt)))
-        return [{"block": r.block, "waves": r.waves, "ms": r.ms} for r in res[:cnt.value]]
+        _check(lib().gr_batch_autotune(self.h, field.h, ctypes.byref(desc), n, njobs, steps, trials,
+                                       res, 64, ctypes.byref(cnt)))
+        return [{"block": r.block, "waves": r.waves, "group": r.group, "ms": r.ms}
+                for r in res[:cnt.value]]
 
     def reorder(self, jobs):
         """Reorders several jobs of this context in one pass (gr_jobs_reorder)."""

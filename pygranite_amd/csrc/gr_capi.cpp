@@ -73,7 +73,10 @@ struct gr_ctx
     int device{0};
     hipStream_t compute{nullptr};
     hipStream_t copy{nullptr};
-    unsigned long long * dcount{nullptr}; // [0] alive count scratch, [1] integrated particle-steps
+    unsigned long long * dcount{nullptr}; // alive count scratch
+    unsigned long long * dwork{nullptr};  // integrated particle-steps, WORK_SLOTS x WORK_STRIDE
+    uint32_t * d_culut{nullptr};          // cu_key -> queue of the batched kernel
+    uint32_t ncu{1};                      // compute units found (dense indices in d_culut)
     // persistent batched launches
     std::vector<gr::BatchItem> h_items;  // the last item list (reused while the spans match)
     std::vector<uint32_t> qstart;         // its queue starts
@@ -108,6 +111,7 @@ struct gr_ctx
     uint32_t nxcd{1}; // XCDs of the device
     uint32_t batch_block{256};
     uint32_t batch_waves{0};
+    uint32_t batch_group{32}; // compute units per queue of the batched kernel
     // device time of every step-kernel launch (event pairs, summed by gr_ctx_take_kernel_ms)
     std::vector<hipEvent_t> kev;
     size_t kpending{0};
@@ -209,6 +213,42 @@ extern "C" int gr_device_count(int * count)
     return GR_OK;
 }
 
+extern "C" int gr_ctx_destroy(gr_ctx * c);
+
+// Which compute units run workgroups: a probe kernel records cu_key() for many small workgroups;
+// every key seen gets a dense index (ordered by XCC, shader engine, CU), which becomes the CU's
+// home queue in the batched kernel. CUs never seen map to no queue (their wavefronts fall back to
+// workgroup index modulo the queue count).
+static int map_compute_units(gr_ctx * c)
+{
+    const uint32_t nprobe = 16384;
+    const size_t nkeys = size_t(1) << gr::CU_KEY_BITS;
+    uint32_t * d = nullptr;
+    int rc = dalloc(&d, nprobe * sizeof(uint32_t));
+    if (rc != GR_OK) return rc;
+    std::vector<uint32_t> h(nprobe);
+    if (gr::launch_cu_probe(d, nprobe, c->compute) != 0 ||
+        hipMemcpyAsync(h.data(), d, nprobe * sizeof(uint32_t), hipMemcpyDeviceToHost, c->compute) != hipSuccess ||
+        hipStreamSynchronize(c->compute) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        dfree(d);
+        return set_err(GR_E_HIP, "compute-unit probe failed");
+    }
+    dfree(d);
+    std::vector<char> seen(nkeys, 0);
+    for (uint32_t k : h) seen[k & (nkeys - 1)] = 1;
+    std::vector<uint32_t> lut(nkeys, 0xffffffffu);
+    uint32_t nq = 0;
+    for (size_t k = 0; k < nkeys; ++k)
+        if (seen[k] && nq < gr::MAX_QUEUES) lut[k] = nq++;
+    c->ncu = std::max(1u, nq);
+    c->batch_group = std::max(1u, c->ncu / std::max(1u, c->nxcd)); // one queue per XCD
+    if ((rc = dalloc(&c->d_culut, nkeys * sizeof(uint32_t))) != GR_OK) return rc;
+    GR_HIP(hipMemcpy(c->d_culut, lut.data(), nkeys * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return GR_OK;
+}
+
 extern "C" int gr_ctx_create(int device, gr_ctx ** out)
 {
     if (!out) return set_err(GR_E_INVALID, "gr_ctx_create: null out");
@@ -227,12 +267,15 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
     // default-priority streams only
     if (hipStreamCreateWithFlags(&c->compute, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&c->dcount), 2 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(c->dcount, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
+        hipMalloc(reinterpret_cast<void **>(&c->dcount), sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&c->dwork), size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess ||
+        hipMemset(c->dwork, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess)
     {
         (void)hipGetLastError();
         if (c->compute) (void)hipStreamDestroy(c->compute);
         if (c->copy) (void)hipStreamDestroy(c->copy);
+        dfree(c->dcount);
+        dfree(c->dwork);
         delete c;
         return set_err(GR_E_HIP, "gr_ctx_create: stream / buffer creation failed");
     }
@@ -240,8 +283,14 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
     int nx = 1;
     if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && nx >= 1)
-        c->nxcd = uint32_t(std::min<int>(nx, int(gr::MAX_QUEUES)));
+        c->nxcd = uint32_t(nx);
     (void)hipGetLastError();
+    int rc = map_compute_units(c);
+    if (rc != GR_OK)
+    {
+        gr_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return GR_OK;
 }
@@ -257,6 +306,8 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     (void)hipStreamSynchronize(c->copy);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     dfree(c->dcount);
+    dfree(c->dwork);
+    dfree(c->d_culut);
     dfree(c->d_stage);
     dfree(c->d_items);
     for (auto & st : c->stage)
@@ -303,6 +354,11 @@ extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
             if (value != 0 && value != 4) return set_err(GR_E_INVALID, "batch waves must be 0 or 4");
             c->batch_waves = uint32_t(value);
             return GR_OK;
+        case GR_CTX_BATCH_GROUP:
+            if (value < 1 || value > int64_t(c->ncu))
+                return set_err(GR_E_INVALID, "batch group must be 1 .. %u compute units", c->ncu);
+            c->batch_group = uint32_t(value);
+            return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -314,6 +370,7 @@ extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
     {
         case GR_CTX_BATCH_BLOCK: *value = c->batch_block; return GR_OK;
         case GR_CTX_BATCH_WAVES: *value = c->batch_waves; return GR_OK;
+        case GR_CTX_BATCH_GROUP: *value = c->batch_group; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -371,11 +428,13 @@ extern "C" int gr_ctx_take_work(gr_ctx * c, uint64_t * particle_steps)
 {
     if (!c || !particle_steps) return set_err(GR_E_INVALID, "gr_ctx_take_work: null argument");
     GR_HIP(hipSetDevice(c->device));
-    unsigned long long h = 0;
-    GR_HIP(hipMemcpyAsync(&h, c->dcount + 1, sizeof(h), hipMemcpyDeviceToHost, c->compute));
-    GR_HIP(hipMemsetAsync(c->dcount + 1, 0, sizeof(h), c->compute));
+    std::vector<unsigned long long> h(size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE);
+    GR_HIP(hipMemcpyAsync(h.data(), c->dwork, h.size() * 8, hipMemcpyDeviceToHost, c->compute));
+    GR_HIP(hipMemsetAsync(c->dwork, 0, h.size() * 8, c->compute));
     GR_HIP(hipStreamSynchronize(c->compute));
-    *particle_steps = h;
+    uint64_t sum = 0;
+    for (uint32_t k = 0; k < gr::WORK_SLOTS; ++k) sum += h[size_t(k) * gr::WORK_STRIDE];
+    *particle_steps = sum;
     return GR_OK;
 }
 
@@ -1078,7 +1137,16 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
 
     std::memset(&P, 0, sizeof(P));
     P.n = uint32_t(j->n);
-    P.work = j->ctx->dcount + 1;
+    P.work = j->ctx->dwork;
+    {
+        // curvature divisors as the step computes them (FP32) and their reciprocals
+        const float dtf = d.dt;
+        P.c2dt = 2.f * dtf;
+        P.cdt2 = dtf * dtf;
+        auto rc_ = [](float c) { return (std::fabs(c) >= 1e-6f && std::fabs(c) <= 1e6f) ? 1.f / c : 0.f; };
+        P.r2dt = rc_(P.c2dt);
+        P.rdt2 = rc_(P.cdt2);
+    }
     P.nsteps = w->nsteps;
     uint32_t fl = 0;
     const bool dyn = d.windfield_mode == 1 || d.uplift_mode == 2 || d.constants_mode == 2 || d.volume_mode == 2;
@@ -1451,7 +1519,8 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     // wavefronts in flight on one XCD then sample one compact region of one field, and long
     // spans start early. The list depends only on the spans' jobs, sizes, variants, fields and
     // length classes, so a launch with the same ones reuses the previous list on the device.
-    const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>(ctx->nxcd, gr::MAX_QUEUES));
+    const uint32_t group = std::max(1u, ctx->batch_group);
+    const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>((ctx->ncu + group - 1) / group, gr::MAX_QUEUES));
     std::vector<uint64_t> key;
     key.push_back(nq);
     std::vector<uint32_t> cls_of(nspans, 0);
@@ -1494,7 +1563,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             return a.pos < b.pos;
         });
         ctx->h_items.resize(keyed.size());
-        ctx->qstart.assign(gr::MAX_QUEUES + 1, 0u);
+        ctx->qstart.assign(nq + 1, 0u);
         for (size_t k = 0; k < keyed.size(); ++k)
         {
             ctx->h_items[k] = keyed[k].item;
@@ -1506,9 +1575,12 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     }
     const uint32_t nitems = uint32_t(ctx->h_items.size());
 
-    // staging block: queue counters (zero) + queue starts | params | window pointers | window
-    // edges | items (only when rebuilt); one copy to the device block, one to the item table
-    const size_t off_p = 256;
+    // staging block: queue data (zeroed counters, queue starts, masks of the queues holding
+    // items) | params | window pointers | window edges | items (only when rebuilt); one copy to
+    // the device block, one to the item table
+    const uint32_t ng = (nq + 31) / 32;
+    const size_t qwords = size_t(nq) * gr::QUEUE_STRIDE + nq + 1 + ng;
+    const size_t off_p = align256(qwords * sizeof(uint32_t));
     const size_t off_w = align256(off_p + size_t(nspans) * sizeof(gr::StepParams));
     const size_t off_e = off_w + wptr.size() * sizeof(void *);
     const size_t off_i = align256(off_e + wedge.size() * sizeof(uint32_t));
@@ -1521,8 +1593,11 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     char * h = static_cast<char *>(ctx->stage[k].h);
     char * dbase = reinterpret_cast<char *>(ctx->d_stage);
     uint32_t * hq = reinterpret_cast<uint32_t *>(h);
-    std::memset(hq, 0, gr::MAX_QUEUES * sizeof(uint32_t));
-    std::memcpy(hq + gr::MAX_QUEUES, ctx->qstart.data(), (gr::MAX_QUEUES + 1) * sizeof(uint32_t));
+    std::memset(hq, 0, qwords * sizeof(uint32_t));
+    std::memcpy(hq + size_t(nq) * gr::QUEUE_STRIDE, ctx->qstart.data(), (nq + 1) * sizeof(uint32_t));
+    uint32_t * hmask = hq + size_t(nq) * gr::QUEUE_STRIDE + nq + 1;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (ctx->qstart[q + 1] > ctx->qstart[q]) hmask[q >> 5] |= 1u << (q & 31u);
     for (uint32_t i = 0; i < nspans; ++i)
         if (span_nwin[i] > 1 && span_steps[i])
         {
@@ -1551,8 +1626,8 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     uint32_t * dq = reinterpret_cast<uint32_t *>(dbase);
     GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
-    if (nitems && gr::launch_batch(dparams, ctx->d_items, dq + gr::MAX_QUEUES, nq, dq, grid, block,
-                                   ctx->batch_waves, s) != 0)
+    if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
+                                   block, ctx->batch_waves, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if (nitems && (rc = kt_mark(ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(ctx->bev1, s));
@@ -1764,11 +1839,13 @@ extern "C" int gr_job_last_advance_ms(gr_job * j, float * ms)
 // computes identical results, so device times compare directly. The context's work counter and
 // kernel-time pool are left as they were.
 extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
-                                 uint32_t steps, uint32_t trials, gr_tune_entry * results,
+                                 uint32_t njobs, uint32_t steps, uint32_t trials, gr_tune_entry * results,
                                  uint32_t max_results, uint32_t * nresults)
 {
     if (!ctx || !wind || !desc) return set_err(GR_E_INVALID, "gr_batch_autotune: null argument");
     if (n == 0 || steps == 0) return set_err(GR_E_INVALID, "gr_batch_autotune: n and steps must be > 0");
+    if (njobs == 0) njobs = 1;
+    if (njobs > 256) return set_err(GR_E_INVALID, "gr_batch_autotune: at most 256 probe jobs");
     if (trials == 0) trials = 2;
     GR_HIP(hipSetDevice(ctx->device));
     int rc;
@@ -1776,57 +1853,72 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
     if ((rc = kt_fold(ctx)) != GR_OK) return rc;
     const double kms_saved = ctx->kms_acc;
     const uint64_t kl_saved = ctx->klaunch_acc;
-    unsigned long long work_saved = 0;
-    GR_HIP(hipMemcpyAsync(&work_saved, ctx->dcount + 1, sizeof(work_saved), hipMemcpyDeviceToHost, ctx->compute));
+    std::vector<unsigned long long> work_saved(size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE);
+    GR_HIP(hipMemcpyAsync(work_saved.data(), ctx->dwork, work_saved.size() * 8, hipMemcpyDeviceToHost, ctx->compute));
     GR_HIP(hipStreamSynchronize(ctx->compute));
-    const uint32_t block0 = ctx->batch_block, waves0 = ctx->batch_waves;
+    const uint32_t block0 = ctx->batch_block, waves0 = ctx->batch_waves, group0 = ctx->batch_group;
 
-    // probe seeds: a deterministic low-discrepancy spread over the domain (inset by 1e-3)
+    // probe seeds of each job: deterministic low-discrepancy spreads over the domain (inset by
+    // 1e-3), a different phase per job
     const uint32_t dim = desc->dim;
-    std::vector<float> seeds(size_t(n) * dim);
+    std::vector<std::vector<float>> seeds(njobs, std::vector<float>(size_t(n) * dim));
     const double alpha[3] = {0.7548776662466927, 0.5698402909980532, 0.4142135623730950};
-    for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t d = 0; d < dim; ++d)
-        {
-            double u = std::fmod(0.5 + alpha[d] * double(i + 1), 1.0);
-            u = 1e-3 + u * (1.0 - 2e-3);
-            seeds[size_t(i) * dim + d] = float(desc->offset[d] + u * double(desc->domain[d]));
-        }
-    gr_job_desc d = *desc;
-    d.save_interval = desc->save_interval;
+    for (uint32_t jb = 0; jb < njobs; ++jb)
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t d = 0; d < dim; ++d)
+            {
+                double u = std::fmod(0.5 + alpha[d] * (double(i + 1) + 0.3819660112501051 * jb * double(n)), 1.0);
+                u = 1e-3 + u * (1.0 - 2e-3);
+                seeds[jb][size_t(i) * dim + d] = float(desc->offset[d] + u * double(desc->domain[d]));
+            }
 
-    struct Cand { uint32_t block, waves; };
-    const Cand cands[] = {{64, 0}, {128, 0}, {256, 0}, {64, 4}, {128, 4}, {256, 4}};
-    const uint32_t ncand = sizeof(cands) / sizeof(cands[0]);
+    struct Cand { uint32_t block, waves, group; };
+    std::vector<Cand> cands;
+    const uint32_t gx = std::max(1u, ctx->ncu / std::max(1u, ctx->nxcd));
+    std::vector<uint32_t> groups{gx};
+    if (gx > 8) groups.push_back(8);
+    if (gx > 1) groups.push_back(1);
+    for (uint32_t g : groups)
+        for (uint32_t w : {0u, 4u})
+            for (uint32_t b : {64u, 128u, 256u}) cands.push_back(Cand{b, w, g});
     float best = 3.4e38f;
-    uint32_t best_i = 0;
+    size_t best_i = 0;
     uint32_t nres = 0;
-    for (uint32_t ci = 0; ci < ncand; ++ci)
+    std::vector<gr_job *> jobs(njobs, nullptr);
+    std::vector<gr_span> sp(njobs);
+    for (size_t ci = 0; ci < cands.size() && rc == GR_OK; ++ci)
     {
         float tmin = 3.4e38f;
-        for (uint32_t t = 0; t < trials; ++t)
+        for (uint32_t t = 0; t < trials && rc == GR_OK; ++t)
         {
-            gr_job * j = nullptr;
-            if ((rc = gr_job_create(ctx, &d, seeds.data(), n, steps, &j)) != GR_OK) break;
-            if (n >= 4096 && (rc = gr_job_reorder(j)) != GR_OK)
+            for (uint32_t jb = 0; jb < njobs && rc == GR_OK; ++jb)
+                rc = gr_job_create(ctx, desc, seeds[jb].data(), n, steps, &jobs[jb]);
+            if (rc == GR_OK && n >= 4096) rc = gr_jobs_reorder(jobs.data(), njobs);
+            if (rc == GR_OK)
             {
-                gr_job_destroy(j);
-                break;
+                ctx->batch_block = cands[ci].block;
+                ctx->batch_waves = cands[ci].waves;
+                ctx->batch_group = cands[ci].group;
+                for (uint32_t jb = 0; jb < njobs; ++jb)
+                {
+                    std::memset(&sp[jb], 0, sizeof(gr_span));
+                    sp[jb].job = jobs[jb];
+                    sp[jb].wind = wind;
+                    sp[jb].max_steps = steps;
+                }
+                float ms = 0.f;
+                rc = gr_batch_run(ctx, sp.data(), njobs, 0, nullptr, &ms);
+                if (rc == GR_OK) tmin = std::min(tmin, ms);
             }
-            ctx->batch_block = cands[ci].block;
-            ctx->batch_waves = cands[ci].waves;
-            gr_span sp{};
-            sp.job = j;
-            sp.wind = wind;
-            sp.max_steps = steps;
-            float ms = 0.f;
-            rc = gr_batch_run(ctx, &sp, 1, 0, nullptr, &ms);
-            gr_job_destroy(j);
-            if (rc != GR_OK) break;
-            tmin = std::min(tmin, ms);
+            for (gr_job *& j : jobs)
+            {
+                if (j) gr_job_destroy(j);
+                j = nullptr;
+            }
         }
         if (rc != GR_OK) break;
-        if (results && nres < max_results) results[nres++] = gr_tune_entry{cands[ci].block, cands[ci].waves, tmin};
+        if (results && nres < max_results)
+            results[nres++] = gr_tune_entry{cands[ci].block, cands[ci].waves, cands[ci].group, tmin};
         if (tmin < best)
         {
             best = tmin;
@@ -1837,17 +1929,19 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
     int rc2 = kt_fold(ctx);
     ctx->kms_acc = kms_saved;
     ctx->klaunch_acc = kl_saved;
-    if (hipMemcpyAsync(ctx->dcount + 1, &work_saved, sizeof(work_saved), hipMemcpyHostToDevice, ctx->compute) != hipSuccess ||
+    if (hipMemcpyAsync(ctx->dwork, work_saved.data(), work_saved.size() * 8, hipMemcpyHostToDevice, ctx->compute) != hipSuccess ||
         hipStreamSynchronize(ctx->compute) != hipSuccess)
         rc2 = set_err(GR_E_HIP, "gr_batch_autotune: restoring the work counter failed");
     if (rc != GR_OK)
     {
         ctx->batch_block = block0;
         ctx->batch_waves = waves0;
+        ctx->batch_group = group0;
         return rc;
     }
     ctx->batch_block = cands[best_i].block;
     ctx->batch_waves = cands[best_i].waves;
+    ctx->batch_group = cands[best_i].group;
     if (nresults) *nresults = nres;
     return rc2;
 }

@@ -92,6 +92,9 @@ struct StepParams
     uint32_t refmask; // reference-vector slots read by the programs (bit per slot)
 
     float dt, sphere_radius, max_length;
+    // curvature divisors 2 * dt and dt * dt (FP32 products) with their correctly rounded
+    // reciprocals; a reciprocal of 0 selects plain division
+    float c2dt, r2dt, cdt2, rdt2;
     float h[3], inv_h[3], off[3], dom[3];
     double off64[3];
 
@@ -165,14 +168,28 @@ struct BatchItem
     uint32_t count;
     uint32_t variant; // batch_variant(dim, integrator, dtype)
 };
-enum : uint32_t { BATCH_ITEM = 64, MAX_QUEUES = 16 };
+// BATCH_ITEM: particles per item (one wavefront). Queue data of a launch (qdata): nq counters
+// QUEUE_STRIDE words apart (zero at launch), nq + 1 queue starts, ceil(nq / 32) mask words (bit
+// set for every queue holding items). The work counter is WORK_SLOTS 64-bit slots WORK_STRIDE
+// apart (summed by the host). CU_KEY_BITS: width of cu_key() values (the CU lookup table size).
+enum : uint32_t
+{
+    BATCH_ITEM = 64,
+    MAX_QUEUES = 1024,
+    QUEUE_STRIDE = 16,
+    WORK_SLOTS = 256,
+    WORK_STRIDE = 8,
+    CU_KEY_BITS = 11
+};
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype);
-// items are split into nq queues, queue q = items[qbegin[q] .. qbegin[q + 1]); counters[q] must
-// be zero at launch
+// culut[cu_key] = dense index of that CU (0xffffffff: none); a CU's queue is index / group;
+// work: the work counter slots
 // waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD)
-int launch_batch(const StepParams * jobs, const BatchItem * items, const uint32_t * qbegin,
-                 uint32_t nq, uint32_t * counters, uint32_t grid, uint32_t block, uint32_t waves,
-                 void * stream);
+int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
+                 const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
+                 uint32_t block, uint32_t waves, void * stream);
+// writes cu_key() of the CU running each of n workgroups
+int launch_cu_probe(uint32_t * out, uint32_t n, void * stream);
 int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
 
 // launchers (gr_kernels.hip)

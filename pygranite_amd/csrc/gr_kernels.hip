@@ -125,6 +125,42 @@ __device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint6
 // --------------------------------------------------------------------------------------------
 __device__ __forceinline__ float lerp(float a, float b, float t) { return a + t * (b - a); }
 
+// x / c correctly rounded, for a wave-uniform divisor c with r = RN(1 / c) (0: plain division).
+// With |x| in [1e-30, 1e30] and |c| in [1e-6, 1e6] nothing under- or overflows, and one FMA
+// correction of q0 = RN(x r) gives RN(x / c) (Markstein's theorem: r correctly rounded, q0
+// faithful, the residual x - c q0 exact in one FMA). Other x (zero included) take the division.
+__device__ __forceinline__ float div_rc(float x, float c, float r)
+{
+    const float ax = fabsf(x);
+    if (r != 0.f && ax >= 1e-30f && ax <= 1e30f)
+    {
+        const float q0 = x * r;
+        return __builtin_fmaf(__builtin_fmaf(-c, q0, x), r, q0);
+    }
+    return x / c;
+}
+__device__ __forceinline__ double div_rc(double x, double c, double r)
+{
+    const double ax = fabs(x);
+    if (r != 0.0 && ax >= 1e-200 && ax <= 1e200)
+    {
+        const double q0 = x * r;
+        return __builtin_fma(__builtin_fma(-c, q0, x), r, q0);
+    }
+    return x / c;
+}
+// the reciprocal div_rc wants for divisor c (0 outside the range it covers)
+__device__ __forceinline__ float recip_rc(float c)
+{
+    const float a = fabsf(c);
+    return (a >= 1e-6f && a <= 1e6f) ? 1.f / c : 0.f;
+}
+__device__ __forceinline__ double recip_rc(double c)
+{
+    const double a = fabs(c);
+    return (a >= 1e-50 && a <= 1e50) ? 1.0 / c : 0.0;
+}
+
 // texel coordinate -> (index, weight). Clamping to [-1, n] leaves every tap outside the grid
 // and keeps the conversion defined (NaN -> -1).
 __device__ __forceinline__ void split(float a, uint32_t n, uint32_t wmode, int & i, float & w)
@@ -430,6 +466,8 @@ struct Win
     const void * pair;
     float tb, tf;
     double tb64, tf64;
+    float rw;    // recip_rc(tf - tb)
+    double rw64; // recip_rc(tf64 - tb64)
 };
 
 // Texel coordinate of height zr (relative to the origin) on a stretched z axis:
@@ -550,11 +588,11 @@ __device__ __forceinline__ float unit_t(const StepParams & P, const Win & W, flo
 {
     if (P.flags & F_TIME64)
     {
-        const double x = (time64 + double(add) - W.tb64) / (W.tf64 - W.tb64);
+        const double x = div_rc(time64 + double(add) - W.tb64, W.tf64 - W.tb64, W.rw64);
         return float(x);
     }
     const float x = time32 + add;
-    return (x - W.tb) / (W.tf - W.tb);
+    return div_rc(x - W.tb, W.tf - W.tb, W.rw);
 }
 
 // 2D loop borders (TrajectoryIntegrator.tpp:20-96); returns true on Domain abort.
@@ -612,7 +650,9 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     const uint32_t oid = (FULL && P.orig) ? GR_LOAD(uint32_t, P.orig, pid) : pid; // caller-order index
 
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
-    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
+    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64, 0.f, 0.0};
+    W.rw = recip_rc(W.tf - W.tb);
+    W.rw64 = recip_rc(W.tf64 - W.tb64);
     uint32_t wi = 0;
     uint32_t edge = P.nwin > 1 ? G(P.win_edge)[0] : 0xffffffffu;
 
@@ -671,6 +711,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             W.tf = float(c + 1) * P.dtd;
             W.tb64 = double(c) * double(P.dtd);
             W.tf64 = double(c + 1) * double(P.dtd);
+            W.rw = recip_rc(W.tf - W.tb);
+            W.rw64 = recip_rc(W.tf64 - W.tb64);
             edge = wi + 1 < P.nwin ? G(P.win_edge)[wi] : 0xffffffffu;
         }
         const bool live = status == GR_STATUS_TIME; // alive at the start of step s
@@ -861,8 +903,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 #pragma unroll
                         for (int d = 0; d < 3; ++d)
                         {
-                            d1[d] = (np[d] - q[d]) / (2.f * dt);
-                            d2[d] = ((q[d] + np[d]) - 2.f * p[d]) / (dt * dt);
+                            d1[d] = div_rc(np[d] - q[d], P.c2dt, P.r2dt);             // / (2 dt)
+                            d2[d] = div_rc((q[d] + np[d]) - 2.f * p[d], P.cdt2, P.rdt2); // / (dt dt)
                         }
                         const float ldp = sqrtf((d1[0] * d1[0] + d1[1] * d1[1]) + d1[2] * d1[2]);
                         const float ldp3 = (ldp * ldp) * ldp;
@@ -1027,7 +1069,8 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nx
     const uint32_t pid = b * blockDim.x + threadIdx.x;
     uint32_t w = 0;
     if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE>(P, pid);
-    if (P.work) wave_add_work(P.work, w); // block sizes are multiples of 64
+    if (P.work) // block sizes are multiples of 64
+        wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
 }
 
 // Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
@@ -1041,37 +1084,68 @@ __device__ __forceinline__ uint32_t run_item(const StepParams & P, uint32_t pid)
     return advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
 }
 
+// Identity of the compute unit running the calling wavefront: XCC, shader engine, shader array
+// and CU fields of the HW_ID / XCC_ID hardware registers packed into CU_KEY_BITS bits.
+__device__ __forceinline__ uint32_t cu_key()
+{
+    const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4); // HW_REG_HW_ID, bits 31:0
+    const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20); // HW_REG_XCC_ID, bits 3:0
+    return ((xcc & 7u) << 8) | (((hw >> 13) & 7u) << 5) | (((hw >> 12) & 1u) << 4) | ((hw >> 8) & 15u);
+}
+
+__global__ void k_cu_probe(uint32_t * out, uint32_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = cu_key();
+}
+
+int launch_cu_probe(uint32_t * out, uint32_t n, void * stream)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_cu_probe, dim3(n), dim3(64), 0, hipStream_t(stream), out, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Items are one wavefront of particles and every wavefront takes its own: a wavefront whose
-// particles are all dead moves on at once instead of waiting at a workgroup barrier. The item
-// list is split into one queue per XCD (queue q = items [qbegin[q], qbegin[q + 1])), each a
-// compact region of the Morton-ordered particle sets; wavefronts drain their own XCD's queue
-// first, then the following ones. WAVES is the minimum number of wavefronts per SIMD the
-// register allocation must allow (1: no limit; 4: at most 128 VGPRs, spilling if needed).
+// particles are all dead moves on at once instead of waiting at a workgroup barrier. There is
+// one queue per group of `group` compute units (queue q = items [qbegin[q], qbegin[q + 1]), a
+// compact region of the Morton-ordered particle sets, the same region for every job); the
+// wavefronts of a group drain its queue first, so the group's caches hold the field tiles of
+// one region (one group per XCD: its L2; one per CU: its L1). A drained queue
+// clears its bit in masks[q / 32]; wavefronts then move to the nearest queue whose bit is still
+// set, and exit when no bit is left. Every wavefront accumulates its integrated particle-steps
+// and adds them once, into one of WORK_SLOTS counters. WAVES is the minimum number of wavefronts
+// per SIMD the register allocation must allow (1: no limit; 4: at most 128 VGPRs).
 template <int WAVES>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ items,
-        const uint32_t * __restrict__ qbegin, uint32_t nq, uint32_t * counters)
+        uint32_t * qdata, uint32_t nq, const uint32_t * __restrict__ culut, uint32_t group,
+        unsigned long long * work)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t home = blockIdx.x % nq;
-    for (uint32_t k = 0; k < nq; ++k)
+    uint32_t * counters = qdata;                                  // nq x QUEUE_STRIDE
+    const uint32_t * qbegin = qdata + size_t(nq) * QUEUE_STRIDE;  // nq + 1
+    uint32_t * masks = qdata + size_t(nq) * QUEUE_STRIDE + nq + 1; // ceil(nq / 32)
+    const uint32_t ng = (nq + 31) / 32;
+    uint32_t home = culut ? culut[cu_key()] : 0xffffffffu;
+    home = home != 0xffffffffu ? home / group : nq;
+    if (home >= nq) home = blockIdx.x % nq;
+    uint32_t q = home;
+    uint32_t acc = 0;
+    for (;;)
     {
-        const uint32_t q = home + k < nq ? home + k : home + k - nq;
-        const uint32_t q0 = qbegin[q], q1 = qbegin[q + 1];
-        for (;;)
+        uint32_t v = 0;
+        if (lane == 0) v = atomicAdd(&counters[size_t(q) * QUEUE_STRIDE], 1u);
+        // every lane is active here, so the first active lane is lane 0; the result is
+        // wavefront-uniform (a scalar register), which keeps the item and job loads scalar
+        const uint32_t it = qbegin[q] + __builtin_amdgcn_readfirstlane(v);
+        if (it < qbegin[q + 1])
         {
-            uint32_t v = 0;
-            if (lane == 0) v = atomicAdd(&counters[q], 1u);
-            // every lane is active here, so the first active lane is lane 0; the result is
-            // wavefront-uniform (a scalar register), which keeps the item and job loads scalar
-            const uint32_t it = q0 + __builtin_amdgcn_readfirstlane(v);
-            if (it >= q1) break; // this queue is drained (the counter only grows)
             const BatchItem b = items[it];
             const uint32_t pid = b.first + lane;
             const StepParams & P = jobs[b.job];
-            uint32_t w = 0;
             if (lane < b.count)
             {
+                uint32_t w;
                 switch (b.variant)
                 {
                     case 0: w = run_item<2, GR_EULER, float>(P, pid); break;
@@ -1087,10 +1161,37 @@ k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ ite
                     case 10: w = run_item<3, GR_RK4, __half>(P, pid); break;
                     default: w = run_item<3, GR_RK4, bf16_t>(P, pid); break;
                 }
+                acc += w;
             }
-            if (P.work) wave_add_work(P.work, w);
+            continue;
         }
+        // queue q is drained (its counter only grows): clear its bit, find the nearest queue
+        // that still has one, starting in q's own mask word
+        if (lane == 0) atomicAnd(&masks[q >> 5], ~(1u << (q & 31u)));
+        uint32_t next = 0xffffffffu;
+        for (uint32_t k = 0; k < ng; ++k)
+        {
+            const uint32_t g = (q >> 5) + k < ng ? (q >> 5) + k : (q >> 5) + k - ng;
+            uint32_t m = 0;
+            if (lane == 0) m = __hip_atomic_load(&masks[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            m = __builtin_amdgcn_readfirstlane(m);
+            if (!m) continue;
+            uint32_t bit;
+            if (k == 0)
+            {
+                const uint32_t sh = q & 31u; // first set bit at or after q's position, cyclically
+                const uint32_t rot = sh ? ((m >> sh) | (m << (32u - sh))) : m;
+                bit = (uint32_t(__builtin_ctz(rot)) + sh) & 31u;
+            }
+            else
+                bit = uint32_t(__builtin_ctz(m));
+            next = g * 32u + bit;
+            break;
+        }
+        if (next >= nq) break; // every queue is drained
+        q = next;
     }
+    if (work) wave_add_work(work + size_t(home % WORK_SLOTS) * WORK_STRIDE, acc);
 }
 
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
@@ -1098,17 +1199,17 @@ int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
     return int((dim == 3 ? 6 : 0) + (integrator == GR_RK4 ? 3 : 0) + (dtype > 2 ? 0 : dtype));
 }
 
-int launch_batch(const StepParams * jobs, const BatchItem * items, const uint32_t * qbegin,
-                 uint32_t nq, uint32_t * counters, uint32_t grid, uint32_t block, uint32_t waves,
-                 void * stream)
+int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
+                 const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
+                 uint32_t block, uint32_t waves, void * stream)
 {
-    if (nq == 0 || grid == 0) return 0;
+    if (nq == 0 || grid == 0 || group == 0) return 0;
     if (waves >= 4)
         hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                           qbegin, nq, counters);
+                           qdata, nq, culut, group, work);
     else
         hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                           qbegin, nq, counters);
+                           qdata, nq, culut, group, work);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

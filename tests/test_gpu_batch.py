@@ -137,8 +137,8 @@ def test_batch_equals_single_job_advances(gpu_available):
     ctx.close()
 
 
-@pytest.mark.parametrize("block,waves", [(64, 0), (128, 4), (256, 4), (192, 0)])
-def test_batch_launch_options_change_nothing(gpu_available, block, waves):
+@pytest.mark.parametrize("block,waves,group", [(64, 0, 1), (128, 4, 8), (256, 4, 0), (192, 0, 3)])
+def test_batch_launch_options_change_nothing(gpu_available, block, waves, group):
     """Workgroup size and register budget of the batched kernel leave every result unchanged,
     and the device work counter equals the integrated particle-steps implied by the final
     status and abort step of every particle."""
@@ -161,6 +161,8 @@ def test_batch_launch_options_change_nothing(gpu_available, block, waves):
         if opts:
             ctx.set_option(capi.CTX_BATCH_BLOCK, opts[0])
             ctx.set_option(capi.CTX_BATCH_WAVES, opts[1])
+            if opts[2]:
+                ctx.set_option(capi.CTX_BATCH_GROUP, opts[2])
         a = capi.Job(ctx, d3, s3)
         a.reorder()
         b = capi.Job(ctx, d2, s2)
@@ -177,7 +179,7 @@ def test_batch_launch_options_change_nothing(gpu_available, block, waves):
 
     steps0, work0, ref = run(None)
     assert ctx.get_option(capi.CTX_BATCH_BLOCK) == 256 and ctx.get_option(capi.CTX_BATCH_WAVES) == 0
-    steps1, work1, got = run((block, waves))
+    steps1, work1, got = run((block, waves, group))
     assert steps0 == steps1 == [640, 333]
     for r, g in zip(ref, got):
         for x, y in zip(r, g):
@@ -191,6 +193,8 @@ def test_batch_launch_options_change_nothing(gpu_available, block, waves):
         ctx.set_option(capi.CTX_BATCH_BLOCK, 96)
     with pytest.raises(capi.GraniteError):
         ctx.set_option(capi.CTX_BATCH_WAVES, 3)
+    with pytest.raises(capi.GraniteError):
+        ctx.set_option(capi.CTX_BATCH_GROUP, 0)
     f.close()
     f2.close()
     ctx.close()
@@ -232,11 +236,13 @@ def test_autotune_picks_a_candidate_and_leaves_counters(gpu_available):
     a = capi.Job(ctx, d, seeds)
     ctx.batch_run([(a, f, dict(max_steps=100))])
     ctx.synchronize()
-    res = ctx.autotune(f, d, n=8192, steps=40, trials=1)
-    assert len(res) == 6 and all(r["ms"] > 0 for r in res)
+    res = ctx.autotune(f, d, n=8192, njobs=3, steps=40, trials=1)
+    assert len(res) >= 6 and all(r["ms"] > 0 for r in res)
+    assert {(r["block"], r["waves"]) for r in res} == {(b_, w_) for b_ in (64, 128, 256) for w_ in (0, 4)}
     best = min(res, key=lambda r: r["ms"])
     assert ctx.get_option(capi.CTX_BATCH_BLOCK) == best["block"]
     assert ctx.get_option(capi.CTX_BATCH_WAVES) == best["waves"]
+    assert ctx.get_option(capi.CTX_BATCH_GROUP) == best["group"]
     # the probe runs did not count: only job a's 100 steps are in the counters
     status, ab = a.read(capi.OUT_STATUS), a.read(capi.OUT_ABORT_STEP)
     assert ctx.take_work() == int(np.where(status == 0, 100, ab.astype(np.int64) + 1).sum())
