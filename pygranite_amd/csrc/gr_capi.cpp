@@ -1138,15 +1138,6 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
     std::memset(&P, 0, sizeof(P));
     P.n = uint32_t(j->n);
     P.work = j->ctx->dwork;
-    {
-        // curvature divisors as the step computes them (FP32) and their reciprocals
-        const float dtf = d.dt;
-        P.c2dt = 2.f * dtf;
-        P.cdt2 = dtf * dtf;
-        auto rc_ = [](float c) { return (std::fabs(c) >= 1e-6f && std::fabs(c) <= 1e6f) ? 1.f / c : 0.f; };
-        P.r2dt = rc_(P.c2dt);
-        P.rdt2 = rc_(P.cdt2);
-    }
     P.nsteps = w->nsteps;
     uint32_t fl = 0;
     const bool dyn = d.windfield_mode == 1 || d.uplift_mode == 2 || d.constants_mode == 2 || d.volume_mode == 2;

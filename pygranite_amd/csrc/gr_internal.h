@@ -92,9 +92,6 @@ struct StepParams
     uint32_t refmask; // reference-vector slots read by the programs (bit per slot)
 
     float dt, sphere_radius, max_length;
-    // curvature divisors 2 * dt and dt * dt (FP32 products) with their correctly rounded
-    // reciprocals; a reciprocal of 0 selects plain division
-    float c2dt, r2dt, cdt2, rdt2;
     float h[3], inv_h[3], off[3], dom[3];
     double off64[3];
 

@@ -125,42 +125,6 @@ __device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint6
 // --------------------------------------------------------------------------------------------
 __device__ __forceinline__ float lerp(float a, float b, float t) { return a + t * (b - a); }
 
-// x / c correctly rounded, for a wave-uniform divisor c with r = RN(1 / c) (0: plain division).
-// With |x| in [1e-30, 1e30] and |c| in [1e-6, 1e6] nothing under- or overflows, and one FMA
-// correction of q0 = RN(x r) gives RN(x / c) (Markstein's theorem: r correctly rounded, q0
-// faithful, the residual x - c q0 exact in one FMA). Other x (zero included) take the division.
-__device__ __forceinline__ float div_rc(float x, float c, float r)
-{
-    const float ax = fabsf(x);
-    if (r != 0.f && ax >= 1e-30f && ax <= 1e30f)
-    {
-        const float q0 = x * r;
-        return __builtin_fmaf(__builtin_fmaf(-c, q0, x), r, q0);
-    }
-    return x / c;
-}
-__device__ __forceinline__ double div_rc(double x, double c, double r)
-{
-    const double ax = fabs(x);
-    if (r != 0.0 && ax >= 1e-200 && ax <= 1e200)
-    {
-        const double q0 = x * r;
-        return __builtin_fma(__builtin_fma(-c, q0, x), r, q0);
-    }
-    return x / c;
-}
-// the reciprocal div_rc wants for divisor c (0 outside the range it covers)
-__device__ __forceinline__ float recip_rc(float c)
-{
-    const float a = fabsf(c);
-    return (a >= 1e-6f && a <= 1e6f) ? 1.f / c : 0.f;
-}
-__device__ __forceinline__ double recip_rc(double c)
-{
-    const double a = fabs(c);
-    return (a >= 1e-50 && a <= 1e50) ? 1.0 / c : 0.0;
-}
-
 // texel coordinate -> (index, weight). Clamping to [-1, n] leaves every tap outside the grid
 // and keeps the conversion defined (NaN -> -1).
 __device__ __forceinline__ void split(float a, uint32_t n, uint32_t wmode, int & i, float & w)
@@ -466,8 +430,6 @@ struct Win
     const void * pair;
     float tb, tf;
     double tb64, tf64;
-    float rw;    // recip_rc(tf - tb)
-    double rw64; // recip_rc(tf64 - tb64)
 };
 
 // Texel coordinate of height zr (relative to the origin) on a stretched z axis:
@@ -588,11 +550,11 @@ __device__ __forceinline__ float unit_t(const StepParams & P, const Win & W, flo
 {
     if (P.flags & F_TIME64)
     {
-        const double x = div_rc(time64 + double(add) - W.tb64, W.tf64 - W.tb64, W.rw64);
+        const double x = (time64 + double(add) - W.tb64) / (W.tf64 - W.tb64);
         return float(x);
     }
     const float x = time32 + add;
-    return div_rc(x - W.tb, W.tf - W.tb, W.rw);
+    return (x - W.tb) / (W.tf - W.tb);
 }
 
 // 2D loop borders (TrajectoryIntegrator.tpp:20-96); returns true on Domain abort.
@@ -650,9 +612,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     const uint32_t oid = (FULL && P.orig) ? GR_LOAD(uint32_t, P.orig, pid) : pid; // caller-order index
 
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
-    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64, 0.f, 0.0};
-    W.rw = recip_rc(W.tf - W.tb);
-    W.rw64 = recip_rc(W.tf64 - W.tb64);
+    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
     uint32_t wi = 0;
     uint32_t edge = P.nwin > 1 ? G(P.win_edge)[0] : 0xffffffffu;
 
@@ -711,8 +671,6 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             W.tf = float(c + 1) * P.dtd;
             W.tb64 = double(c) * double(P.dtd);
             W.tf64 = double(c + 1) * double(P.dtd);
-            W.rw = recip_rc(W.tf - W.tb);
-            W.rw64 = recip_rc(W.tf64 - W.tb64);
             edge = wi + 1 < P.nwin ? G(P.win_edge)[wi] : 0xffffffffu;
         }
         const bool live = status == GR_STATUS_TIME; // alive at the start of step s
@@ -903,8 +861,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 #pragma unroll
                         for (int d = 0; d < 3; ++d)
                         {
-                            d1[d] = div_rc(np[d] - q[d], P.c2dt, P.r2dt);             // / (2 dt)
-                            d2[d] = div_rc((q[d] + np[d]) - 2.f * p[d], P.cdt2, P.rdt2); // / (dt dt)
+                            d1[d] = (np[d] - q[d]) / (2.f * dt);
+                            d2[d] = ((q[d] + np[d]) - 2.f * p[d]) / (dt * dt);
                         }
                         const float ldp = sqrtf((d1[0] * d1[0] + d1[1] * d1[1]) + d1[2] * d1[2]);
                         const float ldp3 = (ldp * ldp) * ldp;

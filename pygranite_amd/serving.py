@@ -71,15 +71,19 @@ class _Active:
         self.bytes = nbytes
 
 
-def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0) -> int:
-    """Device bytes of a job's state (positions, scratch, status, abort step, sort buffers,
-    curvature and expression state); no saved columns."""
-    per = 4 * (dim * 3 + 2 + 5)
+def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0, reorder: bool = True) -> int:
+    """Device bytes of a job's state (positions, scratch, status, abort step, index map,
+    curvature and expression state, and the shadow copy of every state array a reorder
+    gathers into); no saved columns."""
+    state = dim + 2                     # positions, status, abort step
     if curvature in (1, 3):
-        per += 4 * (dim + 2)
+        state += dim + 2                # previous position, total and individual curvature
     elif curvature == 2:
-        per += 4 * (dim + 1)
-    per += 4 * nprog
+        state += dim + 1
+    state += nprog
+    per = 4 * (state + 2 * dim + 1)     # + seed / output scratch + index map
+    if reorder:
+        per += 4 * state                # shadow copies
     return int(n) * per + (1 << 16)
 
 
@@ -133,7 +137,8 @@ class Server:
     def admit(self):
         while self.queue and self.queue[0].arrival <= self.now():
             req = self.queue[0]
-            nb = job_bytes(req.seeds.shape[0], 3, req.curvature, len(req.exprs))
+            nb = job_bytes(req.seeds.shape[0], 3, req.curvature, len(req.exprs),
+                           self.reorder and req.seeds.shape[0] >= 4096)
             if self.in_use + nb > self.hbm_budget and self.active:
                 break  # wait for memory (FIFO: no overtaking)
             self.queue.popleft()
@@ -151,9 +156,8 @@ class Server:
                           dict(first_window=a.window, max_steps=min(left, self.quantum),
                                time_start=a.time, time_start64=a.time64,
                                counter_start=a.counter)))
-        ends, ms = self.ctx.batch_run(spans, block=self.block, timed=True, ends=True)
+        ends, _ = self.ctx.batch_run(spans, block=self.block, timed=False, ends=True)
         self.rounds += 1
-        self.device_ms += ms
         keep = []
         for a, e in zip(self.active, ends):
             a.req.steps_done += e["steps"]
@@ -179,6 +183,7 @@ class Server:
         """Serves `requests` (sorted by arrival) in real time; returns the finished ones."""
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
+        self.ctx.take_kernel_ms()  # device time of this run's launches, summed at the end
         self.t0 = time.perf_counter()
         while self.queue or self.active:
             if deadline is not None and self.now() > deadline:
@@ -190,6 +195,9 @@ class Server:
                 wait = self.queue[0].arrival - self.now()
                 if wait > 0:
                     time.sleep(min(wait, 0.05))
+        self.ctx.synchronize()
+        ms, _ = self.ctx.take_kernel_ms()
+        self.device_ms += ms
         return self.done
 
     def report(self) -> dict:

@@ -69,7 +69,8 @@ def test_admission_respects_budget(gpu_available):
     ctx = capi.Context(0)
     ds, _ = make_datasets(ctx)
     reqs = requests()
-    biggest = max(serving.job_bytes(r.seeds.shape[0], 3, r.curvature, len(r.exprs)) for r in reqs)
+    biggest = max(serving.job_bytes(r.seeds.shape[0], 3, r.curvature, len(r.exprs), r.seeds.shape[0] >= 4096)
+                  for r in reqs)
     srv = serving.Server(ctx, ds, hbm_budget=biggest + 1, quantum=128)
     peak = [0]
     orig_round = srv.round
