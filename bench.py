@@ -213,11 +213,35 @@ def fp64_error(wind, seeds_sub, gs, steps_done, res, sub):
     if ref["steps"] != steps_done:
         return {"error": f"oracle ran {ref['steps']} steps, GPU {steps_done}"}
     p_ref = ref["trajectories"][:, -1, :]
-    same = (ref["status"] == res["status"][:sub]) & (ref["abort_step"] == res["abort_step"][:sub])
-    d = np.abs(res["positions"][:sub][same].astype(np.float64) - p_ref[same]) / (2 * math.pi)
-    return {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max()),
-            "particles": int(same.sum()), "termination_mismatch": int((~same).sum()),
-            "alive_at_end": int((ref["status"] == 0).sum()), "steps": int(steps_done)}
+    return termination_and_error(res["positions"][:sub], res["status"][:sub], res["abort_step"][:sub],
+                                 p_ref, ref["status"], ref["abort_step"], [2 * math.pi] * 3,
+                                 {"steps": int(steps_done)})
+
+
+def termination_and_error(pos, status, ab, pos_ref, status_ref, ab_ref, extent, extra=None):
+    """Domain-relative max / RMS position error over particles whose status and abort step
+    agree with the FP64 oracle, and the termination mismatches split into those inside the
+    threshold band (a stopped particle within 1e-5 of the extent from a domain edge, in either
+    result: exits and boundary stalls whose step depends on the last bits) and outside it."""
+    ext = np.asarray(extent, np.float64)
+    same = (status_ref == status) & (ab_ref == ab)
+
+    def near_edge(p, st):
+        p = np.asarray(p, np.float64)
+        m = np.min(np.minimum(np.abs(p), np.abs(ext - p)) / ext, axis=1) < 1e-5
+        return m & (st != 0)
+
+    band = near_edge(pos, status) | near_edge(pos_ref, status_ref)
+    d = np.abs(np.asarray(pos, np.float64)[same] - np.asarray(pos_ref, np.float64)[same]) / ext
+    out = {"rms": float(np.sqrt(np.mean(d * d))) if d.size else None,
+           "max": float(d.max()) if d.size else None,
+           "particles": int(same.sum()),
+           "termination_mismatch_in_band": int((~same & band).sum()),
+           "termination_mismatch_outside_band": int((~same & ~band).sum()),
+           "alive_at_end": int((status_ref == 0).sum())}
+    if extra:
+        out.update(extra)
+    return out
 
 
 def cpu_baseline(wind, gs):
@@ -412,15 +436,15 @@ def atmosphere_workload(capi, ctx, args):
         job = capi.Job(ctx, d, seeds, save_capacity=chunk)
         job.reorder()
         ctx.synchronize()
+        ctx.take_work()
+        ctx.take_kernel_ms()
         st = dict(window=0, time=0.0, time64=0.0, counter=0)
         t0 = time.perf_counter()
-        kms = []
         while st["counter"] < total_steps:
-            ends, ms = ctx.batch_run([(job, f, dict(first_window=st["window"],
-                                                    max_steps=min(chunk, total_steps - st["counter"]),
-                                                    time_start=st["time"], time_start64=st["time64"],
-                                                    counter_start=st["counter"]))], ends=True)
-            kms.append(ms)
+            ends, _ = ctx.batch_run([(job, f, dict(first_window=st["window"],
+                                                   max_steps=min(chunk, total_steps - st["counter"]),
+                                                   time_start=st["time"], time_start64=st["time64"],
+                                                   counter_start=st["counter"]))], ends=True, timed=False)
             e = ends[0]
             if e["window"] != st["window"]:
                 job.reorder()
@@ -429,11 +453,14 @@ def atmosphere_workload(capi, ctx, args):
                 break
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        out[name] = {"particle_steps_per_s": n * st["counter"] / wall, "wall_s": wall,
-                     "device_s": float(np.sum(kms)) / 1e3, "steps": st["counter"],
+        work = ctx.take_work()
+        kms, nl = ctx.take_kernel_ms()
+        out[name] = {"particle_steps_per_s": work / wall, "nominal_particle_steps_per_s": n * st["counter"] / wall,
+                     "integrated_fraction": work / (n * max(1, st["counter"])), "wall_s": wall,
+                     "device_s": kms / 1e3, "launches": nl, "steps": st["counter"],
                      "field_bytes_per_slice": f.bytes_per_slot()}
-        print(f"#   atmosphere {name}: {out[name]['particle_steps_per_s']:.3e} particle-steps/s",
-              file=sys.stderr, flush=True)
+        print(f"#   atmosphere {name}: {out[name]['particle_steps_per_s']:.3e} integrated particle-steps/s "
+              f"({out[name]['integrated_fraction']:.3f} of nominal)", file=sys.stderr, flush=True)
         finals[name] = (job.read(capi.OUT_POSITIONS), job.read(capi.OUT_STATUS),
                         job.read(capi.OUT_ABORT_STEP), st["counter"])
         job.close()
@@ -447,15 +474,166 @@ def atmosphere_workload(capi, ctx, args):
                   GridScale=list(gs), Offset=[0, 0, 0], SaveInterval=0,
                   MaximumSimulationTime=float(total_steps) * float(dt) + 1.0)
         ref = oracle.run(st, wind_h, seeds[:sub], real64=True, zlevels=zlev, time64=True)
-        ext = np.array(dom, np.float64)
         for name, (pos, status, ab, steps) in finals.items():
-            same = (ref["status"] == status[:sub]) & (ref["abort_step"] == ab[:sub])
-            d = np.abs(pos[:sub][same].astype(np.float64) - ref["trajectories"][same, -1, :]) / ext
-            out[name]["fp64_error"] = {"rms": float(np.sqrt(np.mean(d * d))), "max": float(d.max()),
-                                       "particles": int(same.sum()),
-                                       "termination_mismatch": int((~same).sum())}
+            out[name]["fp64_error"] = termination_and_error(
+                pos[:sub], status[:sub], ab[:sub], ref["trajectories"][:, -1, :], ref["status"],
+                ref["abort_step"], dom)
     out["speedup_fp16_vs_fp32"] = out["fp16"]["particle_steps_per_s"] / out["fp32"]["particle_steps_per_s"]
     print(json.dumps(out), flush=True)
+
+
+def batch_workload(capi, ctx, args):
+    """The batched configuration: 256 independent jobs in one persistent launch. Particle
+    counts log-uniform in [1e3, 4e6], step counts uniform in [100, 5000] (SplitMix64 seed 3);
+    even jobs 2D double gyre (512x256, dt 0.01), odd jobs 3D ABC (128^3, dt 0.005); Euler:RK4 =
+    1:3; FP16 storage, launch configuration from the autotuner. Both analytic fields are
+    sampled over enough slices for the longest job (51 and 26). Compared with the same jobs as
+    256 sequential plain FP32 launches (one job each, FP32 slots sampled separately, caller
+    order). Prints one JSON line."""
+    from pygranite_amd import synthetic
+    nj = args.batch_jobs
+    u = synthetic.uniform01(3, 4 * nj).reshape(nj, 4)
+    pmax, smax = args.batch_pmax, args.batch_smax
+    g2, gs2 = synthetic.double_gyre(512, 256, 51)
+    g3, gs3 = synthetic.abc_flow(128, 26)
+    specs = []
+    for i in range(nj):
+        dim = 2 if i % 2 == 0 else 3
+        integ = 0 if u[i, 0] < 0.25 else 1
+        n = int(round(math.exp(math.log(1e3) + u[i, 1] * (math.log(pmax) - math.log(1e3)))))
+        steps = int(100 + u[i, 2] * (smax - 100))
+        specs.append((dim, integ, n, steps))
+    print(f"# batch: {nj} jobs, {sum(x[2] for x in specs)} particles, "
+          f"{sum(x[2] * x[3] for x in specs):.3e} nominal particle-steps", file=sys.stderr, flush=True)
+    seeds = []
+    for i, (dim, integ, n, steps) in enumerate(specs):
+        if dim == 2:
+            seeds.append(synthetic.uniform_seeds(1000 + i, n, [0.01, 0.01], [1.99, 0.99]))
+        else:
+            seeds.append(synthetic.uniform_seeds(1000 + i, n, [0.0] * 3, [2 * math.pi] * 3))
+
+    def desc(dim, integ):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.windfield_mode, d.dtd = dim, integ, 1, 1.0
+        if dim == 2:
+            d.dt = 0.01
+            d.grid_scale[0], d.grid_scale[1], d.grid_scale[2] = gs2[0], gs2[1], gs2[0]
+            d.domain[0] = float(np.float32(512) * np.float32(gs2[0]))
+            d.domain[1] = float(np.float32(256) * np.float32(gs2[1]))
+        else:
+            d.dt = float(DT)
+            for k in range(3):
+                d.grid_scale[k] = gs3[k]
+                d.domain[k] = float(np.float32(128) * np.float32(gs3[k]))
+        return d
+
+    out = {"workload": f"{nj} jobs (particles log-uniform [1e3, {pmax:.0e}], steps uniform [100, "
+                       f"{smax}], half 2D double gyre / half 3D ABC, Euler:RK4 1:3)",
+           "nominal_particle_steps": int(sum(x[2] * x[3] for x in specs))}
+    for form in ("batched_fp16", "sequential_plain_fp32"):
+        dtype = capi.F16 if form == "batched_fp16" else capi.F32
+        f2, f3 = make_field2(capi, ctx, g2, dtype), make_field(capi, ctx, g3, dtype)
+        if form == "batched_fp16" and not args.no_autotune:
+            tune = ctx.autotune(f3, desc(3, 1), n=65536, njobs=16, steps=50, trials=1)
+            out["autotune"] = {"block": ctx.get_option(capi.CTX_BATCH_BLOCK),
+                               "waves": ctx.get_option(capi.CTX_BATCH_WAVES),
+                               "group": ctx.get_option(capi.CTX_BATCH_GROUP),
+                               "best_probe_ms": min(t["ms"] for t in tune)}
+        jobs = []
+        for (dim, integ, n, steps), sd in zip(specs, seeds):
+            j = capi.Job(ctx, desc(dim, integ), sd)
+            if form != "batched_fp16":
+                j.set_option(capi.OPT_PAIRED, 0)
+            jobs.append((j, f2 if dim == 2 else f3, steps))
+        if form == "batched_fp16":
+            ctx.reorder([j for j, _, _ in jobs if j.n >= 4096])
+        ctx.synchronize()
+        ctx.take_work()
+        ctx.take_kernel_ms()
+        t0 = time.perf_counter()
+        if form == "batched_fp16":
+            ctx.batch_run([(j, f, dict(max_steps=st)) for j, f, st in jobs], timed=False)
+        else:
+            for j, f, st in jobs:
+                ctx.batch_run([(j, f, dict(max_steps=st))], timed=False)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        work = ctx.take_work()
+        kms, nl = ctx.take_kernel_ms()
+        out[form] = {"wall_s": wall, "device_s": kms / 1e3, "launches": nl,
+                     "integrated_particle_steps": work, "integrated_particle_steps_per_s": work / wall}
+        print(f"#   {form}: {wall:.2f} s wall, {kms / 1e3:.2f} s device, {work / wall:.3e} "
+              f"integrated particle-steps/s", file=sys.stderr, flush=True)
+        for j, _, _ in jobs:
+            j.close()
+        f2.close()
+        f3.close()
+    out["speedup"] = out["sequential_plain_fp32"]["wall_s"] / out["batched_fp16"]["wall_s"]
+    print(json.dumps(out), flush=True)
+
+
+def serving_workload(capi, ctx, args):
+    """The serving configuration: Poisson arrivals (lambda jobs/s over a duration) onto four
+    resident atmosphere-like datasets (256x256x96, stretched z, 48 hourly slices, FP16, seeds
+    1-4); sizes as in the batched configuration (particles log-uniform [1e3, 4e6], steps
+    uniform [100, 5000]), RK4:Euler 3:1, dt 60 s; seeds uniform in x-y and log-uniform in height,
+    made on the device at admission. Jobs are admitted by HBM budget and advanced in rounds of
+    the persistent batched kernel. Prints one JSON line with p50/p99 latency and throughput."""
+    import torch
+    from pygranite_amd import serving, synthetic
+    dev = torch.device("cuda")
+    nx = ny = 256
+    nz, S = 96, 48
+    ds = {}
+    for k in range(4):
+        wind_t, gs, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=k + 1, device=dev)
+        f = capi.Field(ctx, nx, ny, nz, 3, dtype=capi.F16, nslots=S, address=0)
+        f.set_z_levels(zlev)
+        for sl in range(S):
+            f.upload_device(sl, [wind_t[sl, c].data_ptr() for c in range(3)])
+        ctx.synchronize()
+        del wind_t
+        dom = [float(np.float32(nx) * np.float32(gs[0])), float(np.float32(ny) * np.float32(gs[1])),
+               float(zlev[-1])]
+        ds[f"d{k}"] = serving.Dataset(f, gs, dom, 3600.0, S, zlev)
+    torch.cuda.empty_cache()
+    arr = serving.poisson_arrivals(4, args.serve_rate, args.serve_duration)
+    u = synthetic.uniform01(5, 5 * len(arr)).reshape(len(arr), 5)
+    reqs = []
+    for i, t in enumerate(arr):
+        n = int(round(math.exp(math.log(1e3) + u[i, 0] * (math.log(args.batch_pmax) - math.log(1e3)))))
+        dsn = f"d{int(u[i, 1] * 4)}"
+        dom, z1 = ds[dsn].domain, float(ds[dsn].zlevels[1])
+
+        def seed_fn(n=n, dom=dom, z1=z1, s=int(u[i, 4] * 2**31)):
+            g = torch.Generator(device=dev)
+            g.manual_seed(s)
+            r = torch.rand((n, 3), generator=g, device=dev, dtype=torch.float64)
+            xyz = torch.empty((n, 3), device=dev, dtype=torch.float32)
+            xyz[:, 0] = (r[:, 0] * dom[0]).float()
+            xyz[:, 1] = (r[:, 1] * dom[1]).float()
+            xyz[:, 2] = (z1 * torch.exp(r[:, 2] * math.log(dom[2] / z1))).float()
+            return xyz
+
+        reqs.append(serving.Request(rid=i, arrival=float(t), dataset=dsn, seeds=None, n=n,
+                                    seed_fn=seed_fn, steps=int(100 + u[i, 2] * (args.batch_smax - 100)),
+                                    integrator=0 if u[i, 3] < 0.25 else 1, dt=60.0))
+    offered = sum(r.n * r.steps for r in reqs) / args.serve_duration
+    print(f"# serving: {len(reqs)} requests over {args.serve_duration} s, offered "
+          f"{offered:.3e} nominal particle-steps/s", file=sys.stderr, flush=True)
+    srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum)
+    ctx.take_work()
+    srv.run(reqs)
+    rep = srv.report()
+    rep["integrated_particle_steps"] = ctx.take_work()
+    rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]
+    rep.update(arrival_rate_per_s=args.serve_rate, duration_s=args.serve_duration,
+               offered_nominal_particle_steps_per_s=offered,
+               workload="4 datasets 256x256x96 FP16 x 48 slices; Poisson arrivals; particles "
+                        f"log-uniform [1e3, {args.batch_pmax:.0e}], steps uniform [100, {args.batch_smax}]")
+    print(json.dumps(rep), flush=True)
+    for d in ds.values():
+        d.field.close()
 
 
 def copy_calibration():
@@ -494,13 +672,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
-    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere", "copy"],
+    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere", "copy", "batch", "serving"],
                     help="copy: a plain 1 GiB device copy (x3), the calibration run for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--atmo-nx", type=int, default=512)
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-pmax", type=float, default=4e6)
+    ap.add_argument("--batch-smax", type=int, default=5000)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-duration", type=float, default=120.0)
+    ap.add_argument("--serve-quantum", type=int, default=250)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the batched-kernel autotuner (default: tune before the batched form)")
     args = ap.parse_args()
@@ -512,6 +697,11 @@ def main():
     if args.workload == "copy":
         copy_calibration()
         return
+    if args.workload in ("atmosphere", "serving"):
+        # torch generates fields and seeds on the device and hands their pointers to the engine:
+        # its HIP runtime must be the process's runtime, so it initialises before the engine
+        import torch
+        torch.zeros(1, device="cuda")
     from pygranite_amd import capi, synthetic
     if capi.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
@@ -524,6 +714,12 @@ def main():
         ctx.set_option(capi.CTX_BATCH_WAVES, args.batch_waves)
     if args.workload == "atmosphere":
         atmosphere_workload(capi, ctx, args)
+        return
+    if args.workload == "batch":
+        batch_workload(capi, ctx, args)
+        return
+    if args.workload == "serving":
+        serving_workload(capi, ctx, args)
         return
     n = args.grid
     wind, gs = synthetic.abc_flow(n, 16)

@@ -224,9 +224,11 @@ typedef enum {
     GR_OUT_VOLUME0 = 16         /* + k: N float, latest value of volume k */
 } gr_output;
 
-/* Creates a job for `n` particles; `seeds` are n*dim host floats (AoS, the last row of the
- * reference's TrajectorySet, cpp:424-442). `save_capacity` = saved columns held on the device
- * between reads. */
+/* Creates a job for `n` particles; `seeds` are n*dim floats (AoS, the last row of the
+ * reference's TrajectorySet, cpp:424-442) in host memory or in device memory of the context's
+ * GPU (then copied on the compute stream, in order, without waiting). `save_capacity` = saved
+ * columns held on the device between reads. The state is initialised on the compute stream: a
+ * call does not wait for launches already queued there. */
 int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds, uint64_t n,
                   uint32_t save_capacity, gr_job ** out);
 int gr_job_destroy(gr_job * job);

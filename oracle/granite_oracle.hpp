@@ -274,10 +274,11 @@ public:
             inv_h_[d] = R(1) / h_[d];
             off_[d] = R(s.offset[d]);
         }
-        // Domain = grid * GridScale computed in float (tpp:526-546).
+        // Domain = grid * GridScale computed in float (tpp:526-546); on a stretched z axis
+        // (extension) the z extent is the top level.
         dom_[0] = R(float(s.nx) * s.grid_scale[0]);
         dom_[1] = R(float(s.ny) * s.grid_scale[1]);
-        dom_[2] = R(float(s.nz) * s.grid_scale[2]);
+        dom_[2] = s.zlev.empty() ? R(float(s.nz) * s.grid_scale[2]) : R(s.zlev.back());
         pos_.assign(n * 3, R(0));
         prev_.assign(n * 3, R(0));
         comp_.assign(n * 3, R(0));

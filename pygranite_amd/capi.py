@@ -347,14 +347,21 @@ class Field:
 
 
 class Job:
-    def __init__(self, ctx: Context, desc: JobDesc, seeds: np.ndarray, save_capacity: int = 1):
+    def __init__(self, ctx: Context, desc: JobDesc, seeds, save_capacity: int = 1):
+        """seeds: (n, dim) float32 numpy array, or a contiguous float32 torch tensor on the
+        context's GPU (copied on the device)."""
         self.ctx = ctx
         self.desc = desc
         self.dim = desc.dim
-        seeds = np.ascontiguousarray(seeds, dtype=np.float32)
-        self.n = seeds.shape[0]
+        if hasattr(seeds, "data_ptr") and getattr(seeds, "is_cuda", False):
+            if str(seeds.dtype) != "torch.float32" or not seeds.is_contiguous():
+                raise ValueError("device seeds must be a contiguous float32 tensor")
+            ptr, self.n = seeds.data_ptr(), int(seeds.shape[0])
+        else:
+            seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+            ptr, self.n = seeds.ctypes.data, seeds.shape[0]
         self.h = ctypes.c_void_p()
-        _check(lib().gr_job_create(ctx.h, ctypes.byref(desc), seeds.ctypes.data, self.n,
+        _check(lib().gr_job_create(ctx.h, ctypes.byref(desc), ptr, self.n,
                                    save_capacity, ctypes.byref(self.h)))
         ctx._adopt(self)
 

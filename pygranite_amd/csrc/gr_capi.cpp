@@ -826,18 +826,34 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
         rc = set_err(GR_E_HIP, "event creation failed");
     if (rc == GR_OK && n)
     {
-        // seeds: host AoS -> device AoS staging -> SoA state (optionally relative to the origin)
-        if (hipMemcpyAsync(j->scratch_aos, seeds, n * D * sizeof(float), hipMemcpyHostToDevice,
-                           ctx->compute) != hipSuccess)
-            rc = set_err(GR_E_HIP, "seed upload failed");
+        // seeds: AoS -> device AoS staging -> SoA state (optionally relative to the origin).
+        // Device seeds are copied on the compute stream (ordered, nothing waits); host seeds
+        // on the copy stream, waiting for that copy alone (not for kernels queued on the compute
+        // stream), so that the caller's buffer is free when the call returns.
+        hipPointerAttribute_t pa;
+        std::memset(&pa, 0, sizeof(pa));
+        const bool on_device = hipPointerGetAttributes(&pa, seeds) == hipSuccess &&
+                               (pa.type == hipMemoryTypeDevice || pa.type == hipMemoryTypeManaged);
+        (void)hipGetLastError();
+        const size_t bytes = n * D * sizeof(float);
+        if (on_device)
+        {
+            if (hipMemcpyAsync(j->scratch_aos, seeds, bytes, hipMemcpyDeviceToDevice, ctx->compute) != hipSuccess)
+                rc = set_err(GR_E_HIP, "seed copy failed");
+        }
+        else
+        {
+            if (hipMemcpyAsync(j->scratch_aos, seeds, bytes, hipMemcpyHostToDevice, ctx->copy) != hipSuccess ||
+                hipEventRecord(j->ev0, ctx->copy) != hipSuccess || hipEventSynchronize(j->ev0) != hipSuccess ||
+                hipStreamWaitEvent(ctx->compute, j->ev0, 0) != hipSuccess)
+                rc = set_err(GR_E_HIP, "seed upload failed");
+        }
         float * soa[3] = {j->p[0], j->p[1], D == 3 ? j->p[2] : nullptr};
         if (rc == GR_OK && gr::launch_seed(j->scratch_aos, soa, n, D, d.offset, d.local_coords, ctx->compute) != 0)
             rc = set_err(GR_E_HIP, "seed kernel failed");
         for (uint32_t k = 0; k < D && rc == GR_OK && curv_pos; ++k)
             if (hipMemcpyAsync(j->q[k], j->p[k], n * sizeof(float), hipMemcpyDeviceToDevice, ctx->compute) != hipSuccess)
                 rc = set_err(GR_E_HIP, "state init failed");
-        if (rc == GR_OK && hipStreamSynchronize(ctx->compute) != hipSuccess)
-            rc = set_err(GR_E_HIP, "gr_job_create: synchronize failed");
     }
     if (rc != GR_OK)
     {

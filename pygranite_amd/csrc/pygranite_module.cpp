@@ -104,6 +104,9 @@ struct IntegratorSettings
     bool AllowAbs{false};
     uint32_t BlockSize{0};
     bool SortParticles{true}; // Morton-reorder particle state at window edges (results unchanged)
+    // stretched z axis (3D): height of every z node relative to Offset.Z, strictly increasing
+    // from 0; empty = uniform spacing GridScale[2]. The z extent of the domain is the top level.
+    std::vector<float> ZLevels;
 };
 
 // ---- DataLoader: source/granite/DataLoader.hpp:26-144 ------------------------------------
@@ -379,6 +382,12 @@ private:
             fd.nslots = 3;
             fd.address = d == 3 ? 0 : 1; // 3D border, 2D clamp (cpp:503-506, 570-574)
             check(gr_field_create(device_ctx(), &fd, &wind_.f));
+            if (d == 3 && !s_.ZLevels.empty())
+            {
+                if (s_.ZLevels.size() != nz_)
+                    user_error("ZLevels needs one height per z node of the windfield.");
+                check(gr_field_set_z_levels(wind_.f, s_.ZLevels.data(), uint32_t(s_.ZLevels.size())));
+            }
         }
         const float * comps[3] = {w[0].data(), w[1].data(), d == 3 ? w[2].data() : nullptr};
         check(gr_field_upload(wind_.f, slot, comps, 1));
@@ -589,6 +598,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     d.domain[0] = float(nx_) * g[0];
     d.domain[1] = float(ny_) * g[1];
     d.domain[2] = float(nz_) * g[2];
+    if (d.dim == 3 && !s_.ZLevels.empty()) d.domain[2] = s_.ZLevels.back();
     d.save_interval = s_.SaveInterval;
 
     // expressions (tpp:647-712)
@@ -949,7 +959,8 @@ PYBIND11_MODULE(pygranite, m)
         .def_readwrite("LocalCoordinates", &IntegratorSettings::LocalCoordinates)
         .def_readwrite("AllowAbs", &IntegratorSettings::AllowAbs)
         .def_readwrite("BlockSize", &IntegratorSettings::BlockSize)
-        .def_readwrite("SortParticles", &IntegratorSettings::SortParticles);
+        .def_readwrite("SortParticles", &IntegratorSettings::SortParticles)
+        .def_readwrite("ZLevels", &IntegratorSettings::ZLevels);
 
     py::class_<TrajectoryIntegrator>(m, "TrajectoryIntegrator")
         .def(py::init<IntegratorSettings &, DataLoader &, TrajectorySet &>(), py::keep_alive<1, 3>())

@@ -39,13 +39,15 @@ class Request:
     rid: int
     arrival: float                  # seconds on the server clock
     dataset: str
-    seeds: np.ndarray               # (n, 3) float32
+    seeds: object                   # (n, 3) float32 numpy array or device tensor, or None ...
     steps: int                      # step budget
     integrator: int = 1             # 0 Euler, 1 RK4
     dt: float = 60.0
     curvature: int = 0
     exprs: tuple = ()
     allow_abs: bool = False
+    n: int = 0                      # ... with seed_fn(): seeds made at admission (n of them)
+    seed_fn: object = None
     # filled by the server
     admitted: float = float("nan")
     finished: float = float("nan")
@@ -56,6 +58,10 @@ class Request:
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+    @property
+    def count(self) -> int:
+        return int(self.seeds.shape[0]) if self.seeds is not None else int(self.n)
 
 
 class _Active:
@@ -129,16 +135,17 @@ class Server:
         for i, e in enumerate(req.exprs):
             d.prog[i] = capi.compile_expr(e, dim=3, curvature=req.curvature,
                                           allow_abs=req.allow_abs)
-        job = capi.Job(self.ctx, d, req.seeds)
-        if self.reorder and req.seeds.shape[0] >= 4096:
+        seeds = req.seeds if req.seeds is not None else req.seed_fn()
+        job = capi.Job(self.ctx, d, seeds)
+        if self.reorder and req.count >= 4096:
             job.reorder()
         return job
 
     def admit(self):
         while self.queue and self.queue[0].arrival <= self.now():
             req = self.queue[0]
-            nb = job_bytes(req.seeds.shape[0], 3, req.curvature, len(req.exprs),
-                           self.reorder and req.seeds.shape[0] >= 4096)
+            nb = job_bytes(req.count, 3, req.curvature, len(req.exprs),
+                           self.reorder and req.count >= 4096)
             if self.in_use + nb > self.hbm_budget and self.active:
                 break  # wait for memory (FIFO: no overtaking)
             self.queue.popleft()
@@ -161,7 +168,7 @@ class Server:
         keep = []
         for a, e in zip(self.active, ends):
             a.req.steps_done += e["steps"]
-            self.particle_steps += e["steps"] * a.req.seeds.shape[0]
+            self.particle_steps += e["steps"] * a.req.count
             a.window, a.time, a.time64, a.counter = e["window"], e["time"], e["time64"], e["counter"]
             if e["done"] or e["steps"] == 0 or a.req.steps_done >= a.req.steps:
                 self._retire(a)

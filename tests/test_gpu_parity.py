@@ -89,3 +89,29 @@ def test_precision_form_kahan_time64_bitexact(pg, gpu_available):
     gpu = run_product(pg, st, wind, seeds)
     ref = oracle.run(st, synthetic.round_fp16(wind), seeds, real64=False, time64=True, kahan=True)
     assert_bitexact(gpu, ref, label="kahan ")
+
+
+def test_stretched_z_precision_form_bitexact(pg, gpu_available):
+    """Stretched z axis (ZLevels extension) on an atmosphere-like field: FP32 and FP16 storage,
+    RK4, Kahan positions, FP64 time, the expression w*1000/(abs(u)+abs(v)+1) saved every step,
+    bit for bit against the FP32 oracle on the same levels (and the same rounded slices)."""
+    nx = ny = 48
+    nz, S = 20, 4
+    wind, gs, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=7, dx=3000.0, dz0=20.0, ratio=1.08)
+    dom = [nx * gs[0], ny * gs[1], float(zlev[-1])]
+    n = 2048
+    xy = synthetic.uniform_seeds(23, n, [0.0, 0.0], [dom[0], dom[1]])
+    z = synthetic.loguniform_heights(24, n, float(zlev[1]), float(zlev[-1]))
+    seeds = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
+    for storage, rounder in (("Float32", lambda a: a), ("Float16", synthetic.round_fp16)):
+        st = dict(Space="Space3D", Integrator="ClassicRungeKutta", DeltaT=60.0,
+                  DataTimeDistance=3600.0, GridScale=list(gs), Offset=[0.0, 0.0, 0.0],
+                  MaximumSimulationTime=150 * 60.0, SaveInterval=1, KahanSummation=True,
+                  TimeFP64=True, WindfieldStorage=storage, AllowAbs=True,
+                  AdditionalCompute=["w*1000/(abs(u)+abs(v)+1)"], ZLevels=[float(v) for v in zlev])
+        gpu = run_product(pg, st, wind, seeds)
+        ref = oracle.run(st, rounder(wind), seeds, real64=False, time64=True, kahan=True,
+                         zlevels=zlev, allow_abs=True)
+        assert_bitexact(gpu, ref, label=f"stretched z {storage} ")
+        np.testing.assert_array_equal(gpu["compute"][0], ref["compute"][0].astype(np.float32))
+        assert (ref["status"] != 0).sum() > 0 and (ref["status"] == 0).sum() > n // 2

@@ -289,3 +289,32 @@ def test_double_gyre_fp32_close_to_fp64():
     assert np.array_equal(a["status"], b["status"])
     d = np.abs(a["trajectories"] - b["trajectories"]).max()
     assert d < 1e-4
+
+
+def test_stretched_z_linear_field_closed_form():
+    """On a stretched z axis the sampler is linear in physical height inside every layer, so a
+    field w = c0 + c1 z is reproduced exactly and the FP64 RK4 path of a particle rising through
+    many unequal layers follows z(t) = (z0 + c0 / c1) exp(c1 t) - c0 / c1. A wrong layer index,
+    fraction or search direction breaks the linearity at layer edges and fails this."""
+    nz, ny, nx, S = 24, 4, 4, 2
+    levels = synthetic.stretched_levels(nz, 10.0, 1.15)
+    c0, c1 = 0.5, 0.02
+    w = (c0 + c1 * levels.astype(np.float64)).astype(np.float32)
+    wind = np.zeros((S, 3, nz, ny, nx), np.float32)
+    wind[:, 2] = w[None, :, None, None]
+    dt, steps = 0.05, 400
+    st = dict(Space=0, Integrator=1, DeltaT=dt, DataTimeDistance=1e6, GridScale=[1.0, 1.0, 1.0],
+              Offset=[0.0, 0.0, 0.0], MaximumSimulationTime=(steps - 0.5) * dt, SaveInterval=50)
+    z0 = np.array([0.3, 2.0, 17.5, 40.0], np.float64)
+    seeds = np.stack([np.full(4, 1.5), np.full(4, 1.5), z0], axis=1).astype(np.float32)
+    r = oracle.run(st, wind, seeds, real64=True, zlevels=levels)
+    assert (r["status"] == 0).all()
+    # column k is saved after step 50 k + 1 (the save precedes the time increment)
+    t = dt * (1 + 50 * np.arange(r["trajectories"].shape[1] - 1))
+    zs = np.asarray(seeds[:, 2], np.float64)
+    want = (zs[:, None] + c0 / c1) * np.exp(c1 * t[None, :]) - c0 / c1
+    got = r["trajectories"][:, 1:, 2]
+    # crossed several layers of different thickness (10 m, 11.5 m, 13.2 m, ...)
+    assert got.max() > levels[4] and got.min() < levels[1]
+    # the field holds float32 values of the line: the relative error stays at FP32 level
+    np.testing.assert_allclose(got, want, rtol=2e-6)
