@@ -114,4 +114,5 @@ def test_stretched_z_precision_form_bitexact(pg, gpu_available):
                          zlevels=zlev, allow_abs=True)
         assert_bitexact(gpu, ref, label=f"stretched z {storage} ")
         np.testing.assert_array_equal(gpu["compute"][0], ref["compute"][0].astype(np.float32))
-        assert (ref["status"] != 0).sum() > 0 and (ref["status"] == 0).sum() > n // 2
+        # a mix of particles alive at the end and stopped on the way (domain exits, stalls)
+        assert (ref["status"] != 0).sum() > 100 and (ref["status"] == 0).sum() > 100
