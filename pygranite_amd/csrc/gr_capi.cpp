@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -1542,14 +1543,56 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
     if (items_dirty)
     {
-        struct ItemKey
-        {
-            uint32_t q, cls;
-            uintptr_t field;
-            float pos;
-            gr::BatchItem item;
+        // One counting sort over a composite bucket key: queue, length class (descending),
+        // field, then the item's position inside the queue's range in 256 steps -- so the
+        // ordering costs O(items) however many jobs share the launch.
+        std::vector<uint32_t> cls_vals, fld_rank(nspans, 0);
+        std::vector<uintptr_t> fld_vals;
+        for (uint32_t i = 0; i < nspans; ++i)
+            if (span_steps[i])
+            {
+                cls_vals.push_back(cls_of[i]);
+                fld_vals.push_back(uintptr_t(spans[i].wind));
+            }
+        std::sort(cls_vals.begin(), cls_vals.end(), std::greater<uint32_t>());
+        cls_vals.erase(std::unique(cls_vals.begin(), cls_vals.end()), cls_vals.end());
+        std::sort(fld_vals.begin(), fld_vals.end());
+        fld_vals.erase(std::unique(fld_vals.begin(), fld_vals.end()), fld_vals.end());
+        const uint64_t ncls = std::max<size_t>(1, cls_vals.size()), nfld = std::max<size_t>(1, fld_vals.size());
+        // position buckets per queue (fewer when many classes / fields would make the count
+        // array large)
+        const uint64_t PB = std::max<uint64_t>(1, std::min<uint64_t>(256, (uint64_t(1) << 24) / (uint64_t(nq) * ncls * nfld)));
+        const uint64_t nbuckets = uint64_t(nq) * ncls * nfld * PB;
+        std::vector<uint32_t> span_base(nspans, 0); // cls/field part of the key, per span
+        for (uint32_t i = 0; i < nspans; ++i)
+            if (span_steps[i])
+            {
+                const uint64_t cr = uint64_t(std::find(cls_vals.begin(), cls_vals.end(), cls_of[i]) - cls_vals.begin());
+                const uint64_t fr = uint64_t(std::lower_bound(fld_vals.begin(), fld_vals.end(),
+                                                              uintptr_t(spans[i].wind)) - fld_vals.begin());
+                span_base[i] = uint32_t((cr * nfld + fr) * PB);
+            }
+        auto bucket_of = [&](uint32_t i, uint64_t f, uint64_t n) -> uint64_t {
+            const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, n - f));
+            const double pos = (double(f) + 0.5 * cnt) / double(n); // in [0, 1)
+            const double qp = pos * double(nq);
+            const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(qp));
+            const uint64_t pb = std::min<uint64_t>(PB - 1, uint64_t((qp - double(q)) * double(PB)));
+            return uint64_t(q) * ncls * nfld * PB + span_base[i] + pb;
         };
-        std::vector<ItemKey> keyed;
+        std::vector<uint32_t> count(nbuckets + 1, 0);
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < nspans; ++i)
+        {
+            if (!span_steps[i]) continue;
+            const uint64_t n = spans[i].job->n;
+            for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM) ++count[bucket_of(i, f, n) + 1];
+            total += (n + gr::BATCH_ITEM - 1) / gr::BATCH_ITEM;
+        }
+        for (uint64_t b = 0; b < nbuckets; ++b) count[b + 1] += count[b];
+        ctx->h_items.resize(total);
+        ctx->qstart.assign(nq + 1, 0u);
+        for (uint32_t q = 0; q < nq; ++q) ctx->qstart[q + 1] = count[uint64_t(q + 1) * ncls * nfld * PB];
         for (uint32_t i = 0; i < nspans; ++i)
         {
             if (!span_steps[i]) continue;
@@ -1558,25 +1601,9 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             for (uint64_t f = 0; f < j->n; f += gr::BATCH_ITEM)
             {
                 const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, j->n - f));
-                const float pos = float((double(f) + 0.5 * cnt) / double(j->n));
-                const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(pos * float(nq)));
-                keyed.push_back(ItemKey{q, cls_of[i], uintptr_t(spans[i].wind), pos, gr::BatchItem{i, uint32_t(f), cnt, var}});
+                ctx->h_items[count[bucket_of(i, f, j->n)]++] = gr::BatchItem{i, uint32_t(f), cnt, var};
             }
         }
-        std::stable_sort(keyed.begin(), keyed.end(), [](const ItemKey & a, const ItemKey & b) {
-            if (a.q != b.q) return a.q < b.q;
-            if (a.cls != b.cls) return a.cls > b.cls;
-            if (a.field != b.field) return a.field < b.field;
-            return a.pos < b.pos;
-        });
-        ctx->h_items.resize(keyed.size());
-        ctx->qstart.assign(nq + 1, 0u);
-        for (size_t k = 0; k < keyed.size(); ++k)
-        {
-            ctx->h_items[k] = keyed[k].item;
-            ++ctx->qstart[keyed[k].q + 1];
-        }
-        for (uint32_t q = 0; q < nq; ++q) ctx->qstart[q + 1] += ctx->qstart[q];
         ctx->items_key = key;
         ctx->items_valid = false; // until the copy below is queued
     }
