@@ -623,7 +623,7 @@ def serving_workload(capi, ctx, args):
           f"{offered:.3e} nominal particle-steps/s", file=sys.stderr, flush=True)
     srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum)
     ctx.take_work()
-    srv.run(reqs)
+    srv.run(reqs, progress=lambda m: print("#   " + m, file=sys.stderr, flush=True))
     rep = srv.report()
     rep["integrated_particle_steps"] = ctx.take_work()
     rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]

@@ -186,15 +186,21 @@ class Server:
         self.done.append(req)
 
     # -- driver ---------------------------------------------------------------------------
-    def run(self, requests, deadline: float | None = None):
-        """Serves `requests` (sorted by arrival) in real time; returns the finished ones."""
+    def run(self, requests, deadline: float | None = None, progress=None, every: float = 10.0):
+        """Serves `requests` (sorted by arrival) in real time; returns the finished ones.
+        progress(text) (e.g. a print to stderr) receives a status line every `every` seconds."""
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
         self.ctx.take_kernel_ms()  # device time of this run's launches, summed at the end
         self.t0 = time.perf_counter()
+        next_report = every
         while self.queue or self.active:
             if deadline is not None and self.now() > deadline:
                 break
+            if progress is not None and self.now() >= next_report:
+                progress(f"t={self.now():.0f}s done={len(self.done)} active={len(self.active)} "
+                         f"queued={len(self.queue)} rounds={self.rounds}")
+                next_report += every
             self.admit()
             if self.active:
                 self.round()
