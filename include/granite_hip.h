@@ -248,6 +248,13 @@ int gr_job_read(gr_job * job, uint32_t kind, void * dst, uint64_t bytes);
 int gr_job_read_saved(gr_job * job, uint32_t kind, uint64_t first, uint64_t count, void * dst);
 /* Number of particles still at status Time (replaces count_if, ToolKernel.tpp:4-25). */
 int gr_job_count_alive(gr_job * job, uint64_t * alive);
+/* Live range without a wait: every particle slot from *range on holds a stopped particle, and
+ * step launches cover only [0, *range). A reorder places the alive particles first and counts
+ * them; the count is taken up once that reorder has completed on the device (without waiting
+ * here, or at the job's next reorder, which waits for it). Equals the particle count until
+ * then. No counterpart in the reference (its launches always cover every particle,
+ * TrajectoryIntegrator.tpp:982). */
+int gr_job_live_range(gr_job * job, uint64_t * range);
 /* Number of saved columns written so far. */
 int gr_job_saved_columns(gr_job * job, uint64_t * cols);
 /* Device time of the last gr_job_advance, measured with HIP events on the compute stream. */

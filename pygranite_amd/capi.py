@@ -31,7 +31,7 @@ EXPORTS = [
     "gr_field_set_z_levels",
     "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
-    "gr_job_count_alive", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
+    "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
 OPT_BLOCK, OPT_PAIRED = 1, 2
@@ -146,6 +146,7 @@ def lib():
         "gr_job_read": ([vp, u32, vp, u64], i32),
         "gr_job_read_saved": ([vp, u32, u64, u64, vp], i32),
         "gr_job_count_alive": ([vp, P(u64)], i32),
+        "gr_job_live_range": ([vp, P(u64)], i32),
         "gr_job_saved_columns": ([vp, P(u64)], i32),
         "gr_job_last_advance_ms": ([vp, P(ctypes.c_float)], i32),
         "gr_job_set_launch": ([vp, u32], i32),
@@ -407,6 +408,12 @@ class Job:
     def count_alive(self) -> int:
         c = ctypes.c_uint64()
         _check(lib().gr_job_count_alive(self.h, ctypes.byref(c)))
+        return c.value
+
+    def live_range(self) -> int:
+        """Particle slots the next launch covers (every later slot holds a stopped particle)."""
+        c = ctypes.c_uint64()
+        _check(lib().gr_job_live_range(self.h, ctypes.byref(c)))
         return c.value
 
     def last_advance_ms(self) -> float:

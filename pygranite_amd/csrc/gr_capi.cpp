@@ -118,6 +118,15 @@ struct gr_ctx
     size_t kpending{0};
     double kms_acc{0.0};
     uint64_t klaunch_acc{0};
+    // alive counts of reordered jobs: the count kernel of a reorder stores each job's count into
+    // its slot of host-coherent pinned memory; reorder number k records alive_ev[k % ALIVE_EVENTS]
+    // after it, so a job whose count is pending knows which event to query
+    static constexpr uint32_t ALIVE_EVENTS = 8;
+    std::vector<uint32_t *> alive_pages; // 4 KiB pages of count slots
+    std::vector<uint32_t *> alive_free;  // free slots (host pointers)
+    hipEvent_t alive_ev[ALIVE_EVENTS]{};
+    uint64_t alive_seq{0};  // reorders that stored counts
+    uint64_t alive_done{0}; // all of the first alive_done of them are known complete
     // live fields and jobs: the context outlives them (gr_ctx_destroy refuses otherwise)
     int children{0};
 };
@@ -190,6 +199,13 @@ struct gr_job
     std::vector<void *> shadow;  // second copy of each state array (reorders gather into it)
     bool reordered{false};
     uint64_t reorder_col0{0};    // first saved column written in the current order
+    // Every particle from run_n on is stopped: a reorder places the alive particles first and
+    // counts them, and stopped particles never run again, so the count of any completed reorder
+    // bounds the launches queued after it. Launches cover [0, run_n) only.
+    uint64_t run_n{0};
+    uint32_t * h_alive{nullptr}; // count slot (host-coherent pinned memory)
+    uint32_t * d_alive{nullptr}; // its device address
+    uint64_t alive_seq{0};       // number of the reorder whose count is pending (0: none)
     float * scratch2{nullptr};   // n * dim floats
     // window tables of multi-window launches (kept alive while copies are queued)
 };
@@ -326,6 +342,9 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     dfree(c->d_gtab);
     if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->bev1) (void)hipEventDestroy(c->bev1);
+    for (hipEvent_t & e : c->alive_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (uint32_t * pg : c->alive_pages) (void)hipHostFree(pg);
     (void)hipStreamDestroy(c->compute);
     (void)hipStreamDestroy(c->copy);
     delete c;
@@ -727,6 +746,8 @@ static void job_free(gr_job * j)
     for (void * a : j->shadow) dfree(a);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
+    // the slot is reused only after the compute stream has drained (gr_job_destroy)
+    if (j->h_alive && j->ctx) j->ctx->alive_free.push_back(j->h_alive);
     delete j;
 }
 
@@ -789,6 +810,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     j->ctx = ctx;
     j->d = d;
     j->n = n;
+    j->run_n = n;
     j->cap = save_capacity ? save_capacity : 1;
     j->full = (d.dim == 2 && (d.border != GR_BLOCK || d.sphere_radius > 0.f)) ||
               d.abort_mode != GR_ABORT_TIME || d.uplift_mode != GR_MODE_OFF ||
@@ -936,14 +958,66 @@ template <class T> static int grow(T *& buf, size_t & cap, size_t need)
     return rc;
 }
 
+// A count slot for job j (host-coherent pinned memory, device-visible): taken from the free list,
+// which grows by 4 KiB pages.
+static int alive_slot(gr_job * j)
+{
+    if (j->h_alive) return GR_OK;
+    gr_ctx * c = j->ctx;
+    if (c->alive_free.empty())
+    {
+        void * pg = nullptr;
+        GR_HIP(hipHostMalloc(&pg, 4096, hipHostMallocMapped | hipHostMallocCoherent));
+        c->alive_pages.push_back(static_cast<uint32_t *>(pg));
+        for (int k = 1023; k >= 0; --k) c->alive_free.push_back(static_cast<uint32_t *>(pg) + k);
+    }
+    uint32_t * h = c->alive_free.back();
+    void * d = nullptr;
+    GR_HIP(hipHostGetDevicePointer(&d, h, 0));
+    c->alive_free.pop_back();
+    j->h_alive = h;
+    j->d_alive = static_cast<uint32_t *>(d);
+    return GR_OK;
+}
+
+// Takes the job's pending alive count once its reorder has completed (with wait: blocks until it
+// has). Events complete in stream order, so a later reorder's event standing in for an
+// overwritten one also proves completion.
+static int alive_resolve(gr_job * j, bool wait)
+{
+    if (!j->alive_seq) return GR_OK;
+    gr_ctx * c = j->ctx;
+    if (j->alive_seq > c->alive_done)
+    {
+        hipEvent_t e = c->alive_ev[j->alive_seq % gr_ctx::ALIVE_EVENTS];
+        if (wait)
+            GR_HIP(hipEventSynchronize(e));
+        else
+        {
+            const hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) return GR_OK;
+            if (q != hipSuccess) return set_err(GR_E_HIP, "alive count event: %s", hipGetErrorString(q));
+        }
+        // the event stands for reorder alive_seq or a later one (which implies it)
+        c->alive_done = std::max(c->alive_done, j->alive_seq);
+    }
+    const uint32_t a = *static_cast<volatile uint32_t *>(j->h_alive);
+    j->run_n = std::min<uint64_t>(j->run_n, a);
+    j->alive_seq = 0;
+    return GR_OK;
+}
+
 // Reorders several jobs of one context by the Morton code of each particle's current cell
 // (stopped particles last): one key kernel, one radix sort over all particles (the job index in
 // the high key bits keeps every job in its own range) and one gather launch for every state
 // array of every job, each gathered into its shadow copy, which then takes the array's place.
+// Only a job's live range [0, run_n) takes part (later slots hold stopped particles, which
+// stay where they are): when it is at most half the job, the gathered part is copied back
+// into the arrays; otherwise the stopped tail is copied into the shadows before the swap.
 extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
 {
     if (njobs && !jobs) return set_err(GR_E_INVALID, "gr_jobs_reorder: null argument");
-    std::vector<gr_job *> js;
+    std::vector<gr_job *> all;
     gr_ctx * ctx = nullptr;
     for (uint32_t i = 0; i < njobs; ++i)
     {
@@ -951,13 +1025,26 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
         if (!j) return set_err(GR_E_INVALID, "gr_jobs_reorder: null job %u", i);
         if (ctx && j->ctx != ctx) return set_err(GR_E_INVALID, "gr_jobs_reorder: jobs of different contexts");
         ctx = j->ctx;
-        if (j->n >= 2) js.push_back(j);
+        if (j->n >= 2) all.push_back(j);
     }
-    if (js.empty()) return GR_OK;
-    if (js.size() > 0xffffu) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many jobs");
+    if (all.empty()) return GR_OK;
+    if (all.size() > 0xffffu) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many jobs");
     GR_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->compute;
     int rc;
+    // the previous reorder's counts first: keeps the host at most one reorder ahead of the
+    // device, so the counts arrive while they still bound the launches being queued
+    std::vector<gr_job *> js;
+    for (gr_job * j : all)
+    {
+        if ((rc = alive_resolve(j, true)) != GR_OK) return rc;
+        if (j->run_n >= 2) js.push_back(j); // fewer live slots: nothing to regroup
+    }
+    if (js.empty()) return GR_OK;
+    for (gr_job * j : js)
+        if ((rc = alive_slot(j)) != GR_OK) return rc;
+    if (!ctx->alive_ev[0])
+        for (hipEvent_t & e : ctx->alive_ev) GR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // index maps and shadow arrays on first use
     for (gr_job * j : js)
     {
@@ -993,10 +1080,11 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
             t.grid[d] = std::max(1u, j->grid[d]);
         }
         t.status = j->status;
+        t.alive_out = j->d_alive;
         t.base = total;
-        t.n = uint32_t(j->n);
+        t.n = uint32_t(j->run_n);
         t.dim = j->d.dim;
-        total += j->n;
+        total += j->run_n;
     }
     if (total > ctx->scap)
     {
@@ -1035,27 +1123,49 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     size_t tb = ctx->stmp_bytes;
     if (gr::sort_pairs64(ctx->stmp, tb, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total, end_bit, s) != 0)
         return set_err(GR_E_HIP, "radix sort failed");
-    // gather every state array (and the index map) into its shadow, then swap
-    std::vector<gr::GatherEntry> g;
-    uint32_t maxn = 0;
+    if (gr::launch_alive_counts(ctx->d_sjobs, uint32_t(tab.size()), ctx->skeys[1], s) != 0)
+        return set_err(GR_E_HIP, "alive count kernel failed");
+    ++ctx->alive_seq;
+    GR_HIP(hipEventRecord(ctx->alive_ev[ctx->alive_seq % gr_ctx::ALIVE_EVENTS], s));
+    for (gr_job * j : js) j->alive_seq = ctx->alive_seq;
+    // gather the live part of every state array (and the index map) into its shadow; then
+    // either copy it back (second table) or complete the shadow with the stopped tail and swap
+    std::vector<gr::GatherEntry> g, back;
+    std::vector<char> swap_mode(js.size(), 0);
+    uint32_t maxn = 0, maxb = 0;
     for (size_t k = 0; k < js.size(); ++k)
     {
         gr_job * j = js[k];
+        const uint32_t m = uint32_t(j->run_n), n = uint32_t(j->n);
+        swap_mode[k] = 2ull * m > n;
         const std::vector<void **> slots = state_slots(j);
         for (size_t a = 0; a < slots.size(); ++a)
-            g.push_back(gr::GatherEntry{static_cast<const uint32_t *>(*slots[a]), static_cast<uint32_t *>(j->shadow[a]),
-                                        ctx->svals[1] + tab[k].base, uint32_t(j->n), 0u});
-        maxn = std::max(maxn, uint32_t(j->n));
+        {
+            uint32_t * arr = static_cast<uint32_t *>(*slots[a]);
+            uint32_t * sh = static_cast<uint32_t *>(j->shadow[a]);
+            g.push_back(gr::GatherEntry{arr, sh, ctx->svals[1] + tab[k].base, m, 0u});
+            if (swap_mode[k] && m < n) g.push_back(gr::GatherEntry{arr + m, sh + m, nullptr, n - m, 0u});
+            if (!swap_mode[k]) back.push_back(gr::GatherEntry{sh, arr, nullptr, m, 0u});
+        }
+        maxn = std::max(maxn, swap_mode[k] ? std::max(m, n - m) : m);
+        if (!swap_mode[k]) maxb = std::max(maxb, m);
     }
-    if (g.size() > 65535) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many arrays in one call");
-    if ((rc = grow(ctx->d_gtab, ctx->gtab_cap, g.size())) != GR_OK) return rc;
+    if (g.size() + back.size() > 65535) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many arrays in one call");
+    if ((rc = grow(ctx->d_gtab, ctx->gtab_cap, g.size() + back.size())) != GR_OK) return rc;
+    g.insert(g.end(), back.begin(), back.end());
     GR_HIP(hipMemcpyAsync(ctx->d_gtab, g.data(), g.size() * sizeof(gr::GatherEntry), hipMemcpyHostToDevice, s));
-    if (gr::launch_gather_multi(ctx->d_gtab, uint32_t(g.size()), maxn, s) != 0)
+    const uint32_t ng = uint32_t(g.size() - back.size());
+    if (gr::launch_gather_multi(ctx->d_gtab, ng, maxn, s) != 0 ||
+        (!back.empty() && gr::launch_gather_multi(ctx->d_gtab + ng, uint32_t(back.size()), maxb, s) != 0))
         return set_err(GR_E_HIP, "gather kernel failed");
-    for (gr_job * j : js)
+    for (size_t k = 0; k < js.size(); ++k)
     {
-        const std::vector<void **> slots = state_slots(j);
-        for (size_t a = 0; a < slots.size(); ++a) std::swap(*slots[a], j->shadow[a]);
+        gr_job * j = js[k];
+        if (swap_mode[k])
+        {
+            const std::vector<void **> slots = state_slots(j);
+            for (size_t a = 0; a < slots.size(); ++a) std::swap(*slots[a], j->shadow[a]);
+        }
         j->reordered = true;
         j->reorder_col0 = j->saved;
     }
@@ -1324,9 +1434,10 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     int rc = prepare_advance(j, w, P, mode, nsaves);
     if (rc != GR_OK) return rc;
     hipStream_t s = j->ctx->compute;
+    if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev0, s));
     if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
-    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, s) != 0)
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if ((rc = kt_mark(j->ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev1, s));
@@ -1454,6 +1565,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if (!j || !sp.wind) return set_err(GR_E_INVALID, "span %u: null job or field", i);
         if (j->ctx != ctx) return set_err(GR_E_INVALID, "span %u: job of another context", i);
         if (j->full) return set_err(GR_E_INVALID, "span %u: the job uses features outside the batched kernel", i);
+        if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
         const gr_job_desc & d = j->d;
         const uint32_t nslices = sp.wind->d.nslots;
         gr_span_end end_i{};
@@ -1521,12 +1633,14 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         }
     }
 
-    // Work items of one wavefront of particles. Items go to one queue per XCD by their position
-    // in their job's particle range (Morton-ordered after gr_job_reorder), and inside a queue
-    // they are ordered by span length class (longest first), field, then position: the
-    // wavefronts in flight on one XCD then sample one compact region of one field, and long
-    // spans start early. The list depends only on the spans' jobs, sizes, variants, fields and
-    // length classes, so a launch with the same ones reuses the previous list on the device.
+    // Work items of one wavefront of particles, covering each job's particles [0, run_n) (the
+    // rest are stopped). Items go to the queue of a group of compute units by their position in
+    // that range (Morton-ordered after gr_job_reorder, so the alive particles of every job spread
+    // over all queues), and inside a queue they are ordered by span length class (longest
+    // first), field, then position: the wavefronts in flight on one group then sample one
+    // compact region of one field, and long spans start early. The list depends only on the
+    // spans' jobs, ranges, variants, fields and length classes, so a launch with the same ones
+    // reuses the previous list on the device.
     const uint32_t group = std::max(1u, ctx->batch_group);
     const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>((ctx->ncu + group - 1) / group, gr::MAX_QUEUES));
     std::vector<uint64_t> key;
@@ -1537,7 +1651,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
         for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls_of[i];
-        key.insert(key.end(), {i, uint64_t(uintptr_t(j)), j->n, uint64_t(uintptr_t(spans[i].wind)),
+        key.insert(key.end(), {i, uint64_t(uintptr_t(j)), j->run_n, uint64_t(uintptr_t(spans[i].wind)),
                                uint64_t(spans[i].wind->d.dtype), cls_of[i]});
     }
     const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
@@ -1585,7 +1699,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         for (uint32_t i = 0; i < nspans; ++i)
         {
             if (!span_steps[i]) continue;
-            const uint64_t n = spans[i].job->n;
+            const uint64_t n = spans[i].job->run_n;
             for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM) ++count[bucket_of(i, f, n) + 1];
             total += (n + gr::BATCH_ITEM - 1) / gr::BATCH_ITEM;
         }
@@ -1598,10 +1712,11 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             if (!span_steps[i]) continue;
             gr_job * j = spans[i].job;
             const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
-            for (uint64_t f = 0; f < j->n; f += gr::BATCH_ITEM)
+            const uint64_t n = j->run_n;
+            for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM)
             {
-                const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, j->n - f));
-                ctx->h_items[count[bucket_of(i, f, j->n)]++] = gr::BatchItem{i, uint32_t(f), cnt, var};
+                const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, n - f));
+                ctx->h_items[count[bucket_of(i, f, n)]++] = gr::BatchItem{i, uint32_t(f), cnt, var};
             }
         }
         ctx->items_key = key;
@@ -1842,6 +1957,16 @@ extern "C" int gr_job_count_alive(gr_job * j, uint64_t * alive)
     GR_HIP(hipMemcpyAsync(&h, j->ctx->dcount, sizeof(h), hipMemcpyDeviceToHost, s));
     GR_HIP(hipStreamSynchronize(s));
     *alive = h;
+    return GR_OK;
+}
+
+extern "C" int gr_job_live_range(gr_job * j, uint64_t * range)
+{
+    if (!j || !range) return set_err(GR_E_INVALID, "gr_job_live_range: null argument");
+    GR_HIP(hipSetDevice(j->ctx->device));
+    const int rc = alive_resolve(j, false);
+    if (rc != GR_OK) return rc;
+    *range = j->run_n;
     return GR_OK;
 }
 

@@ -190,26 +190,29 @@ int launch_cu_probe(uint32_t * out, uint32_t n, void * stream);
 int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
 
 // launchers (gr_kernels.hip)
-// nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range)
+// nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range);
+// nrun (<= P.n): the launch covers particles [0, nrun) -- every particle from nrun on is stopped
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, void * stream);
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, void * stream);
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);
 int launch_pair(const void * back, const void * front, void * out, uint32_t nx, uint32_t ny,
                 uint32_t nz, uint32_t dim, uint32_t address, uint32_t ncomp, uint32_t cstride,
                 uint32_t dtype, void * stream);
-// one job of a multi-job reorder: its particles occupy keys [base, base + n)
+// one job of a multi-job reorder: its particles occupy keys [base, base + n); after the sort
+// the number of its alive particles (all placed first) is stored to *alive_out
 struct SortJob
 {
     const float * p[3];
     const int32_t * status;
+    uint32_t * alive_out;
     uint64_t base;
     uint32_t n, dim;
     float off[3], inv_h[3];
     uint32_t grid[3];
 };
-// dst[i] = src[perm[i]], i < n (4-byte elements)
+// dst[i] = src[perm[i]] (perm null: dst[i] = src[i]), i < n (4-byte elements)
 struct GatherEntry
 {
     const uint32_t * src;
@@ -227,6 +230,9 @@ int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_
                  const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
 // all entries in one launch (maxn: largest entry n)
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
+// per job of a sorted multi-job key array: the number of keys before its first stopped key
+int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const uint64_t * sorted_keys,
+                        void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
                      void * stream);

@@ -17,6 +17,7 @@
 #include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "gr_internal.h"
@@ -1294,7 +1295,7 @@ static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
 }
 
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, void * stream)
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, void * stream)
 {
     AdvanceFn fn;
     if (dim == 3)
@@ -1304,7 +1305,7 @@ int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint
         fn = integrator == GR_RK4 ? pick_type<2, GR_RK4>(P.wind.dtype, mode)
                                   : pick_type<2, GR_EULER>(P.wind.dtype, mode);
     if (block == 0 || block > 256) block = 256;
-    const uint32_t grid = uint32_t((uint64_t(P.n) + block - 1) / block);
+    const uint32_t grid = uint32_t((std::min<uint64_t>(nrun, P.n) + block - 1) / block);
     if (grid == 0) return 0;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, hipStream_t(stream), P, nxcd);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1494,6 +1495,33 @@ __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs
     vals[g] = i;
 }
 
+// One thread per job: its keys [base, base + n) are sorted with the stopped particles (low word
+// 0xffffffff) last; the first of those is found by bisection.
+__global__ void k_alive_counts(const SortJob * __restrict__ jobs, uint32_t njobs,
+                               const uint64_t * __restrict__ keys)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= njobs) return;
+    const SortJob & J = jobs[k];
+    uint64_t lo = 0, hi = J.n; // answer in [lo, hi]
+    while (lo < hi)
+    {
+        const uint64_t mid = (lo + hi) / 2;
+        if (uint32_t(keys[J.base + mid]) == 0xffffffffu) hi = mid;
+        else lo = mid + 1;
+    }
+    if (J.alive_out) *J.alive_out = uint32_t(lo);
+}
+
+int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const uint64_t * sorted_keys, void * stream)
+{
+    if (njobs == 0) return 0;
+    const uint32_t block = 64;
+    hipLaunchKernelGGL(k_alive_counts, dim3((njobs + block - 1) / block), dim3(block), 0,
+                       hipStream_t(stream), jobs, njobs, sorted_keys);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, uint64_t * keys,
                         uint32_t * vals, void * stream)
 {
@@ -1519,7 +1547,7 @@ __global__ void k_gather_multi(const GatherEntry * __restrict__ tab)
 {
     const GatherEntry & E = tab[blockIdx.y];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < E.n) E.dst[i] = E.src[E.perm[i]];
+    if (i < E.n) E.dst[i] = E.src[E.perm ? E.perm[i] : i];
 }
 
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream)

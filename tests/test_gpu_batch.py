@@ -305,3 +305,105 @@ def test_joint_reorder_of_mixed_jobs(gpu_available):
     f.close()
     f2.close()
     ctx.close()
+
+
+
+def _advance_chunks(job, f, steps, chunk, dt, after_chunk):
+    """Advances `job` by `steps` in launches of at most `chunk` steps under the FP32 window
+    schedule (a launch also ends at a window edge); after_chunk(steps_left) runs after each."""
+    t, counter, c, mwt, left = np.float32(0), 0, 0, np.float32(1.0), steps
+    launches = 0
+    while left:
+        k, tt, edge = 0, t, False
+        while k < min(left, chunk):
+            k += 1
+            tt = np.float32(tt + np.float32(dt))
+            if tt >= mwt:
+                edge = True
+                break
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, c, c + 1
+        w.time_back, w.time_front = float(np.float32(c)), float(np.float32(c + 1))
+        w.time_start, w.counter_start, w.nsteps = float(t), counter, k
+        job.advance(w)
+        t, counter, left = tt, counter + k, left - k
+        if edge:
+            c += 1
+            mwt = np.float32(mwt + np.float32(1.0))
+        launches += 1
+        after_chunk(left)
+    return launches
+
+
+def test_live_range_bounds_launches(gpu_available):
+    """Launches that cover only a job's live range (the leading slots a reorder filled with its
+    alive particles) give the results of launches over every particle, on the single-job and
+    the batched path; the range shrinks once a reorder's count has been taken up and always
+    holds every alive particle."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 5)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
+    for s in range(5):
+        f.upload(s, [src[s, c] for c in range(3)])
+    # dt = 2^-7: FP32 time is exact, so window edges fall at steps 128, 256, 384 and every launch
+    # has 32 steps; the last launch saves the columns of counters 450 ... 475
+    n, dt, steps, chunk = 6000, 0.0078125, 480, 32
+    seeds = [synthetic.uniform_seeds(9, n, [0.0] * 3, [2 * math.pi] * 3),
+             synthetic.uniform_seeds(10, 2500, [0.0] * 3, [2 * math.pi] * 3)]
+    d = make_desc(3, 1, gs, src.shape, dt, 3, ("sqrt(u*u+v*v+w*w)",), save_interval=5)
+    kinds = (capi.OUT_POSITIONS, capi.OUT_STATUS, capi.OUT_ABORT_STEP, capi.OUT_TOTAL_CURVATURE,
+             capi.OUT_EXPR0)
+
+    def single(reorder):
+        job = capi.Job(ctx, d, seeds[0], save_capacity=100)
+        ranges = []
+
+        def after(left):
+            if reorder and left:
+                job.reorder()
+                ranges.append((job.live_range(), job.count_alive()))
+
+        assert _advance_chunks(job, f, steps, chunk, dt, after) == steps // chunk
+        out = [job.read(k) for k in kinds]
+        out.append(job.read_saved(capi.OUT_POSITIONS, job.saved_columns() - 6, 6))
+        job.close()
+        return out, ranges
+
+    def batched(reorder):
+        jobs = [capi.Job(ctx, d, sd, save_capacity=100) for sd in seeds]
+        ranges = []
+        e = dict(window=0, time=0.0, time64=0.0, counter=0)
+        for i in range(steps // chunk):
+            ends, _ = ctx.batch_run([(j, f, dict(max_steps=chunk, first_window=e["window"], time_start=e["time"],
+                                                 time_start64=e["time64"], counter_start=e["counter"]))
+                                     for j in jobs], ends=True)
+            e = ends[0]
+            if reorder and i < steps // chunk - 1:
+                ctx.reorder(jobs)
+                ranges.append([(j.live_range(), j.count_alive()) for j in jobs])
+        assert e["counter"] == steps
+        out = [[j.read(k) for k in kinds] + [j.read_saved(capi.OUT_POSITIONS, j.saved_columns() - 6, 6)]
+               for j in jobs]
+        for j in jobs:
+            j.close()
+        return out, ranges
+
+    a, _ = single(False)
+    b, ranges = single(True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert all(c <= r <= n for r, c in ranges)
+    # from the second reorder on, the first one's count has been taken up
+    assert ranges[-1][0] <= ranges[1][0] < n
+
+    a, _ = batched(False)
+    b, ranges = batched(True)
+    for ja, jb in zip(a, b):
+        for x, y in zip(ja, jb):
+            np.testing.assert_array_equal(x, y)
+    for step in ranges:
+        for (r, c), sd in zip(step, seeds):
+            assert c <= r <= sd.shape[0]
+    assert all(r < sd.shape[0] for (r, _), sd in zip(ranges[-1], seeds))
+    f.close()
+    ctx.close()
