@@ -1437,7 +1437,9 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev0, s));
     if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
-    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, s) != 0)
+    // wide: at least two rounds of the wavefronts the device holds at three per SIMD
+    const bool wide = j->run_n >= 2ull * uint64_t(std::max(1, j->ctx->cus)) * 4 * 3 * 64;
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, wide, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if ((rc = kt_mark(j->ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev1, s));

@@ -191,9 +191,11 @@ int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
 
 // launchers (gr_kernels.hip)
 // nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range);
-// nrun (<= P.n): the launch covers particles [0, nrun) -- every particle from nrun on is stopped
+// nrun (<= P.n): the launch covers particles [0, nrun) -- every particle from nrun on is stopped;
+// wide: the launch holds many more wavefronts than the device runs at once (selects a register
+// budget of four wavefronts per SIMD where a kernel variant has one)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, void * stream);
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, void * stream);
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);

@@ -1022,7 +1022,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint
 }
 
 template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nxcd)
+__device__ __forceinline__ void advance_block(const StepParams & P, uint32_t nxcd)
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
     const uint32_t pid = b * blockDim.x + threadIdx.x;
@@ -1030,6 +1030,23 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nx
     if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE>(P, pid);
     if (P.work) // block sizes are multiples of 64
         wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
+}
+
+template <int DIM, int INTEG, class T, int MODE>
+__global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE>(P, nxcd);
+}
+
+// The same kernel held to at most 128 VGPRs, i.e. four wavefronts per SIMD instead of three.
+// Each wavefront's step chain runs a little slower, so this pays only when the launch has many
+// more wavefronts than the compute units hold at once. Instantiated for the FP16 time-paired
+// form only: it is the variant that fits this budget without spilling registers.
+template <int DIM, int INTEG, class T, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_advance_w4(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE>(P, nxcd);
 }
 
 // Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
@@ -1295,10 +1312,19 @@ static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
 }
 
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, void * stream)
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, void * stream)
 {
     AdvanceFn fn;
-    if (dim == 3)
+    if (wide && mode == M_PAIRED && P.wind.dtype == GR_F16)
+    {
+        if (dim == 3)
+            fn = integrator == GR_RK4 ? k_advance_w4<3, GR_RK4, __half, M_PAIRED>
+                                      : k_advance_w4<3, GR_EULER, __half, M_PAIRED>;
+        else
+            fn = integrator == GR_RK4 ? k_advance_w4<2, GR_RK4, __half, M_PAIRED>
+                                      : k_advance_w4<2, GR_EULER, __half, M_PAIRED>;
+    }
+    else if (dim == 3)
         fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, mode)
                                   : pick_type<3, GR_EULER>(P.wind.dtype, mode);
     else

@@ -95,13 +95,15 @@ def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0, reorder:
 
 class Server:
     def __init__(self, ctx: capi.Context, datasets: dict, hbm_budget: int = 64 << 30,
-                 quantum: int = 250, block: int = 0, reorder: bool = True):
+                 quantum: int = 250, block: int = 0, reorder: bool = True, reorder_every: int = 4):
         self.ctx = ctx
         self.datasets = datasets
         self.hbm_budget = int(hbm_budget)
         self.quantum = int(quantum)
         self.block = block
         self.reorder = reorder
+        self.reorder_every = max(1, int(reorder_every))
+        self.early = 0  # jobs retired once every particle had stopped
         self.queue = collections.deque()
         self.active: list[_Active] = []
         self.done: list[Request] = []
@@ -172,9 +174,21 @@ class Server:
             a.window, a.time, a.time64, a.counter = e["window"], e["time"], e["time64"], e["counter"]
             if e["done"] or e["steps"] == 0 or a.req.steps_done >= a.req.steps:
                 self._retire(a)
+            elif a.job.live_range() == 0:
+                # every particle has stopped: the remaining steps change nothing
+                a.req.steps_done = a.req.steps
+                self.early += 1
+                self._retire(a)
             else:
                 keep.append(a)
         self.active = keep
+        # regroup the large jobs by cell every few rounds: their launches then cover only the
+        # particles still alive (the live range), and a job whose particles have all stopped
+        # retires at once
+        if self.reorder and self.rounds % self.reorder_every == 0:
+            big = [a.job for a in self.active if a.req.count >= 4096]
+            if big:
+                self.ctx.reorder(big)
 
     def _retire(self, a: _Active):
         req = a.req
@@ -224,6 +238,7 @@ class Server:
             "particle_steps": int(self.particle_steps),
             "particle_steps_per_s": self.particle_steps / wall if wall > 0 else float("nan"),
             "rounds": self.rounds,
+            "retired_all_stopped": self.early,
             "device_busy_frac": (self.device_ms / 1e3) / wall if wall > 0 else float("nan"),
             "wall_s": wall,
         }

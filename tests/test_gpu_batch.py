@@ -407,3 +407,38 @@ def test_live_range_bounds_launches(gpu_available):
     assert all(r < sd.shape[0] for (r, _), sd in zip(ranges[-1], seeds))
     f.close()
     ctx.close()
+
+
+def test_wide_launch_register_budget_bitexact(gpu_available):
+    """A launch of many more wavefronts than the device holds at once (here 400,000 particles)
+    runs the FP16 time-paired kernel built for four wavefronts per SIMD; its results equal the
+    batched kernel's bit for bit, and a subsample equals the FP32 oracle fed the same rounded
+    slices."""
+    ctx = capi.Context(0)
+    src, gs = synthetic.abc_flow(32, 3)
+    f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=3)
+    for s in range(3):
+        f.upload(s, [src[s, c] for c in range(3)])
+    n, dt = 400000, 0.0078125
+    seeds = synthetic.uniform_seeds(12, n, [0.0] * 3, [2 * math.pi] * 3)
+    d = make_desc(3, 1, gs, src.shape, dt, 3, ("sqrt(u*u+v*v+w*w)",))
+    a = capi.Job(ctx, d, seeds)
+    b = capi.Job(ctx, d, seeds)
+    assert _advance_chunks(a, f, 48, 16, dt, lambda left: None) == 3
+    steps, _ = ctx.batch_run([(b, f, dict(max_steps=48))])
+    assert steps == [48]
+    kinds = (capi.OUT_POSITIONS, capi.OUT_STATUS, capi.OUT_ABORT_STEP, capi.OUT_TOTAL_CURVATURE,
+             capi.OUT_EXPR0)
+    got = {k: a.read(k) for k in kinds}
+    for k in kinds:
+        np.testing.assert_array_equal(got[k], b.read(k))
+    sub = np.arange(0, n, 200)
+    st = oracle_settings(3, 1, gs, dt, 47.5 * dt, 3, ("sqrt(u*u+v*v+w*w)",))
+    ref = oracle.run(st, synthetic.round_fp16(src), seeds[sub], real64=False)
+    assert ref["steps"] == 48
+    np.testing.assert_array_equal(got[capi.OUT_POSITIONS][sub], ref["trajectories"][:, -1].astype(np.float32))
+    np.testing.assert_array_equal(got[capi.OUT_STATUS][sub], ref["status"])
+    a.close()
+    b.close()
+    f.close()
+    ctx.close()
