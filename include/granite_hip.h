@@ -157,6 +157,11 @@ typedef struct {
 int gr_expr_compile(const char * src, const gr_expr_env * env, gr_program * out);
 /* CPU evaluation of compiled bytecode on a 19-slot reference vector (for checking). */
 int gr_expr_eval_host(const gr_program * prog, const float * ref19, float * out);
+/* CPU evaluation of the device form the step kernels run (leaf operands folded into their
+ * operations, the top two stack values in registers), for checking the lowering: the result
+ * equals gr_expr_eval_host's bit for bit. No counterpart in the reference (its interpreter
+ * walks the flattened tree, TrajectoryIntegrator.tpp:379-447). */
+int gr_expr_eval_device_form(const gr_program * prog, const float * ref19, float * out);
 
 /* ---- jobs ----------------------------------------------------------------------------------
  * A job is one particle set advanced through windows of a field. It replaces compute_ /

@@ -29,7 +29,7 @@ EXPORTS = [
     "gr_ctx_get_option", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
-    "gr_expr_compile", "gr_expr_eval_host", "gr_job_create", "gr_job_destroy",
+    "gr_expr_compile", "gr_expr_eval_host", "gr_expr_eval_device_form", "gr_job_create", "gr_job_destroy",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
@@ -139,6 +139,7 @@ def lib():
         "gr_field_set_z_levels": ([vp, P(ctypes.c_float), u32], i32),
         "gr_expr_compile": ([ctypes.c_char_p, P(ExprEnv), P(Program)], i32),
         "gr_expr_eval_host": ([P(Program), P(ctypes.c_float), P(ctypes.c_float)], i32),
+        "gr_expr_eval_device_form": ([P(Program), P(ctypes.c_float), P(ctypes.c_float)], i32),
         "gr_job_create": ([vp, P(JobDesc), vp, u64, u32, P(vp)], i32),
         "gr_job_destroy": ([vp], i32),
         "gr_job_set_particle_data": ([vp, u32, u32, vp], i32),
@@ -202,6 +203,14 @@ def eval_expr_host(prog: Program, ref) -> float:
     r = (ctypes.c_float * 19)(*[float(x) for x in ref])
     out = ctypes.c_float()
     _check(lib().gr_expr_eval_host(ctypes.byref(prog), r, ctypes.byref(out)))
+    return out.value
+
+
+def eval_expr_device_form(prog: Program, ref) -> float:
+    """The program evaluated in the device form the step kernels run (host arithmetic)."""
+    r = (ctypes.c_float * 19)(*[float(x) for x in ref])
+    out = ctypes.c_float()
+    _check(lib().gr_expr_eval_device_form(ctypes.byref(prog), r, ctypes.byref(out)))
     return out.value
 
 

@@ -1366,13 +1366,8 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
         P.cst_f[k] = j->cst[f][k];
     }
     for (uint32_t e = 0; e < d.nprog; ++e)
-    {
-        P.prog[e].n = d.prog[e].n;
-        std::memcpy(P.prog[e].code, d.prog[e].code, sizeof(P.prog[e].code));
-        std::memcpy(P.prog[e].imm, d.prog[e].imm, sizeof(P.prog[e].imm));
-        for (uint32_t i = 0; i < d.prog[e].n; ++i)
-            if ((d.prog[e].code[i] & 0xffu) == gr::OP_REF) P.refmask |= 1u << (d.prog[e].code[i] >> 8);
-    }
+        if (gr::lower_program(d.prog[e], P.prog[e], P.refmask) != GR_OK)
+            return set_err(GR_E_INVALID, "program %u is malformed", e);
 
     hipStream_t s = j->ctx->compute;
     // wait for the slices this window reads

@@ -310,3 +310,177 @@ extern "C" int gr_expr_eval_host(const gr_program * pr, const float * ref, float
     *out = sp > 0 ? st[0] : 0.f;
     return GR_OK;
 }
+
+namespace gr
+{
+
+static uint32_t dense_ref(uint32_t slot)
+{
+    if (slot <= 3) return slot;           // x y z cv
+    if (slot >= 16) return slot - 16 + 4; // u v w
+    return 7 + slot;                      // volumes, constants
+}
+
+static uint32_t unary_sub(uint32_t op)
+{
+    switch (op)
+    {
+        case OP_SQRT: return 0;
+        case OP_EXP: return 1;
+        case OP_LN: return 2;
+        case OP_SIN: return 3;
+        case OP_COS: return 4;
+        case OP_TAN: return 5;
+        case OP_ASIN: return 6;
+        case OP_ACOS: return 7;
+        case OP_ATAN: return 8;
+        default: return 9; // OP_ABS
+    }
+}
+
+// Postfix -> device form. A symbolic stack tracks which values are still unemitted leaves
+// (reference slots or immediates); a binary operation with a leaf operand becomes one leaf-form
+// instruction, and leaves are pushed only when an operation needs them on the stack. The device
+// stack then holds exactly the emitted (materialized) values in their symbolic order.
+int lower_program(const gr_program & in, Program & out, uint32_t & refmask)
+{
+    struct Sym
+    {
+        uint32_t kind; // 0 on the device stack, 1 reference leaf, 2 immediate leaf
+        uint32_t arg;  // slot or immediate index
+    };
+    Sym st[GR_MAX_STACK + 1];
+    int sp = 0;
+    uint32_t n = 0;
+    std::memcpy(out.imm, in.imm, sizeof(out.imm));
+    auto emit = [&](uint32_t cls, uint32_t sub, uint32_t arg) -> bool {
+        if (n >= GR_MAX_CODE) return false;
+        out.code[n++] = (cls << 4 | sub) | (arg << 8);
+        return true;
+    };
+    auto leaf_arg = [&](const Sym & x) -> uint32_t {
+        if (x.kind == 1)
+        {
+            refmask |= 1u << x.arg;
+            return dense_ref(x.arg);
+        }
+        return x.arg;
+    };
+    auto push = [&](Sym & x) -> bool {
+        if (x.kind && !emit(DC_PUSH, x.kind == 2 ? 1u : 0u, leaf_arg(x))) return false;
+        x.kind = 0;
+        return true;
+    };
+    if (in.n > GR_MAX_CODE) return GR_E_INVALID;
+    for (uint32_t i = 0; i < in.n; ++i)
+    {
+        const uint32_t op = in.code[i] & 0xffu, arg = in.code[i] >> 8;
+        if (op == OP_REF || op == OP_CONST)
+        {
+            if (sp >= GR_MAX_STACK || (op == OP_REF && arg >= 19) || (op == OP_CONST && arg >= GR_MAX_CODE))
+                return GR_E_INVALID;
+            st[sp++] = Sym{op == OP_REF ? 1u : 2u, arg};
+            continue;
+        }
+        const bool binary = (op >= OP_ADD && op <= OP_DIV) || op == OP_ATAN2;
+        if (binary)
+        {
+            if (sp < 2) return GR_E_INVALID;
+            Sym r = st[--sp], l = st[--sp];
+            bool ok = true;
+            const uint32_t k = op == OP_ATAN2 ? 4u : op - OP_ADD;
+            if (k == 4)
+            {
+                if (l.kind && !r.kind) ok = push(l) && emit(DC_BINARY, 4u | 8u, 0); // [.., r, l]
+                else ok = push(l) && push(r) && emit(DC_BINARY, 4u, 0);
+            }
+            else if (r.kind) // l is (or becomes) the top: top op leaf
+                ok = push(l) && emit(DC_LEAF, k | (r.kind == 2 ? 8u : 0u), leaf_arg(r));
+            else if (l.kind) // r is the top: leaf op top
+                ok = emit(DC_LEAF, k | 4u | (l.kind == 2 ? 8u : 0u), leaf_arg(l));
+            else
+                ok = emit(DC_BINARY, k, 0);
+            if (!ok) return GR_E_INVALID;
+            st[sp++] = Sym{0, 0};
+            continue;
+        }
+        if (op < OP_SQRT || op > OP_ABS || op == OP_ATAN2 || sp < 1) return GR_E_INVALID;
+        if (!push(st[sp - 1]) || !emit(DC_UNARY, unary_sub(op), 0)) return GR_E_INVALID;
+    }
+    if (sp != 1 || !push(st[0])) return GR_E_INVALID;
+    out.n = n;
+    return GR_OK;
+}
+
+static float apply(uint32_t k, float a, float b)
+{
+    switch (k)
+    {
+        case 0: return a + b;
+        case 1: return a - b;
+        case 2: return a * b;
+        case 3: return a / b;
+        default: return std::atan2(a, b);
+    }
+}
+
+float eval_lowered(const Program & pr, const float * ref)
+{
+    static const uint32_t slot_of[7] = {0, 1, 2, 3, 16, 17, 18};
+    auto refv = [&](uint32_t d) { return d < 7 ? ref[slot_of[d]] : ref[d - 7]; };
+    float acc = 0.f, acc2 = 0.f, deep[GR_MAX_STACK];
+    uint32_t depth = 0;
+    for (uint32_t i = 0; i < pr.n; ++i)
+    {
+        const uint32_t w = pr.code[i], cls = (w >> 4) & 0xfu, sub = w & 0xfu, arg = w >> 8;
+        if (cls == DC_LEAF)
+        {
+            const float x = (sub & 8) ? pr.imm[arg] : refv(arg);
+            acc = (sub & 4) ? apply(sub & 3, x, acc) : apply(sub & 3, acc, x);
+        }
+        else if (cls == DC_PUSH)
+        {
+            const float x = sub ? pr.imm[arg] : refv(arg);
+            if (depth >= 2) deep[depth - 2] = acc2;
+            acc2 = acc;
+            acc = x;
+            ++depth;
+        }
+        else if (cls == DC_BINARY)
+        {
+            const bool rev = sub & 8;
+            acc = rev ? apply(sub & 7, acc, acc2) : apply(sub & 7, acc2, acc);
+            --depth;
+            if (depth >= 2) acc2 = deep[depth - 2];
+        }
+        else
+        {
+            switch (sub)
+            {
+                case 0: acc = std::sqrt(acc); break;
+                case 1: acc = std::exp(acc); break;
+                case 2: acc = std::log(acc); break;
+                case 3: acc = std::sin(acc); break;
+                case 4: acc = std::cos(acc); break;
+                case 5: acc = std::tan(acc); break;
+                case 6: acc = std::asin(acc); break;
+                case 7: acc = std::acos(acc); break;
+                case 8: acc = std::atan(acc); break;
+                default: acc = std::fabs(acc); break;
+            }
+        }
+    }
+    return acc;
+}
+
+} // namespace gr
+
+extern "C" int gr_expr_eval_device_form(const gr_program * pr, const float * ref, float * out)
+{
+    if (!pr || !ref || !out) return GR_E_INVALID;
+    gr::Program lo;
+    uint32_t mask = 0;
+    if (gr::lower_program(*pr, lo, mask) != GR_OK) return GR_E_INVALID;
+    *out = gr::eval_lowered(lo, ref);
+    return GR_OK;
+}

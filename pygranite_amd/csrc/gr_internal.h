@@ -33,6 +33,26 @@ enum Op : uint32_t
     OP_ABS = 17
 };
 
+// Device form of a program (lower_program): the postfix code above with a leaf operand (a
+// reference slot or an immediate) folded into the binary operation that consumes it, so that
+// the two top stack values stay in registers and most programs touch no stack memory. An
+// instruction is (class << 4 | sub) | (arg << 8):
+//   DC_PUSH   sub 0: push ref[arg], 1: push imm[arg]
+//   DC_BINARY sub = k | rev << 3 (k: 0 add, 1 sub, 2 mul, 3 div, 4 atan2): second op top, or
+//             top op second when rev
+//   DC_LEAF   sub = k | rev << 2 | imm << 3 (k < 4): top op leaf, or leaf op top when rev; the
+//             leaf is ref[arg] or imm[arg]
+//   DC_UNARY  sub: 0 sqrt 1 exp 2 ln 3 sin 4 cos 5 tan 6 asin 7 acos 8 atan 9 abs
+// Reference arguments are dense: 0..6 = slots x y z cv u v w (held in registers), 7 + s = slot
+// s (volumes and constants, staged in LDS by the full kernel).
+enum DevClass : uint32_t
+{
+    DC_PUSH = 0,
+    DC_BINARY = 1,
+    DC_LEAF = 2,
+    DC_UNARY = 3
+};
+
 // Slot views of one field: node (i, j, k) of a slot starts at element
 // ((k * ny + j) * nx + i) * cstride of that slot, components consecutive.
 struct FieldView
@@ -78,6 +98,12 @@ struct Program
     uint32_t code[GR_MAX_CODE];
     float imm[GR_MAX_CODE];
 };
+
+// Lowers a compiled program (gr_expr_compile) to its device form (DevOp); the reference slots
+// it reads are or-ed into refmask. Returns GR_OK or GR_E_INVALID (malformed program).
+int lower_program(const gr_program & in, Program & out, uint32_t & refmask);
+// Host evaluation of a device-form program with the step kernels' semantics.
+float eval_lowered(const Program & pr, const float * ref19);
 
 struct StepParams
 {

@@ -292,59 +292,88 @@ __device__ float sample_scalar(const void * base, const FieldView & f, uint32_t 
 // expression interpreter (wavefront-uniform: the program lives in the kernel arguments, every
 // lane executes the same instruction stream)
 // --------------------------------------------------------------------------------------------
-// The top of the stack lives in a register; deeper entries live in LDS, one column per lane
-// (entry d of lane l at stk[d * blockDim.x + l]), so the wave-uniform stack depth turns into a
-// fixed LDS offset instead of dynamically indexed registers. The reference vector is staged in
-// LDS the same way (only the slots the programs read, `refmask`), so OP_REF is one LDS read.
+// Programs run in their device form (lower_program): binary operations with a leaf operand read
+// the leaf directly, and the two top stack values live in registers; deeper entries live in
+// LDS, one column per lane (entry d of lane l at stk[d * blockDim.x + l]), so the wave-uniform
+// stack depth turns into a fixed LDS offset instead of dynamically indexed registers. Reference
+// slots come from a uniform switch over the step's registers (`ref`).
 typedef __attribute__((address_space(3))) float lds_float;
 
 // One LDS area per kernel (a non-template function, so every inlined step variant of the
-// batched kernel shares it): 19 reference slots + (stack depth - 1) stack entries per lane.
+// batched kernel shares it): 19 reference slots (volumes and constants of the full kernel) +
+// (stack depth - 1) stack entries per lane.
 __device__ __forceinline__ lds_float * expr_lds()
 {
     __shared__ float buf[(19 + GR_MAX_STACK - 1) * 256];
     return (lds_float *)buf;
 }
 
-__device__ __forceinline__ float run_program(const Program & pr, const lds_float * refs,
-                                             lds_float * stk, uint32_t lane, uint32_t bs)
+// a op b for k = 0 add, 1 sub, 2 mul (branch-free: a - b is a + (-b) exactly), 3 div
+__device__ __forceinline__ float apply_op(uint32_t k, float a, float b)
 {
-    float acc = 0.f;
-    uint32_t depth = 0; // values on the stack, acc included (uniform)
+    if (k == 3) return a / b; // uniform branch
+    const float s = a + (k == 1 ? -b : b);
+    return k == 2 ? a * b : s;
+}
+
+// Reference slots x y z cv u v w of a step as one vector value: element d for a uniform d is an
+// indexed register read (no branches, no memory)
+typedef float refvec_t __attribute__((ext_vector_type(8)));
+
+// rv: reference slots x y z cv u v w of the step; R: this lane's LDS reference column (volumes
+// and constants, full kernel only)
+template <bool FULL>
+__device__ __forceinline__ float run_program(const Program & pr, const refvec_t rv,
+                                             const lds_float * R, lds_float * stk, uint32_t lane,
+                                             uint32_t bs)
+{
+    auto refv = [&](uint32_t d) -> float {
+        if (!FULL) return rv[d & 7u];
+        return d < 7 ? rv[d] : R[(d - 7) * bs];
+    };
+    float acc = 0.f, acc2 = 0.f;
+    uint32_t depth = 0; // values on the stack (uniform)
     for (uint32_t i = 0; i < pr.n; ++i)
     {
         const uint32_t w = pr.code[i];
-        const uint32_t op = w & 0xffu, arg = w >> 8;
-        if (op == OP_REF || op == OP_CONST)
+        const uint32_t cls = (w >> 4) & 0xfu, sub = w & 0xfu, arg = w >> 8;
+        if (cls == DC_LEAF) // the most common instruction first
         {
-            if (depth) stk[(depth - 1) * bs + lane] = acc;
-            acc = op == OP_REF ? refs[arg * bs + lane] : pr.imm[arg];
+            const float x = (sub & 8) ? pr.imm[arg] : refv(arg);
+            const bool rev = sub & 4;
+            acc = apply_op(sub & 3, rev ? x : acc, rev ? acc : x);
+        }
+        else if (cls == DC_PUSH)
+        {
+            const float x = sub ? pr.imm[arg] : refv(arg);
+            if (depth >= 2) stk[(depth - 2) * bs + lane] = acc2;
+            acc2 = acc;
+            acc = x;
             ++depth;
         }
-        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_ATAN2)
+        else if (cls == DC_BINARY)
         {
+            const bool rev = sub & 8;
+            const float a = rev ? acc : acc2, b = rev ? acc2 : acc;
+            if ((sub & 7) == 4) acc = atan2f(a, b);
+            else acc = apply_op(sub & 3, a, b);
             --depth;
-            const float a = stk[(depth - 1) * bs + lane];
-            if (op == OP_ADD) acc = a + acc;
-            else if (op == OP_SUB) acc = a - acc;
-            else if (op == OP_MUL) acc = a * acc;
-            else if (op == OP_DIV) acc = a / acc;
-            else acc = atan2f(a, acc);
+            if (depth >= 2) acc2 = stk[(depth - 2) * bs + lane];
         }
         else
         {
-            switch (op)
+            switch (sub)
             {
-                case OP_SQRT: acc = sqrtf(acc); break;
-                case OP_EXP: acc = expf(acc); break;
-                case OP_LN: acc = logf(acc); break;
-                case OP_SIN: acc = sinf(acc); break;
-                case OP_COS: acc = cosf(acc); break;
-                case OP_TAN: acc = tanf(acc); break;
-                case OP_ASIN: acc = asinf(acc); break;
-                case OP_ACOS: acc = acosf(acc); break;
-                case OP_ATAN: acc = atanf(acc); break;
-                default: acc = fabsf(acc); break; // OP_ABS
+                case 0: acc = sqrtf(acc); break;
+                case 1: acc = expf(acc); break;
+                case 2: acc = logf(acc); break;
+                case 3: acc = sinf(acc); break;
+                case 4: acc = cosf(acc); break;
+                case 5: acc = tanf(acc); break;
+                case 6: acc = asinf(acc); break;
+                case 7: acc = acosf(acc); break;
+                case 8: acc = atanf(acc); break;
+                default: acc = fabsf(acc); break;
             }
         }
     }
@@ -659,6 +688,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     uint64_t g = P.save_g0;
 
     uint32_t integrated = 0;
+    // |p - q| of the next step is |np - p| of this one (same operands, same order): carried
+    // over while consecutive steps both take the curvature branch
+    bool have_sb = false;
+    float sb_prev = 0.f;
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
         if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
@@ -872,14 +905,20 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                         const float cz = d1[0] * d2[1] - d1[1] * d2[0];
                         k = fabsf(sqrtf((cx * cx + cy * cy) + cz * cz) / ldp3);
                     }
-                    float a2 = 0.f, b2 = 0.f;
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d)
+                    float sa = sb_prev, b2 = 0.f;
+                    if (!have_sb)
                     {
-                        a2 += (p[d] - q[d]) * (p[d] - q[d]);
-                        b2 += (np[d] - p[d]) * (np[d] - p[d]);
+                        float a2 = 0.f;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) a2 += (p[d] - q[d]) * (p[d] - q[d]);
+                        sa = sqrtf(a2);
                     }
-                    const float ds = 0.5f * (sqrtf(a2) + sqrtf(b2));
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) b2 += (np[d] - p[d]) * (np[d] - p[d]);
+                    const float sb = sqrtf(b2);
+                    const float ds = 0.5f * (sa + sb);
+                    sb_prev = sb;
+                    have_sb = true;
                     if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) curv = k;
                     if (!isnan(k)) total += k * ds;
                 }
@@ -899,15 +938,12 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 
                 if (P.nprog > 0) // tpp:379-447
                 {
-                    // reference vector slots (tpp:381-439) staged in LDS, only those read
+                    // reference vector slots (tpp:381-439): position, curvature and velocity from
+                    // registers; volumes and constants (full kernel) staged in LDS, only those read
                     lds_float * s_refs = expr_lds();
                     lds_float * s_stk = s_refs + 19 * 256;
                     const uint32_t lane = threadIdx.x, bs = blockDim.x, m = P.refmask;
                     lds_float * R = s_refs + lane;
-                    if (m & (1u << 0)) R[0 * bs] = global_coord(P, np[0], 0);
-                    if (m & (1u << 1)) R[1 * bs] = global_coord(P, np[1], 1);
-                    if (m & (1u << 2)) R[2 * bs] = DIM == 3 ? global_coord(P, np[2], 2) : 0.f;
-                    if (m & (1u << 3)) R[3 * bs] = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f;
                     if (FULL)
                     {
 #pragma unroll
@@ -926,11 +962,17 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                             R[(12 + k) * bs] = cv;
                         }
                     }
-                    if (m & (1u << 16)) R[16 * bs] = vc[0];
-                    if (m & (1u << 17)) R[17 * bs] = vc[1];
-                    if (m & (1u << 18)) R[18 * bs] = DIM == 3 ? vc[2] : 0.f;
+                    float gx = 0.f, gy = 0.f, gz = 0.f;
+                    if (m & 7u)
+                    {
+                        gx = global_coord(P, np[0], 0);
+                        gy = global_coord(P, np[1], 1);
+                        if (DIM == 3) gz = global_coord(P, np[2], 2);
+                    }
+                    const refvec_t rv = {gx, gy, gz, P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f,
+                                         vc[0], vc[1], DIM == 3 ? vc[2] : 0.f, 0.f};
                     for (uint32_t e = 0; e < P.nprog; ++e)
-                        ex[e] = run_program(P.prog[e], s_refs, s_stk, lane, bs);
+                        ex[e] = run_program<FULL>(P.prog[e], rv, R, s_stk, lane, bs);
                 }
 
                 if (P.flags & F_KAHAN)

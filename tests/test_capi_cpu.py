@@ -91,3 +91,46 @@ def test_expression_postfix_layout():
     assert p.n == 12
     assert ops == [1, 1, 5, 1, 1, 5, 3, 1, 1, 5, 3, 7]
     assert [a for o, a in zip(ops, args) if o == 1] == [16, 16, 17, 17, 18, 18]
+
+
+def _random_expr(rng, depth):
+    """A random expression of the AdditionalCompute language (registers, constants, + - * /,
+    the functions) at most `depth` levels deep."""
+    if depth == 0 or rng.random() < 0.25:
+        if rng.random() < 0.7:
+            return str(rng.choice(["x", "y", "z", "cv", "u", "v", "w"]))
+        return f"{rng.uniform(0.1, 3.0):.3f}"
+    r = rng.random()
+    if r < 0.55:
+        op = str(rng.choice(["+", "-", "*", "/"]))
+        return f"({_random_expr(rng, depth - 1)}{op}{_random_expr(rng, depth - 1)})"
+    if r < 0.65:
+        return f"atan2({_random_expr(rng, depth - 1)}, {_random_expr(rng, depth - 1)})"
+    fn = str(rng.choice(["sqrt", "exp", "ln", "sin", "cos", "tan", "atan", "abs"]))
+    return f"{fn}({_random_expr(rng, depth - 1)})"
+
+
+def test_device_form_equals_postfix_bit_for_bit():
+    """The device form of a program (leaf operands folded into their operations, the two top
+    stack values in registers, reversed forms keeping operand order) computes exactly what the
+    postfix bytecode computes, on the bench expression, the listed expressions and 400 random
+    ones; and agrees with the oracle's own parser in FP32 bit for bit."""
+    rng = np.random.default_rng(7)
+    srcs = list(EXPRS) + [_random_expr(rng, 4) for _ in range(400)]
+    consts = ["temp", "rho", "c_2"]
+    env = {k: 13 + i for i, k in enumerate(consts)}
+    checked = 0
+    for src in srcs:
+        try:
+            prog = capi.compile_expr(src, dim=3, curvature=3, constants=consts, allow_abs=True)
+        except capi.GraniteError:
+            continue  # too long / too deep for the limits
+        for _ in range(6):
+            ref = rng.uniform(-2.0, 2.0, 19).astype(np.float32)
+            a = np.float32(capi.eval_expr_host(prog, ref))
+            b = np.float32(capi.eval_expr_device_form(prog, ref))
+            f32, _ = oracle.eval_expr(src, ref, env, allow_abs=True)
+            assert (a == b) or (np.isnan(a) and np.isnan(b)), src
+            assert (a == np.float32(f32)) or (np.isnan(a) and np.isnan(f32)), src
+        checked += 1
+    assert checked > 300
