@@ -1419,6 +1419,9 @@ static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, bool
     return GR_OK;
 }
 
+static int stage_reserve(gr_ctx * ctx, int k, size_t bytes);
+static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out);
+
 extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
 {
     if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
@@ -1430,6 +1433,18 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if (rc != GR_OK) return rc;
     hipStream_t s = j->ctx->compute;
     if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
+    // the launch's time table through the pinned staging ring (stream-ordered device copy)
+    gr_ctx * ctx = j->ctx;
+    const size_t tbytes = size_t(P.nsteps) * 4 * sizeof(float);
+    const int k = ctx->stage_i;
+    if ((rc = stage_reserve(ctx, k, tbytes)) != GR_OK) return rc;
+    if ((rc = grow(ctx->d_stage, ctx->d_stage_cap, tbytes)) != GR_OK) return rc;
+    time_table(P, nullptr, static_cast<float *>(ctx->stage[k].h));
+    GR_HIP(hipMemcpyAsync(ctx->d_stage, ctx->stage[k].h, tbytes, hipMemcpyHostToDevice, s));
+    GR_HIP(hipEventRecord(ctx->stage[k].done, s));
+    ctx->stage[k].pending = true;
+    ctx->stage_i = 1 - k;
+    P.ttab = reinterpret_cast<const float *>(ctx->d_stage);
     GR_HIP(hipEventRecord(j->ev0, s));
     if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
     // wide: at least two rounds of the wavefronts the device holds at three per SIMD
@@ -1529,6 +1544,54 @@ static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// The time weights of every step of a launch exactly as the step kernel forms them (FP32 time,
+// or FP64 with F_TIME64): t at the step's start, t + dt/2 and t + dt, each mapped into the
+// step's window (unit_interval, tpp:127-144), and 1 / 0 for "the step starts after time 0"
+// (tpp:347); four floats per step. edges: the window edges of a launch that crosses them.
+static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out)
+{
+    const float dt = P.dt, half_dt = 0.5f * dt;
+    const bool t64 = (P.flags & gr::F_TIME64) != 0;
+    float tb = P.tb, tf = P.tf;
+    double tb64 = P.tb64, tf64 = P.tf64;
+    float time32 = P.t_start;
+    double time64 = P.t_start64;
+    uint32_t wi = 0;
+    uint32_t edge = P.nwin > 1 ? edges[0] : 0xffffffffu;
+    auto unit = [&](float add) -> float {
+        if (t64) return float((time64 + double(add) - tb64) / (tf64 - tb64));
+        const float x = time32 + add;
+        return (x - tb) / (tf - tb);
+    };
+    for (uint32_t s = 0; s < P.nsteps; ++s)
+    {
+        if (s == edge)
+        {
+            ++wi;
+            const uint32_t c = P.win_first + wi;
+            tb = float(c) * P.dtd;
+            tf = float(c + 1) * P.dtd;
+            tb64 = double(c) * double(P.dtd);
+            tf64 = double(c + 1) * double(P.dtd);
+            edge = wi + 1 < P.nwin ? edges[wi] : 0xffffffffu;
+        }
+        float t = 0.f;
+        if (P.flags & gr::F_DYNAMIC_ANY) t = unit(0.f);
+        float t1 = t, t2 = t;
+        if (P.flags & gr::F_WIND_DYNAMIC)
+        {
+            t1 = unit(half_dt);
+            t2 = unit(dt);
+        }
+        out[4 * size_t(s) + 0] = t;
+        out[4 * size_t(s) + 1] = t1;
+        out[4 * size_t(s) + 2] = t2;
+        out[4 * size_t(s) + 3] = (t64 ? time64 > 0.0 : time32 > 0.f) ? 1.f : 0.f;
+        time32 += dt;
+        time64 += double(dt);
+    }
+}
 
 extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t block,
                             gr_span_end * ends, float * ms)
@@ -1729,7 +1792,16 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     const size_t off_p = align256(qwords * sizeof(uint32_t));
     const size_t off_w = align256(off_p + size_t(nspans) * sizeof(gr::StepParams));
     const size_t off_e = off_w + wptr.size() * sizeof(void *);
-    const size_t off_i = align256(off_e + wedge.size() * sizeof(uint32_t));
+    // per-step time tables of the spans (four floats per step)
+    std::vector<size_t> tofs(nspans, 0);
+    size_t tfloats = 0;
+    for (uint32_t i = 0; i < nspans; ++i)
+    {
+        tofs[i] = tfloats;
+        tfloats += 4 * size_t(span_steps[i]);
+    }
+    const size_t off_t = align256(off_e + wedge.size() * sizeof(uint32_t));
+    const size_t off_i = align256(off_t + tfloats * sizeof(float));
     const size_t dev_bytes = off_i;
     const size_t host_bytes = off_i + (items_dirty ? size_t(nitems) * sizeof(gr::BatchItem) : 0);
     const int k = ctx->stage_i;
@@ -1749,6 +1821,12 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         {
             params[i].win_ptr = reinterpret_cast<const void * const *>(dbase + off_w + 3 * wofs[i] * sizeof(void *));
             params[i].win_edge = reinterpret_cast<const uint32_t *>(dbase + off_e + wofs[i] * sizeof(uint32_t));
+        }
+    for (uint32_t i = 0; i < nspans; ++i)
+        if (span_steps[i])
+        {
+            time_table(params[i], wedge.data() + wofs[i], reinterpret_cast<float *>(h + off_t) + tofs[i]);
+            params[i].ttab = reinterpret_cast<const float *>(dbase + off_t) + tofs[i];
         }
     if (nspans) std::memcpy(h + off_p, params.data(), size_t(nspans) * sizeof(gr::StepParams));
     if (!wptr.empty()) std::memcpy(h + off_w, wptr.data(), wptr.size() * sizeof(void *));

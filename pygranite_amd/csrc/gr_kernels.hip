@@ -711,18 +711,27 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         if (live)
         {
             ++integrated;
-            float t = 0.f;
-            if (P.flags & F_DYNAMIC_ANY) t = unit_t(P, W, time32, time64, 0.f);
+            // time weights of this step: from the launch's table, or formed here
+            float4 tt = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (P.ttab)
+                tt = *reinterpret_cast<const float4 *>(P.ttab + 4 * size_t(s));
+            else
+            {
+                if (P.flags & F_DYNAMIC_ANY) tt.x = unit_t(P, W, time32, time64, 0.f);
+                tt.y = tt.z = tt.x;
+                if (INTEG == GR_RK4 && (P.flags & F_WIND_DYNAMIC))
+                {
+                    tt.y = unit_t(P, W, time32, time64, half_dt);
+                    tt.z = unit_t(P, W, time32, time64, dt);
+                }
+                tt.w = ((P.flags & F_TIME64) ? (time64 > 0.0) : (time32 > 0.f)) ? 1.f : 0.f;
+            }
+            const float t = tt.x;
             const bool sph = FULL && DIM == 2 && (P.flags & F_SPHERE);
             float v[3] = {0.f, 0.f, 0.f}, vc[3] = {0.f, 0.f, 0.f};
             if (INTEG == GR_RK4)
             {
-                float t1 = t, t2 = t;
-                if (P.flags & F_WIND_DYNAMIC)
-                {
-                    t1 = unit_t(P, W, time32, time64, half_dt);
-                    t2 = unit_t(P, W, time32, time64, dt);
-                }
+                const float t1 = tt.y, t2 = tt.z;
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
                 velocity<DIM, T, MODE>(P, W, p, t, k1, one);
@@ -865,7 +874,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                const bool tpos = (P.flags & F_TIME64) ? (time64 > 0.0) : (time32 > 0.f);
+                const bool tpos = tt.w != 0.f;
                 if (P.curvature == GR_CURV_FAST_TOTAL) // tpp:337-346
                 {
                     float l2 = 0.f;
