@@ -50,6 +50,15 @@ __device__ __forceinline__ float b_hi(uint32_t w) { return __uint_as_float(w & 0
 #define GR_LOAD(T, b, i) (reinterpret_cast<const T *>(b)[i])
 #endif
 
+// Read-only launch data (time tables, window tables) through the constant address space: the
+// compiler may then use scalar loads, which it must not do for global memory the kernel could
+// also write through another pointer.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GR_CONST(T, b, i) (((const __attribute__((address_space(4))) T *)(b))[i])
+#else
+#define GR_CONST(T, b, i) (reinterpret_cast<const T *>(b)[i])
+#endif
+
 // the same for state and output arrays reached through structs in global memory (the batched
 // kernel's job table): G(p)[i] is a global-address-space access
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -714,7 +723,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             // time weights of this step: from the launch's table, or formed here
             float4 tt = make_float4(0.f, 0.f, 0.f, 0.f);
             if (P.ttab)
-                tt = *reinterpret_cast<const float4 *>(P.ttab + 4 * size_t(s));
+                tt = GR_CONST(float4, P.ttab, s);
             else
             {
                 if (P.flags & F_DYNAMIC_ANY) tt.x = unit_t(P, W, time32, time64, 0.f);
