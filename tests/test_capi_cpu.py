@@ -43,7 +43,9 @@ def test_ctx_create_fails_cleanly_without_gpu():
 
 EXPRS = ["sqrt(u*u+v*v+w*w)", "w*1000/(abs(u)+abs(v)+1)", "1-2-3", "x*y+z/2-cv",
          "atan2(y, x) + sin(z)*cos(x) - tan(0.25)", "exp(-1.5)*ln(2.5)+asin(0.5)-acos(0.25)",
-         "atan(x) / (1 + u*u)", "((x))", "2*-3+x", "temp*rho - c_2", "(u+v)*(u-v)/(w+10)"]
+         "atan(x) / (1 + u*u)", "((x))", "2*-3+x", "temp*rho - c_2", "(u+v)*(u-v)/(w+10)",
+         "u*u+v*v", "x*y+z*w+u*v", "1+u*u+v*v", "u*u+v*v+w*w+x*x", "x*(u*u+v*v)",
+         "u*v+w*temp", "sqrt(u*u+v*v)-sqrt(w*w+x*x+y*y)"]
 
 
 @pytest.mark.parametrize("src", EXPRS)

@@ -84,3 +84,37 @@ def test_admission_respects_budget(gpu_available):
     assert len(srv.done) == len(reqs)
     assert peak[0] <= biggest + 1
     ctx.close()
+
+
+
+def test_job_whose_particles_all_stopped_retires_early(gpu_available):
+    """A large request whose particles all leave the domain early (a uniform flow towards the
+    lower x edge) is retired as soon as a regrouping has counted no alive particle (live range
+    0), long before its step budget, with the result of a server that never regroups: the
+    remaining steps cannot change it."""
+    ctx = capi.Context(0)
+    n, S = 24, 6
+    wind = np.zeros((S, 3, n, n, n), np.float32)
+    wind[:, 0] = -1.0
+    gs = [2 * math.pi / n] * 3
+    f = capi.Field(ctx, n, n, n, 3, capi.F16, nslots=S)
+    for s in range(S):
+        f.upload(s, [wind[s, c] for c in range(3)])
+    dom = [float(np.float32(n) * np.float32(g)) for g in gs]
+    ds = {"flow": serving.Dataset(f, gs, dom, 1.0, S)}
+    count = 8192
+    seeds = synthetic.uniform_seeds(77, count, [0.05, 0.1, 0.1], [1.0, 6.0, 6.0])
+
+    def req():
+        return serving.Request(rid=0, arrival=0.0, dataset="flow", seeds=seeds, steps=1000,
+                               integrator=1, dt=0.005)
+
+    lazy = serving.Server(ctx, ds, quantum=50, reorder=False).run([req()])[0]
+    srv = serving.Server(ctx, ds, quantum=50, reorder=True, reorder_every=1)
+    early = srv.run([req()])[0]
+    assert (lazy.status != 0).all()  # every particle left within 1.0 / 0.005 = 200 steps
+    assert early.steps_done == lazy.steps_done == 1000
+    np.testing.assert_array_equal(early.positions, lazy.positions)
+    np.testing.assert_array_equal(early.status, lazy.status)
+    assert srv.early == 1 and srv.rounds < 1000 // 50
+    ctx.close()
