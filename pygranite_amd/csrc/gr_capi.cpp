@@ -1104,9 +1104,25 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     }
     uint32_t jb = 0;
     while ((size_t(1) << jb) < js.size()) ++jb;
-    const uint32_t end_bit = 32 + jb;
+    // 32-bit keys when job index, Morton code and the stopped flag fit: a coordinate along an
+    // axis of g cells needs bitlen(g - 1) bits (at most 10 in 3D, 16 in 2D), a code dim times that
+    uint32_t mbits = 1;
+    for (const gr::SortJob & t : tab)
+        for (uint32_t d = 0; d < t.dim; ++d)
+        {
+            uint32_t b = 0;
+            while (b < (t.dim == 3 ? 10u : 16u) && (t.grid[d] - 1) >> b) ++b;
+            mbits = std::max(mbits, t.dim * b);
+        }
+    const bool k32 = jb + mbits + 1 <= 32;
+    const uint32_t kbits = k32 ? mbits : 0;
+    const uint32_t end_bit = k32 ? jb + mbits + 1 : 32 + jb;
+    uint32_t * k32_in = reinterpret_cast<uint32_t *>(ctx->skeys[0]);
+    uint32_t * k32_out = reinterpret_cast<uint32_t *>(ctx->skeys[1]);
     size_t need = 0;
-    if (gr::sort_pairs64(nullptr, need, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total, end_bit, s) != 0)
+    if ((k32 ? gr::sort_pairs32(nullptr, need, k32_in, k32_out, ctx->svals[0], ctx->svals[1], total, end_bit, s)
+             : gr::sort_pairs64(nullptr, need, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total,
+                                end_bit, s)) != 0)
         return set_err(GR_E_HIP, "radix sort sizing failed");
     if (need > ctx->stmp_bytes)
     {
@@ -1118,12 +1134,14 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     }
     if ((rc = grow(ctx->d_sjobs, ctx->sjobs_cap, tab.size())) != GR_OK) return rc;
     GR_HIP(hipMemcpyAsync(ctx->d_sjobs, tab.data(), tab.size() * sizeof(gr::SortJob), hipMemcpyHostToDevice, s));
-    if (gr::launch_morton_multi(ctx->d_sjobs, uint32_t(tab.size()), total, ctx->skeys[0], ctx->svals[0], s) != 0)
+    if (gr::launch_morton_multi(ctx->d_sjobs, uint32_t(tab.size()), total, ctx->skeys[0], ctx->svals[0], kbits, s) != 0)
         return set_err(GR_E_HIP, "morton key kernel failed");
     size_t tb = ctx->stmp_bytes;
-    if (gr::sort_pairs64(ctx->stmp, tb, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total, end_bit, s) != 0)
+    if ((k32 ? gr::sort_pairs32(ctx->stmp, tb, k32_in, k32_out, ctx->svals[0], ctx->svals[1], total, end_bit, s)
+             : gr::sort_pairs64(ctx->stmp, tb, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total,
+                                end_bit, s)) != 0)
         return set_err(GR_E_HIP, "radix sort failed");
-    if (gr::launch_alive_counts(ctx->d_sjobs, uint32_t(tab.size()), ctx->skeys[1], s) != 0)
+    if (gr::launch_alive_counts(ctx->d_sjobs, uint32_t(tab.size()), ctx->skeys[1], kbits, s) != 0)
         return set_err(GR_E_HIP, "alive count kernel failed");
     ++ctx->alive_seq;
     GR_HIP(hipEventRecord(ctx->alive_ev[ctx->alive_seq % gr_ctx::ALIVE_EVENTS], s));

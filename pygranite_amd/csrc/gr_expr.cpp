@@ -372,9 +372,38 @@ int lower_program(const gr_program & in, Program & out, uint32_t & refmask)
         return true;
     };
     if (in.n > GR_MAX_CODE) return GR_E_INVALID;
+    // register-slot reference at code[k] (dense index < 7), or -1
+    auto reg_ref = [&](uint32_t k) -> int {
+        if (k >= in.n || (in.code[k] & 0xffu) != OP_REF) return -1;
+        const uint32_t a = in.code[k] >> 8;
+        if (a >= 19) return -1;
+        const uint32_t d = dense_ref(a);
+        return d < 7 ? int(d) : -1;
+    };
+    auto op_at = [&](uint32_t k) -> uint32_t { return k < in.n ? in.code[k] & 0xffu : 0u; };
     for (uint32_t i = 0; i < in.n; ++i)
     {
         const uint32_t op = in.code[i] & 0xffu, arg = in.code[i] >> 8;
+        // a*b + c*d [+ e*f] over register slots: postfix r r * r r * + [r r * +]
+        if (op == OP_REF && sp < GR_MAX_STACK && reg_ref(i) >= 0 && reg_ref(i + 1) >= 0 &&
+            op_at(i + 2) == OP_MUL && reg_ref(i + 3) >= 0 && reg_ref(i + 4) >= 0 &&
+            op_at(i + 5) == OP_MUL && op_at(i + 6) == OP_ADD)
+        {
+            uint32_t packed = 0, terms = 2;
+            const uint32_t at[6] = {i, i + 1, i + 3, i + 4, i + 7, i + 8};
+            const bool third = reg_ref(i + 7) >= 0 && reg_ref(i + 8) >= 0 && op_at(i + 9) == OP_MUL &&
+                               op_at(i + 10) == OP_ADD;
+            if (third) terms = 3;
+            for (uint32_t k = 0; k < 2 * terms; ++k)
+            {
+                packed |= uint32_t(reg_ref(at[k])) << (3 * k);
+                refmask |= 1u << (in.code[at[k]] >> 8);
+            }
+            if (!emit(DC_DOT, terms, packed)) return GR_E_INVALID;
+            st[sp++] = Sym{0, 0};
+            i += third ? 10 : 6;
+            continue;
+        }
         if (op == OP_REF || op == OP_CONST)
         {
             if (sp >= GR_MAX_STACK || (op == OP_REF && arg >= 19) || (op == OP_CONST && arg >= GR_MAX_CODE))
@@ -438,9 +467,17 @@ float eval_lowered(const Program & pr, const float * ref)
             const float x = (sub & 8) ? pr.imm[arg] : refv(arg);
             acc = (sub & 4) ? apply(sub & 3, x, acc) : apply(sub & 3, acc, x);
         }
-        else if (cls == DC_PUSH)
+        else if (cls == DC_PUSH || cls == DC_DOT)
         {
-            const float x = sub ? pr.imm[arg] : refv(arg);
+            float x;
+            if (cls == DC_DOT)
+            {
+                auto r = [&](uint32_t k) { return refv((arg >> (3 * k)) & 7u); };
+                x = (r(0) * r(1)) + (r(2) * r(3));
+                if (sub == 3) x = x + (r(4) * r(5));
+            }
+            else
+                x = sub ? pr.imm[arg] : refv(arg);
             if (depth >= 2) deep[depth - 2] = acc2;
             acc2 = acc;
             acc = x;

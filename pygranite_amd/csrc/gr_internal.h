@@ -43,6 +43,8 @@ enum Op : uint32_t
 //   DC_LEAF   sub = k | rev << 2 | imm << 3 (k < 4): top op leaf, or leaf op top when rev; the
 //             leaf is ref[arg] or imm[arg]
 //   DC_UNARY  sub: 0 sqrt 1 exp 2 ln 3 sin 4 cos 5 tan 6 asin 7 acos 8 atan 9 abs
+//   DC_DOT    sub = 2 or 3: push (r0 * r1) + (r2 * r3) [+ (r4 * r5)], the sum formed left to
+//             right, with register slots r_k = arg >> 3k & 7 (the postfix of a*b + c*d [+ e*f])
 // Reference arguments are dense: 0..6 = slots x y z cv u v w (held in registers), 7 + s = slot
 // s (volumes and constants, staged in LDS by the full kernel).
 enum DevClass : uint32_t
@@ -50,7 +52,8 @@ enum DevClass : uint32_t
     DC_PUSH = 0,
     DC_BINARY = 1,
     DC_LEAF = 2,
-    DC_UNARY = 3
+    DC_UNARY = 3,
+    DC_DOT = 4
 };
 
 // Slot views of one field: node (i, j, k) of a slot starts at element
@@ -253,18 +256,23 @@ struct GatherEntry
     uint32_t n;
     uint32_t pad;
 };
-// keys (job << 32 | Morton code of the current cell, stopped particles last) and in-job indices
-int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, uint64_t * keys,
-                        uint32_t * vals, void * stream);
+// keys and in-job indices: 64-bit keys job << 32 | Morton code of the current cell (stopped
+// particles last) when mbits = 0; else 32-bit keys job << (mbits + 1) | code, stopped particles
+// 1 << mbits (every code below 2^mbits)
+int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, void * keys,
+                        uint32_t * vals, uint32_t mbits, void * stream);
 // radix sort of (key, value) pairs on key bits [0, end_bit); temp storage sized by a first call
 // with temp == null
 int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
                  const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
+int sort_pairs32(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
+                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
 // all entries in one launch (maxn: largest entry n)
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
 // per job of a sorted multi-job key array: the number of keys before its first stopped key
-int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const uint64_t * sorted_keys,
-                        void * stream);
+// (mbits as for launch_morton_multi)
+int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const void * sorted_keys,
+                        uint32_t mbits, void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
                      void * stream);

@@ -352,9 +352,16 @@ __device__ __forceinline__ float run_program(const Program & pr, const refvec_t 
             const bool rev = sub & 4;
             acc = apply_op(sub & 3, rev ? x : acc, rev ? acc : x);
         }
-        else if (cls == DC_PUSH)
+        else if (cls == DC_PUSH || cls == DC_DOT)
         {
-            const float x = sub ? pr.imm[arg] : refv(arg);
+            float x;
+            if (cls == DC_DOT) // (r0 * r1) + (r2 * r3) [+ (r4 * r5)], register slots
+            {
+                x = (rv[arg & 7u] * rv[(arg >> 3) & 7u]) + (rv[(arg >> 6) & 7u] * rv[(arg >> 9) & 7u]);
+                if (sub == 3) x = x + (rv[(arg >> 12) & 7u] * rv[(arg >> 15) & 7u]);
+            }
+            else
+                x = sub ? pr.imm[arg] : refv(arg);
             if (depth >= 2) stk[(depth - 2) * bs + lane] = acc2;
             acc2 = acc;
             acc = x;
@@ -1548,8 +1555,12 @@ __device__ __forceinline__ uint32_t spread2(uint32_t x) // 16 bits -> every seco
 // Keys of a reorder of several jobs at once: key = (job << 32) | Morton code of the particle's
 // cell (stopped particles 0xffffffff, i.e. last), value = the particle's index in its job. One
 // radix sort over all keys then orders every job by cell inside its own key range.
+// Key of particle g: its job, then the Morton code of its current cell, stopped particles last
+// within the job. 64-bit keys: job << 32 | code (stopped: low word all ones). 32-bit keys
+// (mbits > 0, every code below 2^mbits): job << (mbits + 1) | code, stopped: 1 << mbits.
+template <class K>
 __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs, uint64_t total,
-                               uint64_t * __restrict__ keys, uint32_t * __restrict__ vals)
+                               K * __restrict__ keys, uint32_t * __restrict__ vals, uint32_t mbits)
 {
     const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= total) return;
@@ -1562,8 +1573,9 @@ __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs
     }
     const SortJob & J = jobs[lo];
     const uint32_t i = uint32_t(g - J.base);
-    uint32_t code = 0xffffffffu;
-    if (J.status[i] == GR_STATUS_TIME)
+    const bool alive = J.status[i] == GR_STATUS_TIME;
+    uint32_t code = 0;
+    if (alive)
     {
         const uint32_t lim = J.dim == 3 ? 1023u : 65535u;
         uint32_t c[3] = {0, 0, 0};
@@ -1575,16 +1587,20 @@ __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs
         }
         code = J.dim == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
                           : (spread2(c[0]) | (spread2(c[1]) << 1));
-        code = min(code, 0xfffffffeu);
     }
-    keys[g] = (uint64_t(lo) << 32) | code;
+    if (sizeof(K) == 8)
+        keys[g] = K((uint64_t(lo) << 32) | (alive ? min(code, 0xfffffffeu) : 0xffffffffu));
+    else
+        keys[g] = K((lo << (mbits + 1)) | (alive ? code : (1u << mbits)));
     vals[g] = i;
 }
 
-// One thread per job: its keys [base, base + n) are sorted with the stopped particles (low word
-// 0xffffffff) last; the first of those is found by bisection.
+// One thread per job: its keys [base, base + n) are sorted with the stopped particles last
+// (64-bit keys: low word 0xffffffff; 32-bit keys: bit mbits set); the first of those is found by
+// bisection.
+template <class K>
 __global__ void k_alive_counts(const SortJob * __restrict__ jobs, uint32_t njobs,
-                               const uint64_t * __restrict__ keys)
+                               const K * __restrict__ keys, uint32_t mbits)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= njobs) return;
@@ -1593,33 +1609,52 @@ __global__ void k_alive_counts(const SortJob * __restrict__ jobs, uint32_t njobs
     while (lo < hi)
     {
         const uint64_t mid = (lo + hi) / 2;
-        if (uint32_t(keys[J.base + mid]) == 0xffffffffu) hi = mid;
+        const K key = keys[J.base + mid];
+        const bool stopped = sizeof(K) == 8 ? uint32_t(key) == 0xffffffffu : ((key >> mbits) & 1u) != 0;
+        if (stopped) hi = mid;
         else lo = mid + 1;
     }
     if (J.alive_out) *J.alive_out = uint32_t(lo);
 }
 
-int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const uint64_t * sorted_keys, void * stream)
+int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const void * sorted_keys,
+                        uint32_t mbits, void * stream)
 {
     if (njobs == 0) return 0;
     const uint32_t block = 64;
-    hipLaunchKernelGGL(k_alive_counts, dim3((njobs + block - 1) / block), dim3(block), 0,
-                       hipStream_t(stream), jobs, njobs, sorted_keys);
+    if (mbits)
+        hipLaunchKernelGGL(k_alive_counts<uint32_t>, dim3((njobs + block - 1) / block), dim3(block), 0,
+                           hipStream_t(stream), jobs, njobs, static_cast<const uint32_t *>(sorted_keys), mbits);
+    else
+        hipLaunchKernelGGL(k_alive_counts<uint64_t>, dim3((njobs + block - 1) / block), dim3(block), 0,
+                           hipStream_t(stream), jobs, njobs, static_cast<const uint64_t *>(sorted_keys), 0u);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, uint64_t * keys,
-                        uint32_t * vals, void * stream)
+int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, void * keys,
+                        uint32_t * vals, uint32_t mbits, void * stream)
 {
     const uint32_t block = 256;
     const uint64_t grid = (total + block - 1) / block;
     if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_morton_multi, dim3(uint32_t(grid)), dim3(block), 0, hipStream_t(stream),
-                       jobs, njobs, total, keys, vals);
+    if (mbits)
+        hipLaunchKernelGGL(k_morton_multi<uint32_t>, dim3(uint32_t(grid)), dim3(block), 0,
+                           hipStream_t(stream), jobs, njobs, total, static_cast<uint32_t *>(keys), vals, mbits);
+    else
+        hipLaunchKernelGGL(k_morton_multi<uint64_t>, dim3(uint32_t(grid)), dim3(block), 0,
+                           hipStream_t(stream), jobs, njobs, total, static_cast<uint64_t *>(keys), vals, 0u);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
+                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream)
+{
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
+                                                            int(n), 0, int(end_bit), hipStream_t(stream));
+    return e == hipSuccess ? 0 : -1;
+}
+
+int sort_pairs32(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
                  const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream)
 {
     const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
