@@ -1465,9 +1465,13 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     P.ttab = reinterpret_cast<const float *>(ctx->d_stage);
     GR_HIP(hipEventRecord(j->ev0, s));
     if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
-    // wide: at least two rounds of the wavefronts the device holds at three per SIMD
-    const bool wide = j->run_n >= 2ull * uint64_t(std::max(1, j->ctx->cus)) * 4 * 3 * 64;
-    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, wide, s) != 0)
+    // wide: at least two rounds of the wavefronts the device holds at three per SIMD; narrow:
+    // every wavefront fits at two per SIMD
+    const uint64_t cus = uint64_t(std::max(1, j->ctx->cus));
+    const bool wide = j->run_n >= 2ull * cus * 4 * 3 * 64;
+    const bool narrow = j->run_n <= cus * 4 * 2 * 64;
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, wide, narrow,
+                           s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if ((rc = kt_mark(j->ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev1, s));
@@ -1868,8 +1872,10 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     uint32_t * dq = reinterpret_cast<uint32_t *>(dbase);
     GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
+    // narrow: every item's wavefront fits on the device at two per SIMD
+    const bool narrow = uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
     if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
-                                   block, ctx->batch_waves, s) != 0)
+                                   block, ctx->batch_waves, narrow, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     if (nitems && (rc = kt_mark(ctx, s, true)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(ctx->bev1, s));

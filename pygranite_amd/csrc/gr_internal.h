@@ -214,10 +214,12 @@ enum : uint32_t
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype);
 // culut[cu_key] = dense index of that CU (0xffffffff: none); a CU's queue is index / group;
 // work: the work counter slots
-// waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD)
+// waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD);
+// narrow: every item's wavefront fits on the device at once (selects the build that reuses FP16
+// stencils across RK4 stages, at two wavefronts per SIMD)
 int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
                  const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
-                 uint32_t block, uint32_t waves, void * stream);
+                 uint32_t block, uint32_t waves, bool narrow, void * stream);
 // writes cu_key() of the CU running each of n workgroups
 int launch_cu_probe(uint32_t * out, uint32_t n, void * stream);
 int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
@@ -226,9 +228,10 @@ int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
 // nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range);
 // nrun (<= P.n): the launch covers particles [0, nrun) -- every particle from nrun on is stopped;
 // wide: the launch holds many more wavefronts than the device runs at once (selects a register
-// budget of four wavefronts per SIMD where a kernel variant has one)
+// budget of four wavefronts per SIMD where a kernel variant has one); narrow: all its wavefronts
+// fit at two per SIMD (selects the FP16 build that reuses stencils across RK4 stages)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, void * stream);
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, void * stream);
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);

@@ -509,9 +509,24 @@ __device__ __forceinline__ void texel_arg(const StepParams & P, const float * x,
     if (DIM == 3 && P.wind.zlev) arg[2] = zcoord(P.wind, x[2] - P.off[2]);
 }
 
-template <int DIM, class T, int MODE>
+// The last FP16 time-paired stencil a lane loaded: its base node and the raw words of its
+// corners (3D: words x, y, z of the eight 16-byte nodes, the fourth being padding; 2D: both words
+// of the four 8-byte nodes). The RK4 stages of a step mostly sample the same cell (a particle
+// crosses about a fifth of a cell per step), so with CACHE a lane whose base node is unchanged
+// reuses the words instead of loading the stencil again: the same data, the same result. Holding
+// the words costs about 50 VGPRs (two wavefronts per SIMD instead of three), so the kernels
+// built with CACHE serve narrow launches, whose wavefronts all fit on the device at once and
+// whose steps are bound by one wavefront's latency.
+template <int DIM>
+struct Stencil16
+{
+    uint32_t base;
+    uint32_t w[DIM == 3 ? 8 : 4][DIM == 3 ? 3 : 2];
+};
+
+template <int DIM, class T, int MODE, bool CACHE>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
-                                         float t, float * v, f16_t one)
+                                         float t, float * v, f16_t one, Stencil16<DIM> & C)
 {
     float arg[3] = {0.f, 0.f, 0.f};
     texel_arg<DIM>(P, x, arg);
@@ -532,19 +547,33 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         float b[DIM], f[DIM];
         if constexpr (std::is_same<T, __half>::value)
         {
+            if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
+            {
+#pragma unroll
+                for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+                {
+                    const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
+                    if (DIM == 3)
+                    {
+                        const uint4 h = GR_LOAD(uint4, W.pair, node);
+                        C.w[c][0] = h.x;
+                        C.w[c][1] = h.y;
+                        C.w[c][2] = h.z;
+                    }
+                    else
+                    {
+                        const uint2 h = GR_LOAD(uint2, W.pair, node);
+                        C.w[c][0] = h.x;
+                        C.w[c][1] = h.y;
+                    }
+                }
+                C.base = base;
+            }
             uint4 raw[8];
 #pragma unroll
             for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
-            {
-                const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
-                if (DIM == 3)
-                    raw[c] = GR_LOAD(uint4, W.pair, node);
-                else
-                {
-                    const uint2 h = GR_LOAD(uint2, W.pair, node);
-                    raw[c] = make_uint4(h.x, h.y, 0u, 0u);
-                }
-            }
+                raw[c] = DIM == 3 ? make_uint4(C.w[c][0], C.w[c][1], C.w[c][2], 0u)
+                                  : make_uint4(C.w[c][0], C.w[c][1], 0u, 0u);
             combine_pair_h<DIM>(raw, S, one, b, f);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
@@ -651,7 +680,7 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
 
 // Advances one particle by the launch's steps; returns the number of steps it integrated
 // (steps begun alive, the aborting step included).
-template <int DIM, int INTEG, class T, int MODE>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false>
 __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint32_t pid)
 {
     constexpr bool FULL = MODE == M_FULL;
@@ -708,6 +737,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     // over while consecutive steps both take the curvature branch
     bool have_sb = false;
     float sb_prev = 0.f;
+    Stencil16<DIM> sc;
+    sc.base = 0xffffffffu; // no stencil loaded (also after every window change)
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
         if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
@@ -722,6 +753,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             W.tb64 = double(c) * double(P.dtd);
             W.tf64 = double(c + 1) * double(P.dtd);
             edge = wi + 1 < P.nwin ? G(P.win_edge)[wi] : 0xffffffffu;
+            sc.base = 0xffffffffu;
         }
         const bool live = status == GR_STATUS_TIME; // alive at the start of step s
         if (live)
@@ -750,22 +782,22 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 const float t1 = tt.y, t2 = tt.z;
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE>(P, W, p, t, k1, one);
+                velocity<DIM, T, MODE, CACHE>(P, W, p, t, k1, one, sc);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE>(P, W, x, t1, k2, one);
+                velocity<DIM, T, MODE, CACHE>(P, W, x, t1, k2, one, sc);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE>(P, W, x, t1, k3, one);
+                velocity<DIM, T, MODE, CACHE>(P, W, x, t1, k3, one, sc);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE>(P, W, x, t2, k4, one);
+                velocity<DIM, T, MODE, CACHE>(P, W, x, t2, k4, one, sc);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -777,7 +809,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                velocity<DIM, T, MODE>(P, W, p, t, vc, one);
+                velocity<DIM, T, MODE, CACHE>(P, W, p, t, vc, one, sc);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -1088,13 +1120,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint
     return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-template <int DIM, int INTEG, class T, int MODE>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false>
 __device__ __forceinline__ void advance_block(const StepParams & P, uint32_t nxcd)
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
     const uint32_t pid = b * blockDim.x + threadIdx.x;
     uint32_t w = 0;
-    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE>(P, pid);
+    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE>(P, pid);
     if (P.work) // block sizes are multiples of 64
         wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
 }
@@ -1116,14 +1148,21 @@ k_advance_w4(const StepParams P, uint32_t nxcd)
     advance_block<DIM, INTEG, T, MODE>(P, nxcd);
 }
 
+// The FP16 time-paired kernel with the stencil reuse (CACHE) for narrow launches.
+template <int DIM, int INTEG, class T, int MODE>
+__global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE, true>(P, nxcd);
+}
+
 // Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
 // until the list is exhausted; each item runs its job's whole span of steps. Jobs of different
 // dimension, integrator and storage type share the launch; the variant is uniform per item.
-template <int DIM, int INTEG, class T>
+template <int DIM, int INTEG, class T, bool CACHE>
 __device__ __forceinline__ uint32_t run_item(const StepParams & P, uint32_t pid)
 {
     // the host builds a paired copy only for jobs that use it (dynamic wind, paired option)
-    if (P.wind.pair) return advance_one<DIM, INTEG, T, M_PAIRED>(P, pid);
+    if (P.wind.pair) return advance_one<DIM, INTEG, T, M_PAIRED, CACHE>(P, pid);
     return advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
 }
 
@@ -1158,7 +1197,7 @@ int launch_cu_probe(uint32_t * out, uint32_t n, void * stream)
 // set, and exit when no bit is left. Every wavefront accumulates its integrated particle-steps
 // and adds them once, into one of WORK_SLOTS counters. WAVES is the minimum number of wavefronts
 // per SIMD the register allocation must allow (1: no limit; 4: at most 128 VGPRs).
-template <int WAVES>
+template <int WAVES, bool CACHE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ items,
         uint32_t * qdata, uint32_t nq, const uint32_t * __restrict__ culut, uint32_t group,
@@ -1191,18 +1230,18 @@ k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ ite
                 uint32_t w;
                 switch (b.variant)
                 {
-                    case 0: w = run_item<2, GR_EULER, float>(P, pid); break;
-                    case 1: w = run_item<2, GR_EULER, __half>(P, pid); break;
-                    case 2: w = run_item<2, GR_EULER, bf16_t>(P, pid); break;
-                    case 3: w = run_item<2, GR_RK4, float>(P, pid); break;
-                    case 4: w = run_item<2, GR_RK4, __half>(P, pid); break;
-                    case 5: w = run_item<2, GR_RK4, bf16_t>(P, pid); break;
-                    case 6: w = run_item<3, GR_EULER, float>(P, pid); break;
-                    case 7: w = run_item<3, GR_EULER, __half>(P, pid); break;
-                    case 8: w = run_item<3, GR_EULER, bf16_t>(P, pid); break;
-                    case 9: w = run_item<3, GR_RK4, float>(P, pid); break;
-                    case 10: w = run_item<3, GR_RK4, __half>(P, pid); break;
-                    default: w = run_item<3, GR_RK4, bf16_t>(P, pid); break;
+                    case 0: w = run_item<2, GR_EULER, float, CACHE>(P, pid); break;
+                    case 1: w = run_item<2, GR_EULER, __half, CACHE>(P, pid); break;
+                    case 2: w = run_item<2, GR_EULER, bf16_t, CACHE>(P, pid); break;
+                    case 3: w = run_item<2, GR_RK4, float, CACHE>(P, pid); break;
+                    case 4: w = run_item<2, GR_RK4, __half, CACHE>(P, pid); break;
+                    case 5: w = run_item<2, GR_RK4, bf16_t, CACHE>(P, pid); break;
+                    case 6: w = run_item<3, GR_EULER, float, CACHE>(P, pid); break;
+                    case 7: w = run_item<3, GR_EULER, __half, CACHE>(P, pid); break;
+                    case 8: w = run_item<3, GR_EULER, bf16_t, CACHE>(P, pid); break;
+                    case 9: w = run_item<3, GR_RK4, float, CACHE>(P, pid); break;
+                    case 10: w = run_item<3, GR_RK4, __half, CACHE>(P, pid); break;
+                    default: w = run_item<3, GR_RK4, bf16_t, CACHE>(P, pid); break;
                 }
                 acc += w;
             }
@@ -1244,14 +1283,17 @@ int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
 
 int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
                  const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
-                 uint32_t block, uint32_t waves, void * stream)
+                 uint32_t block, uint32_t waves, bool narrow, void * stream)
 {
     if (nq == 0 || grid == 0 || group == 0) return 0;
-    if (waves >= 4)
-        hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+    if (narrow)
+        hipLaunchKernelGGL((k_batch<1, true>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+                           qdata, nq, culut, group, work);
+    else if (waves >= 4)
+        hipLaunchKernelGGL((k_batch<4, false>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
                            qdata, nq, culut, group, work);
     else
-        hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
+        hipLaunchKernelGGL((k_batch<1, false>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
                            qdata, nq, culut, group, work);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1259,8 +1301,8 @@ int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qd
 int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu)
 {
     const hipError_t e =
-        waves >= 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<4>, int(block), 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<1>, int(block), 0);
+        waves >= 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<4, false>, int(block), 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<1, false>, int(block), 0);
     return e == hipSuccess ? 0 : -1;
 }
 
@@ -1379,7 +1421,7 @@ static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
 }
 
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, void * stream)
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, void * stream)
 {
     AdvanceFn fn;
     if (wide && mode == M_PAIRED && P.wind.dtype == GR_F16)
@@ -1390,6 +1432,15 @@ int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint
         else
             fn = integrator == GR_RK4 ? k_advance_w4<2, GR_RK4, __half, M_PAIRED>
                                       : k_advance_w4<2, GR_EULER, __half, M_PAIRED>;
+    }
+    else if (narrow && mode == M_PAIRED && P.wind.dtype == GR_F16)
+    {
+        if (dim == 3)
+            fn = integrator == GR_RK4 ? k_advance_narrow<3, GR_RK4, __half, M_PAIRED>
+                                      : k_advance_narrow<3, GR_EULER, __half, M_PAIRED>;
+        else
+            fn = integrator == GR_RK4 ? k_advance_narrow<2, GR_RK4, __half, M_PAIRED>
+                                      : k_advance_narrow<2, GR_EULER, __half, M_PAIRED>;
     }
     else if (dim == 3)
         fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, mode)
