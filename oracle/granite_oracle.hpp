@@ -119,7 +119,9 @@ inline bool any_dynamic(const Settings & s)
 // -------------------------------------------------------------------------------------------
 // Sampling. arg is in texel units: arg = (p - Offset) * (1 / GridScale)  (icc:112-129).
 // -------------------------------------------------------------------------------------------
-template <class R> inline R lerp(R a, R b, R t) { return a + t * (b - a); } // LinearAlgebra.hpp:617-621
+// a + t * (b - a) (LinearAlgebra.hpp:617-621) evaluated as fma(t, b - a, a): the form nvcc
+// compiles it to in the reference's kernels (multiply-add contraction is nvcc's default)
+template <class R> inline R lerp(R a, R b, R t) { return std::fma(t, b - a, a); }
 
 template <class R> inline R quant(R w, int mode)
 {

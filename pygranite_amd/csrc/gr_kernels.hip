@@ -133,7 +133,9 @@ __device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint6
 // --------------------------------------------------------------------------------------------
 // sampling helpers
 // --------------------------------------------------------------------------------------------
-__device__ __forceinline__ float lerp(float a, float b, float t) { return a + t * (b - a); }
+// a + t * (b - a) as one fused multiply-add (the contraction nvcc applies to the reference's
+// lerp, LinearAlgebra.hpp:617-621; the oracle evaluates the same form)
+__device__ __forceinline__ float lerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
 
 // texel coordinate -> (index, weight). Clamping to [-1, n] leaves every tap outside the grid
 // and keeps the conversion defined (NaN -> -1).
@@ -203,17 +205,17 @@ __device__ __forceinline__ void combine(const float (&v)[8][NC], const Taps & T,
 }
 
 // FP16 corner values enter the first (x) lerp without separate conversions: with f16 operands
-// fma(b, 1, -a) is b - a and fma(a, 1, x) is a + x, each rounded once after the exact f16 -> f32
-// conversion, i.e. the same IEEE results as lerp((float)a, (float)b, t); the compiler forms
-// v_fma_mix_f32 from them. `one` is 1.0 hidden from constant folding (which would turn the fma
-// back into an add of converted values).
+// fma(b, 1, -a) is b - a and fma(t, d, a) is lerp's fused form, each rounded once after the
+// exact f16 -> f32 conversion, i.e. the same IEEE results as lerp((float)a, (float)b, t); the
+// compiler forms v_fma_mix_f32 from them. `one` is 1.0 hidden from constant folding (which would
+// turn the first fma back into a subtraction of converted values).
 typedef _Float16 f16_t;
 __device__ __forceinline__ f16_t f16_lo(uint32_t w) { return __builtin_bit_cast(f16_t, uint16_t(w & 0xffffu)); }
 __device__ __forceinline__ f16_t f16_hi(uint32_t w) { return __builtin_bit_cast(f16_t, uint16_t(w >> 16)); }
 __device__ __forceinline__ float lerp_h(f16_t a, f16_t b, float t, f16_t one)
 {
     const float d = __builtin_fmaf(float(b), float(one), -float(a));
-    return __builtin_fmaf(float(a), float(one), t * d);
+    return __builtin_fmaf(t, d, float(a));
 }
 
 // Component c of slice s (0 back, 1 front) of a time-paired FP16 node: 3D words
