@@ -275,13 +275,15 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
 int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
 
 /* Reorders the resident particle state by the Morton code of each particle's current cell
- * (stopped particles last), so that neighbouring lanes gather neighbouring stencils. Results
- * stay addressed by the caller's particle index: gr_job_read and gr_job_read_saved return
- * them in the original order. Saved columns written before the call must be read before it.
- * No counterpart in the reference (its particle order is fixed). */
+ * (stopped particles last), so that neighbouring lanes gather neighbouring stencils, and counts
+ * the alive particles it places first (see gr_job_live_range). Results stay addressed by the
+ * caller's particle index: gr_job_read and gr_job_read_saved return them in the original order.
+ * Saved columns written before the call must be read before it. Waits until the job's previous
+ * reorder has run on the device (so a caller queueing many launches stays at most one reorder
+ * ahead). No counterpart in the reference (its particle order is fixed). */
 int gr_job_reorder(gr_job * job);
-/* gr_job_reorder for several jobs of one context in one pass (one key kernel, one radix sort,
- * one gather launch for all their state arrays). */
+/* gr_job_reorder for several jobs of one context in one pass (one counting sort by cell over
+ * all their particles, one gather launch for all their state arrays). */
 int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs);
 
 /* ---- batches: many jobs in one persistent launch -------------------------------------------

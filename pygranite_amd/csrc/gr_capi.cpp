@@ -105,6 +105,9 @@ struct gr_ctx
     size_t stmp_bytes{0};
     gr::SortJob * d_sjobs{nullptr};
     size_t sjobs_cap{0};
+    uint32_t * bcount{nullptr}; // bucket counts / starts of the reorder's counting sort
+    uint32_t * bstart{nullptr};
+    size_t bcap{0};
     gr::GatherEntry * d_gtab{nullptr};
     size_t gtab_cap{0};
     hipEvent_t bev0{nullptr}, bev1{nullptr};
@@ -339,6 +342,8 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     }
     dfree(c->stmp);
     dfree(c->d_sjobs);
+    dfree(c->bcount);
+    dfree(c->bstart);
     dfree(c->d_gtab);
     if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->bev1) (void)hipEventDestroy(c->bev1);
@@ -1008,9 +1013,9 @@ static int alive_resolve(gr_job * j, bool wait)
 }
 
 // Reorders several jobs of one context by the Morton code of each particle's current cell
-// (stopped particles last): one key kernel, one radix sort over all particles (the job index in
-// the high key bits keeps every job in its own range) and one gather launch for every state
-// array of every job, each gathered into its shadow copy, which then takes the array's place.
+// (stopped particles last): one counting sort over all particles (buckets are per job, which
+// keeps every job in its own range) and one gather launch for every state array of every job,
+// each gathered into its shadow copy, which then takes the array's place.
 // Only a job's live range [0, run_n) takes part (later slots hold stopped particles, which
 // stay where they are): when it is at most half the job, the gathered part is copied back
 // into the arrays; otherwise the stopped tail is copied into the shadows before the swap.
@@ -1102,10 +1107,10 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
                 return rc;
         ctx->scap = total;
     }
-    uint32_t jb = 0;
-    while ((size_t(1) << jb) < js.size()) ++jb;
-    // 32-bit keys when job index, Morton code and the stopped flag fit: a coordinate along an
-    // axis of g cells needs bitlen(g - 1) bits (at most 10 in 3D, 16 in 2D), a code dim times that
+    // Buckets: the Morton code of the cell (a coordinate along an axis of g cells needs
+    // bitlen(g - 1) bits, at most 10 in 3D and 16 in 2D; a code dim times that) with its low
+    // `shift` bits dropped, so that a job has at most 2^18 cell buckets and all jobs at most 2^24;
+    // one more bucket per job takes its stopped particles.
     uint32_t mbits = 1;
     for (const gr::SortJob & t : tab)
         for (uint32_t d = 0; d < t.dim; ++d)
@@ -1114,35 +1119,42 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
             while (b < (t.dim == 3 ? 10u : 16u) && (t.grid[d] - 1) >> b) ++b;
             mbits = std::max(mbits, t.dim * b);
         }
-    const bool k32 = jb + mbits + 1 <= 32;
-    const uint32_t kbits = k32 ? mbits : 0;
-    const uint32_t end_bit = k32 ? jb + mbits + 1 : 32 + jb;
-    uint32_t * k32_in = reinterpret_cast<uint32_t *>(ctx->skeys[0]);
-    uint32_t * k32_out = reinterpret_cast<uint32_t *>(ctx->skeys[1]);
+    uint32_t bbits = 18;
+    while (bbits > 1 && (uint64_t(js.size()) << bbits) > (uint64_t(1) << 24)) --bbits;
+    const uint32_t shift = mbits > bbits ? mbits - bbits : 0;
+    const uint32_t nb = (1u << (mbits - shift)) + 1;
+    const uint64_t nbt = uint64_t(js.size()) * nb;
+    if (nbt > ctx->bcap)
+    {
+        dfree(ctx->bcount);
+        dfree(ctx->bstart);
+        ctx->bcount = ctx->bstart = nullptr;
+        ctx->bcap = 0;
+        if ((rc = dalloc(&ctx->bcount, nbt * sizeof(uint32_t))) != GR_OK ||
+            (rc = dalloc(&ctx->bstart, nbt * sizeof(uint32_t))) != GR_OK)
+            return rc;
+        ctx->bcap = nbt;
+    }
+    uint32_t * keys = reinterpret_cast<uint32_t *>(ctx->skeys[0]);
     size_t need = 0;
-    if ((k32 ? gr::sort_pairs32(nullptr, need, k32_in, k32_out, ctx->svals[0], ctx->svals[1], total, end_bit, s)
-             : gr::sort_pairs64(nullptr, need, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total,
-                                end_bit, s)) != 0)
-        return set_err(GR_E_HIP, "radix sort sizing failed");
-    if (need > ctx->stmp_bytes)
+    if (gr::launch_bucket_sort(ctx->d_sjobs, uint32_t(tab.size()), total, shift, nb, keys, ctx->svals[0],
+                               ctx->bcount, ctx->bstart, nullptr, need, ctx->svals[1], s) != 0)
+        return set_err(GR_E_HIP, "bucket sort sizing failed");
+    if (need > ctx->stmp_bytes || !ctx->stmp)
     {
         dfree(ctx->stmp);
         ctx->stmp = nullptr;
         ctx->stmp_bytes = 0;
+        need = std::max<size_t>(need, 256);
         if ((rc = dalloc(&ctx->stmp, need)) != GR_OK) return rc;
         ctx->stmp_bytes = need;
     }
     if ((rc = grow(ctx->d_sjobs, ctx->sjobs_cap, tab.size())) != GR_OK) return rc;
     GR_HIP(hipMemcpyAsync(ctx->d_sjobs, tab.data(), tab.size() * sizeof(gr::SortJob), hipMemcpyHostToDevice, s));
-    if (gr::launch_morton_multi(ctx->d_sjobs, uint32_t(tab.size()), total, ctx->skeys[0], ctx->svals[0], kbits, s) != 0)
-        return set_err(GR_E_HIP, "morton key kernel failed");
     size_t tb = ctx->stmp_bytes;
-    if ((k32 ? gr::sort_pairs32(ctx->stmp, tb, k32_in, k32_out, ctx->svals[0], ctx->svals[1], total, end_bit, s)
-             : gr::sort_pairs64(ctx->stmp, tb, ctx->skeys[0], ctx->skeys[1], ctx->svals[0], ctx->svals[1], total,
-                                end_bit, s)) != 0)
-        return set_err(GR_E_HIP, "radix sort failed");
-    if (gr::launch_alive_counts(ctx->d_sjobs, uint32_t(tab.size()), ctx->skeys[1], kbits, s) != 0)
-        return set_err(GR_E_HIP, "alive count kernel failed");
+    if (gr::launch_bucket_sort(ctx->d_sjobs, uint32_t(tab.size()), total, shift, nb, keys, ctx->svals[0],
+                               ctx->bcount, ctx->bstart, ctx->stmp, tb, ctx->svals[1], s) != 0)
+        return set_err(GR_E_HIP, "bucket sort failed");
     ++ctx->alive_seq;
     GR_HIP(hipEventRecord(ctx->alive_ev[ctx->alive_seq % gr_ctx::ALIVE_EVENTS], s));
     for (gr_job * j : js) j->alive_seq = ctx->alive_seq;

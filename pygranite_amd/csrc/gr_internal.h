@@ -259,23 +259,18 @@ struct GatherEntry
     uint32_t n;
     uint32_t pad;
 };
-// keys and in-job indices: 64-bit keys job << 32 | Morton code of the current cell (stopped
-// particles last) when mbits = 0; else 32-bit keys job << (mbits + 1) | code, stopped particles
-// 1 << mbits (every code below 2^mbits)
-int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, void * keys,
-                        uint32_t * vals, uint32_t mbits, void * stream);
-// radix sort of (key, value) pairs on key bits [0, end_bit); temp storage sized by a first call
-// with temp == null
-int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
-                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
-int sort_pairs32(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
-                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream);
+// Regroups the particles [base, base + n) of every job by the Morton code of their current cell
+// with the low `shift` bits dropped (stopped particles last), keeping each job in its range: out
+// [base + r] is the in-job index of the job's r-th particle in that order. A counting sort over
+// njobs * nb buckets (nb - 1 cell buckets and the stopped bucket per job); also stores each job's
+// alive count to *alive_out. keys, vals: total entries of scratch; counts, starts: njobs * nb;
+// temp: scan storage sized by a first call with temp == null (temp_bytes set, nothing launched).
+int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uint32_t shift,
+                       uint32_t nb, uint32_t * keys, uint32_t * vals, uint32_t * counts,
+                       uint32_t * starts, void * temp, size_t & temp_bytes, uint32_t * out,
+                       void * stream);
 // all entries in one launch (maxn: largest entry n)
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
-// per job of a sorted multi-job key array: the number of keys before its first stopped key
-// (mbits as for launch_morton_multi)
-int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const void * sorted_keys,
-                        uint32_t mbits, void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3
 int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
                      void * stream);

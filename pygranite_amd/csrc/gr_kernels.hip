@@ -14,7 +14,7 @@
 // 1/256 for emulation).
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
-#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 #include <stdint.h>
 
 #include <algorithm>
@@ -1605,15 +1605,12 @@ __device__ __forceinline__ uint32_t spread2(uint32_t x) // 16 bits -> every seco
     return x;
 }
 
-// Keys of a reorder of several jobs at once: key = (job << 32) | Morton code of the particle's
-// cell (stopped particles 0xffffffff, i.e. last), value = the particle's index in its job. One
-// radix sort over all keys then orders every job by cell inside its own key range.
-// Key of particle g: its job, then the Morton code of its current cell, stopped particles last
-// within the job. 64-bit keys: job << 32 | code (stopped: low word all ones). 32-bit keys
-// (mbits > 0, every code below 2^mbits): job << (mbits + 1) | code, stopped: 1 << mbits.
-template <class K>
-__global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs, uint64_t total,
-                               K * __restrict__ keys, uint32_t * __restrict__ vals, uint32_t mbits)
+// Bucket of particle g of a multi-job reorder: its job, then the Morton code of its current cell
+// with the low `shift` bits dropped; a job's stopped particles go to its last bucket. Counts the
+// bucket's particles.
+__global__ void k_bucket_keys(const SortJob * __restrict__ jobs, uint32_t njobs, uint64_t total,
+                              uint32_t * __restrict__ keys, uint32_t * __restrict__ vals,
+                              uint32_t * __restrict__ counts, uint32_t shift, uint32_t nb)
 {
     const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= total) return;
@@ -1626,9 +1623,8 @@ __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs
     }
     const SortJob & J = jobs[lo];
     const uint32_t i = uint32_t(g - J.base);
-    const bool alive = J.status[i] == GR_STATUS_TIME;
-    uint32_t code = 0;
-    if (alive)
+    uint32_t bucket = nb - 1;
+    if (J.status[i] == GR_STATUS_TIME)
     {
         const uint32_t lim = J.dim == 3 ? 1023u : 65535u;
         uint32_t c[3] = {0, 0, 0};
@@ -1638,81 +1634,89 @@ __global__ void k_morton_multi(const SortJob * __restrict__ jobs, uint32_t njobs
             a = fminf(fmaxf(a, 0.f), float(J.grid[d] - 1));
             c[d] = min(uint32_t(a), lim);
         }
-        code = J.dim == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
-                          : (spread2(c[0]) | (spread2(c[1]) << 1));
+        const uint32_t code = J.dim == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
+                                         : (spread2(c[0]) | (spread2(c[1]) << 1));
+        bucket = min(code >> shift, nb - 2);
     }
-    if (sizeof(K) == 8)
-        keys[g] = K((uint64_t(lo) << 32) | (alive ? min(code, 0xfffffffeu) : 0xffffffffu));
-    else
-        keys[g] = K((lo << (mbits + 1)) | (alive ? code : (1u << mbits)));
+    const uint32_t key = lo * nb + bucket;
+    keys[g] = key;
     vals[g] = i;
+    // one atomic per distinct key of the wavefront (neighbouring particles mostly share a bucket:
+    // they were regrouped by the previous reorder)
+    for (;;)
+    {
+        const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+        const uint64_t m = __ballot(key == k0);
+        if (key == k0)
+        {
+            if ((m & ((1ull << __lane_id()) - 1)) == 0) atomicAdd(&counts[k0], uint32_t(__popcll(m)));
+            break;
+        }
+    }
 }
 
-// One thread per job: its keys [base, base + n) are sorted with the stopped particles last
-// (64-bit keys: low word 0xffffffff; 32-bit keys: bit mbits set); the first of those is found by
-// bisection.
-template <class K>
-__global__ void k_alive_counts(const SortJob * __restrict__ jobs, uint32_t njobs,
-                               const K * __restrict__ keys, uint32_t mbits)
+// alive count of each job: the start of its stopped bucket minus its base (bucket starts from the
+// exclusive scan of the counts, before the scatter advances them)
+__global__ void k_bucket_alive(const SortJob * __restrict__ jobs, uint32_t njobs,
+                               const uint32_t * __restrict__ starts, uint32_t nb)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= njobs) return;
     const SortJob & J = jobs[k];
-    uint64_t lo = 0, hi = J.n; // answer in [lo, hi]
-    while (lo < hi)
+    if (J.alive_out) *J.alive_out = starts[size_t(k) * nb + nb - 1] - uint32_t(J.base);
+}
+
+// each particle to the next free position of its bucket (order inside a bucket is arbitrary)
+__global__ void k_bucket_scatter(const uint32_t * __restrict__ keys, const uint32_t * __restrict__ vals,
+                                 uint32_t * __restrict__ starts, uint64_t total, uint32_t * __restrict__ out)
+{
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t key = keys[g];
+    // the lanes sharing a key take consecutive positions with one atomic (the lowest lane's)
+    for (;;)
     {
-        const uint64_t mid = (lo + hi) / 2;
-        const K key = keys[J.base + mid];
-        const bool stopped = sizeof(K) == 8 ? uint32_t(key) == 0xffffffffu : ((key >> mbits) & 1u) != 0;
-        if (stopped) hi = mid;
-        else lo = mid + 1;
+        const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+        const uint64_t m = __ballot(key == k0);
+        if (key == k0)
+        {
+            const uint32_t lead = uint32_t(__ffsll((unsigned long long)m) - 1);
+            uint32_t base = 0;
+            if (__lane_id() == lead) base = atomicAdd(&starts[k0], uint32_t(__popcll(m)));
+            base = __shfl(base, int(lead), 64);
+            out[base + uint32_t(__popcll(m & ((1ull << __lane_id()) - 1)))] = vals[g];
+            break;
+        }
     }
-    if (J.alive_out) *J.alive_out = uint32_t(lo);
 }
 
-int launch_alive_counts(const SortJob * jobs, uint32_t njobs, const void * sorted_keys,
-                        uint32_t mbits, void * stream)
+__global__ void k_fill_u32(uint32_t * __restrict__ p, uint64_t n, uint32_t v)
 {
-    if (njobs == 0) return 0;
-    const uint32_t block = 64;
-    if (mbits)
-        hipLaunchKernelGGL(k_alive_counts<uint32_t>, dim3((njobs + block - 1) / block), dim3(block), 0,
-                           hipStream_t(stream), jobs, njobs, static_cast<const uint32_t *>(sorted_keys), mbits);
-    else
-        hipLaunchKernelGGL(k_alive_counts<uint64_t>, dim3((njobs + block - 1) / block), dim3(block), 0,
-                           hipStream_t(stream), jobs, njobs, static_cast<const uint64_t *>(sorted_keys), 0u);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
-int launch_morton_multi(const SortJob * jobs, uint32_t njobs, uint64_t total, void * keys,
-                        uint32_t * vals, uint32_t mbits, void * stream)
+int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uint32_t shift,
+                       uint32_t nb, uint32_t * keys, uint32_t * vals, uint32_t * counts,
+                       uint32_t * starts, void * temp, size_t & temp_bytes, uint32_t * out,
+                       void * stream)
 {
+    const uint64_t nbt = uint64_t(njobs) * nb;
+    if (!temp) // sizing call
+        return hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, counts, starts, int(nbt),
+                                                hipStream_t(stream)) == hipSuccess ? 0 : -1;
     const uint32_t block = 256;
-    const uint64_t grid = (total + block - 1) / block;
-    if (grid == 0) return 0;
-    if (mbits)
-        hipLaunchKernelGGL(k_morton_multi<uint32_t>, dim3(uint32_t(grid)), dim3(block), 0,
-                           hipStream_t(stream), jobs, njobs, total, static_cast<uint32_t *>(keys), vals, mbits);
-    else
-        hipLaunchKernelGGL(k_morton_multi<uint64_t>, dim3(uint32_t(grid)), dim3(block), 0,
-                           hipStream_t(stream), jobs, njobs, total, static_cast<uint64_t *>(keys), vals, 0u);
+    hipStream_t st = hipStream_t(stream);
+    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((nbt + block - 1) / block)), dim3(block), 0, st, counts, nbt, 0u);
+    if (total)
+        hipLaunchKernelGGL(k_bucket_keys, dim3(uint32_t((total + block - 1) / block)), dim3(block), 0, st,
+                           jobs, njobs, total, keys, vals, counts, shift, nb);
+    if (hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, starts, int(nbt), st) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_bucket_alive, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, starts, nb);
+    if (total)
+        hipLaunchKernelGGL(k_bucket_scatter, dim3(uint32_t((total + block - 1) / block)), dim3(block), 0, st,
+                           keys, vals, starts, total, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int sort_pairs64(void * temp, size_t & temp_bytes, const uint64_t * kin, uint64_t * kout,
-                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream)
-{
-    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
-                                                            int(n), 0, int(end_bit), hipStream_t(stream));
-    return e == hipSuccess ? 0 : -1;
-}
-
-int sort_pairs32(void * temp, size_t & temp_bytes, const uint32_t * kin, uint32_t * kout,
-                 const uint32_t * vin, uint32_t * vout, uint64_t n, uint32_t end_bit, void * stream)
-{
-    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout,
-                                                            int(n), 0, int(end_bit), hipStream_t(stream));
-    return e == hipSuccess ? 0 : -1;
 }
 
 // Every (array, job) entry of a multi-job reorder in one launch: blockIdx.y picks the entry,

@@ -445,8 +445,8 @@ def test_wide_launch_register_budget_bitexact(gpu_available):
 
 
 def test_reorder_with_wide_keys_changes_nothing(gpu_available):
-    """Three jobs on a field 1024 cells long: job index, 30-bit Morton code and stopped flag
-    need 33 bits, so the joint reorder sorts 64-bit keys (smaller grids use 32-bit keys). The
+    """Three jobs on a field 1024 cells long: 30-bit Morton codes, of which the reorder's
+    counting sort keeps the top 18 bits per job (a coarser regrouping than on small grids). The
     results equal those of unreordered jobs, and the live ranges hold every alive particle."""
     ctx = capi.Context(0)
     nx, ny, nz, S = 1024, 4, 4, 3
