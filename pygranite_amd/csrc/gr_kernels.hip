@@ -328,8 +328,10 @@ __device__ __forceinline__ float apply_op(uint32_t k, float a, float b)
 }
 
 // Reference slots x y z cv u v w of a step as one vector value: element d for a uniform d is an
-// indexed register read (no branches, no memory)
-typedef float refvec_t __attribute__((ext_vector_type(8)));
+// indexed register read (no branches, no memory). Nine elements (the last two zero): a vector
+// of more than eight is read with GPR index mode (s_set_gpr_idx_on, one move), where one of
+// eight would be expanded into a chain of eight compares and selects per read.
+typedef float refvec_t __attribute__((ext_vector_type(9)));
 
 // rv: reference slots x y z cv u v w of the step; R: this lane's LDS reference column (volumes
 // and constants, full kernel only)
@@ -691,7 +693,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
     Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
     uint32_t wi = 0;
-    uint32_t edge = P.nwin > 1 ? G(P.win_edge)[0] : 0xffffffffu;
+    // the window tables are uniform and read-only: scalar loads keep their values in SGPRs
+    uint32_t edge = P.nwin > 1 ? GR_CONST(uint32_t, P.win_edge, 0) : 0xffffffffu;
 
     // A stopped particle's state never changes again (tpp:452-455 copies it unchanged) and its
     // saved columns after the stop are left to the reader, which fills them from the final
@@ -747,14 +750,15 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         {
             ++wi;
             const uint32_t c = P.win_first + wi;
-            W.back = G(P.win_ptr)[3 * wi + 0];
-            W.front = G(P.win_ptr)[3 * wi + 1];
-            W.pair = G(P.win_ptr)[3 * wi + 2];
+            typedef const void * cvp_t;
+            W.back = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 0);
+            W.front = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 1);
+            W.pair = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 2);
             W.tb = float(c) * P.dtd;
             W.tf = float(c + 1) * P.dtd;
             W.tb64 = double(c) * double(P.dtd);
             W.tf64 = double(c + 1) * double(P.dtd);
-            edge = wi + 1 < P.nwin ? G(P.win_edge)[wi] : 0xffffffffu;
+            edge = wi + 1 < P.nwin ? GR_CONST(uint32_t, P.win_edge, wi) : 0xffffffffu;
             sc.base = 0xffffffffu;
         }
         const bool live = status == GR_STATUS_TIME; // alive at the start of step s
@@ -1029,7 +1033,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                         if (DIM == 3) gz = global_coord(P, np[2], 2);
                     }
                     const refvec_t rv = {gx, gy, gz, P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f,
-                                         vc[0], vc[1], DIM == 3 ? vc[2] : 0.f, 0.f};
+                                         vc[0], vc[1], DIM == 3 ? vc[2] : 0.f, 0.f, 0.f};
                     for (uint32_t e = 0; e < P.nprog; ++e)
                         ex[e] = run_program<FULL>(P.prog[e], rv, R, s_stk, lane, bs);
                 }
