@@ -103,7 +103,7 @@ def make_field(capi, ctx, wind, dtype):
     return f
 
 
-def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder):
+def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
@@ -114,8 +114,11 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     job = new_job()
     sch = Schedule()
 
+    count = [0]
+
     def launch():
         left = chunk
+        regrouped = False
         while left:
             k, edge, t = sch.chunk(left)
             w = capi.Window()
@@ -129,6 +132,10 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
             left -= k
             if edge and reorder:
                 job.reorder()  # regroup by cell at every window edge (inside the timing)
+                regrouped = True
+        count[0] += 1
+        if reorder and reorder_every and not regrouped and count[0] % reorder_every == 0:
+            job.reorder()  # and every reorder_every launches
 
     for _ in range(warmup):
         launch()
@@ -136,6 +143,7 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     job.close()
     job = new_job()
     sch = Schedule()
+    count[0] = 0
     ctx.synchronize()
     ctx.take_work()
     ctx.take_kernel_ms()
@@ -153,7 +161,7 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     return out
 
 
-def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
+def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
@@ -162,7 +170,7 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
         return out
 
     jobs = new_jobs()
-    state = dict(window=0, time=0.0, time64=0.0, counter=0)
+    state = dict(window=0, time=0.0, time64=0.0, counter=0, launches=0)
 
     def launch():
         spans = [(j, f, dict(first_window=state["window"], max_steps=chunk,
@@ -170,8 +178,10 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
                              counter_start=state["counter"])) for j in jobs]
         ends, _ = ctx.batch_run(spans, timed=False, ends=True)
         e = ends[0]
-        if e["window"] != state["window"]:
-            ctx.reorder(jobs)  # every job regrouped by cell at the window edge, in one pass
+        state["launches"] += 1
+        if e["window"] != state["window"] or (reorder_every and state["launches"] % reorder_every == 0):
+            ctx.reorder(jobs)  # every job regrouped by cell in one pass: at window edges (and
+            # every reorder_every launches)
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
 
     for _ in range(warmup):
@@ -179,7 +189,7 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs):
     for j in jobs:
         j.close()
     jobs[:] = new_jobs()
-    state.update(window=0, time=0.0, time64=0.0, counter=0)
+    state.update(window=0, time=0.0, time64=0.0, counter=0, launches=0)
     ctx.synchronize()
     ctx.take_work()
     ctx.take_kernel_ms()
@@ -668,6 +678,9 @@ def main():
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--jobs", type=int, default=16)
+    ap.add_argument("--reorder-every", type=int, default=1,
+                    help="regroup particles by cell every K launches as well as at window edges "
+                         "(0: at window edges only)")
     ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -747,11 +760,14 @@ def main():
         if name == "plain_fp32":
             r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, False, False)
         elif name == "fp32_local":
-            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True)
+            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
+                           args.reorder_every)
         elif name == "fp16":
-            r = run_single(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True)
+            r = run_single(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
+                           args.reorder_every)
         elif name == "fp16_batched":
-            r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs)
+            r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs,
+                            args.reorder_every)
         else:
             raise SystemExit(f"unknown form {name}")
         r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
@@ -808,6 +824,7 @@ def main():
                                "0.005, 2000 steps, curvature + sqrt(u*u+v*v+w*w) saved every step",
                    "form": head, "particles": args.particles, "grid": n,
                    "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
+                   "regroup_every_launches": args.reorder_every,
                    "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
                    "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
                    "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP)},

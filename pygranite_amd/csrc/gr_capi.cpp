@@ -1109,8 +1109,12 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     }
     // Buckets: the Morton code of the cell (a coordinate along an axis of g cells needs
     // bitlen(g - 1) bits, at most 10 in 3D and 16 in 2D; a code dim times that) with its low
-    // `shift` bits dropped, so that a job has at most 2^18 cell buckets and all jobs at most 2^24;
-    // one more bucket per job takes its stopped particles.
+    // `shift` bits dropped, so that a job has at most 2^18 cell buckets, all jobs at most 2^24,
+    // and the largest live range about 16 particles per bucket: a wavefront of 64 particles then
+    // spans a few buckets, blocks of neighbouring cells, as close together as a full sort would
+    // place them, while coarser buckets mean fewer counters to clear and scan and more lanes of
+    // a wavefront adding to the same counter (one atomic for them). One more bucket per job
+    // takes its stopped particles.
     uint32_t mbits = 1;
     for (const gr::SortJob & t : tab)
         for (uint32_t d = 0; d < t.dim; ++d)
@@ -1119,8 +1123,11 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
             while (b < (t.dim == 3 ? 10u : 16u) && (t.grid[d] - 1) >> b) ++b;
             mbits = std::max(mbits, t.dim * b);
         }
+    uint64_t nmax = 0;
+    for (gr_job * j : js) nmax = std::max<uint64_t>(nmax, j->run_n);
     uint32_t bbits = 18;
-    while (bbits > 1 && (uint64_t(js.size()) << bbits) > (uint64_t(1) << 24)) --bbits;
+    while (bbits > 1 && ((uint64_t(js.size()) << bbits) > (uint64_t(1) << 24) || (uint64_t(16) << bbits) > nmax * 2))
+        --bbits;
     const uint32_t shift = mbits > bbits ? mbits - bbits : 0;
     const uint32_t nb = (1u << (mbits - shift)) + 1;
     const uint64_t nbt = uint64_t(js.size()) * nb;
