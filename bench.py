@@ -631,13 +631,15 @@ def serving_workload(capi, ctx, args):
     offered = sum(r.n * r.steps for r in reqs) / args.serve_duration
     print(f"# serving: {len(reqs)} requests over {args.serve_duration} s, offered "
           f"{offered:.3e} nominal particle-steps/s", file=sys.stderr, flush=True)
-    srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum)
+    srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum,
+                         reorder_every=args.serve_reorder_every)
     ctx.take_work()
     srv.run(reqs, progress=lambda m: print("#   " + m, file=sys.stderr, flush=True))
     rep = srv.report()
     rep["integrated_particle_steps"] = ctx.take_work()
     rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]
     rep.update(arrival_rate_per_s=args.serve_rate, duration_s=args.serve_duration,
+               regroup_every_rounds=args.serve_reorder_every,
                offered_nominal_particle_steps_per_s=offered,
                workload="4 datasets 256x256x96 FP16 x 48 slices; Poisson arrivals; particles "
                         f"log-uniform [1e3, {args.batch_pmax:.0e}], steps uniform [100, {args.batch_smax}]")
@@ -699,6 +701,8 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=120.0)
     ap.add_argument("--serve-quantum", type=int, default=250)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-reorder-every", type=int, default=1,
+                    help="serving rounds between joint regroupings of the running jobs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the batched-kernel autotuner (default: tune before the batched form)")
     args = ap.parse_args()

@@ -95,7 +95,7 @@ def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0, reorder:
 
 class Server:
     def __init__(self, ctx: capi.Context, datasets: dict, hbm_budget: int = 64 << 30,
-                 quantum: int = 250, block: int = 0, reorder: bool = True, reorder_every: int = 4):
+                 quantum: int = 250, block: int = 0, reorder: bool = True, reorder_every: int = 1):
         self.ctx = ctx
         self.datasets = datasets
         self.hbm_budget = int(hbm_budget)
@@ -182,9 +182,9 @@ class Server:
             else:
                 keep.append(a)
         self.active = keep
-        # regroup the large jobs by cell every few rounds: their launches then cover only the
-        # particles still alive (the live range), and a job whose particles have all stopped
-        # retires at once
+        # regroup the large jobs by cell (every round by default): their launches then cover
+        # only the particles still alive (the live range) with wavefronts of neighbouring
+        # particles, and a job whose particles have all stopped retires at once
         if self.reorder and self.rounds % self.reorder_every == 0:
             big = [a.job for a in self.active if a.req.count >= 4096]
             if big:
