@@ -103,7 +103,7 @@ def make_field(capi, ctx, wind, dtype):
     return f
 
 
-def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0):
+def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0, regroup_min=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
@@ -134,8 +134,9 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
                 job.reorder()  # regroup by cell at every window edge (inside the timing)
                 regrouped = True
         count[0] += 1
-        if reorder and reorder_every and not regrouped and count[0] % reorder_every == 0:
-            job.reorder()  # and every reorder_every launches
+        if (reorder and reorder_every and not regrouped and count[0] % reorder_every == 0
+                and job.live_range() >= regroup_min):
+            job.reorder()  # and every reorder_every launches while the live range is large
 
     for _ in range(warmup):
         launch()
@@ -161,7 +162,7 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     return out
 
 
-def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0):
+def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0, regroup_min=0):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
@@ -179,9 +180,10 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
         ends, _ = ctx.batch_run(spans, timed=False, ends=True)
         e = ends[0]
         state["launches"] += 1
-        if e["window"] != state["window"] or (reorder_every and state["launches"] % reorder_every == 0):
-            ctx.reorder(jobs)  # every job regrouped by cell in one pass: at window edges (and
-            # every reorder_every launches)
+        if e["window"] != state["window"] or (reorder_every and state["launches"] % reorder_every == 0
+                                               and sum(j.live_range() for j in jobs) >= regroup_min):
+            ctx.reorder(jobs)  # every job regrouped by cell in one pass: at window edges, and
+            # every reorder_every launches while the live range is large
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
 
     for _ in range(warmup):
@@ -683,6 +685,9 @@ def main():
     ap.add_argument("--reorder-every", type=int, default=1,
                     help="regroup particles by cell every K launches as well as at window edges "
                          "(0: at window edges only)")
+    ap.add_argument("--regroup-min", type=int, default=65536,
+                    help="regroup every K launches only while the live range (all jobs) holds at "
+                         "least this many particles; smaller launches are latency-bound")
     ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -765,13 +770,13 @@ def main():
             r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, False, False)
         elif name == "fp32_local":
             r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every)
+                           args.reorder_every, args.regroup_min)
         elif name == "fp16":
             r = run_single(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every)
+                           args.reorder_every, args.regroup_min)
         elif name == "fp16_batched":
             r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs,
-                            args.reorder_every)
+                            args.reorder_every, args.regroup_min)
         else:
             raise SystemExit(f"unknown form {name}")
         r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
@@ -828,7 +833,7 @@ def main():
                                "0.005, 2000 steps, curvature + sqrt(u*u+v*v+w*w) saved every step",
                    "form": head, "particles": args.particles, "grid": n,
                    "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
-                   "regroup_every_launches": args.reorder_every,
+                   "regroup_every_launches": args.reorder_every, "regroup_min_live": args.regroup_min,
                    "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
                    "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
                    "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP)},
