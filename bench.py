@@ -115,6 +115,7 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     sch = Schedule()
 
     count = [0]
+    large = [True]  # live range at least regroup_min (it only shrinks)
 
     def launch():
         left = chunk
@@ -134,9 +135,11 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
                 job.reorder()  # regroup by cell at every window edge (inside the timing)
                 regrouped = True
         count[0] += 1
-        if (reorder and reorder_every and not regrouped and count[0] % reorder_every == 0
-                and job.live_range() >= regroup_min):
-            job.reorder()  # and every reorder_every launches while the live range is large
+        if reorder and reorder_every and not regrouped and count[0] % reorder_every == 0 and large[0]:
+            if job.live_range() >= regroup_min:
+                job.reorder()  # and every reorder_every launches while the live range is large
+            else:
+                large[0] = False
 
     for _ in range(warmup):
         launch()
@@ -145,6 +148,7 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
     job = new_job()
     sch = Schedule()
     count[0] = 0
+    large[0] = True
     ctx.synchronize()
     ctx.take_work()
     ctx.take_kernel_ms()
@@ -171,7 +175,7 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
         return out
 
     jobs = new_jobs()
-    state = dict(window=0, time=0.0, time64=0.0, counter=0, launches=0)
+    state = dict(window=0, time=0.0, time64=0.0, counter=0, launches=0, large=True)
 
     def launch():
         spans = [(j, f, dict(first_window=state["window"], max_steps=chunk,
@@ -180,8 +184,12 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
         ends, _ = ctx.batch_run(spans, timed=False, ends=True)
         e = ends[0]
         state["launches"] += 1
-        if e["window"] != state["window"] or (reorder_every and state["launches"] % reorder_every == 0
-                                               and sum(j.live_range() for j in jobs) >= regroup_min):
+        # live ranges only shrink: once below regroup_min they are not queried again
+        if (reorder_every and state["large"] and state["launches"] % reorder_every == 0
+                and sum(j.live_range() for j in jobs) < regroup_min):
+            state["large"] = False
+        if e["window"] != state["window"] or (reorder_every and state["large"]
+                                               and state["launches"] % reorder_every == 0):
             ctx.reorder(jobs)  # every job regrouped by cell in one pass: at window edges, and
             # every reorder_every launches while the live range is large
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
@@ -191,7 +199,7 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
     for j in jobs:
         j.close()
     jobs[:] = new_jobs()
-    state.update(window=0, time=0.0, time64=0.0, counter=0, launches=0)
+    state.update(window=0, time=0.0, time64=0.0, counter=0, launches=0, large=True)
     ctx.synchronize()
     ctx.take_work()
     ctx.take_kernel_ms()

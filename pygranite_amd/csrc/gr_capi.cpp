@@ -65,6 +65,10 @@ void dfree(void * p)
 }
 
 size_t dtype_size(uint32_t t) { return t == GR_F32 ? 4 : 2; }
+
+// Events that only time device work: recorded without the system-scope fence (cache writeback
+// and invalidation) that a default event performs around every launch it brackets.
+constexpr unsigned TIMING_EVENT = hipEventDisableSystemFence;
 } // namespace
 
 extern "C" void gr_set_error(const char * msg) { g_err = msg ? msg : ""; }
@@ -428,7 +432,7 @@ static int kt_mark(gr_ctx * c, hipStream_t s, bool end)
     while (c->kev.size() <= i)
     {
         hipEvent_t e;
-        GR_HIP(hipEventCreate(&e));
+        GR_HIP(hipEventCreateWithFlags(&e, TIMING_EVENT));
         c->kev.push_back(e);
     }
     GR_HIP(hipEventRecord(c->kev[i], s));
@@ -850,7 +854,8 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     if (rc == GR_OK) rc = dalloc(&j->scratch2, N * D * sizeof(float));
     for (int k = 0; k < 3; ++k)
         j->grid[k] = d.grid_scale[k] > 0.f ? uint32_t(std::lround(double(d.domain[k]) / d.grid_scale[k])) : 1;
-    if (rc == GR_OK && (hipEventCreate(&j->ev0) != hipSuccess || hipEventCreate(&j->ev1) != hipSuccess))
+    if (rc == GR_OK && (hipEventCreateWithFlags(&j->ev0, TIMING_EVENT) != hipSuccess ||
+                        hipEventCreateWithFlags(&j->ev1, TIMING_EVENT) != hipSuccess))
         rc = set_err(GR_E_HIP, "event creation failed");
     if (rc == GR_OK && n)
     {
@@ -1580,7 +1585,8 @@ static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
         return set_err(GR_E_NOMEM, "pinned staging of %zu bytes failed", want);
     }
     st.cap = want;
-    if (!st.done) GR_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    // the host only needs to know that the copy has read the staging block: no fence
+    if (!st.done) GR_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming | hipEventDisableSystemFence));
     return GR_OK;
 }
 
@@ -1644,8 +1650,8 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     hipStream_t s = ctx->compute;
     if (!ctx->bev0)
     {
-        GR_HIP(hipEventCreate(&ctx->bev0));
-        GR_HIP(hipEventCreate(&ctx->bev1));
+        GR_HIP(hipEventCreateWithFlags(&ctx->bev0, TIMING_EVENT));
+        GR_HIP(hipEventCreateWithFlags(&ctx->bev1, TIMING_EVENT));
     }
     std::vector<gr::StepParams> params(nspans);
     std::vector<uint64_t> saves(nspans, 0);
