@@ -533,7 +533,10 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
         if (rc == GR_OK && hipMemset(f->slots[s], 0, f->slot_bytes) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_field_create: memset failed");
         if (rc == GR_OK && (hipEventCreateWithFlags(&f->ready[s], hipEventDisableTiming) != hipSuccess ||
-                            hipEventCreateWithFlags(&f->inuse[s], hipEventDisableTiming) != hipSuccess))
+                            // in use until the launches that read the slot have completed; the
+                            // copy stream only waits for that before overwriting it: no fence
+                            hipEventCreateWithFlags(&f->inuse[s], hipEventDisableTiming | hipEventDisableSystemFence) !=
+                                hipSuccess))
             rc = set_err(GR_E_HIP, "gr_field_create: event creation failed");
     }
     if (rc == GR_OK) rc = dalloc(&f->dstage, nodes * d.ncomp * sizeof(float));
