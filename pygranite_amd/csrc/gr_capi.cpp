@@ -420,8 +420,9 @@ static int kt_fold(gr_ctx * c)
     return GR_OK;
 }
 
-// records the start (end = false) or end of a step-kernel launch on stream s
-static int kt_mark(gr_ctx * c, hipStream_t s, bool end)
+// Records the start or end event of a step-kernel launch (end: *done = the event, which also
+// marks the end of everything the launch read).
+static int kt_mark(gr_ctx * c, hipStream_t s, bool end, hipEvent_t * done = nullptr)
 {
     if (!end && c->kpending == 1024)
     {
@@ -437,6 +438,7 @@ static int kt_mark(gr_ctx * c, hipStream_t s, bool end)
     }
     GR_HIP(hipEventRecord(c->kev[i], s));
     if (end) ++c->kpending;
+    if (done) *done = c->kev[i];
     return GR_OK;
 }
 
@@ -497,8 +499,6 @@ static void field_free(gr_field * f)
     if (f->stage_done) (void)hipEventDestroy(f->stage_done);
     for (hipEvent_t e : f->ready)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : f->inuse)
-        if (e) (void)hipEventDestroy(e);
     delete f;
 }
 
@@ -532,11 +532,7 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
         rc = dalloc(&f->slots[s], f->slot_bytes);
         if (rc == GR_OK && hipMemset(f->slots[s], 0, f->slot_bytes) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_field_create: memset failed");
-        if (rc == GR_OK && (hipEventCreateWithFlags(&f->ready[s], hipEventDisableTiming) != hipSuccess ||
-                            // in use until the launches that read the slot have completed; the
-                            // copy stream only waits for that before overwriting it: no fence
-                            hipEventCreateWithFlags(&f->inuse[s], hipEventDisableTiming | hipEventDisableSystemFence) !=
-                                hipSuccess))
+        if (rc == GR_OK && hipEventCreateWithFlags(&f->ready[s], hipEventDisableTiming) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_field_create: event creation failed");
     }
     if (rc == GR_OK) rc = dalloc(&f->dstage, nodes * d.ncomp * sizeof(float));
@@ -1435,31 +1431,36 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
     return GR_OK;
 }
 
-static int mark_inuse(gr_field * f, uint32_t slot, hipStream_t s)
+// A slot read by a launch is in use until the launch's end event (borrowed from the context's
+// kernel-timing events, no record of its own). Those events are recorded again for later
+// launches once folded; an upload that waits on one then waits for a later launch of the same
+// stream, which implies the one that read the slot.
+static void mark_inuse(gr_field * f, uint32_t slot, hipEvent_t done)
 {
-    if (f)
+    if (f && done)
     {
-        GR_HIP(hipEventRecord(f->inuse[slot], s));
+        f->inuse[slot] = done;
         f->has_inuse[slot] = 1;
     }
-    return GR_OK;
 }
 
-// after the launch: later uploads into the slots read by the window wait for it (the batched
-// path marks each slot once per launch itself: mark = false)
-static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, bool mark = true)
+// after the launch: later uploads into the slots read by the window wait for its end event
+// `done` (the batched path marks each slot once per launch itself: done = nullptr)
+static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, hipEvent_t done)
 {
-    hipStream_t s = j->ctx->compute;
-    int rc;
-    if (!mark)
+    if (!done)
     {
         j->saved += nsaves;
         return GR_OK;
     }
-    if ((rc = mark_inuse(w->wind, w->wind_back, s)) || (rc = mark_inuse(w->wind, w->wind_front, s))) return rc;
-    if (j->d.nvol && ((rc = mark_inuse(w->volumes, w->vol_back, s)) || (rc = mark_inuse(w->volumes, w->vol_front, s))))
-        return rc;
-    if (w->topography && (rc = mark_inuse(w->topography, 0, s))) return rc;
+    mark_inuse(w->wind, w->wind_back, done);
+    mark_inuse(w->wind, w->wind_front, done);
+    if (j->d.nvol)
+    {
+        mark_inuse(w->volumes, w->vol_back, done);
+        mark_inuse(w->volumes, w->vol_front, done);
+    }
+    if (w->topography) mark_inuse(w->topography, 0, done);
     j->saved += nsaves;
     return GR_OK;
 }
@@ -1500,10 +1501,11 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, wide, narrow,
                            s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-    if ((rc = kt_mark(j->ctx, s, true)) != GR_OK) return rc;
+    hipEvent_t done = nullptr;
+    if ((rc = kt_mark(j->ctx, s, true, &done)) != GR_OK) return rc;
     GR_HIP(hipEventRecord(j->ev1, s));
     j->timed = true;
-    return finish_advance(j, w, nsaves);
+    return finish_advance(j, w, nsaves, done);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -1898,15 +1900,18 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         1, std::min<uint32_t>((nitems + wpb - 1) / wpb, uint32_t(per_cu) * uint32_t(std::max(1, ctx->cus))));
     const gr::StepParams * dparams = reinterpret_cast<const gr::StepParams *>(dbase + off_p);
     uint32_t * dq = reinterpret_cast<uint32_t *>(dbase);
-    GR_HIP(hipEventRecord(ctx->bev0, s));
+    // the launch's own elapsed time only when the caller asks for it (each record is one more
+    // packet between consecutive launches)
+    if (ms) GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
     // narrow: every item's wavefront fits on the device at two per SIMD
     const bool narrow = uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
     if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
                                    block, ctx->batch_waves, narrow, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-    if (nitems && (rc = kt_mark(ctx, s, true)) != GR_OK) return rc;
-    GR_HIP(hipEventRecord(ctx->bev1, s));
+    hipEvent_t done = nullptr;
+    if (nitems && (rc = kt_mark(ctx, s, true, &done)) != GR_OK) return rc;
+    if (ms) GR_HIP(hipEventRecord(ctx->bev1, s));
     std::vector<std::pair<gr_field *, uint32_t>> used;
     for (uint32_t i = 0; i < nspans; ++i)
     {
@@ -1920,11 +1925,10 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             const std::pair<gr_field *, uint32_t> u(spans[i].wind, c);
             if (std::find(used.begin(), used.end(), u) == used.end()) used.push_back(u);
         }
-        if ((rc = finish_advance(j, &first_win[i], saves[i], false)) != GR_OK) return rc;
+        if ((rc = finish_advance(j, &first_win[i], saves[i], nullptr)) != GR_OK) return rc;
         j->timed = false;
     }
-    for (const auto & u : used)
-        if ((rc = mark_inuse(u.first, u.second, s)) != GR_OK) return rc;
+    for (const auto & u : used) mark_inuse(u.first, u.second, done);
     if (ms)
     {
         GR_HIP(hipEventSynchronize(ctx->bev1));
