@@ -104,6 +104,11 @@ struct IntegratorSettings
     bool AllowAbs{false};
     uint32_t BlockSize{0};
     bool SortParticles{true}; // Morton-reorder particle state at window edges (results unchanged)
+    // and also every RegroupInterval steps inside a window while at least RegroupMinLive particles
+    // are live (0: window edges only); launches of many live particles gather faster when a
+    // wavefront's particles are neighbours, and they drift apart within a window
+    uint32_t RegroupInterval{64};
+    uint64_t RegroupMinLive{65536};
     // stretched z axis (3D): height of every z node relative to Offset.Z, strictly increasing
     // from 0; empty = uniform spacing GridScale[2]. The z extent of the domain is the top level.
     std::vector<float> ZLevels;
@@ -747,7 +752,20 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
             uint32_t k = 0;
             float t = time32;
             double t64 = time64;
-            const uint64_t chunk_limit = s_.MinimumAliveParticles ? 1 : (SI ? cap * SI : (1u << 30));
+            uint64_t chunk_limit = s_.MinimumAliveParticles ? 1 : (SI ? cap * SI : (1u << 30));
+            // chunks of RegroupInterval steps, each followed by a regrouping, while many
+            // particles are live
+            bool regroup = false;
+            if (s_.SortParticles && s_.RegroupInterval && n >= 4096 && !s_.MinimumAliveParticles)
+            {
+                uint64_t live = 0;
+                check(gr_job_live_range(job.j, &live));
+                if (live >= s_.RegroupMinLive)
+                {
+                    regroup = true;
+                    chunk_limit = std::min<uint64_t>(chunk_limit, s_.RegroupInterval);
+                }
+            }
             while (uint64_t(k) < chunk_limit && float(counter + k) < max_steps_f)
             {
                 if (s_.MinimumAliveParticles && !alive_ok()) break;
@@ -779,6 +797,8 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
             // chunk is read after the loader has been stepped (overlapping the next upload)
             if (!window_done) read_columns();
             if (abort_due_alive) window_done = true;
+            // every saved column has been read: regroup for the window's next chunk
+            if (!window_done && regroup) check(gr_job_reorder(job.j));
         }
 
         bool stepped = false;
@@ -960,6 +980,8 @@ PYBIND11_MODULE(pygranite, m)
         .def_readwrite("AllowAbs", &IntegratorSettings::AllowAbs)
         .def_readwrite("BlockSize", &IntegratorSettings::BlockSize)
         .def_readwrite("SortParticles", &IntegratorSettings::SortParticles)
+        .def_readwrite("RegroupInterval", &IntegratorSettings::RegroupInterval)
+        .def_readwrite("RegroupMinLive", &IntegratorSettings::RegroupMinLive)
         .def_readwrite("ZLevels", &IntegratorSettings::ZLevels);
 
     py::class_<TrajectoryIntegrator>(m, "TrajectoryIntegrator")

@@ -116,3 +116,21 @@ def test_stretched_z_precision_form_bitexact(pg, gpu_available):
         np.testing.assert_array_equal(gpu["compute"][0], ref["compute"][0].astype(np.float32))
         # a mix of particles alive at the end and stopped on the way (domain exits, stalls)
         assert (ref["status"] != 0).sum() > 100 and (ref["status"] == 0).sum() > 100
+
+
+def test_regrouping_inside_windows_changes_nothing(pg, gpu_available):
+    """The drop-in path regroups particles by cell every RegroupInterval steps inside a window
+    (here 16, whatever the live count) as well as at window edges: every output equals a run
+    that never regroups, and both equal the FP32 oracle on a subsample."""
+    st, wind, gs = c1_settings(steps=300)
+    seeds = synthetic.uniform_seeds(21, 6000, [0.0] * 3, [2 * math.pi] * 3)
+    regrouped = run_product(pg, dict(st, RegroupInterval=16, RegroupMinLive=0), wind, seeds)
+    plain = run_product(pg, dict(st, SortParticles=False), wind, seeds)
+    for f in ("trajectories", "curvatures", "total_curvature", "abort_step"):
+        np.testing.assert_array_equal(regrouped[f], plain[f], err_msg=f)
+    np.testing.assert_array_equal(regrouped["status"], plain["status"])
+    np.testing.assert_array_equal(regrouped["compute"][0], plain["compute"][0])
+    assert (plain["status"] != 0).sum() > 100  # particles stopped at different steps
+    ref = oracle.run(st, wind, seeds[:64], real64=False)
+    sub = {k: (v[:64] if isinstance(v, np.ndarray) else v) for k, v in regrouped.items()}
+    assert_bitexact(sub, ref, fields=("trajectories", "curvatures"), label="regrouped ")
