@@ -399,9 +399,12 @@ int lower_program(const gr_program & in, Program & out, uint32_t & refmask)
                 packed |= uint32_t(reg_ref(at[k])) << (3 * k);
                 refmask |= 1u << (in.code[at[k]] >> 8);
             }
-            if (!emit(DC_DOT, terms, packed)) return GR_E_INVALID;
+            // a square root of the sum right after it joins the instruction
+            const uint32_t last = i + (third ? 10 : 6);
+            const bool root = op_at(last + 1) == OP_SQRT;
+            if (!emit(DC_DOT, terms | (root ? 4u : 0u), packed)) return GR_E_INVALID;
             st[sp++] = Sym{0, 0};
-            i += third ? 10 : 6;
+            i = last + (root ? 1 : 0);
             continue;
         }
         if (op == OP_REF || op == OP_CONST)
@@ -474,7 +477,8 @@ float eval_lowered(const Program & pr, const float * ref)
             {
                 auto r = [&](uint32_t k) { return refv((arg >> (3 * k)) & 7u); };
                 x = (r(0) * r(1)) + (r(2) * r(3));
-                if (sub == 3) x = x + (r(4) * r(5));
+                if ((sub & 3) == 3) x = x + (r(4) * r(5));
+                if (sub & 4) x = std::sqrt(x);
             }
             else
                 x = sub ? pr.imm[arg] : refv(arg);

@@ -43,8 +43,9 @@ enum Op : uint32_t
 //   DC_LEAF   sub = k | rev << 2 | imm << 3 (k < 4): top op leaf, or leaf op top when rev; the
 //             leaf is ref[arg] or imm[arg]
 //   DC_UNARY  sub: 0 sqrt 1 exp 2 ln 3 sin 4 cos 5 tan 6 asin 7 acos 8 atan 9 abs
-//   DC_DOT    sub = 2 or 3: push (r0 * r1) + (r2 * r3) [+ (r4 * r5)], the sum formed left to
-//             right, with register slots r_k = arg >> 3k & 7 (the postfix of a*b + c*d [+ e*f])
+//   DC_DOT    sub = terms | root << 2, terms = 2 or 3: push (r0 * r1) + (r2 * r3) [+ (r4 * r5)],
+//             the sum formed left to right, with register slots r_k = arg >> 3k & 7 (the postfix
+//             of a*b + c*d [+ e*f]); with root, its square root (the postfix followed by sqrt)
 // Reference arguments are dense: 0..6 = slots x y z cv u v w (held in registers), 7 + s = slot
 // s (volumes and constants, staged in LDS by the full kernel).
 enum DevClass : uint32_t

@@ -359,10 +359,11 @@ __device__ __forceinline__ float run_program(const Program & pr, const refvec_t 
         else if (cls == DC_PUSH || cls == DC_DOT)
         {
             float x;
-            if (cls == DC_DOT) // (r0 * r1) + (r2 * r3) [+ (r4 * r5)], register slots
+            if (cls == DC_DOT) // (r0 * r1) + (r2 * r3) [+ (r4 * r5)], register slots [, sqrt]
             {
                 x = (rv[arg & 7u] * rv[(arg >> 3) & 7u]) + (rv[(arg >> 6) & 7u] * rv[(arg >> 9) & 7u]);
-                if (sub == 3) x = x + (rv[(arg >> 12) & 7u] * rv[(arg >> 15) & 7u]);
+                if ((sub & 3) == 3) x = x + (rv[(arg >> 12) & 7u] * rv[(arg >> 15) & 7u]);
+                if (sub & 4) x = sqrtf(x);
             }
             else
                 x = sub ? pr.imm[arg] : refv(arg);
