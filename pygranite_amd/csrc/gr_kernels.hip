@@ -736,7 +736,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     float time32 = P.t_start;
     double time64 = P.t_start64;
     uint64_t next_save = P.save_first;
-    uint64_t g = P.save_g0;
+    // ring column of the next save (global save index modulo the ring), advanced per save
+    uint32_t col = P.ring_cap ? uint32_t(P.save_g0 % P.ring_cap) : 0u;
 
     uint32_t integrated = 0;
     // |p - q| of the next step is |np - p| of this one (same operands, same order): carried
@@ -1057,11 +1058,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         // state and are filled when read (k_fill_frozen).
         if (s == next_save && live)
         {
-            const uint64_t col = g % P.ring_cap;
-            auto rp = G(P.ring_pos) + (col * P.n + pid) * DIM;
+            auto rp = G(P.ring_pos) + (uint64_t(col) * P.n + pid) * DIM;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) rp[d] = global_coord(P, p[d], d);
-            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.ring_curv)[col * P.n + pid] = curv;
+            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.ring_curv)[uint64_t(col) * P.n + pid] = curv;
             for (uint32_t e = 0; e < P.nprog; ++e)
                 G(P.ring_expr)[(uint64_t(e) * P.ring_cap + col) * P.n + pid] = ex[e];
             if (FULL)
@@ -1070,7 +1070,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         }
         if (s == next_save)
         {
-            g++;
+            col = col + 1 == P.ring_cap ? 0u : col + 1;
             next_save += P.save_interval;
         }
         time32 += dt;
