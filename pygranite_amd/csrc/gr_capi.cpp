@@ -1597,8 +1597,9 @@ static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// The time weights of every step of a launch exactly as the step kernel forms them (FP32 time,
-// or FP64 with F_TIME64): t at the step's start, t + dt/2 and t + dt, each mapped into the
+// The time weights of every step of a launch, which the step kernels read instead of forming
+// them (FP32 time, or FP64 with F_TIME64): t at the step's start, t + dt/2 and t + dt, each
+// mapped into the
 // step's window (unit_interval, tpp:127-144), and 1 / 0 for "the step starts after time 0"
 // (tpp:347); four floats per step. edges: the window edges of a launch that crosses them.
 static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out)

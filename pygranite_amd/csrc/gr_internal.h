@@ -142,9 +142,10 @@ struct StepParams
     float dtd;
     const void * const * win_ptr;
     const uint32_t * win_edge;
-    // per step of this launch, or null: the time weights (t, t + dt/2, t + dt mapped into the
-    // step's window) and 1 / 0 for "the step starts after time 0", four floats per step, formed
-    // on the host with exactly the FP32 / FP64 operations the kernel would perform
+    // per step of this launch (required for every launch that has steps): the time weights
+    // (t, t + dt/2, t + dt mapped into the step's window) and 1 / 0 for "the step starts after
+    // time 0", four floats per step, formed on the host with the reference's FP32 / FP64
+    // operations
     const float * ttab;
 
     // particle state (SoA)

@@ -473,14 +473,12 @@ template <> struct PairLoad<bf16_t, 2>
 // kernel (every reference feature, separate slots)
 enum : int { M_PAIRED = 0, M_SEPARATE = 1, M_FULL = 2 };
 
-// slices and time stamps of the window being integrated (changes at window edges)
+// slices of the window being integrated (change at window edges)
 struct Win
 {
     const void * back;
     const void * front;
     const void * pair;
-    float tb, tf;
-    double tb64, tf64;
 };
 
 // Texel coordinate of height zr (relative to the origin) on a stretched z axis:
@@ -624,19 +622,6 @@ __device__ __forceinline__ void sphere(const StepParams & P, float py, const flo
     o[1] = (v[1] * 180.f) / (PI * P.sphere_radius);
 }
 
-// unit_interval(Time + add, TimeFront, TimeBack) (tpp:132, 141-144)
-__device__ __forceinline__ float unit_t(const StepParams & P, const Win & W, float time32,
-                                        double time64, float add)
-{
-    if (P.flags & F_TIME64)
-    {
-        const double x = (time64 + double(add) - W.tb64) / (W.tf64 - W.tb64);
-        return float(x);
-    }
-    const float x = time32 + add;
-    return (x - W.tb) / (W.tf - W.tb);
-}
-
 // 2D loop borders (TrajectoryIntegrator.tpp:20-96); returns true on Domain abort.
 __device__ bool loop_border(const StepParams & P, const float * o, float * n)
 {
@@ -692,7 +677,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     const uint32_t oid = (FULL && P.orig) ? GR_LOAD(uint32_t, P.orig, pid) : pid; // caller-order index
 
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
-    Win W{P.wind.back, P.wind.front, P.wind.pair, P.tb, P.tf, P.tb64, P.tf64};
+    Win W{P.wind.back, P.wind.front, P.wind.pair};
     uint32_t wi = 0;
     // the window tables are uniform and read-only: scalar loads keep their values in SGPRs
     uint32_t edge = P.nwin > 1 ? GR_CONST(uint32_t, P.win_edge, 0) : 0xffffffffu;
@@ -733,8 +718,6 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 
     const float dt = P.dt;
     const float half_dt = 0.5f * dt;
-    float time32 = P.t_start;
-    double time64 = P.t_start64;
     uint64_t next_save = P.save_first;
     // ring column of the next save (global save index modulo the ring), advanced per save
     uint32_t col = P.ring_cap ? uint32_t(P.save_g0 % P.ring_cap) : 0u;
@@ -751,15 +734,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
         {
             ++wi;
-            const uint32_t c = P.win_first + wi;
             typedef const void * cvp_t;
             W.back = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 0);
             W.front = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 1);
             W.pair = GR_CONST(cvp_t, P.win_ptr, 3 * wi + 2);
-            W.tb = float(c) * P.dtd;
-            W.tf = float(c + 1) * P.dtd;
-            W.tb64 = double(c) * double(P.dtd);
-            W.tf64 = double(c + 1) * double(P.dtd);
             edge = wi + 1 < P.nwin ? GR_CONST(uint32_t, P.win_edge, wi) : 0xffffffffu;
             sc.base = 0xffffffffu;
         }
@@ -767,21 +745,11 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         if (live)
         {
             ++integrated;
-            // time weights of this step: from the launch's table, or formed here
-            float4 tt = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (P.ttab)
-                tt = GR_CONST(float4, P.ttab, s);
-            else
-            {
-                if (P.flags & F_DYNAMIC_ANY) tt.x = unit_t(P, W, time32, time64, 0.f);
-                tt.y = tt.z = tt.x;
-                if (INTEG == GR_RK4 && (P.flags & F_WIND_DYNAMIC))
-                {
-                    tt.y = unit_t(P, W, time32, time64, half_dt);
-                    tt.z = unit_t(P, W, time32, time64, dt);
-                }
-                tt.w = ((P.flags & F_TIME64) ? (time64 > 0.0) : (time32 > 0.f)) ? 1.f : 0.f;
-            }
+            // time weights of this step from the launch's table, which the host forms for every
+            // launch with the reference's operations: unit_interval of Time, Time + dt/2 and
+            // Time + dt in the step's window (tpp:132, 141-144), FP32 or FP64 time, and whether
+            // Time > 0 (curvature, tpp:347)
+            const float4 tt = GR_CONST(float4, P.ttab, s);
             const float t = tt.x;
             const bool sph = FULL && DIM == 2 && (P.flags & F_SPHERE);
             float v[3] = {0.f, 0.f, 0.f}, vc[3] = {0.f, 0.f, 0.f};
@@ -1073,8 +1041,6 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             col = col + 1 == P.ring_cap ? 0u : col + 1;
             next_save += P.save_interval;
         }
-        time32 += dt;
-        time64 += double(dt);
         if (status != GR_STATUS_TIME) break; // stopped: nothing more to compute or save
     }
 
