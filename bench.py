@@ -2,12 +2,15 @@
 """Benchmark: RK4 3D particle-steps/s on the time-modulated ABC flow (one MI355X).
 
 Workload (the 3D ABC configuration): 128^3 grid over [0, 2*pi)^3, 16 slices one time unit
-apart, 1,048,576 uniform seeds (SplitMix64 seed 1), RK4 with dt = 0.005, per-step curvature
-(IndividualAndTotalCurvature) and the expression sqrt(u*u+v*v+w*w), saved at every step into a
-device-resident column ring. All slices are resident in HBM before the timed region. A bench
-"step" is one launch advancing every particle by `--chunk` integration steps (default 50); the
-timed region is the job from its first step: the default 40 timed launches are the whole
-2000-step job. The warm-up launches run on a separate copy of the job before it.
+apart, 1,048,576 uniform seeds (SplitMix64 seed 1), RK4 with dt = 0.005, 2000 steps, per-step
+curvature (IndividualAndTotalCurvature) and the expression sqrt(u*u+v*v+w*w), saved at every
+step into a device-resident column ring. All slices are resident in HBM before the timed region.
+
+A bench "step" is one launch advancing every particle by ceil(2000 / --steps) integration
+steps, so the timed --steps launches are always the whole 2000-step job from its first step
+(the last launch is shorter when the division is not exact). The warm-up launches run on a
+separate copy of the job. The timed job is repeated --repeats times from fresh state; `value`
+is the median.
 
 Every seed of this workload leaves the domain (3D border aborts) within about 1000 steps, so
 the headline counts integrated particle-steps: steps a particle began alive, counted on the
@@ -20,12 +23,14 @@ Forms, all on identical inputs:
   fp16         FP16 slices, time-paired layout, Morton-ordered particles
   fp16_batched the fp16 form run by the persistent batched kernel, the particle set split into
                16 jobs of 65,536 (value)
-The FP16 batched form's final positions are compared with the FP64 CPU oracle on a particle
-subsample (domain-relative error over particles whose termination step agrees).
+Final positions of the FP16 batched and plain FP32 forms are compared with the FP64 CPU oracle
+on 16,384 particles: domain-relative error over particles whose termination agrees, and the
+termination mismatches classified (sensitive at the input precision, FP16 storage rounding,
+other).
 
-Also reported: a lite batched-jobs run (mixed 2D/3D, Euler/RK4 jobs in one persistent launch
-vs sequential plain FP32 launches), a lite serving run (Poisson arrivals, p50/p99 latency), and
-the CPU baseline (the oracle's FP32 restatement on the host's threads, bounded sample).
+Also reported: the 2D double-gyre configuration on the GPU and as the CPU baseline in full
+(65,536 particles x 1000 Euler steps, all host threads and one), the CPU baseline on a bounded
+sample of the headline workload, and lite batched-jobs and serving runs.
 """
 from __future__ import annotations
 
@@ -42,6 +47,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# VALU issue ceiling: 256 CUs x 2.4 GHz x one wave64 (non-packed FP32) instruction per clock per CU
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 EXPRS = ["sqrt(u*u+v*v+w*w)"]
 DT = np.float32(0.005)
 
@@ -103,24 +110,27 @@ def make_field(capi, ctx, wind, dtype):
     return f
 
 
-def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0, regroup_min=0):
+def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
+               regroup_min=0, total=2000, repeats=1, cu_waves=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
+        if cu_waves:
+            j.set_option(capi.OPT_CU_WAVES, cu_waves)
         if reorder:
             j.reorder()
         return j
 
-    job = new_job()
-    sch = Schedule()
+    st = {}
 
-    count = [0]
-    large = [True]  # live range at least regroup_min (it only shrinks)
+    def reset():
+        st.update(job=new_job(), sch=Schedule(), count=0, large=True)
 
     def launch():
-        left = chunk
+        job, sch = st["job"], st["sch"]
+        left = min(chunk, total - sch.counter)
         regrouped = False
-        while left:
+        while left > 0:
             k, edge, t = sch.chunk(left)
             w = capi.Window()
             w.wind, w.wind_back, w.wind_front = f.h, sch.window, sch.window + 1
@@ -134,39 +144,57 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
             if edge and reorder:
                 job.reorder()  # regroup by cell at every window edge (inside the timing)
                 regrouped = True
-        count[0] += 1
-        if reorder and reorder_every and not regrouped and count[0] % reorder_every == 0 and large[0]:
+        st["count"] += 1
+        if reorder and reorder_every and not regrouped and st["count"] % reorder_every == 0 and st["large"]:
             if job.live_range() >= regroup_min:
                 job.reorder()  # and every reorder_every launches while the live range is large
             else:
-                large[0] = False
+                st["large"] = False
 
+    reset()
     for _ in range(warmup):
         launch()
-    # the timed job starts from step 0 on fresh state
-    job.close()
-    job = new_job()
-    sch = Schedule()
-    count[0] = 0
-    large[0] = True
-    ctx.synchronize()
-    ctx.take_work()
-    ctx.take_kernel_ms()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        launch()
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    work = ctx.take_work()
-    kernel_ms, launches = ctx.take_kernel_ms()  # HIP events around every step-kernel launch
-    out = {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
-           "steps_done": sch.counter, "work": work, "positions": job.read(capi.OUT_POSITIONS),
-           "status": job.read(capi.OUT_STATUS), "abort_step": job.read(capi.OUT_ABORT_STEP)}
-    job.close()
+    st["job"].close()
+    runs = []
+    for rep in range(repeats):
+        # each timed job starts from step 0 on fresh state
+        reset()
+        ctx.synchronize()
+        ctx.take_work()
+        ctx.take_kernel_ms()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            launch()
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        work = ctx.take_work()
+        kernel_ms, launches = ctx.take_kernel_ms()  # HIP events around every step-kernel launch
+        r = {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
+             "steps_done": st["sch"].counter, "work": work}
+        if rep == repeats - 1:
+            job = st["job"]
+            r.update(positions=job.read(capi.OUT_POSITIONS), status=job.read(capi.OUT_STATUS),
+                     abort_step=job.read(capi.OUT_ABORT_STEP))
+        st["job"].close()
+        runs.append(r)
+    return median_run(runs)
+
+
+def median_run(runs):
+    """The run of median wall time, with every run's wall time and the final state of the last
+    run (identical in every run: results do not depend on timing)."""
+    order = sorted(range(len(runs)), key=lambda i: runs[i]["wall_s"])
+    out = dict(runs[order[len(runs) // 2]])
+    last = runs[-1]
+    for k in ("positions", "status", "abort_step"):
+        if k in last:
+            out[k] = last[k]
+    out["repeat_walls_s"] = [r["wall_s"] for r in runs]
     return out
 
 
-def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0, regroup_min=0):
+def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0, regroup_min=0,
+                total=2000, repeats=1):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
@@ -174,11 +202,17 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
         ctx.reorder(out)
         return out
 
-    jobs = new_jobs()
-    state = dict(window=0, time=0.0, time64=0.0, counter=0, launches=0, large=True)
+    state = {}
+
+    def reset():
+        state.update(jobs=new_jobs(), window=0, time=0.0, time64=0.0, counter=0, launches=0, large=True)
 
     def launch():
-        spans = [(j, f, dict(first_window=state["window"], max_steps=chunk,
+        jobs = state["jobs"]
+        k = min(chunk, total - state["counter"])
+        if k <= 0:
+            return
+        spans = [(j, f, dict(first_window=state["window"], max_steps=k,
                              time_start=state["time"], time_start64=state["time64"],
                              counter_start=state["counter"])) for j in jobs]
         ends, _ = ctx.batch_run(spans, timed=False, ends=True)
@@ -194,70 +228,104 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
             # every reorder_every launches while the live range is large
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
 
+    reset()
     for _ in range(warmup):
         launch()
-    for j in jobs:
+    for j in state["jobs"]:
         j.close()
-    jobs[:] = new_jobs()
-    state.update(window=0, time=0.0, time64=0.0, counter=0, launches=0, large=True)
-    ctx.synchronize()
-    ctx.take_work()
-    ctx.take_kernel_ms()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        launch()
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    work = ctx.take_work()
-    kernel_ms, launches = ctx.take_kernel_ms()
-    pos = np.concatenate([j.read(capi.OUT_POSITIONS) for j in jobs])
-    status = np.concatenate([j.read(capi.OUT_STATUS) for j in jobs])
-    ab = np.concatenate([j.read(capi.OUT_ABORT_STEP) for j in jobs])
-    for j in jobs:
-        j.close()
-    return {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
-            "steps_done": state["counter"], "work": work, "positions": pos, "status": status,
-            "abort_step": ab}
+    runs = []
+    for rep in range(repeats):
+        reset()
+        ctx.synchronize()
+        ctx.take_work()
+        ctx.take_kernel_ms()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            launch()
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        work = ctx.take_work()
+        kernel_ms, launches = ctx.take_kernel_ms()
+        r = {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
+             "steps_done": state["counter"], "work": work}
+        jobs = state["jobs"]
+        if rep == repeats - 1:
+            r.update(positions=np.concatenate([j.read(capi.OUT_POSITIONS) for j in jobs]),
+                     status=np.concatenate([j.read(capi.OUT_STATUS) for j in jobs]),
+                     abort_step=np.concatenate([j.read(capi.OUT_ABORT_STEP) for j in jobs]))
+        for j in jobs:
+            j.close()
+        runs.append(r)
+    return median_run(runs)
 
 
-def fp64_error(wind, seeds_sub, gs, steps_done, res, sub):
-    """Final-position error of the GPU form vs the FP64 oracle over particles whose status and
-    abort step agree (chaotic / threshold cases counted separately)."""
+def fp64_error(wind, seeds_sub, gs, steps_done, forms, storage_round=None):
+    """Final-position error of GPU forms against the FP64 oracle on the same particles.
+
+    forms: {name: result dict with positions/status/abort_step of at least len(seeds_sub)
+    particles}; storage_round: {name: function rounding the field as that form stores it}.
+    Returns {name: error summary} (termination_and_error)."""
     import oracle
     # the oracle steps while counter < MaximumSimulationTime / dt (FP32): half a step below
     # steps_done * dt it runs exactly steps_done steps
     st = dict(Space=0, Integrator=1, DeltaT=float(DT), DataTimeDistance=1.0, GridScale=gs,
               Offset=[0, 0, 0], MaximumSimulationTime=(steps_done - 0.5) * float(DT),
               SaveInterval=0)
-    ref = oracle.run(st, wind, seeds_sub, real64=True)
+    threads = min(os.cpu_count() or 1, 16)
+    sub = seeds_sub.shape[0]
+    ref = oracle.run(st, wind, seeds_sub, real64=True, nthreads=threads)
     if ref["steps"] != steps_done:
         return {"error": f"oracle ran {ref['steps']} steps, GPU {steps_done}"}
-    p_ref = ref["trajectories"][:, -1, :]
-    return termination_and_error(res["positions"][:sub], res["status"][:sub], res["abort_step"][:sub],
-                                 p_ref, ref["status"], ref["abort_step"], [2 * math.pi] * 3,
-                                 {"steps": int(steps_done)})
+
+    def rerun(idx, seeds=None, field=None):
+        r = oracle.run(st, wind if field is None else field, seeds_sub[idx] if seeds is None else seeds,
+                       real64=True, nthreads=threads)
+        return r["status"], r["abort_step"]
+
+    out = {}
+    for name, res in forms.items():
+        rnd = (storage_round or {}).get(name)
+        out[name] = termination_and_error(
+            res["positions"][:sub], res["status"][:sub], res["abort_step"][:sub],
+            ref["trajectories"][:, -1, :], ref["status"], ref["abort_step"], [2 * math.pi] * 3,
+            seeds_sub, rerun, rnd(wind) if rnd else None, {"steps": int(steps_done)})
+    return out
 
 
-def termination_and_error(pos, status, ab, pos_ref, status_ref, ab_ref, extent, extra=None):
+def termination_and_error(pos, status, ab, pos_ref, status_ref, ab_ref, extent, seeds, rerun,
+                          rounded_field=None, extra=None):
     """Domain-relative max / RMS position error over particles whose status and abort step
-    agree with the FP64 oracle, and the termination mismatches split into those inside the
-    threshold band (a stopped particle within 1e-5 of the extent from a domain edge, in either
-    result: exits and boundary stalls whose step depends on the last bits) and outside it."""
+    agree with the FP64 oracle, and a classification of the termination mismatches:
+      sensitive   the FP64 oracle's own termination changes when the seed moves by one FP32
+                  ulp per axis (up or down): the outcome is not determined at the precision the
+                  seeds are given in (chaotic divergence, or an exit or stall at a threshold);
+      storage     (FP16 / BF16 forms) the FP64 oracle on the field rounded as the form stores it
+                  terminates exactly like the GPU form: the difference is the documented
+                  storage rounding, amplified by the flow;
+      other       neither -- a genuine disagreement."""
     ext = np.asarray(extent, np.float64)
     same = (status_ref == status) & (ab_ref == ab)
-
-    def near_edge(p, st):
-        p = np.asarray(p, np.float64)
-        m = np.min(np.minimum(np.abs(p), np.abs(ext - p)) / ext, axis=1) < 1e-5
-        return m & (st != 0)
-
-    band = near_edge(pos, status) | near_edge(pos_ref, status_ref)
     d = np.abs(np.asarray(pos, np.float64)[same] - np.asarray(pos_ref, np.float64)[same]) / ext
+    mism = np.nonzero(~same)[0]
+    sensitive = np.zeros(mism.size, bool)
+    storage = np.zeros(mism.size, bool)
+    if mism.size:
+        base = np.asarray(seeds[mism], np.float32)
+        for direction in (np.inf, -np.inf):
+            moved = np.nextafter(base, np.float32(direction)).astype(np.float32)
+            s2, a2 = rerun(mism, seeds=moved)
+            sensitive |= (s2 != status_ref[mism]) | (a2 != ab_ref[mism])
+        if rounded_field is not None:
+            s3, a3 = rerun(mism, field=rounded_field)
+            storage = ~sensitive & (s3 == status[mism]) & (a3 == ab[mism])
     out = {"rms": float(np.sqrt(np.mean(d * d))) if d.size else None,
            "max": float(d.max()) if d.size else None,
-           "particles": int(same.sum()),
-           "termination_mismatch_in_band": int((~same & band).sum()),
-           "termination_mismatch_outside_band": int((~same & ~band).sum()),
+           "particles": int(len(status)),
+           "termination_agree": int(same.sum()),
+           "termination_mismatch": int(mism.size),
+           "mismatch_sensitive": int(sensitive.sum()),
+           "mismatch_storage_rounding": int(storage.sum()),
+           "mismatch_other": int((~sensitive & ~storage).sum()),
            "alive_at_end": int((status_ref == 0).sum())}
     if extra:
         out.update(extra)
@@ -286,6 +354,92 @@ def cpu_baseline(wind, gs):
             "sample": f"{n} particles x {r['steps']} RK4 steps of the same 3D ABC workload with "
                       f"curvature + expression saved every step (FP32 oracle restatement, "
                       f"{threads} threads, {el:.1f} s; {work} integrated particle-steps)"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def gyre_config():
+    """The 2D double-gyre configuration: 512 x 256 nodes over [0, 2] x [0, 1], 11 slices one
+    time unit apart, 65,536 seeds uniform in the interior (SplitMix64 seed 0), Euler, dt 0.01,
+    1000 steps, Block borders, saved every 10th step."""
+    from pygranite_amd import synthetic
+    wind, gs = synthetic.double_gyre(512, 256, 11)
+    seeds = synthetic.uniform_seeds(0, 65536, [gs[0], gs[1]], [2.0 - gs[0], 1.0 - gs[1]])
+    st = dict(Space=1, Integrator=0, DeltaT=0.01, DataTimeDistance=1.0, GridScale=gs, Offset=[0, 0],
+              MaximumSimulationTime=10.0, SaveInterval=10)
+    return wind, gs, seeds, st
+
+
+def gyre_cpu_baseline():
+    """The double-gyre configuration in full on the CPU (the oracle's FP32 restatement), on the
+    host threads (at most 16) and on one thread."""
+    import oracle
+    wind, gs, seeds, st = gyre_config()
+    out = {"cpu_model": cpu_model(), "host_threads_visible": os.cpu_count()}
+    for label, threads in (("all", min(os.cpu_count() or 1, 16)), ("one", 1)):
+        t0 = time.perf_counter()
+        r = oracle.run(st, wind, seeds, real64=False, nthreads=threads)
+        el = time.perf_counter() - t0
+        alive = r["status"] == 0
+        work = int(np.where(alive, r["steps"], r["abort_step"].astype(np.int64) + 1).sum())
+        out[label] = {"threads": threads, "seconds": el, "steps": int(r["steps"]),
+                      "particle_steps_per_s": work / el, "nominal_particle_steps_per_s":
+                      seeds.shape[0] * r["steps"] / el, "integrated_particle_steps": work}
+        out["saved"] = r["positions"][1:].astype(np.float32)  # (columns, N, 2), start set dropped
+    return out
+
+
+def gyre_gpu(capi, ctx, reference=None):
+    """The double-gyre configuration on the GPU: the whole 1000-step job in one launch (the
+    launch crosses the ten window edges), plain FP32 (slots sampled separately, caller order)
+    and FP32 with the locality layout; kernel-only time and time including the read of the
+    saved columns. With `reference` (the CPU path's saved columns, (columns, N, 2)) every saved
+    position is compared bit for bit."""
+    wind, gs, seeds, st = gyre_config()
+    f = make_field2(capi, ctx, wind, capi.F32)
+    out = {}
+    for form, paired in (("plain_fp32", 0), ("fp32_local", 1)):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.windfield_mode, d.dt, d.dtd = 2, 0, 1, 0.01, 1.0
+        d.grid_scale[0], d.grid_scale[1], d.grid_scale[2] = gs[0], gs[1], gs[0]
+        d.domain[0] = float(np.float32(512) * np.float32(gs[0]))
+        d.domain[1] = float(np.float32(256) * np.float32(gs[1]))
+        d.save_interval = 10
+        job = capi.Job(ctx, d, seeds, save_capacity=101)
+        job.set_option(capi.OPT_PAIRED, paired)
+        if paired:
+            job.reorder()
+        ctx.synchronize()
+        ctx.take_work()
+        ctx.take_kernel_ms()
+        t0 = time.perf_counter()
+        ends, _ = ctx.batch_run([(job, f, dict(max_steps=1000, max_sim_time=10.0))], timed=False, ends=True)
+        ctx.synchronize()
+        t_kernel = time.perf_counter() - t0
+        cols = job.read_saved(capi.OUT_POSITIONS, 0, job.saved_columns())
+        t_e2e = time.perf_counter() - t0
+        work = ctx.take_work()
+        kms, _ = ctx.take_kernel_ms()
+        out[form] = {"steps": ends[0]["steps"], "device_ms": kms, "wall_s": t_kernel,
+                     "particle_steps_per_s": work / (kms / 1e3),
+                     "end_to_end_s": t_e2e, "saved_columns": int(cols.shape[0]),
+                     "end_to_end_particle_steps_per_s": work / t_e2e,
+                     "bytes_per_particle_step": 84}
+        if reference is not None:
+            out[form]["bitexact_vs_cpu_fp32"] = bool(cols.shape == reference.shape and
+                                                     np.array_equal(cols, reference))
+        job.close()
+    f.close()
+    return out
 
 
 def batched_jobs_lite(capi, ctx, seed=3, njobs=64):
@@ -493,11 +647,21 @@ def atmosphere_workload(capi, ctx, args):
         st = dict(Space=0, Integrator=1, DeltaT=float(dt), DataTimeDistance=3600.0,
                   GridScale=list(gs), Offset=[0, 0, 0], SaveInterval=0,
                   MaximumSimulationTime=float(total_steps) * float(dt) + 1.0)
-        ref = oracle.run(st, wind_h, seeds[:sub], real64=True, zlevels=zlev, time64=True)
+        threads = min(os.cpu_count() or 1, 16)
+        ref = oracle.run(st, wind_h, seeds[:sub], real64=True, zlevels=zlev, time64=True, nthreads=threads)
+
+        def rerun(idx, seeds=None, field=None):
+            r = oracle.run(st, wind_h if field is None else field, seeds_sub[idx] if seeds is None else seeds,
+                           real64=True, zlevels=zlev, time64=True, nthreads=threads)
+            return r["status"], r["abort_step"]
+
+        seeds_sub = seeds[:sub]
+        rounders = {"fp16": synthetic.round_fp16, "bf16": synthetic.round_bf16}
         for name, (pos, status, ab, steps) in finals.items():
+            rnd = rounders.get(name)
             out[name]["fp64_error"] = termination_and_error(
                 pos[:sub], status[:sub], ab[:sub], ref["trajectories"][:, -1, :], ref["status"],
-                ref["abort_step"], dom)
+                ref["abort_step"], dom, seeds_sub, rerun, rnd(wind_h) if rnd else None)
     out["speedup_fp16_vs_fp32"] = out["fp16"]["particle_steps_per_s"] / out["fp32"]["particle_steps_per_s"]
     print(json.dumps(out), flush=True)
 
@@ -671,22 +835,34 @@ def copy_calibration():
     print(json.dumps({"copies": 3, "bytes_read_each": a.numel() * 4, "bytes_written_each": a.numel() * 4}))
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
-    (profiles/traffic_k_batch_abc3d.json), or None."""
-    p = os.path.join(ROOT, "profiles", "traffic_k_batch_abc3d.json")
+def load_counters():
+    """Per-launch hardware counters of the dominant kernel for the default command, from the
+    committed rocprofv3 --pmc summary (profiles/counters_k_batch_abc3d.json): HBM bytes
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration) and VALU wave instructions. None
+    when absent."""
+    p = os.path.join(ROOT, "profiles", "counters_k_batch_abc3d.json")
     if not os.path.exists(p):
         return None
     with open(p) as fh:
-        return json.load(fh)
+        c = json.load(fh)
+    c["source"] = "profiles/counters_k_batch_abc3d.json (" + c.get("command", "rocprofv3 --pmc") + ")"
+    return c
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=40,
+                    help="timed launches; each runs ceil(total / steps) integration steps, so the "
+                         "timed region is always the whole job")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chunk", type=int, default=50)
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="integration steps per launch (0: ceil(total-steps / steps))")
+    ap.add_argument("--total-steps", type=int, default=2000, help="steps of the headline job")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed runs of the whole job per form (value: the median)")
+    ap.add_argument("--error-particles", type=int, default=16384,
+                    help="particles compared with the FP64 oracle")
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--jobs", type=int, default=16)
@@ -700,6 +876,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
+    ap.add_argument("--no-gyre", action="store_true", help="skip the double-gyre GPU / CPU legs")
     ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere", "copy", "batch", "serving"],
                     help="copy: a plain 1 GiB device copy (x3), the calibration run for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -707,6 +884,10 @@ def main():
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
+    ap.add_argument("--batch-cu-waves", type=int, default=-1,
+                    help="batched kernel wavefronts per CU (0: register-limited; default: autotuned)")
+    ap.add_argument("--cu-waves", type=int, default=0,
+                    help="single-job forms: wavefronts per CU of the step kernel (0: register-limited)")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-pmax", type=float, default=4e6)
     ap.add_argument("--batch-smax", type=int, default=5000)
@@ -742,6 +923,8 @@ def main():
         ctx.set_option(capi.CTX_BATCH_BLOCK, args.batch_block)
     if args.batch_waves >= 0:
         ctx.set_option(capi.CTX_BATCH_WAVES, args.batch_waves)
+    if args.batch_cu_waves >= 0:
+        ctx.set_option(capi.CTX_BATCH_CU_WAVES, args.batch_cu_waves)
     if args.workload == "atmosphere":
         atmosphere_workload(capi, ctx, args)
         return
@@ -752,6 +935,8 @@ def main():
         serving_workload(capi, ctx, args)
         return
     n = args.grid
+    total = args.total_steps
+    chunk = args.chunk or -(-total // max(1, args.steps))
     wind, gs = synthetic.abc_flow(n, 16)
     seeds = synthetic.uniform_seeds(1, args.particles, [0.0] * 3, [2 * math.pi] * 3)
     print(f"# device {info['name']} CUs={info['compute_units']}", file=sys.stderr, flush=True)
@@ -760,67 +945,113 @@ def main():
 
     forms = args.forms.split(",")
     tune = None
-    if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0):
+    if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0
+                                                                 or args.batch_cu_waves >= 0):
         # launch configuration of the batched kernel from the autotuner (probe jobs of the same
         # count, size and description over the same FP16 field, one launch, before any timing)
         tune = ctx.autotune(f16, build_desc(capi, gs, n), n=max(1, args.particles // args.jobs),
-                            njobs=args.jobs, steps=args.chunk, trials=2)
-        print("# autotune: " + ", ".join(f"b{t['block']}/w{t['waves']}/g{t['group']} {t['ms']:.3f} ms"
+                            njobs=args.jobs, steps=min(chunk, 100), trials=2)
+        print("# autotune: " + ", ".join(f"b{t['block']}/g{t['group']}/cw{t['cu_waves']} {t['ms']:.3f} ms"
                                          for t in tune)
-              + f" -> block {ctx.get_option(capi.CTX_BATCH_BLOCK)}, waves "
-              f"{ctx.get_option(capi.CTX_BATCH_WAVES)}, group {ctx.get_option(capi.CTX_BATCH_GROUP)}",
+              + f" -> block {ctx.get_option(capi.CTX_BATCH_BLOCK)}, group "
+              f"{ctx.get_option(capi.CTX_BATCH_GROUP)}, CU waves {ctx.get_option(capi.CTX_BATCH_CU_WAVES)}",
               file=sys.stderr, flush=True)
     res = {}
     for name in forms:
-        print(f"# form {name}: {args.warmup}+{args.steps} launches of {args.chunk} steps, "
-              f"{args.particles} particles", file=sys.stderr, flush=True)
+        reps = args.repeats if name == forms[-1] or name == "fp16_batched" else 1
+        print(f"# form {name}: {args.warmup}+{args.steps} launches of {chunk} steps ({total}-step job, "
+              f"{reps} timed run(s)), {args.particles} particles", file=sys.stderr, flush=True)
+        kw = dict(total=total, repeats=reps)
         if name == "plain_fp32":
-            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, False, False)
+            r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, False, False, **kw)
         elif name == "fp32_local":
-            r = run_single(capi, ctx, f32, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every, args.regroup_min)
+            kw["cu_waves"] = args.cu_waves
+            r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
+                           args.reorder_every, args.regroup_min, **kw)
         elif name == "fp16":
-            r = run_single(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every, args.regroup_min)
+            kw["cu_waves"] = args.cu_waves
+            r = run_single(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
+                           args.reorder_every, args.regroup_min, **kw)
         elif name == "fp16_batched":
-            r = run_batched(capi, ctx, f16, seeds, gs, n, args.chunk, args.steps, args.warmup, args.jobs,
-                            args.reorder_every, args.regroup_min)
+            r = run_batched(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, args.jobs,
+                            args.reorder_every, args.regroup_min, **kw)
         else:
             raise SystemExit(f"unknown form {name}")
         r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
         r["psps"] = r["work"] / r["wall_s"]  # integrated particle-steps per second
         r["avg_launch_ms"] = r["kernel_ms"] / max(1, r["launches"])
         res[name] = r
-        print(f"#   {name}: {r['psps']:.3e} integrated particle-steps/s (wall; "
+        print(f"#   {name}: {r['psps']:.3e} integrated particle-steps/s (wall, median; "
               f"{r['work'] / (args.particles * max(1, r['steps_done'])):.3f} of nominal), device "
-              f"{r['kernel_ms']:.1f} ms in {r['launches']} launches", file=sys.stderr, flush=True)
+              f"{r['kernel_ms']:.1f} ms in {r['launches']} launches, {r['steps_done']} steps",
+              file=sys.stderr, flush=True)
 
     head = "fp16_batched" if "fp16_batched" in res else forms[-1]
     hv = res[head]
     outputs = 3 * 4 + 4 + 4  # saved position, curvature, expression per particle-step
     B16 = algorithmic_bytes(2, outputs)
-    # algorithmic bytes of the integrated particle-steps over the kernel's device time
+    # algorithmic bytes of the integrated particle-steps over the kernel's device time, both
+    # from this run (HIP events around every step-kernel launch of the timed job)
     ps_per_launch = hv["work"] / max(1, hv["launches"])
     achieved = ps_per_launch * B16 / (hv["avg_launch_ms"] / 1e3) / 1e9
 
     err = None
     if not args.no_error:
-        print("# FP64 oracle error check", file=sys.stderr, flush=True)
-        sub = 384
-        err = fp64_error(wind, seeds[:sub], gs, hv["steps_done"], hv, sub)
+        print(f"# FP64 oracle error check ({args.error_particles} particles)", file=sys.stderr, flush=True)
+        sub = min(args.error_particles, args.particles)
+        cmp_forms = {head: hv}
+        if "plain_fp32" in res and head != "plain_fp32":
+            cmp_forms["plain_fp32"] = res["plain_fp32"]
+        rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched")}
+        err = fp64_error(wind, seeds[:sub], gs, hv["steps_done"], cmp_forms, rounders)
     extras = {}
     if not args.no_extras:
         print("# lite batched-jobs run", file=sys.stderr, flush=True)
         extras["batched_jobs"] = batched_jobs_lite(capi, ctx)
         print("# lite serving run", file=sys.stderr, flush=True)
         extras["serving"] = serving_lite(capi, ctx)
+    gyre = None
+    if not args.no_gyre:
+        print("# double-gyre configuration: CPU (all threads, one thread) and GPU", file=sys.stderr, flush=True)
+        gcpu = gyre_cpu_baseline() if not args.no_cpu else None
+        ggpu = gyre_gpu(capi, ctx, gcpu["saved"] if gcpu else None)
+        gyre = {"workload": "2D double gyre 512x256 x 11 slices, 65,536 particles, Euler dt 0.01, "
+                            "1000 steps, saved every 10th step", "gpu": ggpu}
+        if gcpu:
+            gcpu.pop("saved")
+            gyre["cpu"] = gcpu
+            gyre["gpu_vs_cpu_all_threads"] = (ggpu["fp32_local"]["particle_steps_per_s"]
+                                              / gcpu["all"]["particle_steps_per_s"])
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wind, gs)
-    traffic = load_traffic()
+        cpu["cpu_model"] = cpu_model()
+    counters = load_counters()
 
     plain = res.get("plain_fp32", {}).get("psps")
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": counters.get("hbm_bytes_per_launch") if counters else None,
+            "achieved_is": "algorithmic bytes (stencil gathers at the storage width, state and "
+                           "saved outputs) of the launch's integrated particle-steps over its "
+                           "average device time; most gathers are served by L2 / MALL",
+            "kernel": "k_batch (RK4 3D FP16 items)" if head == "fp16_batched" else "k_advance",
+            "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
+            "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
+            "algorithmic_bytes_per_launch": ps_per_launch * B16}
+    if counters:
+        # measured per-launch counters of the same command (rocprofv3 --pmc, summarised under
+        # profiles/): actual HBM bytes and VALU instructions against the launch time of this run
+        roof["traffic_source"] = counters.get("source")
+        roof["hbm_actual_GBs"] = counters["hbm_bytes_per_launch"] / (hv["avg_launch_ms"] / 1e3) / 1e9
+        if counters.get("valu_insts_per_launch"):
+            rate = counters["valu_insts_per_launch"] / (hv["avg_launch_ms"] / 1e3)
+            peak = VALU_PEAK_WAVE_INSTS
+            roof["valu"] = {"wave_insts_per_launch": counters["valu_insts_per_launch"],
+                            "achieved_G_per_s": rate / 1e9, "peak_G_per_s": peak / 1e9,
+                            "frac": rate / peak,
+                            "peak_is": "256 CUs x 2.4 GHz x one wave64 VALU instruction per clock"}
     line = {
         "metric": "RK4 3D particle-steps/s vs plain FP32 at <=1e-4 RMS err; p50/p99 job latency",
         "value": hv["psps"],
@@ -834,36 +1065,37 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 arithmetic, fp16 field storage",
+        "dtype": "f32",
         "data": "synthetic (analytic ABC flow, SplitMix64 seeds); value counts integrated "
-                "particle-steps (steps begun alive), counted on the device",
-        "config": {"workload": "3D ABC flow RK4, 128^3 x 16 slices, 1,048,576 particles, dt "
-                               "0.005, 2000 steps, curvature + sqrt(u*u+v*v+w*w) saved every step",
-                   "form": head, "particles": args.particles, "grid": n,
-                   "steps_per_launch": args.chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
+                "particle-steps (steps begun alive), counted on the device; median of "
+                f"{len(hv['repeat_walls_s'])} timed runs of the whole job",
+        "config": {"workload": f"3D ABC flow RK4, {n}^3 x 16 slices, {args.particles:,} particles, dt "
+                               f"0.005, {hv['steps_done']} steps, curvature + sqrt(u*u+v*v+w*w) saved "
+                               "every step",
+                   "form": head, "storage": "fp16" if head.startswith("fp16") else "fp32",
+                   "particles": args.particles, "grid": n, "steps_done": hv["steps_done"],
+                   "steps_per_launch": chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
                    "regroup_every_launches": args.reorder_every, "regroup_min_live": args.regroup_min,
                    "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
                    "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
-                   "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP)},
+                   "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP),
+                   "batch_cu_waves": ctx.get_option(capi.CTX_BATCH_CU_WAVES),
+                   "single_job_cu_waves": args.cu_waves},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
         "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},
+        "forms_steps_done": {k: v["steps_done"] for k, v in res.items()},
+        "repeat_walls_s": hv["repeat_walls_s"],
         "autotune": tune,
         "plain_fp32_value": plain,
         "speedup_vs_plain_fp32": hv["psps"] / plain if plain else None,
         "fp64_error_domain_relative": err,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "k_batch (RK4 3D FP16 items)" if head == "fp16_batched" else "k_advance",
-                     "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
-                     "integrated_particle_steps_per_launch": ps_per_launch,
-                     "algorithmic_bytes_per_launch": ps_per_launch * B16},
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "double_gyre": gyre,
         "extras": extras,
         "device": info["name"],
     }
     print(json.dumps(line), flush=True)
-
 
 if __name__ == "__main__":
     main()

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define GR_ABI_VERSION 1
+#define GR_ABI_VERSION 2
 
 /* error codes */
 #define GR_OK 0
@@ -272,6 +272,15 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
  * for both slices; 0 samples the two slots separately. */
 #define GR_OPT_BLOCK 1u
 #define GR_OPT_PAIRED 2u
+/* GR_OPT_CU_WAVES (0 .. 32, default 0): cap on the step kernel's wavefronts per compute unit,
+ * applied by reserving LDS per workgroup; 0 = as many as the registers allow. Fewer wavefronts
+ * per CU keep their stencil gathers inside the CU's cache. Results do not depend on it. */
+#define GR_OPT_CU_WAVES 3u
+/* GR_OPT_SPECIALIZE (default 1): launches whose features are the plain set (exact sampler
+ * weights, uniform z, no Kahan, no reverse, global coordinates) with curvature Off or
+ * IndividualAndTotal run kernels built for that set; 0 runs the generic kernels, which read every
+ * flag at run time. Results are identical either way. */
+#define GR_OPT_SPECIALIZE 4u
 int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
 
 /* Reorders the resident particle state by the Morton code of each particle's current cell
@@ -333,18 +342,21 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
  * GR_CTX_BATCH_WAVES: register budget of the batched kernel: 0 = the compiler's allocation
  *   (default), 4 = at most 128 VGPRs so that 4 wavefronts fit per SIMD.
  * GR_CTX_BATCH_GROUP: compute units that share one work queue (1 .. number of CUs; default:
- *   the CUs of one XCD). Each queue holds one compact region of the particle sets. */
+ *   the CUs of one XCD). Each queue holds one compact region of the particle sets.
+ * GR_CTX_BATCH_CU_WAVES: cap on the batched kernel's wavefronts per compute unit (0 .. 32;
+ *   default 0 = as many as the registers allow), applied by reserving LDS per workgroup. */
 #define GR_CTX_BATCH_BLOCK 1u
 #define GR_CTX_BATCH_WAVES 2u
 #define GR_CTX_BATCH_GROUP 3u
+#define GR_CTX_BATCH_CU_WAVES 4u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 
 /* Autotuner of the batched kernel (no counterpart in the reference). Times every candidate
- * launch configuration (workgroup size 64 / 128 / 256 x register budget 0 / 4 x compute units
- * per queue: one XCD, 8, 1) on njobs probe jobs of `desc` with n seeds each spread over the
- * domain, advanced `steps` steps through `wind` from window 0 (best of `trials` runs each, all
- * from identical state), and sets the fastest as the context's batch options. results[0 ..
+ * launch configuration (workgroup size 64 / 128 / 256 x compute units per queue: one XCD, 8 x
+ * wavefronts per CU: register-limited, 4, 6, 8) on njobs probe jobs of `desc` with n seeds each
+ * spread over the domain, advanced `steps` steps through `wind` from window 0 (best of `trials`
+ * runs each, all from identical state), and sets the fastest as the context's batch options. results[0 ..
  * *nresults) receive every candidate's device time (may be NULL). The context's work counter
  * and kernel-time sums are unchanged. */
 typedef struct {
@@ -352,6 +364,7 @@ typedef struct {
     uint32_t waves;
     uint32_t group;
     float ms;
+    uint32_t cu_waves;
 } gr_tune_entry;
 int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
                       uint32_t njobs, uint32_t steps, uint32_t trials, gr_tune_entry * results,

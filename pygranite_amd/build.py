@@ -1,6 +1,6 @@
 """Builds the native parts of pygranite_amd in-tree.
 
-* ``libgranite_hip.so`` -- gfx950 HIP kernels + the C ABI of include/granite_hip.h
+* ``libgranite_hip.so`` -- gfx950 HIP kernels (csrc/gr_k_*.hip) + the C ABI of include/granite_hip.h
   (hipcc, --offload-arch=gfx950).
 * ``pygranite.<ext>`` -- the pybind11 ``pygranite`` module (g++), linked against
   libgranite_hip.so with an $ORIGIN rpath.
@@ -36,6 +36,11 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
+# kernel translation units (compiled in parallel): the single-job step kernels per storage type
+# (the FP32 one also holds their launcher), the batched kernel's three builds, and the support
+# kernels; all share gr_step.h
+HIP_SOURCES = ("gr_k_adv_f32.hip", "gr_k_adv_f16.hip", "gr_k_adv_bf16.hip", "gr_k_batch.hip",
+               "gr_k_batch_w4.hip", "gr_k_batch_narrow.hip", "gr_k_support.hip")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 
@@ -53,14 +58,15 @@ def build(verbose: bool = False) -> None:
     inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
     objs = []
     jobs = []
-    k_obj = os.path.join(BUILD, "gr_kernels.o")
-    jobs.append([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "gr_kernels.hip"), "-o", k_obj])
-    objs.append(k_obj)
+    for src in HIP_SOURCES:
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        jobs.append([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     for src in ("gr_capi.cpp", "gr_expr.cpp"):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
         jobs.append(["g++", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    with ThreadPoolExecutor(max_workers=min(10, (os.cpu_count() or 4) + 2)) as ex:
         list(ex.map(_run, jobs))
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", LIB + ".tmp"])
     os.replace(LIB + ".tmp", LIB)

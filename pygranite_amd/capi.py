@@ -34,8 +34,8 @@ EXPORTS = [
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
-OPT_BLOCK, OPT_PAIRED = 1, 2
-CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP = 1, 2, 3
+OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE = 1, 2, 3, 4
+CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP, CTX_BATCH_CU_WAVES = 1, 2, 3, 4
 
 
 class FieldDesc(ctypes.Structure):
@@ -102,7 +102,7 @@ class SpanEnd(ctypes.Structure):
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("waves", ctypes.c_uint32), ("group", ctypes.c_uint32),
-                ("ms", ctypes.c_float)]
+                ("ms", ctypes.c_float), ("cu_waves", ctypes.c_uint32)]
 
 
 _lib = None
@@ -277,8 +277,8 @@ class Context:
         cnt = ctypes.c_uint32(0)
         _check(lib().gr_batch_autotune(self.h, field.h, ctypes.byref(desc), n, njobs, steps, trials,
                                        res, 64, ctypes.byref(cnt)))
-        return [{"block": r.block, "waves": r.waves, "group": r.group, "ms": r.ms}
-                for r in res[:cnt.value]]
+        return [{"block": r.block, "waves": r.waves, "group": r.group, "cu_waves": r.cu_waves,
+                 "ms": r.ms} for r in res[:cnt.value]]
 
     def reorder(self, jobs):
         """Reorders several jobs of this context in one pass (gr_jobs_reorder)."""

@@ -1,0 +1,93 @@
+// Step kernels of single-job launches (gr_job_advance): one thread per particle, the
+// workgroups of a launch renumbered per XCD. Instantiated per storage type in
+// gr_k_adv_{f32,f16,bf16}.hip (compiled in parallel).
+#pragma once
+
+#include <type_traits>
+
+#include "gr_step.h"
+
+namespace gr
+{
+
+template <int DIM, int INTEG, class T, int MODE, int SPEC>
+__global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE, false, SPEC>(P, nxcd);
+}
+
+// The same kernel held to at most 128 VGPRs, i.e. four wavefronts per SIMD instead of three.
+// Each wavefront's step chain runs a little slower, so this pays only when the launch has many
+// more wavefronts than the compute units hold at once. Instantiated for the FP16 time-paired
+// form only: it is the variant that fits this budget without spilling registers.
+template <int DIM, int INTEG, class T, int MODE, int SPEC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_advance_w4(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE, false, SPEC>(P, nxcd);
+}
+
+// The FP16 time-paired kernel with the stencil reuse (CACHE) for narrow launches.
+template <int DIM, int INTEG, class T, int MODE, int SPEC>
+__global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE, true, SPEC>(P, nxcd);
+}
+
+typedef void (*AdvanceFn)(const StepParams, uint32_t);
+
+// lean kernels exist generic (spec < 0) and for the plain feature set with curvature Off (0) or
+// IndividualAndTotal (3); the full kernel is generic only
+template <template <int, int, class, int, int> class K, int DIM, int INTEG, class T, int MODE>
+struct SpecPick
+{
+    static AdvanceFn get(int spec)
+    {
+        if (spec == 0) return K<DIM, INTEG, T, MODE, 0>::fn();
+        if (spec == 3) return K<DIM, INTEG, T, MODE, 3>::fn();
+        return K<DIM, INTEG, T, MODE, -1>::fn();
+    }
+};
+template <int DIM, int INTEG, class T, int MODE, int SPEC> struct KPlainFn
+{
+    static AdvanceFn fn() { return k_advance<DIM, INTEG, T, MODE, SPEC>; }
+};
+template <int DIM, int INTEG, class T, int MODE, int SPEC> struct KW4Fn
+{
+    static AdvanceFn fn() { return k_advance_w4<DIM, INTEG, T, MODE, SPEC>; }
+};
+template <int DIM, int INTEG, class T, int MODE, int SPEC> struct KNarrowFn
+{
+    static AdvanceFn fn() { return k_advance_narrow<DIM, INTEG, T, MODE, SPEC>; }
+};
+
+template <int DIM, int INTEG, class T>
+AdvanceFn pick_mode(uint32_t mode, int spec, bool wide, bool narrow)
+{
+    if constexpr (std::is_same<T, __half>::value)
+    {
+        if (mode == M_PAIRED && wide) return SpecPick<KW4Fn, DIM, INTEG, T, M_PAIRED>::get(spec);
+        if (mode == M_PAIRED && narrow) return SpecPick<KNarrowFn, DIM, INTEG, T, M_PAIRED>::get(spec);
+    }
+    if (mode == M_FULL) return k_advance<DIM, INTEG, T, M_FULL, -1>;
+    if (mode == M_PAIRED) return SpecPick<KPlainFn, DIM, INTEG, T, M_PAIRED>::get(spec);
+    return SpecPick<KPlainFn, DIM, INTEG, T, M_SEPARATE>::get(spec);
+}
+
+// the kernel of storage type T for a launch (wide / narrow: see launch_advance)
+template <class T>
+AdvanceFn pick_advance_t(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow)
+{
+    if (dim == 3)
+        return integrator == GR_RK4 ? pick_mode<3, GR_RK4, T>(mode, spec, wide, narrow)
+                                    : pick_mode<3, GR_EULER, T>(mode, spec, wide, narrow);
+    return integrator == GR_RK4 ? pick_mode<2, GR_RK4, T>(mode, spec, wide, narrow)
+                                : pick_mode<2, GR_EULER, T>(mode, spec, wide, narrow);
+}
+
+// one per storage type, each in its own translation unit
+AdvanceFn pick_advance_f32(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
+AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
+AdvanceFn pick_advance_bf16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
+
+} // namespace gr

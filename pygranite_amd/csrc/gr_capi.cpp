@@ -116,10 +116,12 @@ struct gr_ctx
     size_t gtab_cap{0};
     hipEvent_t bev0{nullptr}, bev1{nullptr};
     int cus{0};
+    uint32_t lds_per_cu{0}; // LDS bytes of one compute unit (occupancy caps)
     uint32_t nxcd{1}; // XCDs of the device
     uint32_t batch_block{256};
     uint32_t batch_waves{0};
     uint32_t batch_group{32}; // compute units per queue of the batched kernel
+    uint32_t batch_cu_waves{0}; // cap on the batched kernel's wavefronts per CU (0: none)
     // device time of every step-kernel launch (event pairs, summed by gr_ctx_take_kernel_ms)
     std::vector<hipEvent_t> kev;
     size_t kpending{0};
@@ -177,6 +179,11 @@ struct gr_job
     uint64_t n{0};
     uint32_t full{0};
     uint32_t block{0};
+    // interpreter stack columns in LDS: the deepest program's stack depth - 2 (the two top values
+    // live in registers), from the validated bytecode
+    uint32_t expr_cols{0};
+    uint32_t cu_waves{0}; // cap on the step kernel's wavefronts per CU (0: none)
+    uint32_t specialize{1}; // use the compile-time feature sets when they apply
     float * p[3]{};
     float * q[3]{};
     float * c[3]{};
@@ -304,7 +311,11 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
         return set_err(GR_E_HIP, "gr_ctx_create: stream / buffer creation failed");
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    {
+        c->cus = prop.multiProcessorCount;
+        c->lds_per_cu = uint32_t(prop.maxSharedMemoryPerMultiProcessor);
+    }
     int nx = 1;
     if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && nx >= 1)
         c->nxcd = uint32_t(nx);
@@ -388,6 +399,10 @@ extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
                 return set_err(GR_E_INVALID, "batch group must be 1 .. %u compute units", c->ncu);
             c->batch_group = uint32_t(value);
             return GR_OK;
+        case GR_CTX_BATCH_CU_WAVES:
+            if (value < 0 || value > 32) return set_err(GR_E_INVALID, "batch CU waves must be 0 .. 32");
+            c->batch_cu_waves = uint32_t(value);
+            return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -400,6 +415,7 @@ extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
         case GR_CTX_BATCH_BLOCK: *value = c->batch_block; return GR_OK;
         case GR_CTX_BATCH_WAVES: *value = c->batch_waves; return GR_OK;
         case GR_CTX_BATCH_GROUP: *value = c->batch_group; return GR_OK;
+        case GR_CTX_BATCH_CU_WAVES: *value = c->batch_cu_waves; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -782,6 +798,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
         return set_err(GR_E_INVALID, "gr_job_create: too many programs / volumes / constants");
     if (n >= (1ull << 31)) return set_err(GR_E_INVALID, "gr_job_create: too many particles");
     if (!(d.dt == d.dt)) return set_err(GR_E_INVALID, "gr_job_create: DeltaT is NaN");
+    uint32_t expr_cols = 0;
     for (uint32_t e = 0; e < d.nprog; ++e)
     {
         const gr_program & pr = d.prog[e];
@@ -809,6 +826,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
             if (sp < pops) return set_err(GR_E_INVALID, "program %u: stack underflow at %u", e, i);
             sp += pushes - pops;
             if (sp > GR_MAX_STACK) return set_err(GR_E_INVALID, "program %u: stack overflow at %u", e, i);
+            if (sp > 2) expr_cols = std::max(expr_cols, uint32_t(sp - 2));
         }
         if (sp != 1) return set_err(GR_E_INVALID, "program %u leaves %d values on the stack", e, sp);
     }
@@ -819,6 +837,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     j->d = d;
     j->n = n;
     j->run_n = n;
+    j->expr_cols = expr_cols;
     j->cap = save_capacity ? save_capacity : 1;
     j->full = (d.dim == 2 && (d.border != GR_BLOCK || d.sphere_radius > 0.f)) ||
               d.abort_mode != GR_ABORT_TIME || d.uplift_mode != GR_MODE_OFF ||
@@ -926,6 +945,11 @@ extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
     {
         case GR_OPT_BLOCK: return gr_job_set_launch(j, uint32_t(value));
         case GR_OPT_PAIRED: j->paired = value ? 1 : 0; return GR_OK;
+        case GR_OPT_CU_WAVES:
+            if (value < 0 || value > 32) return set_err(GR_E_INVALID, "CU waves must be 0 .. 32");
+            j->cu_waves = uint32_t(value);
+            return GR_OK;
+        case GR_OPT_SPECIALIZE: j->specialize = value ? 1 : 0; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown option %u", option);
     }
 }
@@ -1498,8 +1522,12 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     const uint64_t cus = uint64_t(std::max(1, j->ctx->cus));
     const bool wide = j->run_n >= 2ull * cus * 4 * 3 * 64;
     const bool narrow = j->run_n <= cus * 4 * 2 * 64;
-    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, j->block, j->ctx->nxcd, j->run_n, wide, narrow,
-                           s) != 0)
+    const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
+    const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
+    const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, j->ctx->lds_per_cu);
+    const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, j->ctx->nxcd, j->run_n, wide, narrow,
+                           lds, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     hipEvent_t done = nullptr;
     if ((rc = kt_mark(j->ctx, s, true, &done)) != GR_OK) return rc;
@@ -1765,7 +1793,8 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         gr_job * j = spans[i].job;
         for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls_of[i];
         key.insert(key.end(), {i, uint64_t(uintptr_t(j)), j->run_n, uint64_t(uintptr_t(spans[i].wind)),
-                               uint64_t(spans[i].wind->d.dtype), cls_of[i]});
+                               uint64_t(spans[i].wind->d.dtype), cls_of[i],
+                               uint64_t(int64_t(j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1))});
     }
     const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
     if (items_dirty)
@@ -1824,7 +1853,8 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         {
             if (!span_steps[i]) continue;
             gr_job * j = spans[i].job;
-            const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype));
+            const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype,
+                                                            j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1));
             const uint64_t n = j->run_n;
             for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM)
             {
@@ -1893,8 +1923,14 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     ctx->stage_i = 1 - k;
     ctx->items_valid = true;
 
+    // interpreter LDS of the launch: the deepest stack of its jobs' programs (batched jobs use the
+    // lean kernel, which stages no reference slots)
+    uint32_t lds_cols = 0;
+    for (uint32_t i = 0; i < nspans; ++i)
+        if (span_steps[i] && spans[i].job->d.nprog) lds_cols = std::max(lds_cols, spans[i].job->expr_cols);
     int per_cu = 0;
-    if (gr::batch_occupancy(block, ctx->batch_waves, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
+    const uint32_t lds = gr::launch_lds_bytes(block, lds_cols, ctx->batch_cu_waves, ctx->lds_per_cu);
+    if (gr::batch_occupancy(block, ctx->batch_waves, lds, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
     // enough workgroups to fill the device, no more than one wavefront per item
     const uint32_t wpb = block / 64u;
     const uint32_t grid = std::max<uint32_t>(
@@ -1908,7 +1944,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     // narrow: every item's wavefront fits on the device at two per SIMD
     const bool narrow = uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
     if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
-                                   block, ctx->batch_waves, narrow, s) != 0)
+                                   block, ctx->batch_waves, narrow, lds, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     hipEvent_t done = nullptr;
     if (nitems && (rc = kt_mark(ctx, s, true, &done)) != GR_OK) return rc;
@@ -2123,12 +2159,12 @@ extern "C" int gr_job_last_advance_ms(gr_job * j, float * ms)
 // autotuner of the batched kernel
 // --------------------------------------------------------------------------------------------
 
-// Runs every candidate launch configuration of the batched kernel (workgroup size x register
-// budget) on the same probe workload -- a job of the caller's description with n seeds spread
-// uniformly over the domain, Morton-ordered, advanced `steps` steps from window 0 -- and keeps
-// the fastest as the context's batch options. Every candidate starts from identical state and
-// computes identical results, so device times compare directly. The context's work counter and
-// kernel-time pool are left as they were.
+// Runs every candidate launch configuration of the batched kernel (workgroup size x compute
+// units per queue x wavefronts per CU) on the same probe workload -- jobs of the caller's
+// description with n seeds each spread uniformly over the domain, Morton-ordered, advanced
+// `steps` steps from window 0 -- and keeps the fastest as the context's batch options. Every
+// candidate starts from identical state and computes identical results, so device times compare
+// directly. The context's work counter and kernel-time pool are left as they were.
 extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, uint32_t n,
                                  uint32_t njobs, uint32_t steps, uint32_t trials, gr_tune_entry * results,
                                  uint32_t max_results, uint32_t * nresults)
@@ -2148,6 +2184,7 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
     GR_HIP(hipMemcpyAsync(work_saved.data(), ctx->dwork, work_saved.size() * 8, hipMemcpyDeviceToHost, ctx->compute));
     GR_HIP(hipStreamSynchronize(ctx->compute));
     const uint32_t block0 = ctx->batch_block, waves0 = ctx->batch_waves, group0 = ctx->batch_group;
+    const uint32_t cuw0 = ctx->batch_cu_waves;
 
     // probe seeds of each job: deterministic low-discrepancy spreads over the domain (inset by
     // 1e-3), a different phase per job
@@ -2163,15 +2200,21 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
                 seeds[jb][size_t(i) * dim + d] = float(desc->offset[d] + u * double(desc->domain[d]));
             }
 
-    struct Cand { uint32_t block, waves, group; };
+    // The register budget stays the compiler's allocation (the 128-VGPR build spills and was never
+    // faster); the occupancy cap is the wavefronts-per-CU dimension: 0 = as many as registers
+    // allow (12 at block 64), or 4 / 6 / 8 (fewer concurrent stencil sets per CU cache).
+    struct Cand { uint32_t block, waves, group, cu_waves; };
     std::vector<Cand> cands;
     const uint32_t gx = std::max(1u, ctx->ncu / std::max(1u, ctx->nxcd));
     std::vector<uint32_t> groups{gx};
     if (gx > 8) groups.push_back(8);
-    if (gx > 1) groups.push_back(1);
     for (uint32_t g : groups)
-        for (uint32_t w : {0u, 4u})
-            for (uint32_t b : {64u, 128u, 256u}) cands.push_back(Cand{b, w, g});
+        for (uint32_t cw : {0u, 4u, 6u, 8u})
+            for (uint32_t b : {64u, 128u, 256u})
+            {
+                if (cw && cw < b / 64) continue; // the cap admits one workgroup at least
+                cands.push_back(Cand{b, 0u, g, cw});
+            }
     float best = 3.4e38f;
     size_t best_i = 0;
     uint32_t nres = 0;
@@ -2190,6 +2233,7 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
                 ctx->batch_block = cands[ci].block;
                 ctx->batch_waves = cands[ci].waves;
                 ctx->batch_group = cands[ci].group;
+                ctx->batch_cu_waves = cands[ci].cu_waves;
                 for (uint32_t jb = 0; jb < njobs; ++jb)
                 {
                     std::memset(&sp[jb], 0, sizeof(gr_span));
@@ -2209,7 +2253,8 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
         }
         if (rc != GR_OK) break;
         if (results && nres < max_results)
-            results[nres++] = gr_tune_entry{cands[ci].block, cands[ci].waves, cands[ci].group, tmin};
+            results[nres++] = gr_tune_entry{cands[ci].block, cands[ci].waves, cands[ci].group, tmin,
+                                            cands[ci].cu_waves};
         if (tmin < best)
         {
             best = tmin;
@@ -2228,11 +2273,13 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
         ctx->batch_block = block0;
         ctx->batch_waves = waves0;
         ctx->batch_group = group0;
+        ctx->batch_cu_waves = cuw0;
         return rc;
     }
     ctx->batch_block = cands[best_i].block;
     ctx->batch_waves = cands[best_i].waves;
     ctx->batch_group = cands[best_i].group;
+    ctx->batch_cu_waves = cands[best_i].cu_waves;
     if (nresults) *nresults = nres;
     return rc2;
 }

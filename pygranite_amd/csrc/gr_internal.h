@@ -188,8 +188,26 @@ struct StepParams
     unsigned long long * work;
 };
 
-// kernel modes (see gr_kernels.hip)
+// LDS of the expression interpreter, in columns of one float per lane of the workgroup: the
+// full kernel stages reference slots 4..15 (EXPR_REF_SLOTS columns), and every kernel holds the
+// stack entries below the two register-held values (maximum stack depth - 2 columns).
+enum : uint32_t { EXPR_REF_SLOTS = 12 };
+
+// kernel modes (see gr_advance.h)
 enum : uint32_t { MODE_PAIRED = 0, MODE_SEPARATE = 1, MODE_FULL = 2 };
+
+// Compile-time feature set of a lean launch (the step kernels' SPEC parameter): the curvature
+// mode when the launch uses the plain features only -- exact sampler weights, uniform z spacing,
+// no Kahan compensation, no reverse integration, global coordinates -- and a specialised build
+// exists for that mode (Off, IndividualAndTotal); -1 (generic build) otherwise.
+inline int lean_spec(const StepParams & P, uint32_t mode)
+{
+    if (mode == MODE_FULL) return -1;
+    if (P.flags & (F_KAHAN | F_REVERSE | F_LOCAL)) return -1;
+    if (P.wind.zlev || P.weights != GR_WEIGHTS_EXACT) return -1;
+    if (P.curvature == GR_CURV_OFF || P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) return int(P.curvature);
+    return -1;
+}
 
 // one work item of the persistent batched kernel: `count` (<= 64, one wavefront) particles of
 // job `job`
@@ -213,27 +231,52 @@ enum : uint32_t
     WORK_STRIDE = 8,
     CU_KEY_BITS = 11
 };
-int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype);
+// item variant: 12 * specialisation (0 generic, 1 plain with curvature off, 2 plain with
+// individual-and-total curvature) + 6 * (dim == 3) + 3 * (RK4) + storage type
+int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype, int spec);
+enum : uint32_t { BATCH_VARIANTS = 36 };
 // culut[cu_key] = dense index of that CU (0xffffffff: none); a CU's queue is index / group;
 // work: the work counter slots
 // waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD);
 // narrow: every item's wavefront fits on the device at once (selects the build that reuses FP16
 // stencils across RK4 stages, at two wavefronts per SIMD)
+// lds_bytes: dynamic shared memory of each workgroup (the interpreter's columns, possibly
+// padded to cap the workgroups per CU, see launch_lds_bytes)
 int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
                  const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
-                 uint32_t block, uint32_t waves, bool narrow, void * stream);
+                 uint32_t block, uint32_t waves, bool narrow, uint32_t lds_bytes, void * stream);
 // writes cu_key() of the CU running each of n workgroups
 int launch_cu_probe(uint32_t * out, uint32_t n, void * stream);
-int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu);
+int batch_occupancy(uint32_t block, uint32_t waves, uint32_t lds_bytes, int * blocks_per_cu);
+// Dynamic shared memory of a workgroup of `block` threads: the interpreter's `lds_cols` columns,
+// and, with cu_waves > 0, at least enough that no more than max(1, cu_waves / (block / 64))
+// workgroups fit in the CU's `lds_per_cu` bytes (an occupancy cap: fewer wavefronts per CU keep
+// their stencil gathers inside the CU's cache). Padding is rounded down to 1 KiB so that the cap
+// holds whatever the allocation granule.
+inline uint32_t launch_lds_bytes(uint32_t block, uint32_t lds_cols, uint32_t cu_waves, uint32_t lds_per_cu)
+{
+    uint32_t need = lds_cols * block * uint32_t(sizeof(float));
+    if (cu_waves && lds_per_cu)
+    {
+        const uint32_t wpb = block / 64 ? block / 64 : 1;
+        const uint32_t blocks = cu_waves / wpb ? cu_waves / wpb : 1;
+        const uint32_t pad = (lds_per_cu / blocks) & ~1023u;
+        if (pad > need) need = pad;
+    }
+    return need;
+}
 
-// launchers (gr_kernels.hip)
+// launchers (gr_k_advance.hip, gr_k_support.hip)
 // nxcd: XCDs of the device (workgroups are renumbered so each XCD runs one contiguous range);
 // nrun (<= P.n): the launch covers particles [0, nrun) -- every particle from nrun on is stopped;
 // wide: the launch holds many more wavefronts than the device runs at once (selects a register
 // budget of four wavefronts per SIMD where a kernel variant has one); narrow: all its wavefronts
 // fit at two per SIMD (selects the FP16 build that reuses stencils across RK4 stages)
-int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, void * stream);
+// spec: the kernel's compile-time feature set (lean_spec, or -1 for the generic build);
+// lds_bytes: as for launch_batch
+int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode, int spec,
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow,
+                   uint32_t lds_bytes, void * stream);
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);

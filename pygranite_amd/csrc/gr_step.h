@@ -1,20 +1,21 @@
-// gfx950 kernels of the trajectory engine.
+// Device code shared by the gfx950 kernel translation units: storage types, the software
+// sampler, the expression interpreter and the step body (advance_one) with its launch helpers.
 //
-// k_advance runs `nsteps` integration steps of one window for every particle, keeping the
-// particle state in registers across steps and writing saved columns to a device ring. It
-// replaces the one-launch-per-step `granite::integrate` kernel of the reference
-// (source/granite/TrajectoryIntegrator.tpp:108-457) together with the host step loop that
-// launched it (tpp:973-994). Sampling is done in software from node-interleaved FP32 / FP16 /
-// BF16 slices (one vector load per stencil corner and slice) instead of texture fetches
-// (TrajectoryIntegrator.icc:22-129).
+// The step kernels (gr_k_advance.hip, gr_k_batch.hip) replace the one-launch-per-step
+// `granite::integrate` kernel of the reference (source/granite/TrajectoryIntegrator.tpp:108-457)
+// together with the host step loop that launched it (tpp:973-994): they run `nsteps`
+// integration steps of one or more windows per launch, keeping the particle state in registers
+// and writing saved columns to a device ring. Sampling is done in software from node-interleaved
+// FP32 / FP16 / BF16 slices (one vector load per stencil corner and slice) instead of texture
+// fetches (TrajectoryIntegrator.icc:22-129).
 //
 // Arithmetic follows the reference's operation order and is compiled without FP contraction,
 // so the FP32 form performs exactly the IEEE operations of the step (reference semantics);
 // only the sampler weights differ from the texture unit (exact here, optionally quantised to
 // 1/256 for emulation).
+#pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
-#include <hipcub/device/device_scan.hpp>
 #include <stdint.h>
 
 #include <algorithm>
@@ -310,14 +311,13 @@ __device__ float sample_scalar(const void * base, const FieldView & f, uint32_t 
 // slots come from a uniform switch over the step's registers (`ref`).
 typedef __attribute__((address_space(3))) float lds_float;
 
-// One LDS area per kernel (a non-template function, so every inlined step variant of the
-// batched kernel shares it): 19 reference slots (volumes and constants of the full kernel) +
-// (stack depth - 1) stack entries per lane.
-__device__ __forceinline__ lds_float * expr_lds()
-{
-    __shared__ float buf[(19 + GR_MAX_STACK - 1) * 256];
-    return (lds_float *)buf;
-}
+// The interpreter's LDS is the launch's dynamic shared memory, sized by the host for the block
+// and the launch's programs (expr_lds_bytes): the full kernel stages the reference slots 4..15
+// (volumes, constants; EXPR_REF_SLOTS columns) and every kernel keeps stack entries below the
+// two register-held values (depth - 2 columns); one column holds one float per lane. A launch
+// without programs gets none.
+extern __shared__ float gr_dyn_lds[];
+__device__ __forceinline__ lds_float * expr_lds() { return (lds_float *)gr_dyn_lds; }
 
 // a op b for k = 0 add, 1 sub, 2 mul (branch-free: a - b is a + (-b) exactly), 3 div
 __device__ __forceinline__ float apply_op(uint32_t k, float a, float b)
@@ -342,7 +342,7 @@ __device__ __forceinline__ float run_program(const Program & pr, const refvec_t 
 {
     auto refv = [&](uint32_t d) -> float {
         if (!FULL) return rv[d & 7u];
-        return d < 7 ? rv[d] : R[(d - 7) * bs];
+        return d < 7 ? rv[d] : R[(d - 11) * bs]; // argument 7 + s reads slot s (4..15): column s - 4
     };
     float acc = 0.f, acc2 = 0.f;
     uint32_t depth = 0; // values on the stack (uniform)
@@ -404,11 +404,6 @@ __device__ __forceinline__ float run_program(const Program & pr, const refvec_t 
 // --------------------------------------------------------------------------------------------
 // the step kernel
 // --------------------------------------------------------------------------------------------
-struct Ctx
-{
-    const StepParams & P;
-};
-
 // Time-paired node layout: both slices of a node in one vector, 3D as
 // [u0 v0 w0 u1 v1 w1 0 0], 2D as [u0 v0 u1 v1] (slice 0 = back, 1 = front).
 template <class T, int NC> struct PairLoad;
@@ -504,12 +499,34 @@ __device__ __forceinline__ float zcoord(const FieldView & f, float zr)
     return float(k) + (zr - z0) / (z1 - z0);
 }
 
-template <int DIM>
+// Feature set of a step kernel. SPEC < 0: every flag is read from the launch parameters (the
+// generic kernels). SPEC >= 0: the plain feature set fixed at compile time -- exact sampler
+// weights, uniform z spacing, no Kahan compensation, no reverse integration, global coordinates
+// -- with curvature mode SPEC (lean_spec picks it on the host). Fixed flags remove the uniform
+// branches of the unused features from the step loop (fewer scalar instructions, no register
+// moves at their joins, lower SGPR pressure); results are the same as the generic kernel's.
+template <int SPEC> struct Feat
+{
+    static __device__ __forceinline__ bool kahan(const StepParams & P) { return SPEC < 0 && (P.flags & F_KAHAN); }
+    static __device__ __forceinline__ bool reverse(const StepParams & P) { return SPEC < 0 && (P.flags & F_REVERSE); }
+    static __device__ __forceinline__ bool local(const StepParams & P) { return SPEC < 0 && (P.flags & F_LOCAL); }
+    static __device__ __forceinline__ bool zlev(const StepParams & P) { return SPEC < 0 && P.wind.zlev; }
+    static __device__ __forceinline__ uint32_t weights(const StepParams & P)
+    {
+        return SPEC < 0 ? P.weights : uint32_t(GR_WEIGHTS_EXACT);
+    }
+    static __device__ __forceinline__ uint32_t curvature(const StepParams & P)
+    {
+        return SPEC < 0 ? P.curvature : uint32_t(SPEC);
+    }
+};
+
+template <int DIM, int SPEC>
 __device__ __forceinline__ void texel_arg(const StepParams & P, const float * x, float * arg)
 {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) arg[d] = (x[d] - P.off[d]) * P.inv_h[d];
-    if (DIM == 3 && P.wind.zlev) arg[2] = zcoord(P.wind, x[2] - P.off[2]);
+    if (DIM == 3 && Feat<SPEC>::zlev(P)) arg[2] = zcoord(P.wind, x[2] - P.off[2]);
 }
 
 // The last FP16 time-paired stencil a lane loaded: its base node and the raw words of its
@@ -527,12 +544,13 @@ struct Stencil16
     uint32_t w[DIM == 3 ? 8 : 4][DIM == 3 ? 3 : 2];
 };
 
-template <int DIM, class T, int MODE, bool CACHE>
+template <int DIM, class T, int MODE, bool CACHE, int SPEC>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
                                          float t, float * v, f16_t one, Stencil16<DIM> & C)
 {
     float arg[3] = {0.f, 0.f, 0.f};
-    texel_arg<DIM>(P, x, arg);
+    texel_arg<DIM, SPEC>(P, x, arg);
+    const uint32_t wmode = Feat<SPEC>::weights(P);
     if (MODE == M_PAIRED)
     {
         // Padded paired copy: node (i, j, k), i in [-1, n + 1], at
@@ -541,10 +559,10 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         // fixed offsets from the base node and need no per-corner clamp or select.
         Taps S;
         int i, j, k = -1;
-        split(arg[0], P.wind.nx, P.weights, i, S.wx);
-        split(arg[1], P.wind.ny, P.weights, j, S.wy);
+        split(arg[0], P.wind.nx, wmode, i, S.wx);
+        split(arg[1], P.wind.ny, wmode, j, S.wy);
         S.wz = 0.f;
-        if (DIM == 3) split(arg[2], P.wind.nz, P.weights, k, S.wz);
+        if (DIM == 3) split(arg[2], P.wind.nz, wmode, k, S.wz);
         const uint32_t PX = P.wind.nx + 3, PXY = PX * (P.wind.ny + 3);
         const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
         float b[DIM], f[DIM];
@@ -596,7 +614,7 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         return;
     }
     Taps S;
-    taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, P.weights, S);
+    taps<DIM>(arg, P.wind.nx, P.wind.ny, P.wind.nz, P.wind.address, wmode, S);
     float b[DIM];
     sample_slot<DIM, T, DIM>(W.back, S, b);
     if (P.flags & F_WIND_DYNAMIC)
@@ -663,17 +681,22 @@ __device__ bool loop_border(const StepParams & P, const float * o, float * n)
     return false;
 }
 
+template <int SPEC>
 __device__ __forceinline__ float global_coord(const StepParams & P, float x, int d)
 {
-    return (P.flags & F_LOCAL) ? float(P.off64[d] + double(x)) : x;
+    return Feat<SPEC>::local(P) ? float(P.off64[d] + double(x)) : x;
 }
 
 // Advances one particle by the launch's steps; returns the number of steps it integrated
 // (steps begun alive, the aborting step included).
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1>
 __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint32_t pid)
 {
     constexpr bool FULL = MODE == M_FULL;
+    static_assert(!FULL || SPEC < 0, "the full kernel reads every flag at run time");
+    typedef Feat<SPEC> F;
+    const uint32_t curvm = F::curvature(P);
+    const bool kahan = F::kahan(P), reverse = F::reverse(P);
     const uint32_t oid = (FULL && P.orig) ? GR_LOAD(uint32_t, P.orig, pid) : pid; // caller-order index
 
     // window state; with nwin > 1 the launch crosses window edges (tpp:933-969 per window)
@@ -694,20 +717,20 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     float ld[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) p[d] = G(P.p[d])[pid];
-    const bool curv_on = P.curvature != GR_CURV_OFF;
-    const bool curv_pos = P.curvature == GR_CURV_TOTAL || P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
+    const bool curv_on = curvm != GR_CURV_OFF;
+    const bool curv_pos = curvm == GR_CURV_TOTAL || curvm == GR_CURV_INDIVIDUAL_AND_TOTAL;
     if (curv_pos)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) q[d] = G(P.q[d])[pid];
-    if (P.curvature == GR_CURV_FAST_TOTAL)
+    if (curvm == GR_CURV_FAST_TOTAL)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) ld[d] = G(P.ld[d])[pid];
-    if (P.flags & F_KAHAN)
+    if (kahan)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) cmp[d] = G(P.c[d])[pid];
     int32_t abort_step = G(P.abort_step)[pid];
     float total = curv_on ? G(P.total)[pid] : 0.f;
-    float curv = P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? G(P.curv)[pid] : 0.f;
+    float curv = curvm == GR_CURV_INDIVIDUAL_AND_TOTAL ? G(P.curv)[pid] : 0.f;
     float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? G(P.len)[pid] : 0.f;
     float ex[GR_MAX_PROGRAMS];
 #pragma unroll
@@ -758,22 +781,22 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 const float t1 = tt.y, t2 = tt.z;
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE, CACHE>(P, W, p, t, k1, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, p, t, k1, one, sc);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE>(P, W, x, t1, k2, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t1, k2, one, sc);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE>(P, W, x, t1, k3, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t1, k3, one, sc);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE, CACHE>(P, W, x, t2, k4, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t2, k4, one, sc);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -785,7 +808,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                velocity<DIM, T, MODE, CACHE>(P, W, p, t, vc, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, p, t, vc, one, sc);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -793,7 +816,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 
             // dp (tpp:233-250)
             float dp[3] = {0.f, 0.f, 0.f};
-            const float sgn = (P.flags & F_REVERSE) ? -1.f : 1.f;
+            const float sgn = reverse ? -1.f : 1.f;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) dp[d] = v[d] * dt * sgn;
             if (FULL && DIM == 3 && (P.flags & (F_UPLIFT_CONST | F_UPLIFT_DYN)))
@@ -838,19 +861,19 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 
             float np[3] = {p[0], p[1], p[2]};
             float y[3] = {0.f, 0.f, 0.f};
-            if (P.flags & F_KAHAN)
+            if (kahan)
             {
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
                 {
-                    y[d] = ((P.flags & F_REVERSE) ? -dp[d] : dp[d]) - cmp[d];
+                    y[d] = (reverse ? -dp[d] : dp[d]) - cmp[d];
                     np[d] = p[d] + y[d];
                 }
             }
             else
             {
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) np[d] = (P.flags & F_REVERSE) ? p[d] - dp[d] : p[d] + dp[d];
+                for (int d = 0; d < DIM; ++d) np[d] = reverse ? p[d] - dp[d] : p[d] + dp[d];
             }
 
             float dp2 = 0.f;
@@ -870,10 +893,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                     bool topo = false;
                     if (FULL && DIM == 3 && (P.flags & F_TOPO))
                     {
-                        float ta[3] = {global_coord(P, np[0], 0) / P.h[0],
-                                       global_coord(P, np[1], 1) / P.h[1], 0.f};
+                        float ta[3] = {global_coord<SPEC>(P, np[0], 0) / P.h[0],
+                                       global_coord<SPEC>(P, np[1], 1) / P.h[1], 0.f};
                         topo = sample_scalar<2>(P.topo.back, P.topo, 0, ta, P.weights) >
-                               global_coord(P, np[2], 2);
+                               global_coord<SPEC>(P, np[2], 2);
                     }
                     if (topo)
                         abort = GR_STATUS_TOPOGRAPHY;
@@ -899,7 +922,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             else
             {
                 const bool tpos = tt.w != 0.f;
-                if (P.curvature == GR_CURV_FAST_TOTAL) // tpp:337-346
+                if (curvm == GR_CURV_FAST_TOTAL) // tpp:337-346
                 {
                     float l2 = 0.f;
 #pragma unroll
@@ -952,14 +975,14 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                     const float ds = 0.5f * (sa + sb);
                     sb_prev = sb;
                     have_sb = true;
-                    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) curv = k;
+                    if (curvm == GR_CURV_INDIVIDUAL_AND_TOTAL) curv = k;
                     if (!isnan(k)) total += k * ds;
                 }
 
                 if (FULL && (P.flags & (F_VOL_CONST | F_VOL_DYN))) // tpp:363-377
                 {
                     float arg[3] = {0.f, 0.f, 0.f};
-                    texel_arg<DIM>(P, np, arg);
+                    texel_arg<DIM, SPEC>(P, np, arg);
                     for (uint32_t k = 0; k < P.nvol; ++k)
                     {
                         float val = sample_scalar<DIM>(P.vols.back, P.vols, k, arg, P.weights);
@@ -973,15 +996,15 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 {
                     // reference vector slots (tpp:381-439): position, curvature and velocity from
                     // registers; volumes and constants (full kernel) staged in LDS, only those read
-                    lds_float * s_refs = expr_lds();
-                    lds_float * s_stk = s_refs + 19 * 256;
                     const uint32_t lane = threadIdx.x, bs = blockDim.x, m = P.refmask;
-                    lds_float * R = s_refs + lane;
+                    // column c of the reference area holds slot c + 4 (volumes 4..11, constants 12..15)
+                    lds_float * s_stk = expr_lds() + (FULL ? EXPR_REF_SLOTS * bs : 0u);
+                    lds_float * R = expr_lds() + lane;
                     if (FULL)
                     {
 #pragma unroll
                         for (int k = 0; k < GR_MAX_VOLUMES; ++k)
-                            if (m & (1u << (4 + k))) R[(4 + k) * bs] = k < int(P.nvol) ? vo[k] : 0.f;
+                            if (m & (1u << (4 + k))) R[k * bs] = k < int(P.nvol) ? vo[k] : 0.f;
 #pragma unroll
                         for (int k = 0; k < GR_MAX_CONSTANTS; ++k)
                         {
@@ -992,23 +1015,23 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                                 cv = G(P.cst_b[k])[oid];
                                 if (P.flags & F_CONST_DYN) cv = lerp(cv, G(P.cst_f[k])[oid], t);
                             }
-                            R[(12 + k) * bs] = cv;
+                            R[(8 + k) * bs] = cv;
                         }
                     }
                     float gx = 0.f, gy = 0.f, gz = 0.f;
                     if (m & 7u)
                     {
-                        gx = global_coord(P, np[0], 0);
-                        gy = global_coord(P, np[1], 1);
-                        if (DIM == 3) gz = global_coord(P, np[2], 2);
+                        gx = global_coord<SPEC>(P, np[0], 0);
+                        gy = global_coord<SPEC>(P, np[1], 1);
+                        if (DIM == 3) gz = global_coord<SPEC>(P, np[2], 2);
                     }
-                    const refvec_t rv = {gx, gy, gz, P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f,
+                    const refvec_t rv = {gx, gy, gz, curvm == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f,
                                          vc[0], vc[1], DIM == 3 ? vc[2] : 0.f, 0.f, 0.f};
                     for (uint32_t e = 0; e < P.nprog; ++e)
                         ex[e] = run_program<FULL>(P.prog[e], rv, R, s_stk, lane, bs);
                 }
 
-                if (P.flags & F_KAHAN)
+                if (kahan)
                 {
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) cmp[d] = (np[d] - p[d]) - y[d];
@@ -1028,8 +1051,8 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         {
             auto rp = G(P.ring_pos) + (uint64_t(col) * P.n + pid) * DIM;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) rp[d] = global_coord(P, p[d], d);
-            if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.ring_curv)[uint64_t(col) * P.n + pid] = curv;
+            for (int d = 0; d < DIM; ++d) rp[d] = global_coord<SPEC>(P, p[d], d);
+            if (curvm == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.ring_curv)[uint64_t(col) * P.n + pid] = curv;
             for (uint32_t e = 0; e < P.nprog; ++e)
                 G(P.ring_expr)[(uint64_t(e) * P.ring_cap + col) * P.n + pid] = ex[e];
             if (FULL)
@@ -1049,16 +1072,16 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     if (curv_pos)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) G(P.q[d])[pid] = q[d];
-    if (P.curvature == GR_CURV_FAST_TOTAL)
+    if (curvm == GR_CURV_FAST_TOTAL)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) G(P.ld[d])[pid] = ld[d];
-    if (P.flags & F_KAHAN)
+    if (kahan)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) G(P.c[d])[pid] = cmp[d];
     G(P.status)[pid] = status;
     G(P.abort_step)[pid] = abort_step;
     if (curv_on) G(P.total)[pid] = total;
-    if (P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.curv)[pid] = curv;
+    if (curvm == GR_CURV_INDIVIDUAL_AND_TOTAL) G(P.curv)[pid] = curv;
     if (FULL && P.abort_mode != GR_ABORT_TIME) G(P.len)[pid] = len;
     for (uint32_t e = 0; e < P.nprog; ++e) G(P.expr[e])[pid] = ex[e];
     if (FULL)
@@ -1093,639 +1116,15 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint
     return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1>
 __device__ __forceinline__ void advance_block(const StepParams & P, uint32_t nxcd)
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
     const uint32_t pid = b * blockDim.x + threadIdx.x;
     uint32_t w = 0;
-    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE>(P, pid);
+    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE, SPEC>(P, pid);
     if (P.work) // block sizes are multiples of 64
         wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
-}
-
-template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nxcd)
-{
-    advance_block<DIM, INTEG, T, MODE>(P, nxcd);
-}
-
-// The same kernel held to at most 128 VGPRs, i.e. four wavefronts per SIMD instead of three.
-// Each wavefront's step chain runs a little slower, so this pays only when the launch has many
-// more wavefronts than the compute units hold at once. Instantiated for the FP16 time-paired
-// form only: it is the variant that fits this budget without spilling registers.
-template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-k_advance_w4(const StepParams P, uint32_t nxcd)
-{
-    advance_block<DIM, INTEG, T, MODE>(P, nxcd);
-}
-
-// The FP16 time-paired kernel with the stencil reuse (CACHE) for narrow launches.
-template <int DIM, int INTEG, class T, int MODE>
-__global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint32_t nxcd)
-{
-    advance_block<DIM, INTEG, T, MODE, true>(P, nxcd);
-}
-
-// Persistent batched kernel: workgroups pull (job, particle block) items from a global counter
-// until the list is exhausted; each item runs its job's whole span of steps. Jobs of different
-// dimension, integrator and storage type share the launch; the variant is uniform per item.
-template <int DIM, int INTEG, class T, bool CACHE>
-__device__ __forceinline__ uint32_t run_item(const StepParams & P, uint32_t pid)
-{
-    // the host builds a paired copy only for jobs that use it (dynamic wind, paired option)
-    if (P.wind.pair) return advance_one<DIM, INTEG, T, M_PAIRED, CACHE>(P, pid);
-    return advance_one<DIM, INTEG, T, M_SEPARATE>(P, pid);
-}
-
-// Identity of the compute unit running the calling wavefront: XCC, shader engine, shader array
-// and CU fields of the HW_ID / XCC_ID hardware registers packed into CU_KEY_BITS bits.
-__device__ __forceinline__ uint32_t cu_key()
-{
-    const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4); // HW_REG_HW_ID, bits 31:0
-    const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20); // HW_REG_XCC_ID, bits 3:0
-    return ((xcc & 7u) << 8) | (((hw >> 13) & 7u) << 5) | (((hw >> 12) & 1u) << 4) | ((hw >> 8) & 15u);
-}
-
-__global__ void k_cu_probe(uint32_t * out, uint32_t n)
-{
-    if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = cu_key();
-}
-
-int launch_cu_probe(uint32_t * out, uint32_t n, void * stream)
-{
-    if (n == 0) return 0;
-    hipLaunchKernelGGL(k_cu_probe, dim3(n), dim3(64), 0, hipStream_t(stream), out, n);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-// Items are one wavefront of particles and every wavefront takes its own: a wavefront whose
-// particles are all dead moves on at once instead of waiting at a workgroup barrier. There is
-// one queue per group of `group` compute units (queue q = items [qbegin[q], qbegin[q + 1]), a
-// compact region of the Morton-ordered particle sets, the same region for every job); the
-// wavefronts of a group drain its queue first, so the group's caches hold the field tiles of
-// one region (one group per XCD: its L2; one per CU: its L1). A drained queue
-// clears its bit in masks[q / 32]; wavefronts then move to the nearest queue whose bit is still
-// set, and exit when no bit is left. Every wavefront accumulates its integrated particle-steps
-// and adds them once, into one of WORK_SLOTS counters. WAVES is the minimum number of wavefronts
-// per SIMD the register allocation must allow (1: no limit; 4: at most 128 VGPRs).
-template <int WAVES, bool CACHE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
-k_batch(const StepParams * __restrict__ jobs, const BatchItem * __restrict__ items,
-        uint32_t * qdata, uint32_t nq, const uint32_t * __restrict__ culut, uint32_t group,
-        unsigned long long * work)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t * counters = qdata;                                  // nq x QUEUE_STRIDE
-    const uint32_t * qbegin = qdata + size_t(nq) * QUEUE_STRIDE;  // nq + 1
-    uint32_t * masks = qdata + size_t(nq) * QUEUE_STRIDE + nq + 1; // ceil(nq / 32)
-    const uint32_t ng = (nq + 31) / 32;
-    uint32_t home = culut ? culut[cu_key()] : 0xffffffffu;
-    home = home != 0xffffffffu ? home / group : nq;
-    if (home >= nq) home = blockIdx.x % nq;
-    uint32_t q = home;
-    uint32_t acc = 0;
-    for (;;)
-    {
-        uint32_t v = 0;
-        if (lane == 0) v = atomicAdd(&counters[size_t(q) * QUEUE_STRIDE], 1u);
-        // every lane is active here, so the first active lane is lane 0; the result is
-        // wavefront-uniform (a scalar register), which keeps the item and job loads scalar
-        const uint32_t it = qbegin[q] + __builtin_amdgcn_readfirstlane(v);
-        if (it < qbegin[q + 1])
-        {
-            const BatchItem b = items[it];
-            const uint32_t pid = b.first + lane;
-            const StepParams & P = jobs[b.job];
-            if (lane < b.count)
-            {
-                uint32_t w;
-                switch (b.variant)
-                {
-                    case 0: w = run_item<2, GR_EULER, float, CACHE>(P, pid); break;
-                    case 1: w = run_item<2, GR_EULER, __half, CACHE>(P, pid); break;
-                    case 2: w = run_item<2, GR_EULER, bf16_t, CACHE>(P, pid); break;
-                    case 3: w = run_item<2, GR_RK4, float, CACHE>(P, pid); break;
-                    case 4: w = run_item<2, GR_RK4, __half, CACHE>(P, pid); break;
-                    case 5: w = run_item<2, GR_RK4, bf16_t, CACHE>(P, pid); break;
-                    case 6: w = run_item<3, GR_EULER, float, CACHE>(P, pid); break;
-                    case 7: w = run_item<3, GR_EULER, __half, CACHE>(P, pid); break;
-                    case 8: w = run_item<3, GR_EULER, bf16_t, CACHE>(P, pid); break;
-                    case 9: w = run_item<3, GR_RK4, float, CACHE>(P, pid); break;
-                    case 10: w = run_item<3, GR_RK4, __half, CACHE>(P, pid); break;
-                    default: w = run_item<3, GR_RK4, bf16_t, CACHE>(P, pid); break;
-                }
-                acc += w;
-            }
-            continue;
-        }
-        // queue q is drained (its counter only grows): clear its bit, find the nearest queue
-        // that still has one, starting in q's own mask word
-        if (lane == 0) atomicAnd(&masks[q >> 5], ~(1u << (q & 31u)));
-        uint32_t next = 0xffffffffu;
-        for (uint32_t k = 0; k < ng; ++k)
-        {
-            const uint32_t g = (q >> 5) + k < ng ? (q >> 5) + k : (q >> 5) + k - ng;
-            uint32_t m = 0;
-            if (lane == 0) m = __hip_atomic_load(&masks[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            m = __builtin_amdgcn_readfirstlane(m);
-            if (!m) continue;
-            uint32_t bit;
-            if (k == 0)
-            {
-                const uint32_t sh = q & 31u; // first set bit at or after q's position, cyclically
-                const uint32_t rot = sh ? ((m >> sh) | (m << (32u - sh))) : m;
-                bit = (uint32_t(__builtin_ctz(rot)) + sh) & 31u;
-            }
-            else
-                bit = uint32_t(__builtin_ctz(m));
-            next = g * 32u + bit;
-            break;
-        }
-        if (next >= nq) break; // every queue is drained
-        q = next;
-    }
-    if (work) wave_add_work(work + size_t(home % WORK_SLOTS) * WORK_STRIDE, acc);
-}
-
-int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype)
-{
-    return int((dim == 3 ? 6 : 0) + (integrator == GR_RK4 ? 3 : 0) + (dtype > 2 ? 0 : dtype));
-}
-
-int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
-                 const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
-                 uint32_t block, uint32_t waves, bool narrow, void * stream)
-{
-    if (nq == 0 || grid == 0 || group == 0) return 0;
-    if (narrow)
-        hipLaunchKernelGGL((k_batch<1, true>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                           qdata, nq, culut, group, work);
-    else if (waves >= 4)
-        hipLaunchKernelGGL((k_batch<4, false>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                           qdata, nq, culut, group, work);
-    else
-        hipLaunchKernelGGL((k_batch<1, false>), dim3(grid), dim3(block), 0, hipStream_t(stream), jobs, items,
-                           qdata, nq, culut, group, work);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int batch_occupancy(uint32_t block, uint32_t waves, int * blocks_per_cu)
-{
-    const hipError_t e =
-        waves >= 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<4, false>, int(block), 0)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<1, false>, int(block), 0);
-    return e == hipSuccess ? 0 : -1;
-}
-
-// --------------------------------------------------------------------------------------------
-// support kernels
-// --------------------------------------------------------------------------------------------
-
-// FP32 component-major staging -> node-interleaved storage of the slot
-template <class T>
-__global__ void k_pack(const float * __restrict__ src, T * __restrict__ dst, uint64_t nodes,
-                       uint32_t ncomp, uint32_t cstride)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nodes) return;
-    for (uint32_t c = 0; c < cstride; ++c)
-    {
-        const float v = c < ncomp ? src[uint64_t(c) * nodes + i] : 0.f;
-        if constexpr (sizeof(T) == 4)
-            reinterpret_cast<float *>(dst)[i * cstride + c] = v;
-        else if constexpr (std::is_same<T, __half>::value)
-            dst[i * cstride + c] = __float2half_rn(v);
-        else
-        {
-            // round to nearest even to bfloat16
-            uint32_t u = __float_as_uint(v);
-            uint16_t b;
-            if ((u & 0x7fffffffu) > 0x7f800000u)
-                b = uint16_t((u >> 16) | 0x0040u);
-            else
-                b = uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-            reinterpret_cast<uint16_t *>(dst)[i * cstride + c] = b;
-        }
-    }
-}
-
-__global__ void k_seed(const float * __restrict__ aos, float * x, float * y, float * z, uint64_t n,
-                       uint32_t dim, double o0, double o1, double o2, uint32_t local)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float a = aos[i * dim + 0], b = aos[i * dim + 1];
-    x[i] = local ? float(double(a) - o0) : a;
-    y[i] = local ? float(double(b) - o1) : b;
-    if (dim == 3)
-    {
-        const float c = aos[i * dim + 2];
-        z[i] = local ? float(double(c) - o2) : c;
-    }
-}
-
-__global__ void k_gather(const float * x, const float * y, const float * z, float * aos, uint64_t n,
-                         uint32_t dim, double o0, double o1, double o2, uint32_t local)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    aos[i * dim + 0] = local ? float(o0 + double(x[i])) : x[i];
-    aos[i * dim + 1] = local ? float(o1 + double(y[i])) : y[i];
-    if (dim == 3) aos[i * dim + 2] = local ? float(o2 + double(z[i])) : z[i];
-}
-
-__global__ void k_count_alive(const int32_t * __restrict__ status, uint64_t n,
-                              unsigned long long * out)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool alive = i < n && status[i] == GR_STATUS_TIME;
-    const unsigned long long m = __ballot(alive);
-    if ((threadIdx.x & (warpSize - 1)) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
-}
-
-// Completes saved columns: a particle that stopped before column c's step keeps its final
-// state, which the step kernels do not write into the ring. ring: `count` columns starting at
-// global column `first` of a ring of `cap` columns of n x w floats; src[d]: final values (w
-// arrays); column c was saved at step c * save_interval.
-__global__ void k_fill_frozen(float * __restrict__ ring, uint32_t cap, uint64_t first, uint32_t count,
-                              uint64_t n, uint32_t w, const int32_t * __restrict__ status,
-                              const int32_t * __restrict__ abort_step, const float * src0,
-                              const float * src1, const float * src2, uint64_t save_interval,
-                              uint32_t local, double o0, double o1, double o2)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || status[i] == GR_STATUS_TIME) return;
-    const int64_t stop = abort_step[i];
-    const float * src[3] = {src0, src1, src2};
-    const double off[3] = {o0, o1, o2};
-    float v[3];
-    for (uint32_t d = 0; d < w; ++d)
-        v[d] = local ? float(off[d] + double(src[d][i])) : src[d][i];
-    for (uint32_t c = 0; c < count; ++c)
-    {
-        const uint64_t g = first + c;
-        if (int64_t(g * save_interval) <= stop) continue; // written by the step kernel
-        float * col = ring + (g % cap) * n * w;
-        for (uint32_t d = 0; d < w; ++d) col[i * w + d] = v[d];
-    }
-}
-
-// --------------------------------------------------------------------------------------------
-// launchers
-// --------------------------------------------------------------------------------------------
-typedef void (*AdvanceFn)(const StepParams, uint32_t);
-
-template <int DIM, int INTEG, class T>
-static AdvanceFn pick_mode(uint32_t mode)
-{
-    if (mode == M_FULL) return k_advance<DIM, INTEG, T, M_FULL>;
-    if (mode == M_PAIRED) return k_advance<DIM, INTEG, T, M_PAIRED>;
-    return k_advance<DIM, INTEG, T, M_SEPARATE>;
-}
-
-template <int DIM, int INTEG>
-static AdvanceFn pick_type(uint32_t dtype, uint32_t mode)
-{
-    if (dtype == GR_F16) return pick_mode<DIM, INTEG, __half>(mode);
-    if (dtype == GR_BF16) return pick_mode<DIM, INTEG, bf16_t>(mode);
-    return pick_mode<DIM, INTEG, float>(mode);
-}
-
-int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, void * stream)
-{
-    AdvanceFn fn;
-    if (wide && mode == M_PAIRED && P.wind.dtype == GR_F16)
-    {
-        if (dim == 3)
-            fn = integrator == GR_RK4 ? k_advance_w4<3, GR_RK4, __half, M_PAIRED>
-                                      : k_advance_w4<3, GR_EULER, __half, M_PAIRED>;
-        else
-            fn = integrator == GR_RK4 ? k_advance_w4<2, GR_RK4, __half, M_PAIRED>
-                                      : k_advance_w4<2, GR_EULER, __half, M_PAIRED>;
-    }
-    else if (narrow && mode == M_PAIRED && P.wind.dtype == GR_F16)
-    {
-        if (dim == 3)
-            fn = integrator == GR_RK4 ? k_advance_narrow<3, GR_RK4, __half, M_PAIRED>
-                                      : k_advance_narrow<3, GR_EULER, __half, M_PAIRED>;
-        else
-            fn = integrator == GR_RK4 ? k_advance_narrow<2, GR_RK4, __half, M_PAIRED>
-                                      : k_advance_narrow<2, GR_EULER, __half, M_PAIRED>;
-    }
-    else if (dim == 3)
-        fn = integrator == GR_RK4 ? pick_type<3, GR_RK4>(P.wind.dtype, mode)
-                                  : pick_type<3, GR_EULER>(P.wind.dtype, mode);
-    else
-        fn = integrator == GR_RK4 ? pick_type<2, GR_RK4>(P.wind.dtype, mode)
-                                  : pick_type<2, GR_EULER>(P.wind.dtype, mode);
-    if (block == 0 || block > 256) block = 256;
-    const uint32_t grid = uint32_t((std::min<uint64_t>(nrun, P.n) + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, hipStream_t(stream), P, nxcd);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_fill_frozen(float * ring, uint32_t cap, uint64_t first, uint32_t count, uint64_t n,
-                       uint32_t w, const int32_t * status, const int32_t * abort_step,
-                       const float * const * src, uint64_t save_interval, uint32_t local,
-                       const double * off64, void * stream)
-{
-    if (n == 0 || count == 0) return 0;
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    hipLaunchKernelGGL(k_fill_frozen, dim3(grid), dim3(block), 0, hipStream_t(stream), ring, cap,
-                       first, count, n, w, status, abort_step, src[0], w > 1 ? src[1] : src[0],
-                       w > 2 ? src[2] : src[0], save_interval, local, off64[0], off64[1], off64[2]);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_pack(const float * src, void * dst, uint64_t nodes, uint32_t ncomp, uint32_t cstride,
-                uint32_t dtype, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((nodes + block - 1) / block);
-    if (grid == 0) return 0;
-    if (dtype == GR_F16)
-        hipLaunchKernelGGL(k_pack<__half>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
-                           reinterpret_cast<__half *>(dst), nodes, ncomp, cstride);
-    else if (dtype == GR_BF16)
-        hipLaunchKernelGGL(k_pack<bf16_t>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
-                           reinterpret_cast<bf16_t *>(dst), nodes, ncomp, cstride);
-    else
-        hipLaunchKernelGGL(k_pack<float>, dim3(grid), dim3(block), 0, hipStream_t(stream), src,
-                           reinterpret_cast<float *>(dst), nodes, ncomp, cstride);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_seed(const float * aos, float * const * soa, uint64_t n, uint32_t dim,
-                const double * off64, uint32_t local, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_seed, dim3(grid), dim3(block), 0, hipStream_t(stream), aos, soa[0],
-                       soa[1], soa[2], n, dim, off64[0], off64[1], off64[2], local);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_gather_positions(float * const * soa, float * aos, uint64_t n, uint32_t dim,
-                            const double * off64, uint32_t local, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_gather, dim3(grid), dim3(block), 0, hipStream_t(stream), soa[0], soa[1],
-                       soa[2], aos, n, dim, off64[0], off64[1], off64[2], local);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_count_alive(const int32_t * status, uint64_t n, unsigned long long * out, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_count_alive, dim3(grid), dim3(block), 0, hipStream_t(stream), status, n,
-                       out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-// --------------------------------------------------------------------------------------------
-// locality: time-paired slices and Morton reordering of particles
-// --------------------------------------------------------------------------------------------
-
-// Padded paired copy. Output node (pi, pj, pk) of the (nx + 3) x (ny + 3) x (nz + 3) grid
-// (2D: one plane) stands for grid node (pi - 1, pj - 1, pk - 1) and holds
-// [back comps, front comps, zero pad] with `so` elements; outside the grid it holds zeros
-// (border addressing) or the nearest edge node (clamp addressing). W is the raw element word
-// (uint32_t for FP32, uint16_t for FP16 / BF16): bits are copied, not converted.
-template <class W>
-__global__ void k_pair(const W * __restrict__ back, const W * __restrict__ front, W * __restrict__ out,
-                       uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim, uint32_t address,
-                       uint32_t ncomp, uint32_t cstride, uint32_t so)
-{
-    const uint32_t PX = nx + 3, PY = ny + 3, PZ = dim == 3 ? nz + 3 : 1;
-    const uint64_t o = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (o >= uint64_t(PX) * PY * PZ) return;
-    const int pi = int(o % PX), pj = int((o / PX) % PY), pk = int(o / (uint64_t(PX) * PY));
-    int i = pi - 1, j = pj - 1, k = dim == 3 ? pk - 1 : 0;
-    const bool inside = i >= 0 && i < int(nx) && j >= 0 && j < int(ny) && k >= 0 && k < int(nz);
-    if (address == 1)
-    {
-        i = min(max(i, 0), int(nx) - 1);
-        j = min(max(j, 0), int(ny) - 1);
-        k = min(max(k, 0), int(nz) - 1);
-    }
-    const bool copy = inside || address == 1;
-    const uint64_t src = copy ? (uint64_t(k) * ny + uint64_t(j)) * nx + uint64_t(i) : 0;
-    for (uint32_t c = 0; c < so; ++c)
-    {
-        W v = 0;
-        if (copy && c < ncomp)
-            v = back[src * cstride + c];
-        else if (copy && c < 2 * ncomp)
-            v = front[src * cstride + (c - ncomp)];
-        out[o * so + c] = v;
-    }
-}
-
-uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim)
-{
-    return uint64_t(nx + 3) * (ny + 3) * (dim == 3 ? nz + 3 : 1);
-}
-
-int launch_pair(const void * back, const void * front, void * out, uint32_t nx, uint32_t ny,
-                uint32_t nz, uint32_t dim, uint32_t address, uint32_t ncomp, uint32_t cstride,
-                uint32_t dtype, void * stream)
-{
-    const uint64_t nodes = pair_nodes(nx, ny, nz, dim);
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((nodes + block - 1) / block);
-    if (grid == 0) return 0;
-    const uint32_t so = ncomp == 3 ? 8 : 4;
-    if (dtype == GR_F32)
-        hipLaunchKernelGGL(k_pair<uint32_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
-                           static_cast<const uint32_t *>(back), static_cast<const uint32_t *>(front),
-                           static_cast<uint32_t *>(out), nx, ny, nz, dim, address, ncomp, cstride, so);
-    else
-        hipLaunchKernelGGL(k_pair<uint16_t>, dim3(grid), dim3(block), 0, hipStream_t(stream),
-                           static_cast<const uint16_t *>(back), static_cast<const uint16_t *>(front),
-                           static_cast<uint16_t *>(out), nx, ny, nz, dim, address, ncomp, cstride, so);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-__device__ __forceinline__ uint32_t spread3(uint32_t x) // 10 bits -> every third bit
-{
-    x &= 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
-__device__ __forceinline__ uint32_t spread2(uint32_t x) // 16 bits -> every second bit
-{
-    x &= 0xffffu;
-    x = (x | (x << 8)) & 0x00ff00ffu;
-    x = (x | (x << 4)) & 0x0f0f0f0fu;
-    x = (x | (x << 2)) & 0x33333333u;
-    x = (x | (x << 1)) & 0x55555555u;
-    return x;
-}
-
-// Bucket of particle g of a multi-job reorder: its job, then the Morton code of its current cell
-// with the low `shift` bits dropped; a job's stopped particles go to its last bucket. Counts the
-// bucket's particles.
-__global__ void k_bucket_keys(const SortJob * __restrict__ jobs, uint32_t njobs, uint64_t total,
-                              uint32_t * __restrict__ keys, uint32_t * __restrict__ vals,
-                              uint32_t * __restrict__ counts, uint32_t shift, uint32_t nb)
-{
-    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    uint32_t lo = 0, hi = njobs; // last job with base <= g
-    while (hi - lo > 1)
-    {
-        const uint32_t mid = (lo + hi) / 2;
-        if (jobs[mid].base <= g) lo = mid;
-        else hi = mid;
-    }
-    const SortJob & J = jobs[lo];
-    const uint32_t i = uint32_t(g - J.base);
-    uint32_t bucket = nb - 1;
-    if (J.status[i] == GR_STATUS_TIME)
-    {
-        const uint32_t lim = J.dim == 3 ? 1023u : 65535u;
-        uint32_t c[3] = {0, 0, 0};
-        for (uint32_t d = 0; d < J.dim; ++d)
-        {
-            float a = (J.p[d][i] - J.off[d]) * J.inv_h[d];
-            a = fminf(fmaxf(a, 0.f), float(J.grid[d] - 1));
-            c[d] = min(uint32_t(a), lim);
-        }
-        const uint32_t code = J.dim == 3 ? (spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2))
-                                         : (spread2(c[0]) | (spread2(c[1]) << 1));
-        bucket = min(code >> shift, nb - 2);
-    }
-    const uint32_t key = lo * nb + bucket;
-    keys[g] = key;
-    vals[g] = i;
-    // one atomic per distinct key of the wavefront (neighbouring particles mostly share a bucket:
-    // they were regrouped by the previous reorder)
-    for (;;)
-    {
-        const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-        const uint64_t m = __ballot(key == k0);
-        if (key == k0)
-        {
-            if ((m & ((1ull << __lane_id()) - 1)) == 0) atomicAdd(&counts[k0], uint32_t(__popcll(m)));
-            break;
-        }
-    }
-}
-
-// alive count of each job: the start of its stopped bucket minus its base (bucket starts from the
-// exclusive scan of the counts, before the scatter advances them)
-__global__ void k_bucket_alive(const SortJob * __restrict__ jobs, uint32_t njobs,
-                               const uint32_t * __restrict__ starts, uint32_t nb)
-{
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= njobs) return;
-    const SortJob & J = jobs[k];
-    if (J.alive_out) *J.alive_out = starts[size_t(k) * nb + nb - 1] - uint32_t(J.base);
-}
-
-// each particle to the next free position of its bucket (order inside a bucket is arbitrary)
-__global__ void k_bucket_scatter(const uint32_t * __restrict__ keys, const uint32_t * __restrict__ vals,
-                                 uint32_t * __restrict__ starts, uint64_t total, uint32_t * __restrict__ out)
-{
-    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const uint32_t key = keys[g];
-    // the lanes sharing a key take consecutive positions with one atomic (the lowest lane's)
-    for (;;)
-    {
-        const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-        const uint64_t m = __ballot(key == k0);
-        if (key == k0)
-        {
-            const uint32_t lead = uint32_t(__ffsll((unsigned long long)m) - 1);
-            uint32_t base = 0;
-            if (__lane_id() == lead) base = atomicAdd(&starts[k0], uint32_t(__popcll(m)));
-            base = __shfl(base, int(lead), 64);
-            out[base + uint32_t(__popcll(m & ((1ull << __lane_id()) - 1)))] = vals[g];
-            break;
-        }
-    }
-}
-
-__global__ void k_fill_u32(uint32_t * __restrict__ p, uint64_t n, uint32_t v)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uint32_t shift,
-                       uint32_t nb, uint32_t * keys, uint32_t * vals, uint32_t * counts,
-                       uint32_t * starts, void * temp, size_t & temp_bytes, uint32_t * out,
-                       void * stream)
-{
-    const uint64_t nbt = uint64_t(njobs) * nb;
-    if (!temp) // sizing call
-        return hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, counts, starts, int(nbt),
-                                                hipStream_t(stream)) == hipSuccess ? 0 : -1;
-    const uint32_t block = 256;
-    hipStream_t st = hipStream_t(stream);
-    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((nbt + block - 1) / block)), dim3(block), 0, st, counts, nbt, 0u);
-    if (total)
-        hipLaunchKernelGGL(k_bucket_keys, dim3(uint32_t((total + block - 1) / block)), dim3(block), 0, st,
-                           jobs, njobs, total, keys, vals, counts, shift, nb);
-    if (hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, starts, int(nbt), st) != hipSuccess) return -1;
-    hipLaunchKernelGGL(k_bucket_alive, dim3((njobs + 63) / 64), dim3(64), 0, st, jobs, njobs, starts, nb);
-    if (total)
-        hipLaunchKernelGGL(k_bucket_scatter, dim3(uint32_t((total + block - 1) / block)), dim3(block), 0, st,
-                           keys, vals, starts, total, out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-// Every (array, job) entry of a multi-job reorder in one launch: blockIdx.y picks the entry,
-// dst[i] = src[perm[i]] for its n elements.
-__global__ void k_gather_multi(const GatherEntry * __restrict__ tab)
-{
-    const GatherEntry & E = tab[blockIdx.y];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < E.n) E.dst[i] = E.src[E.perm ? E.perm[i] : i];
-}
-
-int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = (maxn + block - 1) / block;
-    if (grid == 0 || nent == 0) return 0;
-    hipLaunchKernelGGL(k_gather_multi, dim3(grid, nent), dim3(block), 0, hipStream_t(stream), tab);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-__global__ void k_scatter32(const uint32_t * __restrict__ in, uint32_t * __restrict__ out,
-                            const uint32_t * __restrict__ orig, uint64_t n, uint32_t w)
-{
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t o = orig[i];
-    for (uint32_t c = 0; c < w; ++c) out[o * w + c] = in[i * w + c];
-}
-
-int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_t n, uint32_t w,
-                     void * stream)
-{
-    const uint32_t block = 256;
-    const uint32_t grid = uint32_t((n + block - 1) / block);
-    if (grid == 0) return 0;
-    hipLaunchKernelGGL(k_scatter32, dim3(grid), dim3(block), 0, hipStream_t(stream),
-                       static_cast<const uint32_t *>(in), static_cast<uint32_t *>(out), orig, n, w);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 } // namespace gr

@@ -137,11 +137,13 @@ def test_batch_equals_single_job_advances(gpu_available):
     ctx.close()
 
 
-@pytest.mark.parametrize("block,waves,group", [(64, 0, 1), (128, 4, 8), (256, 4, 0), (192, 0, 3)])
-def test_batch_launch_options_change_nothing(gpu_available, block, waves, group):
-    """Workgroup size and register budget of the batched kernel leave every result unchanged,
-    and the device work counter equals the integrated particle-steps implied by the final
-    status and abort step of every particle."""
+@pytest.mark.parametrize("block,waves,group,cuw,spec", [(64, 0, 1, 6, 1), (128, 4, 8, 4, 1), (256, 4, 0, 0, 0),
+                                                  (192, 0, 3, 8, 0), (64, 0, 0, 0, 0)])
+def test_batch_launch_options_change_nothing(gpu_available, block, waves, group, cuw, spec):
+    """Workgroup size, register budget, wavefronts per CU and the compile-time feature sets
+    (spec 0: the generic kernels) of the batched kernel leave every result unchanged, and the
+    device work counter equals the integrated particle-steps implied by the final status and
+    abort step of every particle."""
     ctx = capi.Context(0)
     src, gs = synthetic.abc_flow(32, 5)
     f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
@@ -163,9 +165,13 @@ def test_batch_launch_options_change_nothing(gpu_available, block, waves, group)
             ctx.set_option(capi.CTX_BATCH_WAVES, opts[1])
             if opts[2]:
                 ctx.set_option(capi.CTX_BATCH_GROUP, opts[2])
+            ctx.set_option(capi.CTX_BATCH_CU_WAVES, opts[3])
         a = capi.Job(ctx, d3, s3)
         a.reorder()
         b = capi.Job(ctx, d2, s2)
+        if opts:
+            a.set_option(capi.OPT_SPECIALIZE, opts[4])
+            b.set_option(capi.OPT_SPECIALIZE, opts[4])
         ctx.take_work()
         steps, _ = ctx.batch_run([(a, f, dict(max_steps=640)), (b, f2, dict(max_steps=333))])
         work = ctx.take_work()
@@ -179,7 +185,7 @@ def test_batch_launch_options_change_nothing(gpu_available, block, waves, group)
 
     steps0, work0, ref = run(None)
     assert ctx.get_option(capi.CTX_BATCH_BLOCK) == 256 and ctx.get_option(capi.CTX_BATCH_WAVES) == 0
-    steps1, work1, got = run((block, waves, group))
+    steps1, work1, got = run((block, waves, group, cuw, spec))
     assert steps0 == steps1 == [640, 333]
     for r, g in zip(ref, got):
         for x, y in zip(r, g):
@@ -238,11 +244,14 @@ def test_autotune_picks_a_candidate_and_leaves_counters(gpu_available):
     ctx.synchronize()
     res = ctx.autotune(f, d, n=8192, njobs=3, steps=40, trials=1)
     assert len(res) >= 6 and all(r["ms"] > 0 for r in res)
-    assert {(r["block"], r["waves"]) for r in res} == {(b_, w_) for b_ in (64, 128, 256) for w_ in (0, 4)}
+    # workgroup size x wavefronts per CU (register-limited, 4, 6, 8; a cap admits one workgroup)
+    want = {(b_, c_) for b_ in (64, 128, 256) for c_ in (0, 4, 6, 8) if c_ == 0 or c_ >= b_ // 64}
+    assert {(r["block"], r["cu_waves"]) for r in res} == want
     best = min(res, key=lambda r: r["ms"])
     assert ctx.get_option(capi.CTX_BATCH_BLOCK) == best["block"]
     assert ctx.get_option(capi.CTX_BATCH_WAVES) == best["waves"]
     assert ctx.get_option(capi.CTX_BATCH_GROUP) == best["group"]
+    assert ctx.get_option(capi.CTX_BATCH_CU_WAVES) == best["cu_waves"]
     # the probe runs did not count: only job a's 100 steps are in the counters
     status, ab = a.read(capi.OUT_STATUS), a.read(capi.OUT_ABORT_STEP)
     assert ctx.take_work() == int(np.where(status == 0, 100, ab.astype(np.int64) + 1).sum())

@@ -97,7 +97,9 @@ def test_reverse_and_constant_windfield(pg, gpu_available):
         assert_bitexact(gpu, ref, label=str(extra) + " ")
 
 
-def test_uplift_constants_volumes_topography(pg, gpu_available):
+@pytest.mark.parametrize("modes", [dict(UpLiftMode="Dynamic", ConstantsMode="Dynamic", AdditionalVolumeMode="Constant"),
+                                   dict(UpLiftMode="Constant", ConstantsMode="Constant", AdditionalVolumeMode="Dynamic")])
+def test_uplift_constants_volumes_topography(pg, gpu_available, modes):
     wind, gs = abc(24, 4)
     n = 512
     s = seeds3(n)
@@ -109,8 +111,7 @@ def test_uplift_constants_volumes_topography(pg, gpu_available):
     vols = {"temp": rng.uniform(250, 300, (S, 24, 24, 24)).astype(np.float32),
             "qv": rng.uniform(0, 0.01, (S, 24, 24, 24)).astype(np.float32)}
     topo = np.full((24, 24), 0.3, np.float32)
-    st = base3(gs, UpLiftMode="Dynamic", ConstantsMode="Dynamic", AdditionalVolumeMode="Constant",
-               AdditionalCompute=["alpha*temp + beta*qv", "x+y+z-u*v"])
+    st = base3(gs, AdditionalCompute=["alpha*temp + beta*qv", "x+y+z-u*v"], **modes)
     gpu = run_product(pg, st, wind, s, uplift=uplift, constants=consts, volumes=vols,
                       topography=topo)
     ref = oracle.run(st, wind, s, real64=False, uplift=uplift, constants=consts, volumes=vols,
@@ -233,5 +234,60 @@ def test_saved_columns_of_stopped_particles(gpu_available):
         np.testing.assert_array_equal(ex, ref["compute"][0][:, :ncols].astype(np.float32))
         np.testing.assert_array_equal(job.read(capi.OUT_STATUS), ref["status"])
         job.close()
+    f.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,dtype,paired,curv", [(3, capi.F16, 1, 3), (3, capi.F32, 0, 3), (3, capi.BF16, 1, 0),
+                                                   (2, capi.F32, 1, 0), (2, capi.F16, 0, 3)])
+def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, curv):
+    """The step kernels built for the plain feature set (curvature Off / IndividualAndTotal)
+    give every output bit for bit equal to the generic kernels (GR_OPT_SPECIALIZE = 0), over
+    a launch that crosses window edges and saves every step."""
+    ctx = capi.Context(0)
+    if dim == 3:
+        src, gs = abc(24, 4)
+        f = capi.Field(ctx, 24, 24, 24, 3, dtype, nslots=4)
+        seeds = seeds3(3000)
+    else:
+        src, gs = gyre(64, 32, 4)
+        f = capi.Field(ctx, 64, 32, 1, 2, dtype, nslots=4, address=1)
+        seeds = seeds2(3000)
+    for sl in range(4):
+        f.upload(sl, [src[sl, c] for c in range(dim)])
+    outs = []
+    for spec in (1, 0):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode = dim, 1, curv, 1
+        d.dt, d.dtd = 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k] = gs[k % len(gs)]
+            d.domain[k] = float(np.float32(src.shape[-1 - k] if k < dim else 1) * np.float32(gs[k % len(gs)]))
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v)*2-x", dim=dim, curvature=curv)
+        job = capi.Job(ctx, d, seeds, save_capacity=250)
+        job.set_option(capi.OPT_PAIRED, paired)
+        job.set_option(capi.OPT_SPECIALIZE, spec)
+        ends, _ = ctx.batch_run([(job, f, dict(max_steps=250))], ends=True)
+        assert ends[0]["steps"] == 250
+        o = [job.read_saved(capi.OUT_POSITIONS, 0, 250), job.read_saved(capi.OUT_EXPR0, 0, 250),
+             job.read(capi.OUT_STATUS), job.read(capi.OUT_ABORT_STEP)]
+        if curv:
+            o += [job.read_saved(capi.OUT_CURVATURE, 0, 250), job.read(capi.OUT_TOTAL_CURVATURE)]
+        # the single-job kernels from the same state: one more window through gr_job_advance
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, 2, 3
+        w.time_back, w.time_front = 2.0, 3.0
+        w.time_start = w.time_start64 = float(ends[0]["time"])
+        w.counter_start, w.nsteps = 250, 40
+        job.advance(w)
+        o += [job.read(capi.OUT_POSITIONS), job.read(capi.OUT_EXPR0), job.read(capi.OUT_STATUS)]
+        outs.append(o)
+        job.close()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    if dim == 3:
+        assert (outs[0][2] != 0).sum() > 50  # particles stopped along the way (3D border)
     f.close()
     ctx.close()

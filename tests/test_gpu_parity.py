@@ -92,7 +92,7 @@ def test_precision_form_kahan_time64_bitexact(pg, gpu_available):
 
 
 def test_stretched_z_precision_form_bitexact(pg, gpu_available):
-    """Stretched z axis (ZLevels extension) on an atmosphere-like field: FP32 and FP16 storage,
+    """Stretched z axis (ZLevels extension) on an atmosphere-like field: FP32, FP16 and BF16 storage,
     RK4, Kahan positions, FP64 time, the expression w*1000/(abs(u)+abs(v)+1) saved every step,
     bit for bit against the FP32 oracle on the same levels (and the same rounded slices)."""
     nx = ny = 48
@@ -103,7 +103,8 @@ def test_stretched_z_precision_form_bitexact(pg, gpu_available):
     xy = synthetic.uniform_seeds(23, n, [0.0, 0.0], [dom[0], dom[1]])
     z = synthetic.loguniform_heights(24, n, float(zlev[1]), float(zlev[-1]))
     seeds = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
-    for storage, rounder in (("Float32", lambda a: a), ("Float16", synthetic.round_fp16)):
+    for storage, rounder in (("Float32", lambda a: a), ("Float16", synthetic.round_fp16),
+                             ("BFloat16", synthetic.round_bf16)):
         st = dict(Space="Space3D", Integrator="ClassicRungeKutta", DeltaT=60.0,
                   DataTimeDistance=3600.0, GridScale=list(gs), Offset=[0.0, 0.0, 0.0],
                   MaximumSimulationTime=150 * 60.0, SaveInterval=1, KahanSummation=True,

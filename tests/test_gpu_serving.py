@@ -57,11 +57,14 @@ def test_alone_equals_in_the_mix(gpu_available):
     r = requests()[3]
     wind, gs = raw[r.dataset]
     st = dict(Space=0, Integrator=r.integrator, DeltaT=r.dt, DataTimeDistance=1.0, GridScale=gs,
-              Offset=[0, 0, 0], MaximumSimulationTime=r.steps * r.dt, SaveInterval=0)
+              Offset=[0, 0, 0], MaximumSimulationTime=(r.steps - 0.5) * r.dt, SaveInterval=0)
+    # the oracle steps while counter < MaximumSimulationTime / dt (FP32): half a step below
+    # steps * dt it runs exactly r.steps steps
     ref = oracle.run(st, synthetic.round_fp16(wind), r.seeds, real64=False)
-    if ref["steps"] == r.steps:
-        np.testing.assert_array_equal(done_mix[r.rid].positions,
-                                      ref["trajectories"][:, -1].astype(np.float32))
+    assert ref["steps"] == r.steps
+    np.testing.assert_array_equal(done_mix[r.rid].positions,
+                                  ref["trajectories"][:, -1].astype(np.float32))
+    np.testing.assert_array_equal(done_mix[r.rid].status, ref["status"])
     ctx.close()
 
 
