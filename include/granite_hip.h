@@ -41,6 +41,8 @@ extern "C" {
 #define GR_MAX_CONSTANTS 4
 #define GR_MAX_CODE 64
 #define GR_MAX_STACK 8
+/* steps of one gr_job_advance_noalloc call */
+#define GR_NOALLOC_MAX_STEPS 16384
 #define GR_ABORT_VALUE 1e-9f
 
 /* storage types of field slices */
@@ -244,6 +246,18 @@ int gr_job_set_particle_data(gr_job * job, uint32_t kind, uint32_t slot, const f
 /* Runs w->nsteps steps on the compute stream (asynchronously). Saved columns (steps whose
  * counter % save_interval == 0) go to the device column ring. */
 int gr_job_advance(gr_job * job, const gr_window * w);
+/* gr_job_advance without any allocation, for callers that cannot allocate device or host
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. No
+ * counterpart in the reference (one job per TrajectoryIntegrator, allocations in compute_,
+ * TrajectoryIntegrator.tpp:467-715). Same window, same results as gr_job_advance; the launch's
+ * time table is formed on the device in a fixed per-GPU scratch array of GR_NOALLOC_MAX_STEPS
+ * steps (w->nsteps beyond it: GR_E_INVALID), the time-paired copy of the window's slots is used
+ * only if it is already resident (otherwise the slots are sampled separately, with identical
+ * results), and the launch is not timed (gr_job_last_advance_ms, gr_ctx_take_kernel_ms do not
+ * see it). Every call on one GPU shares the scratch array: calls queued on one context are
+ * ordered by its stream; calls that overlap on one GPU from several contexts race. The job's
+ * state, ring and the fields are allocated beforehand (gr_job_create, gr_field_create). */
+int gr_job_advance_noalloc(gr_job * job, const gr_window * w);
 /* Reads the current per-particle state (blocking). */
 int gr_job_read(gr_job * job, uint32_t kind, void * dst, uint64_t bytes);
 /* Copies saved columns [first, first + count) of `kind` (positions, curvature, expressions,

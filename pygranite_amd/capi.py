@@ -30,7 +30,7 @@ EXPORTS = [
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
     "gr_expr_compile", "gr_expr_eval_host", "gr_expr_eval_device_form", "gr_job_create", "gr_job_destroy",
-    "gr_job_set_particle_data", "gr_job_advance", "gr_job_read", "gr_job_read_saved",
+    "gr_job_set_particle_data", "gr_job_advance", "gr_job_advance_noalloc", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
@@ -144,6 +144,7 @@ def lib():
         "gr_job_destroy": ([vp], i32),
         "gr_job_set_particle_data": ([vp, u32, u32, vp], i32),
         "gr_job_advance": ([vp, P(Window)], i32),
+        "gr_job_advance_noalloc": ([vp, P(Window)], i32),
         "gr_job_read": ([vp, u32, vp, u64], i32),
         "gr_job_read_saved": ([vp, u32, u64, u64, vp], i32),
         "gr_job_count_alive": ([vp, P(u64)], i32),
@@ -390,6 +391,10 @@ class Job:
 
     def advance(self, w: Window):
         _check(lib().gr_job_advance(self.h, ctypes.byref(w)))
+
+    def advance_noalloc(self, w: Window):
+        """gr_job_advance_noalloc: the same window without any allocation."""
+        _check(lib().gr_job_advance_noalloc(self.h, ctypes.byref(w)))
 
     def read(self, kind: int) -> np.ndarray:
         if kind == OUT_POSITIONS:

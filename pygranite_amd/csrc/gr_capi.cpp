@@ -138,6 +138,9 @@ struct gr_ctx
     uint64_t alive_done{0}; // all of the first alive_done of them are known complete
     // live fields and jobs: the context outlives them (gr_ctx_destroy refuses otherwise)
     int children{0};
+    // end of the last no-allocation launch (created with the context, recorded by each such
+    // launch): later uploads into the slots it read wait for it
+    hipEvent_t noalloc_done{nullptr};
 };
 
 struct gr_field
@@ -321,6 +324,11 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
         c->nxcd = uint32_t(nx);
     (void)hipGetLastError();
     int rc = map_compute_units(c);
+    // the no-allocation entry points: this device's scratch address, and their launch event
+    if (rc == GR_OK && gr::noalloc_setup(device) != 0)
+        rc = set_err(GR_E_HIP, "gr_ctx_create: scratch address of device %d not found", device);
+    if (rc == GR_OK && hipEventCreateWithFlags(&c->noalloc_done, hipEventDisableTiming) != hipSuccess)
+        rc = set_err(GR_E_HIP, "gr_ctx_create: event creation failed");
     if (rc != GR_OK)
     {
         gr_ctx_destroy(c);
@@ -362,6 +370,7 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     dfree(c->d_gtab);
     if (c->bev0) (void)hipEventDestroy(c->bev0);
     if (c->bev1) (void)hipEventDestroy(c->bev1);
+    if (c->noalloc_done) (void)hipEventDestroy(c->noalloc_done);
     for (hipEvent_t & e : c->alive_ev)
         if (e) (void)hipEventDestroy(e);
     for (uint32_t * pg : c->alive_pages) (void)hipHostFree(pg);
@@ -1310,8 +1319,10 @@ static int f_wait(gr_field * f, uint32_t slot, hipStream_t s)
     return GR_OK;
 }
 
+// With allow_alloc = false nothing is allocated: the time-paired copy is used only when it is
+// already resident, else the window's slots are sampled separately (the same results).
 static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, uint32_t & mode,
-                           uint64_t & nsaves)
+                           uint64_t & nsaves, bool allow_alloc = true)
 {
     if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
     const gr_job_desc & d = j->d;
@@ -1445,7 +1456,13 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
 
     mode = j->full ? gr::MODE_FULL
                                   : ((d.windfield_mode == 1 && j->paired) ? gr::MODE_PAIRED : gr::MODE_SEPARATE);
-    if (mode == gr::MODE_PAIRED)
+    if (mode == gr::MODE_PAIRED && !allow_alloc)
+    {
+        for (auto & pr : wf->pairs)
+            if (pr.valid && pr.b == w->wind_back && pr.f == w->wind_front) P.wind.pair = pr.buf;
+        if (!P.wind.pair) mode = gr::MODE_SEPARATE;
+    }
+    else if (mode == gr::MODE_PAIRED)
         if ((rc = field_pair(wf, w->wind_back, w->wind_front, s, &P.wind.pair)) != GR_OK) return rc;
     P.orig = j->reordered ? j->orig : nullptr;
     j->grid[0] = wf->d.nx;
@@ -1534,6 +1551,42 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     GR_HIP(hipEventRecord(j->ev1, s));
     j->timed = true;
     return finish_advance(j, w, nsaves, done);
+}
+
+extern "C" int gr_job_advance_noalloc(gr_job * j, const gr_window * w)
+{
+    if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance_noalloc: null argument");
+    if (w->nsteps == 0) return GR_OK;
+    if (w->nsteps > GR_NOALLOC_MAX_STEPS)
+        return set_err(GR_E_INVALID, "gr_job_advance_noalloc: %u steps exceed the scratch of %u steps",
+                       w->nsteps, unsigned(GR_NOALLOC_MAX_STEPS));
+    gr::StepParams P;
+    uint32_t mode = 0;
+    uint64_t nsaves = 0;
+    int rc = prepare_advance(j, w, P, mode, nsaves, false);
+    if (rc != GR_OK) return rc;
+    gr_ctx * ctx = j->ctx;
+    hipStream_t s = ctx->compute;
+    if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
+    float * tab = static_cast<float *>(gr::noalloc_scratch(ctx->device));
+    if (!tab) return set_err(GR_E_STATE, "gr_job_advance_noalloc: no scratch on device %d", ctx->device);
+    // the whole table of this launch is written before the step kernel reads it (stream order)
+    if (gr::launch_time_table(P, tab, s) != 0)
+        return set_err(GR_E_HIP, "time-table kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    P.ttab = tab;
+    const uint64_t cus = uint64_t(std::max(1, ctx->cus));
+    const bool wide = j->run_n >= 2ull * cus * 4 * 3 * 64;
+    const bool narrow = j->run_n <= cus * 4 * 2 * 64;
+    const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
+    const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
+    const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, ctx->lds_per_cu);
+    const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
+    if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, ctx->nxcd, j->run_n, wide, narrow,
+                           lds, s) != 0)
+        return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    GR_HIP(hipEventRecord(ctx->noalloc_done, s));
+    j->timed = false;
+    return finish_advance(j, w, nsaves, ctx->noalloc_done);
 }
 
 // --------------------------------------------------------------------------------------------

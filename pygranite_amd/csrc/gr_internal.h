@@ -277,6 +277,15 @@ inline uint32_t launch_lds_bytes(uint32_t block, uint32_t lds_cols, uint32_t cu_
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode, int spec,
                    uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow,
                    uint32_t lds_bytes, void * stream);
+// Scratch of the no-allocation entry points (gr_k_noalloc.hip): NOALLOC_MAX_STEPS steps of time
+// table in a file-scope device array. noalloc_setup takes its address on the current device
+// (once per device, at context creation); noalloc_scratch returns it (null before setup).
+// launch_time_table forms the time table of a single-window launch P in `out` on the device.
+enum : uint32_t { NOALLOC_MAX_STEPS = GR_NOALLOC_MAX_STEPS };
+int noalloc_setup(int device);
+void * noalloc_scratch(int device);
+int launch_time_table(const StepParams & P, float * out, void * stream);
+
 // builds the padded time-paired layout of (back, front) slots: (nx+3)(ny+3)(nz+3) nodes (2D: one
 // plane), elements per node 8 (3D) / 4 (2D)
 uint64_t pair_nodes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t dim);
