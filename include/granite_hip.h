@@ -239,6 +239,11 @@ typedef enum {
 int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds, uint64_t n,
                   uint32_t save_capacity, gr_job ** out);
 int gr_job_destroy(gr_job * job);
+/* Device bytes gr_job_create allocates for a job of `desc` with n particles and save_capacity
+ * saved columns, plus (with_reorder != 0) what its first reorder adds (index map and shadow
+ * copies). For admission by memory budget; no counterpart in the reference. */
+int gr_job_bytes(const gr_job_desc * desc, uint64_t n, uint32_t save_capacity, int with_reorder,
+                 uint64_t * bytes);
 /* Per-particle data with slots: kind 0 = uplift (n*dim floats), 1 + k = constant k
  * (n floats). Replaces the uplift/constant buffers of TrajectoryIntegrator.cpp:195-245 and
  * 305-359. */

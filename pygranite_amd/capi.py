@@ -29,7 +29,7 @@ EXPORTS = [
     "gr_ctx_get_option", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
-    "gr_expr_compile", "gr_expr_eval_host", "gr_expr_eval_device_form", "gr_job_create", "gr_job_destroy",
+    "gr_expr_compile", "gr_expr_eval_host", "gr_expr_eval_device_form", "gr_job_create", "gr_job_destroy", "gr_job_bytes",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_advance_noalloc", "gr_job_read", "gr_job_read_saved",
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
@@ -142,6 +142,7 @@ def lib():
         "gr_expr_eval_device_form": ([P(Program), P(ctypes.c_float), P(ctypes.c_float)], i32),
         "gr_job_create": ([vp, P(JobDesc), vp, u64, u32, P(vp)], i32),
         "gr_job_destroy": ([vp], i32),
+        "gr_job_bytes": ([P(JobDesc), u64, u32, i32, P(u64)], i32),
         "gr_job_set_particle_data": ([vp, u32, u32, vp], i32),
         "gr_job_advance": ([vp, P(Window)], i32),
         "gr_job_advance_noalloc": ([vp, P(Window)], i32),
@@ -173,6 +174,14 @@ class GraniteError(RuntimeError):
 def _check(rc):
     if rc != 0:
         raise GraniteError(f"granite error {rc}: {lib().gr_last_error().decode()}")
+
+
+def job_bytes(desc, n: int, save_capacity: int = 1, reorder: bool = True) -> int:
+    """Device bytes of a job (gr_job_bytes): what gr_job_create allocates, plus what its first
+    reorder adds when `reorder`."""
+    b = ctypes.c_uint64()
+    _check(lib().gr_job_bytes(ctypes.byref(desc), n, save_capacity, 1 if reorder else 0, ctypes.byref(b)))
+    return b.value
 
 
 def device_count() -> int:

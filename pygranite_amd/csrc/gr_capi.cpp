@@ -792,6 +792,72 @@ static int zalloc(float ** p, size_t n)
     return rc;
 }
 
+// Every device array of a job at creation: where its pointer goes, its size, and the byte its
+// contents start as (-1: left uninitialised). `state` marks the per-particle state arrays that a
+// reorder gathers (each gets a shadow copy then). gr_job_create allocates exactly these and
+// gr_job_bytes sums them, so the two cannot drift apart.
+struct JobArray
+{
+    void ** slot;
+    size_t bytes;
+    int fill;
+    bool state;
+};
+static std::vector<JobArray> job_arrays(gr_job * j)
+{
+    const gr_job_desc & d = j->d;
+    const size_t N = size_t(j->n ? j->n : 1), D = d.dim, cap = j->cap;
+    std::vector<JobArray> a;
+    auto add = [&](auto *& p, size_t bytes, int fill, bool state) {
+        a.push_back(JobArray{reinterpret_cast<void **>(&p), bytes, fill, state});
+    };
+    const bool curv_pos = d.curvature == GR_CURV_TOTAL || d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
+    for (size_t k = 0; k < D; ++k) add(j->p[k], N * 4, 0, true);
+    for (size_t k = 0; k < D && curv_pos; ++k) add(j->q[k], N * 4, 0, true);
+    for (size_t k = 0; k < D && d.kahan; ++k) add(j->c[k], N * 4, 0, true);
+    for (size_t k = 0; k < D && d.curvature == GR_CURV_FAST_TOTAL; ++k) add(j->ld[k], N * 4, 0, true);
+    add(j->status, N * 4, 0, true);
+    add(j->abort_step, N * 4, 0xff, true);
+    if (d.curvature != GR_CURV_OFF) add(j->total, N * 4, 0, true);
+    if (d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) add(j->curv, N * 4, 0, true);
+    if (d.abort_mode != GR_ABORT_TIME) add(j->len, N * 4, 0, true);
+    for (uint32_t e = 0; e < d.nprog; ++e) add(j->expr[e], N * 4, 0, true);
+    for (uint32_t k = 0; k < d.nvol; ++k) add(j->vol[k], N * 4, 0, true);
+    if (d.save_interval != 0)
+    {
+        add(j->ring_pos, cap * N * D * 4, 0, false);
+        if (d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) add(j->ring_curv, cap * N * 4, 0, false);
+        if (d.nprog) add(j->ring_expr, cap * N * d.nprog * 4, 0, false);
+        if (d.nvol) add(j->ring_vol, cap * N * d.nvol * 4, 0, false);
+    }
+    add(j->scratch_aos, N * D * 4, -1, false);
+    add(j->scratch2, N * D * 4, -1, false);
+    return a;
+}
+
+extern "C" int gr_job_bytes(const gr_job_desc * desc, uint64_t n, uint32_t save_capacity,
+                            int with_reorder, uint64_t * bytes)
+{
+    if (!desc || !bytes) return set_err(GR_E_INVALID, "gr_job_bytes: null argument");
+    if (desc->dim != 2 && desc->dim != 3) return set_err(GR_E_INVALID, "dim must be 2 or 3");
+    if (desc->nprog > GR_MAX_PROGRAMS || desc->nvol > GR_MAX_VOLUMES)
+        return set_err(GR_E_INVALID, "gr_job_bytes: too many programs / volumes");
+    gr_job j;
+    j.d = *desc;
+    j.n = n;
+    j.cap = save_capacity ? save_capacity : 1;
+    uint64_t total = 0, nstate = 0;
+    for (const JobArray & a : job_arrays(&j))
+    {
+        total += a.bytes;
+        nstate += a.state ? 1 : 0;
+    }
+    // a reorder adds the index map and a shadow copy of every state array and of the map
+    if (with_reorder && n >= 2) total += (nstate + 2) * uint64_t(n) * 4;
+    *bytes = total;
+    return GR_OK;
+}
+
 extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds,
                              uint64_t n, uint32_t save_capacity, gr_job ** out)
 {
@@ -854,31 +920,14 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
               d.nconst > 0;
     const uint32_t D = d.dim;
     int rc = GR_OK;
-    const size_t N = size_t(n ? n : 1);
-    for (uint32_t k = 0; k < D && rc == GR_OK; ++k) rc = zalloc(&j->p[k], N);
     const bool curv_pos = d.curvature == GR_CURV_TOTAL || d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL;
-    for (uint32_t k = 0; k < D && rc == GR_OK && curv_pos; ++k) rc = zalloc(&j->q[k], N);
-    for (uint32_t k = 0; k < D && rc == GR_OK && d.kahan; ++k) rc = zalloc(&j->c[k], N);
-    for (uint32_t k = 0; k < D && rc == GR_OK && d.curvature == GR_CURV_FAST_TOTAL; ++k)
-        rc = zalloc(&j->ld[k], N);
-    if (rc == GR_OK) rc = dalloc(&j->status, N * sizeof(int32_t));
-    if (rc == GR_OK && hipMemset(j->status, 0, N * sizeof(int32_t)) != hipSuccess) rc = GR_E_HIP;
-    if (rc == GR_OK) rc = dalloc(&j->abort_step, N * sizeof(int32_t));
-    if (rc == GR_OK && hipMemset(j->abort_step, 0xff, N * sizeof(int32_t)) != hipSuccess) rc = GR_E_HIP;
-    if (rc == GR_OK && d.curvature != GR_CURV_OFF) rc = zalloc(&j->total, N);
-    if (rc == GR_OK && d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) rc = zalloc(&j->curv, N);
-    if (rc == GR_OK && d.abort_mode != GR_ABORT_TIME) rc = zalloc(&j->len, N);
-    for (uint32_t e = 0; e < d.nprog && rc == GR_OK; ++e) rc = zalloc(&j->expr[e], N);
-    for (uint32_t k = 0; k < d.nvol && rc == GR_OK; ++k) rc = zalloc(&j->vol[k], N);
-    if (rc == GR_OK && d.save_interval != 0)
+    for (const JobArray & a : job_arrays(j))
     {
-        rc = zalloc(&j->ring_pos, size_t(j->cap) * N * D);
-        if (rc == GR_OK && d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) rc = zalloc(&j->ring_curv, size_t(j->cap) * N);
-        if (rc == GR_OK && d.nprog) rc = zalloc(&j->ring_expr, size_t(j->cap) * N * d.nprog);
-        if (rc == GR_OK && d.nvol) rc = zalloc(&j->ring_vol, size_t(j->cap) * N * d.nvol);
+        if (rc != GR_OK) break;
+        rc = dalloc(a.slot, a.bytes);
+        if (rc == GR_OK && a.fill >= 0 && hipMemset(*a.slot, a.fill, a.bytes) != hipSuccess)
+            rc = set_err(GR_E_HIP, "gr_job_create: memset failed");
     }
-    if (rc == GR_OK) rc = dalloc(&j->scratch_aos, N * D * sizeof(float));
-    if (rc == GR_OK) rc = dalloc(&j->scratch2, N * D * sizeof(float));
     for (int k = 0; k < 3; ++k)
         j->grid[k] = d.grid_scale[k] > 0.f ? uint32_t(std::lround(double(d.domain[k]) / d.grid_scale[k])) : 1;
     if (rc == GR_OK && (hipEventCreateWithFlags(&j->ev0, TIMING_EVENT) != hipSuccess ||

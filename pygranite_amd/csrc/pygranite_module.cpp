@@ -262,20 +262,29 @@ public:
     }
 };
 
-// ---- process-wide device context ----------------------------------------------------------
-static gr_ctx * device_ctx()
+// ---- device context ------------------------------------------------------------------------
+// Every TrajectoryIntegrator owns its own context (its own streams and scratch), so integrators
+// used from several Python threads never share a context: the C ABI's one-submitting-thread rule
+// holds per integrator, and their device work may overlap. The reference holds the GIL through
+// compute() (source/pygranite.cpp:126-130); here the GIL is released around device work.
+struct CtxHolder
 {
-    static gr_ctx * ctx = nullptr;
-    if (!ctx)
+    gr_ctx * c{nullptr};
+    CtxHolder()
     {
         int n = 0;
         gr_device_count(&n);
         if (n == 0)
             throw std::runtime_error("ERROR < RUNTIME | pygranite_amd needs a HIP GPU (none visible) >");
-        check(gr_ctx_create(0, &ctx));
+        check(gr_ctx_create(0, &c));
     }
-    return ctx;
-}
+    CtxHolder(const CtxHolder &) = delete;
+    CtxHolder & operator=(const CtxHolder &) = delete;
+    ~CtxHolder()
+    {
+        if (c) gr_ctx_destroy(c);
+    }
+};
 
 struct FieldHolder
 {
@@ -316,6 +325,9 @@ public:
 private:
     IntegratorSettings s_;
     DataLoader & loader_;
+    // declared before the fields so that it is destroyed after them (the context outlives its
+    // fields and jobs)
+    CtxHolder ctx_;
     std::unique_ptr<TrajectorySet> set_;
     std::vector<float> start_;
     uint32_t nx_{0}, ny_{0}, nz_{1};
@@ -386,7 +398,7 @@ private:
             fd.dtype = uint32_t(s_.WindfieldStorage);
             fd.nslots = 3;
             fd.address = d == 3 ? 0 : 1; // 3D border, 2D clamp (cpp:503-506, 570-574)
-            check(gr_field_create(device_ctx(), &fd, &wind_.f));
+            check(gr_field_create(ctx_.c, &fd, &wind_.f));
             if (d == 3 && !s_.ZLevels.empty())
             {
                 if (s_.ZLevels.size() != nz_)
@@ -419,7 +431,7 @@ private:
             fd.dtype = GR_F32;
             fd.nslots = 3;
             fd.address = D() == 3 ? 0 : 1;
-            check(gr_field_create(device_ctx(), &fd, &vols_.f));
+            check(gr_field_create(ctx_.c, &fd, &vols_.f));
         }
         if (!vols_.f) return;
         std::vector<const float *> comps(vol_keys_.size(), nullptr);
@@ -512,7 +524,7 @@ private:
         fd.dtype = GR_F32;
         fd.nslots = 1;
         fd.address = 1; // 2D texture (clamp)
-        check(gr_field_create(device_ctx(), &fd, &topo_.f));
+        check(gr_field_create(ctx_.c, &fd, &topo_.f));
         const float * comps[1] = {t.data()};
         check(gr_field_upload(topo_.f, 0, comps, 0));
     }
@@ -640,7 +652,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     cap = std::min<uint64_t>(cap, 4096);
 
     JobHolder job;
-    check(gr_job_create(device_ctx(), &d, start_.data(), n, uint32_t(cap), &job.j));
+    check(gr_job_create(ctx_.c, &d, start_.data(), n, uint32_t(cap), &job.j));
     if (s_.BlockSize) check(gr_job_set_launch(job.j, s_.BlockSize));
     for (uint32_t slot = 0; slot < 3; ++slot)
     {
@@ -835,7 +847,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     // final outputs (tpp:719-764, 914-915)
     {
         py::gil_scoped_release nogil;
-        check(gr_ctx_synchronize(device_ctx()));
+        check(gr_ctx_synchronize(ctx_.c));
     }
     if (SI == 0)
     {

@@ -77,20 +77,11 @@ class _Active:
         self.bytes = nbytes
 
 
-def job_bytes(n: int, dim: int = 3, curvature: int = 0, nprog: int = 0, reorder: bool = True) -> int:
-    """Device bytes of a job's state (positions, scratch, status, abort step, index map,
-    curvature and expression state, and the shadow copy of every state array a reorder
-    gathers into); no saved columns."""
-    state = dim + 2                     # positions, status, abort step
-    if curvature in (1, 3):
-        state += dim + 2                # previous position, total and individual curvature
-    elif curvature == 2:
-        state += dim + 1
-    state += nprog
-    per = 4 * (state + 2 * dim + 1)     # + seed / output scratch + index map
-    if reorder:
-        per += 4 * state                # shadow copies
-    return int(n) * per + (1 << 16)
+def job_bytes(desc, n: int, reorder: bool = True) -> int:
+    """Device bytes of a job of `desc` with n particles as the library allocates it (state,
+    scratch, one saved column; with `reorder` also the index map and shadow copies a reorder
+    adds): gr_job_bytes, the same allocation list gr_job_create uses."""
+    return capi.job_bytes(desc, n, 1, reorder)
 
 
 class Server:
@@ -120,7 +111,7 @@ class Server:
         self.queue.append(req)
 
     # -- admission ------------------------------------------------------------------------
-    def _make_job(self, req: Request):
+    def _desc(self, req: Request):
         ds = self.datasets[req.dataset]
         d = capi.JobDesc()
         d.dim = 3
@@ -137,6 +128,9 @@ class Server:
         for i, e in enumerate(req.exprs):
             d.prog[i] = capi.compile_expr(e, dim=3, curvature=req.curvature,
                                           allow_abs=req.allow_abs)
+        return d
+
+    def _make_job(self, req: Request, d):
         seeds = req.seeds if req.seeds is not None else req.seed_fn()
         job = capi.Job(self.ctx, d, seeds)
         if self.reorder and req.count >= 4096:
@@ -146,12 +140,12 @@ class Server:
     def admit(self):
         while self.queue and self.queue[0].arrival <= self.now():
             req = self.queue[0]
-            nb = job_bytes(req.count, 3, req.curvature, len(req.exprs),
-                           self.reorder and req.count >= 4096)
+            d = self._desc(req)
+            nb = job_bytes(d, req.count, self.reorder and req.count >= 4096)
             if self.in_use + nb > self.hbm_budget and self.active:
                 break  # wait for memory (FIFO: no overtaking)
             self.queue.popleft()
-            job = self._make_job(req)
+            job = self._make_job(req, d)
             req.admitted = self.now()
             self.active.append(_Active(req, job, nb))
             self.in_use += nb

@@ -139,3 +139,24 @@ def test_device_form_equals_postfix_bit_for_bit():
             assert (a == np.float32(f32)) or (np.isnan(a) and np.isnan(f32)), src
         checked += 1
     assert checked > 300
+
+
+def test_job_bytes_hand_counted():
+    """gr_job_bytes without a GPU, against byte counts derived by hand from the job layout:
+    SoA positions, status and abort step, curvature / expression state, the saved-column ring
+    (capacity x particles), two AoS scratch arrays, and a reorder's index map plus one shadow
+    copy per state array and of the map."""
+    d = capi.JobDesc()
+    d.dim, d.integrator, d.windfield_mode, d.dt, d.dtd = 3, 1, 1, 0.01, 1.0
+    n = 1000
+    # positions 3x4, status 4, abort step 4 -> 20 B; scratch 2x12 -> 24 B
+    assert capi.job_bytes(d, n, 1, reorder=False) == 44 * n
+    # 5 state arrays: index map + 6 shadow copies
+    assert capi.job_bytes(d, n, 1, reorder=True) == 44 * n + (5 + 2) * 4 * n
+    d.curvature, d.save_interval, d.nprog = 3, 1, 1
+    d.prog[0] = capi.compile_expr("sqrt(u*u+v*v+w*w)", dim=3, curvature=3)
+    # state: p, q (2x12), status, abort, total, curv, expr (5x4) = 44; ring of 10 columns:
+    # positions 12, curvature 4, expression 4 -> 200; scratch 24
+    assert capi.job_bytes(d, n, 10, reorder=False) == (44 + 200 + 24) * n
+    d.dim = 2
+    assert capi.job_bytes(d, n, 10, reorder=False) == ((8 + 8 + 20) + 10 * (8 + 4 + 4) + 16) * n

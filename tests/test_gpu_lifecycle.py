@@ -25,3 +25,40 @@ def test_context_outlives_children(gpu_available):
     # their own close / __del__ are now no-ops
     j.close()
     f.close()
+
+
+def test_integrators_on_two_threads(pg, gpu_available):
+    """Two Python threads run TrajectoryIntegrator.compute() at the same time (each integrator
+    owns its context; the GIL is released around device work): each result equals the same
+    integration run alone."""
+    import math
+    import threading
+
+    from helpers import run_product
+    from pygranite_amd import synthetic
+
+    wind, gs = synthetic.abc_flow(24, 4)
+    st = dict(Space="Space3D", Integrator="ClassicRungeKutta", DeltaT=0.01, DataTimeDistance=1.0,
+              GridScale=gs, Offset=[0.0, 0.0, 0.0], MaximumSimulationTime=2.5, SaveInterval=5,
+              CurvatureMode="IndividualAndTotalCurvature", AdditionalCompute=["sqrt(u*u+v*v+w*w)"])
+    seeds = [synthetic.uniform_seeds(60 + k, 6000, [0.0] * 3, [2 * math.pi] * 3) for k in range(2)]
+    alone = [run_product(pg, st, wind, s) for s in seeds]
+    got, errs = [None, None], []
+
+    def work(k):
+        try:
+            for _ in range(3):
+                got[k] = run_product(pg, st, wind, seeds[k])
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs, errs
+    for k in range(2):
+        for f in ("trajectories", "status", "abort_step", "curvatures", "total_curvature"):
+            np.testing.assert_array_equal(got[k][f], alone[k][f], err_msg=f)
+        np.testing.assert_array_equal(got[k]["compute"][0], alone[k]["compute"][0])

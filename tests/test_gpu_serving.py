@@ -72,8 +72,8 @@ def test_admission_respects_budget(gpu_available):
     ctx = capi.Context(0)
     ds, _ = make_datasets(ctx)
     reqs = requests()
-    biggest = max(serving.job_bytes(r.seeds.shape[0], 3, r.curvature, len(r.exprs), r.seeds.shape[0] >= 4096)
-                  for r in reqs)
+    probe = serving.Server(ctx, ds)
+    biggest = max(serving.job_bytes(probe._desc(r), r.seeds.shape[0], r.seeds.shape[0] >= 4096) for r in reqs)
     srv = serving.Server(ctx, ds, hbm_budget=biggest + 1, quantum=128)
     peak = [0]
     orig_round = srv.round
@@ -120,4 +120,41 @@ def test_job_whose_particles_all_stopped_retires_early(gpu_available):
     np.testing.assert_array_equal(early.positions, lazy.positions)
     np.testing.assert_array_equal(early.status, lazy.status)
     assert srv.early == 1 and srv.rounds < 1000 // 50
+    ctx.close()
+
+
+def test_job_bytes_matches_device_memory(gpu_available):
+    """gr_job_bytes (the library's count that admission uses) agrees with the device memory a
+    job's creation takes, measured with hipMemGetInfo around gr_job_create, within the
+    allocator's rounding."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+
+    def free_bytes():
+        capi.lib()  # the engine's runtime is loaded
+        assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+        return free.value
+
+    ctx = capi.Context(0)
+    for curv, exprs, cap in ((3, ("sqrt(u*u+v*v+w*w)",), 8), (0, (), 1)):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode, d.dt, d.dtd = 3, 1, curv, 1, 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k], d.domain[k] = 1.0, 64.0
+        d.save_interval = 1
+        d.nprog = len(exprs)
+        for i, e in enumerate(exprs):
+            d.prog[i] = capi.compile_expr(e, dim=3, curvature=curv)
+        n = 1 << 20
+        want = capi.job_bytes(d, n, cap, reorder=False)
+        ctx.synchronize()
+        before = free_bytes()
+        job = capi.Job(ctx, d, np.ones((n, 3), np.float32), save_capacity=cap)
+        ctx.synchronize()
+        used = before - free_bytes()
+        job.close()
+        # per array at most a 2 MiB allocation granule of rounding
+        assert want <= used <= want + 24 * (2 << 20), (want, used)
+        assert capi.job_bytes(d, n, cap, reorder=True) > want
     ctx.close()
