@@ -270,6 +270,18 @@ int gr_job_read(gr_job * job, uint32_t kind, void * dst, uint64_t bytes);
  * save index since job creation; a column stays readable until `save_capacity` newer
  * columns were written. */
 int gr_job_read_saved(gr_job * job, uint32_t kind, uint64_t first, uint64_t count, void * dst);
+/* gr_job_read_saved without waiting: queues the completion of the columns, their return to
+ * the caller's particle order and the device-to-host copy (into pinned staging, on the copy
+ * stream) behind the launches that wrote them, and returns a ticket. dst_cols[c] receives
+ * column first + c (N(*dim) floats) when gr_job_read_wait(job, ticket) returns; the pointers
+ * must stay valid until then. Later launches of the job wait only for copies of ring columns
+ * they would overwrite, so a caller that reads each chunk of saves while the next chunk
+ * computes overlaps the copies with integration (the ring must hold both chunks). Replaces the
+ * synchronous per-save copies of TrajectoryIntegrator.tpp:997-1049 (copyParticleInfo). */
+int gr_job_read_saved_async(gr_job * job, uint32_t kind, uint64_t first, uint64_t count,
+                            void * const * dst_cols, uint64_t * ticket);
+/* Blocks until the read `ticket` is complete and its columns are in the caller's memory. */
+int gr_job_read_wait(gr_job * job, uint64_t ticket);
 /* Number of particles still at status Time (replaces count_if, ToolKernel.tpp:4-25). */
 int gr_job_count_alive(gr_job * job, uint64_t * alive);
 /* Live range without a wait: every particle slot from *range on holds a stopped particle, and

@@ -31,6 +31,7 @@ EXPORTS = [
     "gr_field_set_z_levels",
     "gr_expr_compile", "gr_expr_eval_host", "gr_expr_eval_device_form", "gr_job_create", "gr_job_destroy", "gr_job_bytes",
     "gr_job_set_particle_data", "gr_job_advance", "gr_job_advance_noalloc", "gr_job_read", "gr_job_read_saved",
+    "gr_job_read_saved_async", "gr_job_read_wait",
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
@@ -148,6 +149,8 @@ def lib():
         "gr_job_advance_noalloc": ([vp, P(Window)], i32),
         "gr_job_read": ([vp, u32, vp, u64], i32),
         "gr_job_read_saved": ([vp, u32, u64, u64, vp], i32),
+        "gr_job_read_saved_async": ([vp, u32, u64, u64, P(vp), P(u64)], i32),
+        "gr_job_read_wait": ([vp, u64], i32),
         "gr_job_count_alive": ([vp, P(u64)], i32),
         "gr_job_live_range": ([vp, P(u64)], i32),
         "gr_job_saved_columns": ([vp, P(u64)], i32),
@@ -422,6 +425,21 @@ class Job:
             out = np.empty((count, self.n), np.float32)
         _check(lib().gr_job_read_saved(self.h, kind, first, count, out.ctypes.data))
         return out
+
+    def read_saved_async(self, kind: int, first: int, count: int):
+        """Starts gr_job_read_saved_async into a new array; returns (ticket, array). The array
+        holds the columns once wait(ticket) has returned."""
+        if kind == OUT_POSITIONS:
+            out = np.empty((count, self.n, self.dim), np.float32)
+        else:
+            out = np.empty((count, self.n), np.float32)
+        cols = (ctypes.c_void_p * max(1, count))(*[out[c].ctypes.data for c in range(count)])
+        t = ctypes.c_uint64()
+        _check(lib().gr_job_read_saved_async(self.h, kind, first, count, cols, ctypes.byref(t)))
+        return t.value, out
+
+    def wait(self, ticket: int):
+        _check(lib().gr_job_read_wait(self.h, ticket))
 
     def saved_columns(self) -> int:
         c = ctypes.c_uint64()

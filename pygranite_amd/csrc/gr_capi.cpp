@@ -17,6 +17,7 @@
 #include <functional>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gr_internal.h"
@@ -224,7 +225,26 @@ struct gr_job
     uint32_t * d_alive{nullptr}; // its device address
     uint64_t alive_seq{0};       // number of the reorder whose count is pending (0: none)
     float * scratch2{nullptr};   // n * dim floats
-    // window tables of multi-window launches (kept alive while copies are queued)
+    // Reads of saved columns in flight (gr_job_read_saved_async) and finished ones kept for
+    // their buffers: pinned host memory the copy stream fills, a device staging copy in the
+    // caller's particle order (reordered jobs), and the events that order them.
+    struct Read
+    {
+        uint64_t ticket{0};
+        bool from_ring{false};        // the copy reads ring columns [first, first + count)
+        uint64_t first{0}, count{0};
+        hipEvent_t ready{nullptr};    // the columns are complete (compute stream)
+        hipEvent_t done{nullptr};     // the device-to-host copy has finished (copy stream)
+        void * host{nullptr};
+        size_t host_cap{0};
+        void * dstage{nullptr};
+        size_t dstage_cap{0};
+        size_t col_bytes{0};
+        std::vector<void *> dst;
+    };
+    std::vector<Read> reads;
+    std::vector<Read> read_pool;
+    uint64_t next_ticket{1};
 };
 
 // --------------------------------------------------------------------------------------------
@@ -779,6 +799,14 @@ static void job_free(gr_job * j)
     for (void * a : j->shadow) dfree(a);
     if (j->ev0) (void)hipEventDestroy(j->ev0);
     if (j->ev1) (void)hipEventDestroy(j->ev1);
+    for (auto * v : {&j->reads, &j->read_pool})
+        for (gr_job::Read & r : *v)
+        {
+            if (r.host) (void)hipHostFree(r.host);
+            dfree(r.dstage);
+            if (r.ready) (void)hipEventDestroy(r.ready);
+            if (r.done) (void)hipEventDestroy(r.done);
+        }
     // the slot is reused only after the compute stream has drained (gr_job_destroy)
     if (j->h_alive && j->ctx) j->ctx->alive_free.push_back(j->h_alive);
     delete j;
@@ -982,6 +1010,7 @@ extern "C" int gr_job_destroy(gr_job * j)
     gr_ctx * c = j->ctx;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->compute);
+    if (!j->reads.empty()) (void)hipStreamSynchronize(c->copy); // copies of its columns in flight
     job_free(j);
     c->children--;
     return GR_OK;
@@ -1557,6 +1586,7 @@ static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, hipE
 
 static int stage_reserve(gr_ctx * ctx, int k, size_t bytes);
 static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out);
+static int wait_ring_reads(gr_job * j, uint64_t nsaves, hipStream_t s);
 
 extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
 {
@@ -1569,6 +1599,7 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if (rc != GR_OK) return rc;
     hipStream_t s = j->ctx->compute;
     if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
+    if ((rc = wait_ring_reads(j, nsaves, s)) != GR_OK) return rc;
     // the launch's time table through the pinned staging ring (stream-ordered device copy)
     gr_ctx * ctx = j->ctx;
     const size_t tbytes = size_t(P.nsteps) * 4 * sizeof(float);
@@ -1617,6 +1648,7 @@ extern "C" int gr_job_advance_noalloc(gr_job * j, const gr_window * w)
     gr_ctx * ctx = j->ctx;
     hipStream_t s = ctx->compute;
     if ((rc = alive_resolve(j, false)) != GR_OK) return rc;
+    if ((rc = wait_ring_reads(j, nsaves, s)) != GR_OK) return rc;
     float * tab = static_cast<float *>(gr::noalloc_scratch(ctx->device));
     if (!tab) return set_err(GR_E_STATE, "gr_job_advance_noalloc: no scratch on device %d", ctx->device);
     // the whole table of this launch is written before the step kernel reads it (stream order)
@@ -1852,6 +1884,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
             continue;
         }
         if ((rc = prepare_advance(j, &w, P, mode, saves[i])) != GR_OK) return rc;
+        if ((rc = wait_ring_reads(j, saves[i], s)) != GR_OK) return rc;
         // the remaining windows of the span: slices c, c + 1 for c = c0 + 1 ...
         const uint32_t nwin = uint32_t(win_steps.size());
         P.nwin = nwin;
@@ -2126,14 +2159,16 @@ extern "C" int gr_job_read(gr_job * j, uint32_t kind, void * dst, uint64_t bytes
     return GR_OK;
 }
 
-extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint64_t count, void * dst)
+// The ring and column size of saved output `kind`, after checking that columns [first, first +
+// count) are written and not yet overwritten.
+static int saved_source(gr_job * j, uint32_t kind, uint64_t first, uint64_t count, const float ** base_out,
+                        uint64_t * colsz_out)
 {
-    if (!j || (count && !dst)) return set_err(GR_E_INVALID, "gr_job_read_saved: null argument");
-    if (count == 0) return GR_OK;
     if (first + count > j->saved) return set_err(GR_E_STATE, "columns [%llu, %llu) not written yet",
                                                  (unsigned long long)first, (unsigned long long)(first + count));
     if (j->saved - first > j->cap) return set_err(GR_E_STATE, "column %llu was overwritten", (unsigned long long)first);
-    GR_HIP(hipSetDevice(j->ctx->device));
+    if (j->reordered && first < j->reorder_col0)
+        return set_err(GR_E_STATE, "column %llu was written before the last reorder", (unsigned long long)first);
     const uint64_t n = j->n;
     const float * base;
     uint64_t colsz;
@@ -2160,41 +2195,55 @@ extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint
     else
         return set_err(GR_E_INVALID, "gr_job_read_saved: kind %u has no saved columns", kind);
     if (!base) return set_err(GR_E_INVALID, "gr_job_read_saved: output %u is not saved by this job", kind);
+    *base_out = base;
+    *colsz_out = colsz;
+    return GR_OK;
+}
+
+// The step kernels save only particles alive at the saving step: complete columns [first,
+// first + count) with the final state of the particles stopped before them (compute stream).
+static int complete_frozen(gr_job * j, uint32_t kind, const float * base, uint64_t first, uint64_t count)
+{
+    const float * src[3] = {nullptr, nullptr, nullptr};
+    uint32_t w = 1, local = 0;
+    if (kind == GR_OUT_POSITIONS)
+    {
+        w = j->d.dim;
+        for (uint32_t d = 0; d < w; ++d) src[d] = j->p[d];
+        local = j->d.local_coords ? 1u : 0u;
+    }
+    else if (kind == GR_OUT_CURVATURE)
+        src[0] = j->curv;
+    else if (kind >= GR_OUT_VOLUME0)
+        src[0] = j->vol[kind - GR_OUT_VOLUME0];
+    else
+        src[0] = j->expr[kind - GR_OUT_EXPR0];
+    for (uint64_t c = 0; c < count; c += 65536)
+        if (gr::launch_fill_frozen(const_cast<float *>(base), j->cap, first + c,
+                                   uint32_t(std::min<uint64_t>(65536, count - c)), j->n, w, j->status,
+                                   j->abort_step, src, j->d.save_interval, local, j->d.offset, j->ctx->compute) != 0)
+            return set_err(GR_E_HIP, "fill kernel failed");
+    return GR_OK;
+}
+
+extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint64_t count, void * dst)
+{
+    if (!j || (count && !dst)) return set_err(GR_E_INVALID, "gr_job_read_saved: null argument");
+    if (count == 0) return GR_OK;
+    GR_HIP(hipSetDevice(j->ctx->device));
+    const float * base = nullptr;
+    uint64_t colsz = 0;
+    int rc = saved_source(j, kind, first, count, &base, &colsz);
+    if (rc != GR_OK || (rc = complete_frozen(j, kind, base, first, count)) != GR_OK) return rc;
     hipStream_t s = j->ctx->compute;
     float * out = static_cast<float *>(dst);
-    // the step kernels save only particles alive at the saving step; complete the columns with
-    // the final state of the particles stopped before them
-    {
-        const float * src[3] = {nullptr, nullptr, nullptr};
-        uint32_t w = 1, local = 0;
-        if (kind == GR_OUT_POSITIONS)
-        {
-            w = j->d.dim;
-            for (uint32_t d = 0; d < w; ++d) src[d] = j->p[d];
-            local = j->d.local_coords ? 1u : 0u;
-        }
-        else if (kind == GR_OUT_CURVATURE)
-            src[0] = j->curv;
-        else if (kind >= GR_OUT_VOLUME0)
-            src[0] = j->vol[kind - GR_OUT_VOLUME0];
-        else
-            src[0] = j->expr[kind - GR_OUT_EXPR0];
-        for (uint64_t c = 0; c < count; c += 65536)
-            if (gr::launch_fill_frozen(const_cast<float *>(base), j->cap, first + c,
-                                       uint32_t(std::min<uint64_t>(65536, count - c)), n, w, j->status,
-                                       j->abort_step, src, j->d.save_interval, local, j->d.offset, s) != 0)
-                return set_err(GR_E_HIP, "fill kernel failed");
-    }
     if (j->reordered)
     {
-        if (first < j->reorder_col0)
-            return set_err(GR_E_STATE, "column %llu was written before the last reorder",
-                           (unsigned long long)first);
         const uint32_t w = kind == GR_OUT_POSITIONS ? j->d.dim : 1;
         for (uint64_t c = 0; c < count; ++c)
         {
             const uint64_t slot = (first + c) % j->cap;
-            if (gr::launch_scatter32(base + slot * colsz, j->scratch2, j->orig, n, w, s) != 0)
+            if (gr::launch_scatter32(base + slot * colsz, j->scratch2, j->orig, j->n, w, s) != 0)
                 return set_err(GR_E_HIP, "scatter kernel failed");
             GR_HIP(hipMemcpyAsync(out + c * colsz, j->scratch2, colsz * sizeof(float),
                                   hipMemcpyDeviceToHost, s));
@@ -2212,6 +2261,193 @@ extern "C" int gr_job_read_saved(gr_job * j, uint32_t kind, uint64_t first, uint
         done += run;
     }
     GR_HIP(hipStreamSynchronize(s));
+    return GR_OK;
+}
+
+// A read record with buffers of at least `host` pinned and `dev` device bytes, from the pool
+// when one fits.
+static int read_record(gr_job * j, size_t host, size_t dev, gr_job::Read & out)
+{
+    size_t best = j->read_pool.size();
+    for (size_t i = 0; i < j->read_pool.size(); ++i)
+    {
+        const gr_job::Read & r = j->read_pool[i];
+        if (r.host_cap >= host && r.dstage_cap >= dev &&
+            (best == j->read_pool.size() || r.host_cap < j->read_pool[best].host_cap))
+            best = i;
+    }
+    if (best == j->read_pool.size() && !j->read_pool.empty()) best = 0; // regrow one
+    gr_job::Read r;
+    if (best < j->read_pool.size())
+    {
+        r = std::move(j->read_pool[best]);
+        j->read_pool.erase(j->read_pool.begin() + long(best));
+    }
+    int rc = GR_OK;
+    if (r.host_cap < host)
+    {
+        if (r.host) (void)hipHostFree(r.host);
+        r.host = nullptr;
+        r.host_cap = 0;
+        if (hipHostMalloc(&r.host, host, hipHostMallocDefault) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            r.host = nullptr;
+            rc = set_err(GR_E_NOMEM, "pinned read buffer of %zu bytes failed", host);
+        }
+        else
+            r.host_cap = host;
+    }
+    if (rc == GR_OK && r.dstage_cap < dev)
+    {
+        dfree(r.dstage);
+        r.dstage = nullptr;
+        r.dstage_cap = 0;
+        if ((rc = dalloc(&r.dstage, dev)) == GR_OK) r.dstage_cap = dev;
+    }
+    if (rc == GR_OK && !r.ready && hipEventCreateWithFlags(&r.ready, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess)
+        rc = set_err(GR_E_HIP, "event creation failed");
+    if (rc == GR_OK && !r.done && hipEventCreateWithFlags(&r.done, hipEventDisableTiming) != hipSuccess)
+        rc = set_err(GR_E_HIP, "event creation failed");
+    if (rc != GR_OK)
+    {
+        j->read_pool.push_back(std::move(r));
+        return rc;
+    }
+    out = std::move(r);
+    return GR_OK;
+}
+
+extern "C" int gr_job_read_saved_async(gr_job * j, uint32_t kind, uint64_t first, uint64_t count,
+                                       void * const * dst_cols, uint64_t * ticket)
+{
+    if (!j || !ticket || (count && !dst_cols)) return set_err(GR_E_INVALID, "gr_job_read_saved_async: null argument");
+    *ticket = 0;
+    if (count == 0) return GR_OK;
+    for (uint64_t c = 0; c < count; ++c)
+        if (!dst_cols[c]) return set_err(GR_E_INVALID, "gr_job_read_saved_async: column %llu has no destination",
+                                         (unsigned long long)c);
+    gr_ctx * ctx = j->ctx;
+    GR_HIP(hipSetDevice(ctx->device));
+    const float * base = nullptr;
+    uint64_t colsz = 0;
+    int rc = saved_source(j, kind, first, count, &base, &colsz);
+    if (rc != GR_OK) return rc;
+    const size_t bytes = size_t(count) * colsz * sizeof(float);
+    gr_job::Read r;
+    if ((rc = read_record(j, bytes, j->reordered ? bytes : 0, r)) != GR_OK) return rc;
+    auto fail = [&](int code) {
+        j->read_pool.push_back(std::move(r));
+        return code;
+    };
+    if ((rc = complete_frozen(j, kind, base, first, count)) != GR_OK) return fail(rc);
+    hipStream_t s = ctx->compute;
+    r.from_ring = !j->reordered;
+    r.first = first;
+    r.count = count;
+    r.col_bytes = colsz * sizeof(float);
+    r.dst.assign(dst_cols, dst_cols + count);
+    if (j->reordered)
+    {
+        // back to the caller's particle order, column by column into the staging copy
+        const uint32_t w = kind == GR_OUT_POSITIONS ? j->d.dim : 1;
+        float * st = static_cast<float *>(r.dstage);
+        for (uint64_t c = 0; c < count; ++c)
+            if (gr::launch_scatter32(base + ((first + c) % j->cap) * colsz, st + c * colsz, j->orig, j->n, w, s) != 0)
+                return fail(set_err(GR_E_HIP, "scatter kernel failed"));
+    }
+    if (hipEventRecord(r.ready, s) != hipSuccess || hipStreamWaitEvent(ctx->copy, r.ready, 0) != hipSuccess)
+        return fail(set_err(GR_E_HIP, "read event failed"));
+    char * host = static_cast<char *>(r.host);
+    if (j->reordered)
+    {
+        if (hipMemcpyAsync(host, r.dstage, bytes, hipMemcpyDeviceToHost, ctx->copy) != hipSuccess)
+            return fail(set_err(GR_E_HIP, "device-to-host copy failed"));
+    }
+    else
+    {
+        uint64_t done = 0;
+        while (done < count)
+        {
+            const uint64_t slot = (first + done) % j->cap;
+            const uint64_t run = std::min<uint64_t>(count - done, j->cap - slot);
+            if (hipMemcpyAsync(host + done * r.col_bytes, base + slot * colsz, run * r.col_bytes,
+                               hipMemcpyDeviceToHost, ctx->copy) != hipSuccess)
+                return fail(set_err(GR_E_HIP, "device-to-host copy failed"));
+            done += run;
+        }
+    }
+    if (hipEventRecord(r.done, ctx->copy) != hipSuccess) return fail(set_err(GR_E_HIP, "read event failed"));
+    r.ticket = j->next_ticket++;
+    *ticket = r.ticket;
+    j->reads.push_back(std::move(r));
+    return GR_OK;
+}
+
+// Copies `count` columns of `col_bytes` from contiguous src to the column pointers, on several
+// threads when the amount is large.
+static void copy_columns(const char * src, const std::vector<void *> & dst, size_t col_bytes)
+{
+    const size_t count = dst.size(), total = count * col_bytes;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = total < (64u << 20) ? 1 : std::min<size_t>(std::min<size_t>(16, hw), total / (16u << 20));
+    auto part = [&](size_t t) {
+        // byte range [b, e) of the concatenated columns
+        const size_t b = total * t / nt, e = total * (t + 1) / nt;
+        size_t c = b / col_bytes, off = b % col_bytes, at = b;
+        while (at < e)
+        {
+            const size_t len = std::min(col_bytes - off, e - at);
+            std::memcpy(static_cast<char *>(dst[c]) + off, src + at, len);
+            at += len;
+            ++c;
+            off = 0;
+        }
+    };
+    if (nt <= 1)
+    {
+        part(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(part, t);
+    part(0);
+    for (auto & x : th) x.join();
+}
+
+extern "C" int gr_job_read_wait(gr_job * j, uint64_t ticket)
+{
+    if (!j) return set_err(GR_E_INVALID, "gr_job_read_wait: null job");
+    if (ticket == 0) return GR_OK;
+    size_t i = 0;
+    while (i < j->reads.size() && j->reads[i].ticket != ticket) ++i;
+    if (i == j->reads.size()) return set_err(GR_E_INVALID, "gr_job_read_wait: unknown ticket %llu", (unsigned long long)ticket);
+    GR_HIP(hipSetDevice(j->ctx->device));
+    gr_job::Read r = std::move(j->reads[i]);
+    j->reads.erase(j->reads.begin() + long(i));
+    const hipError_t e = hipEventSynchronize(r.done);
+    if (e == hipSuccess) copy_columns(static_cast<const char *>(r.host), r.dst, r.col_bytes);
+    r.dst.clear();
+    j->read_pool.push_back(std::move(r));
+    if (e != hipSuccess) return set_err(GR_E_HIP, "gr_job_read_wait: %s", hipGetErrorString(e));
+    return GR_OK;
+}
+
+// Before a launch that saves `nsaves` columns from column j->saved on: the compute stream waits
+// for the in-flight copies of ring columns that those saves would overwrite (the same ring slot,
+// a multiple of the capacity later).
+static int wait_ring_reads(gr_job * j, uint64_t nsaves, hipStream_t s)
+{
+    if (!nsaves) return GR_OK;
+    const uint64_t s0 = j->saved, cap = j->cap;
+    for (const gr_job::Read & r : j->reads)
+    {
+        if (!r.from_ring || !r.count) continue;
+        // writes g in [s0, s0 + nsaves), reads c in [first, first + count), c < s0 <= g: g hits
+        // the slot of c when g - c is a positive multiple of cap
+        const uint64_t lo = s0 - (r.first + r.count - 1), hi = s0 + nsaves - 1 - r.first;
+        if (hi / cap >= (lo + cap - 1) / cap && hi >= cap) GR_HIP(hipStreamWaitEvent(s, r.done, 0));
+    }
     return GR_OK;
 }
 

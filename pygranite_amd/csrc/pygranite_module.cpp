@@ -650,6 +650,9 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     if (!dyn && SI) per_window = uint64_t(std::max(0.f, max_steps_f)) / SI + 2;
     uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(per_window, col_bytes ? (1ull << 30) / col_bytes : 1));
     cap = std::min<uint64_t>(cap, 4096);
+    // the ring holds two chunks of saves: one being copied to the host while the next computes
+    cap = std::max<uint64_t>(2, cap + (cap & 1));
+    const uint64_t chunk_cols = cap / 2;
 
     JobHolder job;
     check(gr_job_create(ctx_.c, &d, start_.data(), n, uint32_t(cap), &job.j));
@@ -671,55 +674,51 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     uint64_t cols_read = 0;
     bool abort_due_alive = false;
 
-    // reads saved columns [cols_read, saved) into the set
-    auto read_columns = [&]() {
+    // Saved columns travel to the set asynchronously: the columns of a chunk are queued for
+    // reading (device-to-host on the copy stream, straight into the set's column vectors) as soon
+    // as the chunk is launched, and collected after the next chunk has been launched, so copies
+    // overlap integration. Curvature has no column for the save of step 0 (Time == 0, tpp:1004).
+    std::vector<uint64_t> pending;
+    auto issue_reads = [&]() {
         uint64_t saved = 0;
         check(gr_job_saved_columns(job.j, &saved));
         const uint64_t cnt = saved - cols_read;
         if (!cnt) return;
-        std::vector<float> buf;
-        {
-            buf.resize(cnt * n * dim);
-            py::gil_scoped_release nogil;
-            check(gr_job_read_saved(job.j, GR_OUT_POSITIONS, cols_read, cnt, buf.data()));
-        }
-        for (uint64_t c = 0; c < cnt; ++c)
-            set_->traj.emplace_back(buf.begin() + c * n * dim, buf.begin() + (c + 1) * n * dim);
-        if (d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL)
-        {
-            buf.resize(cnt * n);
+        auto issue = [&](uint32_t kind, std::vector<std::vector<float>> & dst, uint64_t first, size_t width) {
+            if (first >= saved) return;
+            const uint64_t c = saved - first;
+            std::vector<void *> cols(c);
+            for (uint64_t k = 0; k < c; ++k)
             {
-                py::gil_scoped_release nogil;
-                check(gr_job_read_saved(job.j, GR_OUT_CURVATURE, cols_read, cnt, buf.data()));
+                dst.emplace_back(n * width);
+                cols[k] = dst.back().data();
             }
-            for (uint64_t c = 0; c < cnt; ++c)
-            {
-                // the save of step 0 happens at Time == 0: no curvature column (tpp:1004)
-                if (cols_read + c == 0) continue;
-                set_->curv.emplace_back(buf.begin() + c * n, buf.begin() + (c + 1) * n);
-            }
-        }
-        for (uint32_t e = 0; e < d.nprog; ++e)
-        {
-            buf.resize(cnt * n);
-            {
-                py::gil_scoped_release nogil;
-                check(gr_job_read_saved(job.j, GR_OUT_EXPR0 + e, cols_read, cnt, buf.data()));
-            }
-            for (uint64_t c = 0; c < cnt; ++c)
-                set_->compute[e].emplace_back(buf.begin() + c * n, buf.begin() + (c + 1) * n);
-        }
-        for (uint32_t k = 0; k < d.nvol; ++k)
-        {
-            buf.resize(cnt * n);
-            {
-                py::gil_scoped_release nogil;
-                check(gr_job_read_saved(job.j, GR_OUT_VOLUME0 + k, cols_read, cnt, buf.data()));
-            }
-            for (uint64_t c = 0; c < cnt; ++c)
-                set_->volumes[k].emplace_back(buf.begin() + c * n, buf.begin() + (c + 1) * n);
-        }
+            uint64_t t = 0;
+            check(gr_job_read_saved_async(job.j, kind, first, c, cols.data(), &t));
+            pending.push_back(t);
+        };
+        issue(GR_OUT_POSITIONS, set_->traj, cols_read, dim);
+        if (d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) issue(GR_OUT_CURVATURE, set_->curv, std::max<uint64_t>(cols_read, 1), 1);
+        for (uint32_t e = 0; e < d.nprog; ++e) issue(GR_OUT_EXPR0 + e, set_->compute[e], cols_read, 1);
+        for (uint32_t k = 0; k < d.nvol; ++k) issue(GR_OUT_VOLUME0 + k, set_->volumes[k], cols_read, 1);
         cols_read = saved;
+    };
+    auto collect = [&](size_t keep) {
+        // waits for all but the `keep` most recent reads
+        if (pending.size() <= keep) return;
+        const size_t m = pending.size() - keep;
+        {
+            py::gil_scoped_release nogil;
+            for (size_t i = 0; i < m; ++i) check(gr_job_read_wait(job.j, pending[i]));
+        }
+        pending.erase(pending.begin(), pending.begin() + long(m));
+    };
+    size_t reads_last = 0; // reads issued for the latest chunk
+    auto read_columns = [&]() {
+        const size_t before = pending.size();
+        issue_reads();
+        reads_last = pending.size() - before;
+        collect(reads_last); // the previous chunk's columns are in the set now
     };
 
     auto alive_ok = [&]() {
@@ -764,7 +763,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
             uint32_t k = 0;
             float t = time32;
             double t64 = time64;
-            uint64_t chunk_limit = s_.MinimumAliveParticles ? 1 : (SI ? cap * SI : (1u << 30));
+            uint64_t chunk_limit = s_.MinimumAliveParticles ? 1 : (SI ? chunk_cols * SI : (1u << 30));
             // chunks of RegroupInterval steps, each followed by a regrouping, while many
             // particles are live
             bool regroup = false;
@@ -845,6 +844,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     }
 
     // final outputs (tpp:719-764, 914-915)
+    collect(0);
     {
         py::gil_scoped_release nogil;
         check(gr_ctx_synchronize(ctx_.c));

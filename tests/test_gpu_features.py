@@ -291,3 +291,66 @@ def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, cu
         assert (outs[0][2] != 0).sum() > 50  # particles stopped along the way (3D border)
     f.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+def test_async_column_reads_overlapping_launches(gpu_available, reorder):
+    """Saved columns read with gr_job_read_saved_async while the next chunk is already queued
+    (a ring of two chunks: each launch overwrites the slots of the chunk before the one being
+    read) equal the columns a twin job reads synchronously after each chunk."""
+    wind, gs = abc(24, 4)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=4)
+    for sl in range(4):
+        f.upload(sl, [wind[sl, c] for c in range(3)])
+    s = seeds3(5000, seed=12)
+    chunk, nchunks = 20, 6
+
+    def make():
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode = 3, 1, 3, 1
+        d.dt, d.dtd = 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k] = gs[k]
+            d.domain[k] = float(np.float32(24) * np.float32(gs[k]))
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v+w*w)", dim=3, curvature=3)
+        j = capi.Job(ctx, d, s, save_capacity=2 * chunk)
+        if reorder:
+            j.reorder()
+        return j
+
+    a, b = make(), make()
+    sync_cols, async_cols, pending = [], [], []
+    counter, t = 0, np.float32(0)
+    for k in range(nchunks):
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+        w.time_back, w.time_front = 0.0, 1.0
+        w.time_start = w.time_start64 = float(t)
+        w.counter_start, w.nsteps = counter, chunk
+        a.advance(w)
+        b.advance(w)
+        for _ in range(chunk):
+            t = np.float32(t + np.float32(0.01))
+        first = counter
+        counter += chunk
+        sync_cols.append([b.read_saved(kind, first, chunk) for kind in (capi.OUT_POSITIONS, capi.OUT_CURVATURE, capi.OUT_EXPR0)])
+        pending.append([a.read_saved_async(kind, first, chunk) for kind in (capi.OUT_POSITIONS, capi.OUT_CURVATURE, capi.OUT_EXPR0)])
+        if len(pending) == 2:  # collect the previous chunk while this one computes
+            done = pending.pop(0)
+            for tk, _ in done:
+                a.wait(tk)
+            async_cols.append([arr for _, arr in done])
+    for tk, _ in pending[0]:
+        a.wait(tk)
+    async_cols.append([arr for _, arr in pending[0]])
+    assert len(async_cols) == nchunks
+    for x, y in zip(sync_cols, async_cols):
+        for u, v in zip(x, y):
+            np.testing.assert_array_equal(u, v)
+    a.close()
+    b.close()
+    f.close()
+    ctx.close()
