@@ -822,6 +822,108 @@ def serving_workload(capi, ctx, args):
         d.field.close()
 
 
+def e2e_workload(args):
+    """End-to-end through the drop-in API (TrajectoryIntegrator(...).compute()), against the
+    kernel-only device time of the same call (TrajectorySet.deviceTime()):
+      * the 3D ABC configuration with every step saved (positions, curvature, expression) on a
+        stated particle subsample: upload, integration, saved columns streamed to the host while
+        integration continues, and the (N, T, D) export;
+      * the atmosphere configuration (512x512x64 stretched z, 25 hourly slices handed over one
+        per window by a Python DataLoader, FP16 storage, Kahan + FP64 time, expression):
+        per-window uploads overlapping integration, final positions only.
+    Prints one JSON line."""
+    import torch
+    torch.zeros(1, device="cuda")
+    import pygranite_amd as pg
+    from pygranite_amd import synthetic
+    Loader = synthetic.make_loader_class(pg.DataLoader)
+    out = {}
+
+    def integrated(res, steps):
+        st = np.array([int(r.value) for r in res.abortReasons()])
+        ab = res.abortSteps().astype(np.int64)
+        return int(np.where(st == 0, steps, ab + 1).sum())
+
+    wind, gs = synthetic.abc_flow(128, 16)
+    n = args.e2e_particles
+    seeds = np.ascontiguousarray(synthetic.uniform_seeds(1, 1 << 20, [0.0] * 3, [2 * math.pi] * 3)[:n])
+    for storage in ("Float16", "Float32"):
+        st = pg.IntegratorSettings()
+        st.Space, st.Integrator = pg.Space.Space3D, pg.Integrator.ClassicRungeKutta
+        st.DeltaT, st.DataTimeDistance, st.GridScale, st.Offset = 0.005, 1.0, gs, [0.0, 0.0, 0.0]
+        st.MaximumSimulationTime, st.SaveInterval = 10.0, 1
+        st.CurvatureMode = pg.CurvatureMode.IndividualAndTotalCurvature
+        st.AdditionalCompute = ["sqrt(u*u+v*v+w*w)"]
+        st.WindfieldStorage = getattr(pg.StorageType, storage)
+        t0 = time.perf_counter()
+        integ = pg.TrajectoryIntegrator(st, Loader(wind), pg.TrajectorySet(seeds))
+        t1 = time.perf_counter()
+        res = integ.compute()
+        t2 = time.perf_counter()
+        traj = res.trajectories()
+        curv = res.curvatures()
+        ex = res.computeInfo(0)
+        t3 = time.perf_counter()
+        nbytes = traj.nbytes + curv.nbytes + ex.nbytes
+        work = integrated(res, 2000)
+        out[f"abc_{storage.lower()}"] = {
+            "particles": n, "steps": 2000, "saved_columns": int(traj.shape[1]),
+            "output_bytes": int(nbytes), "construct_s": t1 - t0, "compute_s": t2 - t1,
+            "device_s": res.deviceTime(), "export_s": t3 - t2,
+            "integrated_particle_steps": work,
+            "end_to_end_particle_steps_per_s": work / (t3 - t0),
+            "compute_particle_steps_per_s": work / (t2 - t1),
+            "kernel_only_particle_steps_per_s": work / max(1e-9, res.deviceTime()),
+            "output_GB_per_s_end_to_end": nbytes / (t3 - t0) / 1e9}
+        print(f"#   e2e ABC {storage}: construct {t1 - t0:.2f} s, compute {t2 - t1:.2f} s (device "
+              f"{res.deviceTime():.3f} s), export {t3 - t2:.2f} s, {nbytes / 1e9:.1f} GB out",
+              file=sys.stderr, flush=True)
+        del traj, curv, ex, res, integ
+    if not args.e2e_no_atmosphere:
+        nx = ny = args.atmo_nx
+        nz, S = 64, 25
+        t0 = time.perf_counter()
+        wind_t, gs_a, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=1, device=torch.device("cuda"))
+        host = [wind_t[s].cpu().numpy() for s in range(S)]
+        del wind_t
+        torch.cuda.empty_cache()
+        slices = np.stack(host)
+        del host
+        print(f"# e2e atmosphere field on the host in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        dom = [float(np.float32(nx) * np.float32(gs_a[0])), float(np.float32(ny) * np.float32(gs_a[1]))]
+        na = args.atmo_particles
+        xy = synthetic.uniform_seeds(21, na, [0.0, 0.0], dom)
+        z = synthetic.loguniform_heights(22, na, float(zlev[1]), float(zlev[-1]))
+        seeds_a = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
+        st = pg.IntegratorSettings()
+        st.Space, st.Integrator = pg.Space.Space3D, pg.Integrator.ClassicRungeKutta
+        st.DeltaT, st.DataTimeDistance, st.GridScale, st.Offset = 60.0, 3600.0, list(gs_a), [0.0, 0.0, 0.0]
+        st.MaximumSimulationTime, st.SaveInterval = 1440 * 60.0, 0
+        st.KahanSummation, st.TimeFP64, st.AllowAbs = True, True, True
+        st.WindfieldStorage = pg.StorageType.Float16
+        st.ZLevels = [float(v) for v in zlev]
+        st.AdditionalCompute = ["w*1000/(abs(u)+abs(v)+1)"]
+        t0 = time.perf_counter()
+        integ = pg.TrajectoryIntegrator(st, Loader(slices), pg.TrajectorySet(seeds_a))
+        t1 = time.perf_counter()
+        res = integ.compute()
+        t2 = time.perf_counter()
+        pos = res.trajectories()
+        t3 = time.perf_counter()
+        work = integrated(res, 1440)
+        out["atmosphere_fp16"] = {
+            "particles": na, "steps": 1440, "windows": S - 1, "slice_bytes_fp32": int(slices[0].nbytes),
+            "construct_s": t1 - t0, "compute_s": t2 - t1, "device_s": res.deviceTime(), "export_s": t3 - t2,
+            "integrated_particle_steps": work,
+            "end_to_end_particle_steps_per_s": work / (t3 - t0),
+            "compute_particle_steps_per_s": work / (t2 - t1),
+            "kernel_only_particle_steps_per_s": work / max(1e-9, res.deviceTime())}
+        print(f"#   e2e atmosphere: construct {t1 - t0:.2f} s, compute {t2 - t1:.2f} s (device "
+              f"{res.deviceTime():.3f} s), export {t3 - t2:.2f} s", file=sys.stderr, flush=True)
+        del pos
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration():
     """Three copies of a 1 GiB device buffer: the reference for scaling HBM byte counters."""
     import torch
@@ -877,10 +979,14 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
     ap.add_argument("--no-gyre", action="store_true", help="skip the double-gyre GPU / CPU legs")
-    ap.add_argument("--workload", default="abc", choices=["abc", "atmosphere", "copy", "batch", "serving"],
+    ap.add_argument("--workload", default="abc",
+                    choices=["abc", "atmosphere", "copy", "batch", "serving", "e2e"],
                     help="copy: a plain 1 GiB device copy (x3), the calibration run for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--atmo-nx", type=int, default=512)
+    ap.add_argument("--e2e-particles", type=int, default=1 << 18,
+                    help="e2e workload: particles of the ABC run with every step saved")
+    ap.add_argument("--e2e-no-atmosphere", action="store_true")
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
@@ -907,6 +1013,9 @@ def main():
 
     if args.workload == "copy":
         copy_calibration()
+        return
+    if args.workload == "e2e":
+        e2e_workload(args)
         return
     if args.workload in ("atmosphere", "serving"):
         # torch generates fields and seeds on the device and hands their pointers to the engine:

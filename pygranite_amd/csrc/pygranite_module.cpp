@@ -28,6 +28,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "granite_hip.h"
@@ -141,22 +142,81 @@ public:
     map_t additionalVolumes() override { PYBIND11_OVERRIDE(map_t, DataLoader, additionalVolumes); }
 };
 
+// Saved columns are filled by the library's threaded copies straight from pinned memory: their
+// storage is left uninitialised on allocation (default-initialised floats), so the first touch
+// of the pages happens in those parallel copies rather than in a serial zero fill.
+template <class T> struct NoInitAlloc : std::allocator<T>
+{
+    template <class U> struct rebind
+    {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U> void construct(U * p) noexcept { ::new (static_cast<void *>(p)) U; }
+    template <class U, class... A> void construct(U * p, A &&... a)
+    {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+using column_t = std::vector<float, NoInitAlloc<float>>;
+
+// Runs f(begin, end) over [0, n) split across the host's threads (at most 16) when n is large.
+template <class F> static void parallel_for(size_t n, size_t grain, F && f)
+{
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), (n + grain - 1) / std::max<size_t>(1, grain));
+    if (nt <= 1)
+    {
+        f(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt); });
+    f(0, n / nt);
+    for (auto & x : th) x.join();
+}
+
+// out[(i * S + s) * w + d] = cols[s][i * w + d]: columns of n particles (w floats each) to the
+// particle-major layout of the exports, in tiles of particles x columns, on the host's threads.
+static void transpose_columns(const std::vector<column_t> & cols, size_t n, size_t w, float * out)
+{
+    const size_t S = cols.size();
+    parallel_for(n, 4096, [&](size_t b, size_t e) {
+        for (size_t i0 = b; i0 < e; i0 += 64)
+        {
+            const size_t i1 = std::min(e, i0 + 64);
+            for (size_t s0 = 0; s0 < S; s0 += 64)
+            {
+                const size_t s1 = std::min(S, s0 + 64);
+                for (size_t s = s0; s < s1; ++s)
+                {
+                    const float * c = cols[s].data();
+                    for (size_t i = i0; i < i1; ++i)
+                        for (size_t d = 0; d < w; ++d) out[(i * S + s) * w + d] = c[i * w + d];
+                }
+            }
+        }
+    });
+}
+
 // ---- TrajectorySet: source/granite/TrajectorySet.{hpp,cpp} -------------------------------
 class TrajectorySet
 {
 public:
     Space space{Space::Space2D};
     size_t num{0};
-    std::vector<std::vector<float>> traj;   // per column N*D
-    std::vector<std::vector<float>> curv;   // per column N
-    std::vector<std::vector<std::vector<float>>> compute; // [program][column] N
+    std::vector<column_t> traj;   // per column N*D
+    std::vector<column_t> curv;   // per column N
+    std::vector<std::vector<column_t>> compute; // [program][column] N
     std::vector<std::string> vol_keys;
-    std::vector<std::vector<std::vector<float>>> volumes; // [volume][column] N
+    std::vector<std::vector<column_t>> volumes; // [volume][column] N
     std::vector<float> total_curvature;
     std::vector<float> lengths;
     std::vector<int32_t> abort_reasons;
     std::vector<int32_t> abort_steps;
     size_t num_compute{0};
+    double device_seconds{0.0}; // summed step-kernel time of the compute() that produced the set
 
     TrajectorySet() = default;
     TrajectorySet(Space s, size_t n) : space(s), num(n) {}
@@ -192,12 +252,8 @@ public:
         const size_t T = traj.size(), D = dim();
         py::array_t<float> r({py::ssize_t(num), py::ssize_t(T), py::ssize_t(D)});
         float * o = r.mutable_data();
-        for (size_t t = 0; t < T; ++t)
-        {
-            const float * c = traj[t].data();
-            for (size_t i = 0; i < num; ++i)
-                for (size_t d = 0; d < D; ++d) o[(i * T + t) * D + d] = c[i * D + d];
-        }
+        py::gil_scoped_release nogil;
+        transpose_columns(traj, num, D, o);
         return r;
     }
 
@@ -219,13 +275,12 @@ public:
         return r;
     }
 
-    static py::array_t<float> columns(const std::vector<std::vector<float>> & cols, size_t n)
+    static py::array_t<float> columns(const std::vector<column_t> & cols, size_t n)
     {
-        const size_t S = cols.size();
-        py::array_t<float> r({py::ssize_t(n), py::ssize_t(S)});
+        py::array_t<float> r({py::ssize_t(n), py::ssize_t(cols.size())});
         float * o = r.mutable_data();
-        for (size_t s = 0; s < S; ++s)
-            for (size_t i = 0; i < n; ++i) o[i * S + s] = cols[s][i];
+        py::gil_scoped_release nogil;
+        transpose_columns(cols, n, 1, o);
         return r;
     }
 
@@ -355,8 +410,8 @@ private:
     {
         set_ = std::make_unique<TrajectorySet>(set.space, set.num);
         if (set.traj.empty()) user_error("Provided TrajectorySet does not contain any data.");
-        start_ = set.traj.back();
-        set_->traj.push_back(start_);
+        start_.assign(set.traj.back().begin(), set.traj.back().end());
+        set_->traj.emplace_back(start_.begin(), start_.end());
         if (s_.Space != set.space)
             user_error("Dimensions of IntegratorSettings and provided TrajectorySet do not match.");
     }
@@ -684,7 +739,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
         check(gr_job_saved_columns(job.j, &saved));
         const uint64_t cnt = saved - cols_read;
         if (!cnt) return;
-        auto issue = [&](uint32_t kind, std::vector<std::vector<float>> & dst, uint64_t first, size_t width) {
+        auto issue = [&](uint32_t kind, std::vector<column_t> & dst, uint64_t first, size_t width) {
             if (first >= saved) return;
             const uint64_t c = saved - first;
             std::vector<void *> cols(c);
@@ -848,29 +903,32 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
     {
         py::gil_scoped_release nogil;
         check(gr_ctx_synchronize(ctx_.c));
+        double ms = 0.0;
+        check(gr_ctx_take_kernel_ms(ctx_.c, &ms, nullptr));
+        set_->device_seconds = ms / 1e3;
     }
     if (SI == 0)
     {
         std::vector<float> pos(n * dim);
         check(gr_job_read(job.j, GR_OUT_POSITIONS, pos.data(), pos.size() * 4));
-        set_->traj.push_back(pos);
+        set_->traj.emplace_back(pos.begin(), pos.end());
         if (d.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL && time32 > 0.f)
         {
             std::vector<float> c(n);
             check(gr_job_read(job.j, GR_OUT_CURVATURE, c.data(), n * 4));
-            set_->curv.push_back(c);
+            set_->curv.emplace_back(c.begin(), c.end());
         }
         for (uint32_t e = 0; e < d.nprog; ++e)
         {
             std::vector<float> c(n);
             check(gr_job_read(job.j, GR_OUT_EXPR0 + e, c.data(), n * 4));
-            set_->compute[e].push_back(c);
+            set_->compute[e].emplace_back(c.begin(), c.end());
         }
         for (uint32_t k = 0; k < d.nvol; ++k)
         {
             std::vector<float> c(n);
             check(gr_job_read(job.j, GR_OUT_VOLUME0 + k, c.data(), n * 4));
-            set_->volumes[k].push_back(c);
+            set_->volumes[k].emplace_back(c.begin(), c.end());
         }
     }
     set_->abort_reasons.resize(n);
@@ -1021,7 +1079,10 @@ PYBIND11_MODULE(pygranite, m)
         .def("lengthTrajectories", &TrajectorySet::lengthTrajectories)
         .def("abortReasons", &TrajectorySet::abortReasons)
         .def("numberTrajectories", &TrajectorySet::numberTrajectories)
-        .def("abortSteps", &TrajectorySet::abortSteps);
+        .def("abortSteps", &TrajectorySet::abortSteps)
+        .def("deviceTime", [](const TrajectorySet & t) { return t.device_seconds; },
+             "Summed device time in seconds of the step kernels of the compute() that produced "
+             "this set (extension; 0 for a set built from an array).");
 
     m.attr("ABI_VERSION") = gr_abi_version();
 }
