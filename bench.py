@@ -111,12 +111,14 @@ def make_field(capi, ctx, wind, dtype):
 
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
-               regroup_min=0, total=2000, repeats=1, cu_waves=0):
+               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
         if cu_waves:
             j.set_option(capi.OPT_CU_WAVES, cu_waves)
+        if reuse > 0:
+            j.set_option(capi.OPT_REUSE, 1)
         if reorder:
             j.reorder()
         return j
@@ -990,8 +992,12 @@ def main():
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
+    ap.add_argument("--batch-group", type=int, default=0,
+                    help="batched kernel compute units per queue (0: default / autotuned)")
     ap.add_argument("--batch-cu-waves", type=int, default=-1,
                     help="batched kernel wavefronts per CU (0: register-limited; default: autotuned)")
+    ap.add_argument("--reuse", type=int, default=-1,
+                    help="stencil-reuse build for every launch: batched (context) and single-job forms")
     ap.add_argument("--cu-waves", type=int, default=0,
                     help="single-job forms: wavefronts per CU of the step kernel (0: register-limited)")
     ap.add_argument("--batch-jobs", type=int, default=256)
@@ -1034,6 +1040,10 @@ def main():
         ctx.set_option(capi.CTX_BATCH_WAVES, args.batch_waves)
     if args.batch_cu_waves >= 0:
         ctx.set_option(capi.CTX_BATCH_CU_WAVES, args.batch_cu_waves)
+    if args.batch_group:
+        ctx.set_option(capi.CTX_BATCH_GROUP, args.batch_group)
+    if args.reuse >= 0:
+        ctx.set_option(capi.CTX_BATCH_REUSE, args.reuse)
     if args.workload == "atmosphere":
         atmosphere_workload(capi, ctx, args)
         return
@@ -1055,7 +1065,7 @@ def main():
     forms = args.forms.split(",")
     tune = None
     if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0
-                                                                 or args.batch_cu_waves >= 0):
+                                                                 or args.batch_cu_waves >= 0 or args.batch_group):
         # launch configuration of the batched kernel from the autotuner (probe jobs of the same
         # count, size and description over the same FP16 field, one launch, before any timing)
         tune = ctx.autotune(f16, build_desc(capi, gs, n), n=max(1, args.particles // args.jobs),
@@ -1079,6 +1089,7 @@ def main():
                            args.reorder_every, args.regroup_min, **kw)
         elif name == "fp16":
             kw["cu_waves"] = args.cu_waves
+            kw["reuse"] = args.reuse
             r = run_single(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
         elif name == "fp16_batched":

@@ -312,6 +312,11 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
  * IndividualAndTotal run kernels built for that set; 0 runs the generic kernels, which read every
  * flag at run time. Results are identical either way. */
 #define GR_OPT_SPECIALIZE 4u
+/* GR_OPT_REUSE (default 1): time-paired launches run the kernel that keeps each lane's last
+ * stencil and reloads it only when the lane's cell changes (fewer gathers; 24 or 48 more VGPRs).
+ * 0: only launches whose wavefronts all fit on the device at once do. Results do not depend on
+ * it. */
+#define GR_OPT_REUSE 5u
 int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
 
 /* Reorders the resident particle state by the Morton code of each particle's current cell
@@ -375,11 +380,15 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
  * GR_CTX_BATCH_GROUP: compute units that share one work queue (1 .. number of CUs; default:
  *   the CUs of one XCD). Each queue holds one compact region of the particle sets.
  * GR_CTX_BATCH_CU_WAVES: cap on the batched kernel's wavefronts per compute unit (0 .. 32;
- *   default 0 = as many as the registers allow), applied by reserving LDS per workgroup. */
+ *   default 0 = as many as the registers allow), applied by reserving LDS per workgroup.
+ * GR_CTX_BATCH_REUSE: 1 (default) = every batched launch uses the build that keeps each lane's
+ *   last time-paired stencil across RK4 stages (fewer gathers, more registers); 0 = only launches
+ *   whose wavefronts all fit on the device at once. */
 #define GR_CTX_BATCH_BLOCK 1u
 #define GR_CTX_BATCH_WAVES 2u
 #define GR_CTX_BATCH_GROUP 3u
 #define GR_CTX_BATCH_CU_WAVES 4u
+#define GR_CTX_BATCH_REUSE 5u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 

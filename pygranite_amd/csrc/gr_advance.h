@@ -16,10 +16,8 @@ __global__ void __launch_bounds__(256) k_advance(const StepParams P, uint32_t nx
     advance_block<DIM, INTEG, T, MODE, false, SPEC>(P, nxcd);
 }
 
-// The same kernel held to at most 128 VGPRs, i.e. four wavefronts per SIMD instead of three.
-// Each wavefront's step chain runs a little slower, so this pays only when the launch has many
-// more wavefronts than the compute units hold at once. Instantiated for the FP16 time-paired
-// form only: it is the variant that fits this budget without spilling registers.
+// The kernel without stencil reuse held to at most 128 VGPRs, i.e. four wavefronts per SIMD
+// instead of three, for wide FP16 time-paired launches of jobs that turn the reuse off.
 template <int DIM, int INTEG, class T, int MODE, int SPEC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_advance_w4(const StepParams P, uint32_t nxcd)
@@ -27,7 +25,8 @@ k_advance_w4(const StepParams P, uint32_t nxcd)
     advance_block<DIM, INTEG, T, MODE, false, SPEC>(P, nxcd);
 }
 
-// The FP16 time-paired kernel with the stencil reuse (CACHE) for narrow launches.
+// The time-paired kernel with the stencil reuse (CACHE): a lane reloads its stencil only when
+// its cell changes. The default for every time-paired launch (GR_OPT_REUSE).
 template <int DIM, int INTEG, class T, int MODE, int SPEC>
 __global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint32_t nxcd)
 {
@@ -64,11 +63,9 @@ template <int DIM, int INTEG, class T, int MODE, int SPEC> struct KNarrowFn
 template <int DIM, int INTEG, class T>
 AdvanceFn pick_mode(uint32_t mode, int spec, bool wide, bool narrow)
 {
+    if (mode == M_PAIRED && narrow) return SpecPick<KNarrowFn, DIM, INTEG, T, M_PAIRED>::get(spec);
     if constexpr (std::is_same<T, __half>::value)
-    {
         if (mode == M_PAIRED && wide) return SpecPick<KW4Fn, DIM, INTEG, T, M_PAIRED>::get(spec);
-        if (mode == M_PAIRED && narrow) return SpecPick<KNarrowFn, DIM, INTEG, T, M_PAIRED>::get(spec);
-    }
     if (mode == M_FULL) return k_advance<DIM, INTEG, T, M_FULL, -1>;
     if (mode == M_PAIRED) return SpecPick<KPlainFn, DIM, INTEG, T, M_PAIRED>::get(spec);
     return SpecPick<KPlainFn, DIM, INTEG, T, M_SEPARATE>::get(spec);

@@ -123,6 +123,7 @@ struct gr_ctx
     uint32_t batch_waves{0};
     uint32_t batch_group{32}; // compute units per queue of the batched kernel
     uint32_t batch_cu_waves{0}; // cap on the batched kernel's wavefronts per CU (0: none)
+    uint32_t batch_reuse{1};    // stencil-reuse build: 1 always (default), 0 for narrow launches only
     // device time of every step-kernel launch (event pairs, summed by gr_ctx_take_kernel_ms)
     std::vector<hipEvent_t> kev;
     size_t kpending{0};
@@ -188,6 +189,7 @@ struct gr_job
     uint32_t expr_cols{0};
     uint32_t cu_waves{0}; // cap on the step kernel's wavefronts per CU (0: none)
     uint32_t specialize{1}; // use the compile-time feature sets when they apply
+    uint32_t reuse{1};      // stencil-reuse build for every launch (0: narrow launches only)
     float * p[3]{};
     float * q[3]{};
     float * c[3]{};
@@ -432,6 +434,10 @@ extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
             if (value < 0 || value > 32) return set_err(GR_E_INVALID, "batch CU waves must be 0 .. 32");
             c->batch_cu_waves = uint32_t(value);
             return GR_OK;
+        case GR_CTX_BATCH_REUSE:
+            if (value != 0 && value != 1) return set_err(GR_E_INVALID, "batch reuse must be 0 or 1");
+            c->batch_reuse = uint32_t(value);
+            return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -445,6 +451,7 @@ extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
         case GR_CTX_BATCH_WAVES: *value = c->batch_waves; return GR_OK;
         case GR_CTX_BATCH_GROUP: *value = c->batch_group; return GR_OK;
         case GR_CTX_BATCH_CU_WAVES: *value = c->batch_cu_waves; return GR_OK;
+        case GR_CTX_BATCH_REUSE: *value = c->batch_reuse; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -1037,6 +1044,7 @@ extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
             j->cu_waves = uint32_t(value);
             return GR_OK;
         case GR_OPT_SPECIALIZE: j->specialize = value ? 1 : 0; return GR_OK;
+        case GR_OPT_REUSE: j->reuse = value ? 1 : 0; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown option %u", option);
     }
 }
@@ -1618,7 +1626,7 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     // every wavefront fits at two per SIMD
     const uint64_t cus = uint64_t(std::max(1, j->ctx->cus));
     const bool wide = j->run_n >= 2ull * cus * 4 * 3 * 64;
-    const bool narrow = j->run_n <= cus * 4 * 2 * 64;
+    const bool narrow = j->reuse || j->run_n <= cus * 4 * 2 * 64;
     const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
     const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
     const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, j->ctx->lds_per_cu);
@@ -1657,7 +1665,7 @@ extern "C" int gr_job_advance_noalloc(gr_job * j, const gr_window * w)
     P.ttab = tab;
     const uint64_t cus = uint64_t(std::max(1, ctx->cus));
     const bool wide = j->run_n >= 2ull * cus * 4 * 3 * 64;
-    const bool narrow = j->run_n <= cus * 4 * 2 * 64;
+    const bool narrow = j->reuse || j->run_n <= cus * 4 * 2 * 64;
     const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
     const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
     const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, ctx->lds_per_cu);
@@ -2077,7 +2085,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     if (ms) GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
     // narrow: every item's wavefront fits on the device at two per SIMD
-    const bool narrow = uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
+    const bool narrow = ctx->batch_reuse || uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
     if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
                                    block, ctx->batch_waves, narrow, lds, s) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));

@@ -404,66 +404,6 @@ __device__ __forceinline__ float run_program(const Program & pr, const refvec_t 
 // --------------------------------------------------------------------------------------------
 // the step kernel
 // --------------------------------------------------------------------------------------------
-// Time-paired node layout: both slices of a node in one vector, 3D as
-// [u0 v0 w0 u1 v1 w1 0 0], 2D as [u0 v0 u1 v1] (slice 0 = back, 1 = front).
-template <class T, int NC> struct PairLoad;
-
-template <> struct PairLoad<float, 3>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
-    {
-        const float4 a = GR_LOAD(float4, b, 2 * uint64_t(node));
-        const float4 c = GR_LOAD(float4, b, 2 * uint64_t(node) + 1);
-        x[0] = a.x; x[1] = a.y; x[2] = a.z;
-        y[0] = a.w; y[1] = c.x; y[2] = c.y;
-    }
-};
-template <> struct PairLoad<float, 2>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
-    {
-        const float4 a = GR_LOAD(float4, b, node);
-        x[0] = a.x; x[1] = a.y;
-        y[0] = a.z; y[1] = a.w;
-    }
-};
-template <> struct PairLoad<__half, 3>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
-    {
-        const uint4 a = GR_LOAD(uint4, b, node);
-        x[0] = h_lo(a.x); x[1] = h_hi(a.x); x[2] = h_lo(a.y);
-        y[0] = h_hi(a.y); y[1] = h_lo(a.z); y[2] = h_hi(a.z);
-    }
-};
-template <> struct PairLoad<__half, 2>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
-    {
-        const uint2 a = GR_LOAD(uint2, b, node);
-        x[0] = h_lo(a.x); x[1] = h_hi(a.x);
-        y[0] = h_lo(a.y); y[1] = h_hi(a.y);
-    }
-};
-template <> struct PairLoad<bf16_t, 3>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[3], float (&y)[3])
-    {
-        const uint4 a = GR_LOAD(uint4, b, node);
-        x[0] = b_lo(a.x); x[1] = b_hi(a.x); x[2] = b_lo(a.y);
-        y[0] = b_hi(a.y); y[1] = b_lo(a.z); y[2] = b_hi(a.z);
-    }
-};
-template <> struct PairLoad<bf16_t, 2>
-{
-    static __device__ __forceinline__ void get(const void * b, uint32_t node, float (&x)[2], float (&y)[2])
-    {
-        const uint2 a = GR_LOAD(uint2, b, node);
-        x[0] = b_lo(a.x); x[1] = b_hi(a.x);
-        y[0] = b_lo(a.y); y[1] = b_hi(a.y);
-    }
-};
-
 // kernel modes: lean kernels on the time-paired layout or on separate slots, and the full
 // kernel (every reference feature, separate slots)
 enum : int { M_PAIRED = 0, M_SEPARATE = 1, M_FULL = 2 };
@@ -529,24 +469,101 @@ __device__ __forceinline__ void texel_arg(const StepParams & P, const float * x,
     if (DIM == 3 && Feat<SPEC>::zlev(P)) arg[2] = zcoord(P.wind, x[2] - P.off[2]);
 }
 
-// The last FP16 time-paired stencil a lane loaded: its base node and the raw words of its
-// corners (3D: words x, y, z of the eight 16-byte nodes, the fourth being padding; 2D: both words
-// of the four 8-byte nodes). The RK4 stages of a step mostly sample the same cell (a particle
-// crosses about a fifth of a cell per step), so with CACHE a lane whose base node is unchanged
-// reuses the words instead of loading the stencil again: the same data, the same result. Holding
-// the words costs about 50 VGPRs (two wavefronts per SIMD instead of three), so the kernels
-// built with CACHE serve narrow launches, whose wavefronts all fit on the device at once and
-// whose steps are bound by one wavefront's latency.
-template <int DIM>
-struct Stencil16
+// The last time-paired stencil a lane loaded: its base node and the raw 32-bit words of its
+// corners. 16-bit storage (FP16 / BF16): words x, y, z of the 16-byte 3D nodes [u0 v0 | w0 u1 |
+// v1 w1 | pad], both words of the 8-byte 2D nodes [u0 v0 | u1 v1]; FP32: the six (3D) or four
+// (2D) floats of the node's two slices. The RK4 stages of a step mostly sample the same cell (a
+// particle crosses about a fifth of a cell per step), so with CACHE a lane whose base node is
+// unchanged reuses the words instead of loading the stencil again -- the same data, the same
+// result -- and the wavefront's gathers touch only the cache lines of the lanes that moved to
+// another cell. That is what bounds the dense launches (the L1 works through the distinct lines
+// of every gather), so the kernels are built with CACHE unless a caller asks otherwise; holding
+// the words costs 24 (16-bit) or 48 (FP32) VGPRs in 3D.
+template <int DIM, class T>
+struct StencilCache
 {
+    static constexpr int NCOR = DIM == 3 ? 8 : 4;
+    static constexpr int NW = sizeof(T) == 2 ? (DIM == 3 ? 3 : 2) : (DIM == 3 ? 6 : 4);
     uint32_t base;
-    uint32_t w[DIM == 3 ? 8 : 4][DIM == 3 ? 3 : 2];
+    uint32_t w[NCOR][NW];
 };
+
+// the raw words of corner node `node` of a time-paired slot copy
+template <int DIM, class T>
+__device__ __forceinline__ void load_pair_node(const void * pair, uint32_t node,
+                                               uint32_t (&w)[StencilCache<DIM, T>::NW])
+{
+    if constexpr (sizeof(T) == 2)
+    {
+        if constexpr (DIM == 3)
+        {
+            const uint4 h = GR_LOAD(uint4, pair, node);
+            w[0] = h.x;
+            w[1] = h.y;
+            w[2] = h.z;
+        }
+        else
+        {
+            const uint2 h = GR_LOAD(uint2, pair, node);
+            w[0] = h.x;
+            w[1] = h.y;
+        }
+    }
+    else
+    {
+        if constexpr (DIM == 3)
+        {
+            // [u0 v0 w0 u1 | v1 w1 0 0]: the padding half is not read
+            const uint4 a = GR_LOAD(uint4, pair, 2 * uint64_t(node));
+            const uint2 c = GR_LOAD(uint2, pair, 4 * uint64_t(node) + 2);
+            w[0] = a.x;
+            w[1] = a.y;
+            w[2] = a.z;
+            w[3] = a.w;
+            w[4] = c.x;
+            w[5] = c.y;
+        }
+        else
+        {
+            const uint4 a = GR_LOAD(uint4, pair, node);
+            w[0] = a.x;
+            w[1] = a.y;
+            w[2] = a.z;
+            w[3] = a.w;
+        }
+    }
+}
+
+// back (x) and front (y) slice values of a cached BF16 / FP32 corner (FP16 goes through
+// combine_pair_h, which reads the words directly)
+template <int DIM, class T>
+__device__ __forceinline__ void decode_pair_node(const uint32_t (&w)[StencilCache<DIM, T>::NW],
+                                                 float (&x)[DIM], float (&y)[DIM])
+{
+    if constexpr (sizeof(T) == 4)
+    {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+        {
+            x[d] = __uint_as_float(w[d]);
+            y[d] = __uint_as_float(w[DIM + d]);
+        }
+    }
+    else if constexpr (DIM == 3)
+    {
+        x[0] = b_lo(w[0]); x[1] = b_hi(w[0]); x[2] = b_lo(w[1]);
+        y[0] = b_hi(w[1]); y[1] = b_lo(w[2]); y[2] = b_hi(w[2]);
+    }
+    else
+    {
+        x[0] = b_lo(w[0]); x[1] = b_hi(w[0]);
+        y[0] = b_lo(w[1]); y[1] = b_hi(w[1]);
+    }
+}
 
 template <int DIM, class T, int MODE, bool CACHE, int SPEC>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
-                                         float t, float * v, f16_t one, Stencil16<DIM> & C)
+                                         float t, float * v, f16_t one, StencilCache<DIM, T> & C)
 {
     float arg[3] = {0.f, 0.f, 0.f};
     texel_arg<DIM, SPEC>(P, x, arg);
@@ -565,50 +582,34 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         if (DIM == 3) split(arg[2], P.wind.nz, wmode, k, S.wz);
         const uint32_t PX = P.wind.nx + 3, PXY = PX * (P.wind.ny + 3);
         const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
+        if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
+        {
+#pragma unroll
+            for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
+            {
+                const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
+                load_pair_node<DIM, T>(W.pair, node, C.w[c]);
+            }
+            C.base = base;
+        }
         float b[DIM], f[DIM];
         if constexpr (std::is_same<T, __half>::value)
         {
-            if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
-            {
-#pragma unroll
-                for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
-                {
-                    const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
-                    if (DIM == 3)
-                    {
-                        const uint4 h = GR_LOAD(uint4, W.pair, node);
-                        C.w[c][0] = h.x;
-                        C.w[c][1] = h.y;
-                        C.w[c][2] = h.z;
-                    }
-                    else
-                    {
-                        const uint2 h = GR_LOAD(uint2, W.pair, node);
-                        C.w[c][0] = h.x;
-                        C.w[c][1] = h.y;
-                    }
-                }
-                C.base = base;
-            }
             uint4 raw[8];
 #pragma unroll
-            for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+            for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
                 raw[c] = DIM == 3 ? make_uint4(C.w[c][0], C.w[c][1], C.w[c][2], 0u)
                                   : make_uint4(C.w[c][0], C.w[c][1], 0u, 0u);
             combine_pair_h<DIM>(raw, S, one, b, f);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
-            return;
         }
-        float vb[8][DIM], vf[8][DIM];
-#pragma unroll
-        for (int c = 0; c < (DIM == 3 ? 8 : 4); ++c)
+        else
         {
-            const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
-            PairLoad<T, DIM>::get(W.pair, node, vb[c], vf[c]);
+            float vb[8][DIM], vf[8][DIM];
+#pragma unroll
+            for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c) decode_pair_node<DIM, T>(C.w[c], vb[c], vf[c]);
+            combine<DIM, DIM>(vb, S, b);
+            combine<DIM, DIM>(vf, S, f);
         }
-        combine<DIM, DIM>(vb, S, b);
-        combine<DIM, DIM>(vf, S, f);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) v[d] = lerp(b[d], f[d], t);
         return;
@@ -750,7 +751,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     // over while consecutive steps both take the curvature branch
     bool have_sb = false;
     float sb_prev = 0.f;
-    Stencil16<DIM> sc;
+    StencilCache<DIM, T> sc;
     sc.base = 0xffffffffu; // no stencil loaded (also after every window change)
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
