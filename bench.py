@@ -47,16 +47,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+ERR_TARGET = 1e-4      # domain-relative RMS position error bound against the FP64 oracle
 # VALU issue ceiling: 256 CUs x 2.4 GHz x one wave64 (non-packed FP32) instruction per clock per CU
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 EXPRS = ["sqrt(u*u+v*v+w*w)"]
 DT = np.float32(0.005)
 
 
-def build_desc(capi, gs, n, save_interval=1, curvature=3, exprs=EXPRS):
+def build_desc(capi, gs, n, save_interval=1, curvature=3, exprs=EXPRS, kahan=False):
     d = capi.JobDesc()
     d.dim, d.integrator, d.curvature, d.windfield_mode = 3, 1, curvature, 1
     d.dt, d.dtd = float(DT), 1.0
+    d.kahan = 1 if kahan else 0
     for k in range(3):
         d.grid_scale[k] = gs[k]
         d.domain[k] = float(np.float32(n) * np.float32(gs[k]))
@@ -196,11 +198,12 @@ def median_run(runs):
 
 
 def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0, regroup_min=0,
-                total=2000, repeats=1):
+                total=2000, repeats=1, kahan=False):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
-        out = [capi.Job(ctx, build_desc(capi, gs, n), seeds[idx], save_capacity=chunk) for idx in parts]
+        out = [capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds[idx], save_capacity=chunk)
+               for idx in parts]
         ctx.reorder(out)
         return out
 
@@ -261,12 +264,13 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
     return median_run(runs)
 
 
-def fp64_error(wind, seeds_sub, gs, steps_done, forms, storage_round=None):
+def fp64_error(wind, seeds_sub, gs, steps_done, forms, storage_round=None, kahan_forms=()):
     """Final-position error of GPU forms against the FP64 oracle on the same particles.
 
     forms: {name: result dict with positions/status/abort_step of at least len(seeds_sub)
-    particles}; storage_round: {name: function rounding the field as that form stores it}.
-    Returns {name: error summary} (termination_and_error)."""
+    particles}; storage_round: {name: function rounding the field as that form stores it};
+    kahan_forms: names of the forms with Kahan-compensated positions. Returns {name: error
+    summary} (termination_and_error)."""
     import oracle
     # the oracle steps while counter < MaximumSimulationTime / dt (FP32): half a step below
     # steps_done * dt it runs exactly steps_done steps
@@ -278,56 +282,69 @@ def fp64_error(wind, seeds_sub, gs, steps_done, forms, storage_round=None):
     ref = oracle.run(st, wind, seeds_sub, real64=True, nthreads=threads)
     if ref["steps"] != steps_done:
         return {"error": f"oracle ran {ref['steps']} steps, GPU {steps_done}"}
-
-    def rerun(idx, seeds=None, field=None):
-        r = oracle.run(st, wind if field is None else field, seeds_sub[idx] if seeds is None else seeds,
-                       real64=True, nthreads=threads)
-        return r["status"], r["abort_step"]
-
     out = {}
     for name, res in forms.items():
         rnd = (storage_round or {}).get(name)
+        field = rnd(wind) if rnd else wind
+
+        def paths(idx, field=field, kahan=name in kahan_forms):
+            # every step of the mismatched particles: the FP64 oracle on the exact field, and the
+            # FP32 restatement of the GPU form (bit for bit the GPU path: the field as the form
+            # stores it, the form's summation)
+            s1 = dict(st, SaveInterval=1)
+            r64 = oracle.run(s1, wind, seeds_sub[idx], real64=True, nthreads=threads)
+            r32 = oracle.run(s1, field, seeds_sub[idx], real64=False, kahan=kahan, nthreads=threads)
+            return r64, r32
+
         out[name] = termination_and_error(
             res["positions"][:sub], res["status"][:sub], res["abort_step"][:sub],
             ref["trajectories"][:, -1, :], ref["status"], ref["abort_step"], [2 * math.pi] * 3,
-            seeds_sub, rerun, rnd(wind) if rnd else None, {"steps": int(steps_done)})
+            paths, {"steps": int(steps_done)})
     return out
 
 
-def termination_and_error(pos, status, ab, pos_ref, status_ref, ab_ref, extent, seeds, rerun,
-                          rounded_field=None, extra=None):
+def termination_and_error(pos, status, ab, pos_ref, status_ref, ab_ref, extent, paths, extra=None):
     """Domain-relative max / RMS position error over particles whose status and abort step
-    agree with the FP64 oracle, and a classification of the termination mismatches:
-      sensitive   the FP64 oracle's own termination changes when the seed moves by one FP32
-                  ulp per axis (up or down): the outcome is not determined at the precision the
-                  seeds are given in (chaotic divergence, or an exit or stall at a threshold);
-      storage     (FP16 / BF16 forms) the FP64 oracle on the field rounded as the form stores it
-                  terminates exactly like the GPU form: the difference is the documented
-                  storage rounding, amplified by the flow;
-      other       neither -- a genuine disagreement."""
+    agree with the FP64 oracle, and a classification of the termination mismatches, from both
+    paths of each mismatched particle up to the first of the two stops (step a):
+      divergence  the paths are more than one step length apart at step a: the stops differ
+                  because the trajectories had already separated (chaotic growth of rounding or
+                  storage differences);
+      threshold   within one step length and the same abort reason: which step crosses the
+                  domain edge (or stalls) is decided by less than the paths' difference;
+      other       the reasons differ while the paths are within a step -- a genuine
+                  disagreement."""
     ext = np.asarray(extent, np.float64)
     same = (status_ref == status) & (ab_ref == ab)
     d = np.abs(np.asarray(pos, np.float64)[same] - np.asarray(pos_ref, np.float64)[same]) / ext
     mism = np.nonzero(~same)[0]
-    sensitive = np.zeros(mism.size, bool)
-    storage = np.zeros(mism.size, bool)
+    cls = {"divergence": 0, "threshold": 0, "other": 0}
+    gap = []
     if mism.size:
-        base = np.asarray(seeds[mism], np.float32)
-        for direction in (np.inf, -np.inf):
-            moved = np.nextafter(base, np.float32(direction)).astype(np.float32)
-            s2, a2 = rerun(mism, seeds=moved)
-            sensitive |= (s2 != status_ref[mism]) | (a2 != ab_ref[mism])
-        if rounded_field is not None:
-            s3, a3 = rerun(mism, field=rounded_field)
-            storage = ~sensitive & (s3 == status[mism]) & (a3 == ab[mism])
+        r64, r32 = paths(mism)
+        T64, T32 = r64["trajectories"], r32["trajectories"]  # (m, steps + 1, 3), column k = after k steps
+        for m in range(mism.size):
+            a = [x for x in (int(r64["abort_step"][m]), int(r32["abort_step"][m])) if x >= 0]
+            k = min(a) if a else T64.shape[1] - 1  # position column of the first stop
+            k = max(1, min(k, T64.shape[1] - 1))
+            sep = np.max(np.abs(T64[m, k] - T32[m, k]))
+            step_len = np.max(np.abs(T64[m, k] - T64[m, k - 1]))
+            gap.append(sep / max(step_len, 1e-30))
+            if sep > step_len:
+                cls["divergence"] += 1
+            elif r64["status"][m] == r32["status"][m] or 0 in (r64["status"][m], r32["status"][m]):
+                cls["threshold"] += 1
+            else:
+                cls["other"] += 1
     out = {"rms": float(np.sqrt(np.mean(d * d))) if d.size else None,
            "max": float(d.max()) if d.size else None,
            "particles": int(len(status)),
            "termination_agree": int(same.sum()),
            "termination_mismatch": int(mism.size),
-           "mismatch_sensitive": int(sensitive.sum()),
-           "mismatch_storage_rounding": int(storage.sum()),
-           "mismatch_other": int((~sensitive & ~storage).sum()),
+           "mismatch_divergence": cls["divergence"],
+           "mismatch_threshold": cls["threshold"],
+           "mismatch_other": cls["other"],
+           "median_separation_in_steps": float(np.median(gap)) if gap else None,
            "alive_at_end": int((status_ref == 0).sum())}
     if extra:
         out.update(extra)
@@ -652,18 +669,22 @@ def atmosphere_workload(capi, ctx, args):
         threads = min(os.cpu_count() or 1, 16)
         ref = oracle.run(st, wind_h, seeds[:sub], real64=True, zlevels=zlev, time64=True, nthreads=threads)
 
-        def rerun(idx, seeds=None, field=None):
-            r = oracle.run(st, wind_h if field is None else field, seeds_sub[idx] if seeds is None else seeds,
-                           real64=True, zlevels=zlev, time64=True, nthreads=threads)
-            return r["status"], r["abort_step"]
-
         seeds_sub = seeds[:sub]
         rounders = {"fp16": synthetic.round_fp16, "bf16": synthetic.round_bf16}
         for name, (pos, status, ab, steps) in finals.items():
             rnd = rounders.get(name)
+
+            def paths(idx, field=rnd(wind_h) if rnd else wind_h):
+                s1 = dict(st, SaveInterval=1)
+                r64 = oracle.run(s1, wind_h, seeds_sub[idx], real64=True, zlevels=zlev, time64=True,
+                                 nthreads=threads)
+                r32 = oracle.run(s1, field, seeds_sub[idx], real64=False, kahan=True, zlevels=zlev,
+                                 time64=True, nthreads=threads)
+                return r64, r32
+
             out[name]["fp64_error"] = termination_and_error(
                 pos[:sub], status[:sub], ab[:sub], ref["trajectories"][:, -1, :], ref["status"],
-                ref["abort_step"], dom, seeds_sub, rerun, rnd(wind_h) if rnd else None)
+                ref["abort_step"], dom, paths)
     out["speedup_fp16_vs_fp32"] = out["fp16"]["particle_steps_per_s"] / out["fp32"]["particle_steps_per_s"]
     print(json.dumps(out), flush=True)
 
@@ -976,7 +997,7 @@ def main():
     ap.add_argument("--regroup-min", type=int, default=65536,
                     help="regroup every K launches only while the live range (all jobs) holds at "
                          "least this many particles; smaller launches are latency-bound")
-    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched")
+    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched,fp32k_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
@@ -1077,7 +1098,7 @@ def main():
               file=sys.stderr, flush=True)
     res = {}
     for name in forms:
-        reps = args.repeats if name == forms[-1] or name == "fp16_batched" else 1
+        reps = args.repeats if name.endswith("_batched") else 1
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {chunk} steps ({total}-step job, "
               f"{reps} timed run(s)), {args.particles} particles", file=sys.stderr, flush=True)
         kw = dict(total=total, repeats=reps)
@@ -1092,9 +1113,10 @@ def main():
             kw["reuse"] = args.reuse
             r = run_single(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
-        elif name == "fp16_batched":
-            r = run_batched(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, args.jobs,
-                            args.reorder_every, args.regroup_min, **kw)
+        elif name in ("fp16_batched", "fp32k_batched"):
+            r = run_batched(capi, ctx, f16 if name == "fp16_batched" else f32, seeds, gs, n, chunk, args.steps,
+                            args.warmup, args.jobs, args.reorder_every, args.regroup_min,
+                            kahan=name == "fp32k_batched", **kw)
         else:
             raise SystemExit(f"unknown form {name}")
         r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
@@ -1106,24 +1128,30 @@ def main():
               f"{r['kernel_ms']:.1f} ms in {r['launches']} launches, {r['steps_done']} steps",
               file=sys.stderr, flush=True)
 
-    head = "fp16_batched" if "fp16_batched" in res else forms[-1]
+    err = None
+    if not args.no_error:
+        print(f"# FP64 oracle error check ({args.error_particles} particles)", file=sys.stderr, flush=True)
+        sub = min(args.error_particles, args.particles)
+        cmp_forms = {k: res[k] for k in ("fp16_batched", "fp32k_batched", "plain_fp32", "fp16", "fp32_local")
+                     if k in res}
+        rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched")}
+        # every form is integrated over the same number of steps
+        err = fp64_error(wind, seeds[:sub], gs, res[forms[0]]["steps_done"], cmp_forms, rounders,
+                         kahan_forms=("fp32k_batched",))
+    # the headline: the fastest batched form whose RMS position error against the FP64 oracle
+    # meets the 1e-4 (domain-relative) bound; the FP16 batched form when no error was measured
+    batched = [k for k in ("fp16_batched", "fp32k_batched") if k in res]
+    ok = [k for k in batched if err and err.get(k, {}).get("rms") is not None and err[k]["rms"] <= ERR_TARGET]
+    head = max(ok, key=lambda k: res[k]["psps"]) if ok else (batched[0] if batched else forms[-1])
     hv = res[head]
     outputs = 3 * 4 + 4 + 4  # saved position, curvature, expression per particle-step
-    B16 = algorithmic_bytes(2, outputs)
+    # + Kahan compensation read and written with the positions
+    B16 = algorithmic_bytes(2 if "fp16" in head else 4, outputs) + (2 * 3 * 4 if "k_" in head else 0)
     # algorithmic bytes of the integrated particle-steps over the kernel's device time, both
     # from this run (HIP events around every step-kernel launch of the timed job)
     ps_per_launch = hv["work"] / max(1, hv["launches"])
     achieved = ps_per_launch * B16 / (hv["avg_launch_ms"] / 1e3) / 1e9
 
-    err = None
-    if not args.no_error:
-        print(f"# FP64 oracle error check ({args.error_particles} particles)", file=sys.stderr, flush=True)
-        sub = min(args.error_particles, args.particles)
-        cmp_forms = {head: hv}
-        if "plain_fp32" in res and head != "plain_fp32":
-            cmp_forms["plain_fp32"] = res["plain_fp32"]
-        rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched")}
-        err = fp64_error(wind, seeds[:sub], gs, hv["steps_done"], cmp_forms, rounders)
     extras = {}
     if not args.no_extras:
         print("# lite batched-jobs run", file=sys.stderr, flush=True)
@@ -1156,7 +1184,8 @@ def main():
             "achieved_is": "algorithmic bytes (stencil gathers at the storage width, state and "
                            "saved outputs) of the launch's integrated particle-steps over its "
                            "average device time; most gathers are served by L2 / MALL",
-            "kernel": "k_batch (RK4 3D FP16 items)" if head == "fp16_batched" else "k_advance",
+            "kernel": ("k_batch (RK4 3D " + ("FP16" if "fp16" in head else "FP32 + Kahan") + " items)")
+                      if head.endswith("_batched") else "k_advance",
             "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
             "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
             "algorithmic_bytes_per_launch": ps_per_launch * B16}
@@ -1193,8 +1222,11 @@ def main():
                                f"0.005, {hv['steps_done']} steps, curvature + sqrt(u*u+v*v+w*w) saved "
                                "every step",
                    "form": head, "storage": "fp16" if head.startswith("fp16") else "fp32",
+                   "kahan": "k_" in head, "error_target_rms": ERR_TARGET,
+                   "meets_error_target": bool(err and err.get(head, {}).get("rms") is not None
+                                              and err[head]["rms"] <= ERR_TARGET),
                    "particles": args.particles, "grid": n, "steps_done": hv["steps_done"],
-                   "steps_per_launch": chunk, "jobs": args.jobs if head == "fp16_batched" else 1,
+                   "steps_per_launch": chunk, "jobs": args.jobs if head.endswith("_batched") else 1,
                    "regroup_every_launches": args.reorder_every, "regroup_min_live": args.regroup_min,
                    "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
                    "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
