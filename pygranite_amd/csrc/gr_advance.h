@@ -36,7 +36,8 @@ __global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint
 typedef void (*AdvanceFn)(const StepParams, uint32_t);
 
 // lean kernels exist generic (spec < 0) and for the plain feature set with curvature Off (0) or
-// IndividualAndTotal (3); the full kernel is generic only
+// IndividualAndTotal (3), each with and without Kahan-compensated positions (+16); the full
+// kernel is generic only
 template <template <int, int, class, int, int> class K, int DIM, int INTEG, class T, int MODE>
 struct SpecPick
 {
@@ -44,6 +45,8 @@ struct SpecPick
     {
         if (spec == 0) return K<DIM, INTEG, T, MODE, 0>::fn();
         if (spec == 3) return K<DIM, INTEG, T, MODE, 3>::fn();
+        if (spec == 16) return K<DIM, INTEG, T, MODE, 16>::fn();
+        if (spec == 19) return K<DIM, INTEG, T, MODE, 19>::fn();
         return K<DIM, INTEG, T, MODE, -1>::fn();
     }
 };

@@ -20,11 +20,13 @@ __device__ __forceinline__ uint32_t run_item(const StepParams & P, uint32_t pid)
     return advance_one<DIM, INTEG, T, M_SEPARATE, false, SPEC>(P, pid);
 }
 
-// the three feature sets of item variant V (V + 12 * specialisation, see batch_variant)
+// the five feature sets of item variant V (V + 12 * specialisation, see batch_variant)
 #define GR_BATCH_CASES(V, DIM, INTEG, T)                                                            \
     case V: w = run_item<DIM, INTEG, T, CACHE, -1>(P, pid); break;                                 \
     case V + 12: w = run_item<DIM, INTEG, T, CACHE, 0>(P, pid); break;                             \
-    case V + 24: w = run_item<DIM, INTEG, T, CACHE, 3>(P, pid); break;
+    case V + 24: w = run_item<DIM, INTEG, T, CACHE, 3>(P, pid); break;                             \
+    case V + 36: w = run_item<DIM, INTEG, T, CACHE, 16>(P, pid); break;                            \
+    case V + 48: w = run_item<DIM, INTEG, T, CACHE, 19>(P, pid); break;
 
 // Identity of the compute unit running the calling wavefront: XCC, shader engine, shader array
 // and CU fields of the HW_ID / XCC_ID hardware registers packed into CU_KEY_BITS bits.

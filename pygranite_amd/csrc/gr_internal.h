@@ -196,17 +196,18 @@ enum : uint32_t { EXPR_REF_SLOTS = 12 };
 // kernel modes (see gr_advance.h)
 enum : uint32_t { MODE_PAIRED = 0, MODE_SEPARATE = 1, MODE_FULL = 2 };
 
-// Compile-time feature set of a lean launch (the step kernels' SPEC parameter): the curvature
-// mode when the launch uses the plain features only -- exact sampler weights, uniform z spacing,
-// no Kahan compensation, no reverse integration, global coordinates -- and a specialised build
-// exists for that mode (Off, IndividualAndTotal); -1 (generic build) otherwise.
+// Compile-time feature set of a lean launch (the step kernels' SPEC parameter) when the launch
+// uses the plain features only -- exact sampler weights, uniform z spacing, no reverse
+// integration, global coordinates -- and a specialised build exists for its curvature mode (Off,
+// IndividualAndTotal): that mode, plus 16 with Kahan-compensated positions (the kernels'
+// SPEC_KAHAN bit); -1 (generic build) otherwise. The specialised sets are 0, 3, 16 and 19.
 inline int lean_spec(const StepParams & P, uint32_t mode)
 {
     if (mode == MODE_FULL) return -1;
-    if (P.flags & (F_KAHAN | F_REVERSE | F_LOCAL)) return -1;
+    if (P.flags & (F_REVERSE | F_LOCAL)) return -1;
     if (P.wind.zlev || P.weights != GR_WEIGHTS_EXACT) return -1;
-    if (P.curvature == GR_CURV_OFF || P.curvature == GR_CURV_INDIVIDUAL_AND_TOTAL) return int(P.curvature);
-    return -1;
+    if (P.curvature != GR_CURV_OFF && P.curvature != GR_CURV_INDIVIDUAL_AND_TOTAL) return -1;
+    return int(P.curvature) + ((P.flags & F_KAHAN) ? 16 : 0);
 }
 
 // one work item of the persistent batched kernel: `count` (<= 64, one wavefront) particles of
@@ -234,7 +235,7 @@ enum : uint32_t
 // item variant: 12 * specialisation (0 generic, 1 plain with curvature off, 2 plain with
 // individual-and-total curvature) + 6 * (dim == 3) + 3 * (RK4) + storage type
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype, int spec);
-enum : uint32_t { BATCH_VARIANTS = 36 };
+enum : uint32_t { BATCH_VARIANTS = 60 };
 // culut[cu_key] = dense index of that CU (0xffffffff: none); a CU's queue is index / group;
 // work: the work counter slots
 // waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD);

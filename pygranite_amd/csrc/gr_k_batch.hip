@@ -22,7 +22,7 @@ GR_BATCH_OCCUPANCY(occupancy_batch_default, 1)
 
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype, int spec)
 {
-    const int s = spec == 0 ? 1 : (spec == 3 ? 2 : 0);
+    const int s = spec == 0 ? 1 : spec == 3 ? 2 : spec == 16 ? 3 : spec == 19 ? 4 : 0;
     return int(12 * s + (dim == 3 ? 6 : 0) + (integrator == GR_RK4 ? 3 : 0) + (dtype > 2 ? 0 : dtype));
 }
 

@@ -441,13 +441,18 @@ __device__ __forceinline__ float zcoord(const FieldView & f, float zr)
 
 // Feature set of a step kernel. SPEC < 0: every flag is read from the launch parameters (the
 // generic kernels). SPEC >= 0: the plain feature set fixed at compile time -- exact sampler
-// weights, uniform z spacing, no Kahan compensation, no reverse integration, global coordinates
-// -- with curvature mode SPEC (lean_spec picks it on the host). Fixed flags remove the uniform
-// branches of the unused features from the step loop (fewer scalar instructions, no register
-// moves at their joins, lower SGPR pressure); results are the same as the generic kernel's.
+// weights, uniform z spacing, no reverse integration, global coordinates -- with curvature mode
+// SPEC & SPEC_CURVATURE and Kahan-compensated positions when SPEC_KAHAN is set (lean_spec picks
+// it on the host). Fixed flags remove the uniform branches of the unused features from the step
+// loop (fewer scalar instructions, no register moves at their joins, lower SGPR pressure);
+// results are the same as the generic kernel's.
+enum : int { SPEC_CURVATURE = 15, SPEC_KAHAN = 16 };
 template <int SPEC> struct Feat
 {
-    static __device__ __forceinline__ bool kahan(const StepParams & P) { return SPEC < 0 && (P.flags & F_KAHAN); }
+    static __device__ __forceinline__ bool kahan(const StepParams & P)
+    {
+        return SPEC < 0 ? (P.flags & F_KAHAN) != 0 : (SPEC & SPEC_KAHAN) != 0;
+    }
     static __device__ __forceinline__ bool reverse(const StepParams & P) { return SPEC < 0 && (P.flags & F_REVERSE); }
     static __device__ __forceinline__ bool local(const StepParams & P) { return SPEC < 0 && (P.flags & F_LOCAL); }
     static __device__ __forceinline__ bool zlev(const StepParams & P) { return SPEC < 0 && P.wind.zlev; }
@@ -457,7 +462,7 @@ template <int SPEC> struct Feat
     }
     static __device__ __forceinline__ uint32_t curvature(const StepParams & P)
     {
-        return SPEC < 0 ? P.curvature : uint32_t(SPEC);
+        return SPEC < 0 ? P.curvature : uint32_t(SPEC & SPEC_CURVATURE);
     }
 };
 

@@ -238,12 +238,14 @@ def test_saved_columns_of_stopped_particles(gpu_available):
     ctx.close()
 
 
-@pytest.mark.parametrize("dim,dtype,paired,curv", [(3, capi.F16, 1, 3), (3, capi.F32, 0, 3), (3, capi.BF16, 1, 0),
-                                                   (2, capi.F32, 1, 0), (2, capi.F16, 0, 3)])
-def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, curv):
-    """The step kernels built for the plain feature set (curvature Off / IndividualAndTotal)
-    give every output bit for bit equal to the generic kernels (GR_OPT_SPECIALIZE = 0), over
-    a launch that crosses window edges and saves every step."""
+@pytest.mark.parametrize("dim,dtype,paired,curv,kahan", [
+    (3, capi.F16, 1, 3, 0), (3, capi.F32, 0, 3, 0), (3, capi.BF16, 1, 0, 0), (2, capi.F32, 1, 0, 0),
+    (2, capi.F16, 0, 3, 0), (3, capi.F32, 1, 3, 1), (3, capi.F16, 0, 0, 1), (2, capi.BF16, 1, 3, 1)])
+def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, curv, kahan):
+    """The step kernels built for the plain feature set (curvature Off / IndividualAndTotal,
+    with or without Kahan-compensated positions) give every output bit for bit equal to the
+    generic kernels (GR_OPT_SPECIALIZE = 0), over a launch that crosses window edges and saves
+    every step."""
     ctx = capi.Context(0)
     if dim == 3:
         src, gs = abc(24, 4)
@@ -260,6 +262,7 @@ def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, cu
         d = capi.JobDesc()
         d.dim, d.integrator, d.curvature, d.windfield_mode = dim, 1, curv, 1
         d.dt, d.dtd = 0.01, 1.0
+        d.kahan = kahan
         for k in range(3):
             d.grid_scale[k] = gs[k % len(gs)]
             d.domain[k] = float(np.float32(src.shape[-1 - k] if k < dim else 1) * np.float32(gs[k % len(gs)]))
