@@ -1084,18 +1084,25 @@ def main():
     f16 = make_field(capi, ctx, wind, capi.F16)
 
     forms = args.forms.split(",")
-    tune = None
-    if "fp16_batched" in forms and not args.no_autotune and not (args.batch_block or args.batch_waves >= 0
-                                                                 or args.batch_cu_waves >= 0 or args.batch_group):
-        # launch configuration of the batched kernel from the autotuner (probe jobs of the same
-        # count, size and description over the same FP16 field, one launch, before any timing)
-        tune = ctx.autotune(f16, build_desc(capi, gs, n), n=max(1, args.particles // args.jobs),
-                            njobs=args.jobs, steps=min(chunk, 100), trials=2)
-        print("# autotune: " + ", ".join(f"b{t['block']}/g{t['group']}/cw{t['cu_waves']} {t['ms']:.3f} ms"
-                                         for t in tune)
-              + f" -> block {ctx.get_option(capi.CTX_BATCH_BLOCK)}, group "
-              f"{ctx.get_option(capi.CTX_BATCH_GROUP)}, CU waves {ctx.get_option(capi.CTX_BATCH_CU_WAVES)}",
+    tune = {}
+    fixed = bool(args.no_autotune or args.batch_block or args.batch_waves >= 0 or args.batch_cu_waves >= 0
+                 or args.batch_group)
+
+    def autotune(name):
+        """Launch configuration of the batched kernel for one form, from the autotuner: probe jobs
+        of the same count, size and description over the form's own field, one launch, before
+        any timing of the form."""
+        kahan = name == "fp32k_batched"
+        t = ctx.autotune(f32 if kahan else f16, build_desc(capi, gs, n, kahan=kahan),
+                         n=max(1, args.particles // args.jobs), njobs=args.jobs, steps=min(chunk, 100), trials=2)
+        chosen = {"block": ctx.get_option(capi.CTX_BATCH_BLOCK), "group": ctx.get_option(capi.CTX_BATCH_GROUP),
+                  "cu_waves": ctx.get_option(capi.CTX_BATCH_CU_WAVES), "waves": ctx.get_option(capi.CTX_BATCH_WAVES)}
+        print(f"# autotune {name}: " + ", ".join(f"b{c['block']}/g{c['group']}/cw{c['cu_waves']} {c['ms']:.3f} ms"
+                                                for c in t)
+              + f" -> block {chosen['block']}, group {chosen['group']}, CU waves {chosen['cu_waves']}",
               file=sys.stderr, flush=True)
+        tune[name] = {"candidates": t, "chosen": chosen}
+
     res = {}
     for name in forms:
         reps = args.repeats if name.endswith("_batched") else 1
@@ -1114,11 +1121,16 @@ def main():
             r = run_single(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
         elif name in ("fp16_batched", "fp32k_batched"):
+            if not fixed:
+                autotune(name)
             r = run_batched(capi, ctx, f16 if name == "fp16_batched" else f32, seeds, gs, n, chunk, args.steps,
                             args.warmup, args.jobs, args.reorder_every, args.regroup_min,
                             kahan=name == "fp32k_batched", **kw)
         else:
             raise SystemExit(f"unknown form {name}")
+        r["options"] = {"block": ctx.get_option(capi.CTX_BATCH_BLOCK), "group": ctx.get_option(capi.CTX_BATCH_GROUP),
+                        "cu_waves": ctx.get_option(capi.CTX_BATCH_CU_WAVES),
+                        "waves": ctx.get_option(capi.CTX_BATCH_WAVES)}
         r["nominal_psps"] = args.particles * r["steps_done"] / r["wall_s"]
         r["psps"] = r["work"] / r["wall_s"]  # integrated particle-steps per second
         r["avg_launch_ms"] = r["kernel_ms"] / max(1, r["launches"])
@@ -1228,10 +1240,8 @@ def main():
                    "particles": args.particles, "grid": n, "steps_done": hv["steps_done"],
                    "steps_per_launch": chunk, "jobs": args.jobs if head.endswith("_batched") else 1,
                    "regroup_every_launches": args.reorder_every, "regroup_min_live": args.regroup_min,
-                   "batch_block": ctx.get_option(capi.CTX_BATCH_BLOCK),
-                   "batch_waves": ctx.get_option(capi.CTX_BATCH_WAVES),
-                   "batch_group": ctx.get_option(capi.CTX_BATCH_GROUP),
-                   "batch_cu_waves": ctx.get_option(capi.CTX_BATCH_CU_WAVES),
+                   "batch_block": hv["options"]["block"], "batch_waves": hv["options"]["waves"],
+                   "batch_group": hv["options"]["group"], "batch_cu_waves": hv["options"]["cu_waves"],
                    "single_job_cu_waves": args.cu_waves},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
         "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},

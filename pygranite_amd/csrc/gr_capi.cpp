@@ -92,12 +92,14 @@ struct gr_ctx
     size_t items_cap{0};
     struct Stage
     {
-        void * h{nullptr}; // pinned
+        void * h{nullptr};  // pinned, mapped into the device's address space
+        void * dh{nullptr}; // the device's address of h
         size_t cap{0};
         hipEvent_t done{nullptr};
         bool pending{false};
     };
-    Stage stage[2];
+    static constexpr int NSTAGE = 4;
+    Stage stage[NSTAGE];
     int stage_i{0};
     char * d_stage{nullptr}; // device copy of the staging block (queues, params, window tables)
     size_t d_stage_cap{0};
@@ -325,7 +327,7 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
         hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&c->dcount), sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&c->dwork), size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess ||
-        hipMemset(c->dwork, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess)
+        hipMemsetAsync(c->dwork, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8, c->compute) != hipSuccess)
     {
         (void)hipGetLastError();
         if (c->compute) (void)hipStreamDestroy(c->compute);
@@ -582,7 +584,8 @@ extern "C" int gr_field_create(gr_ctx * ctx, const gr_field_desc * desc, gr_fiel
     for (uint32_t s = 0; s < d.nslots && rc == GR_OK; ++s)
     {
         rc = dalloc(&f->slots[s], f->slot_bytes);
-        if (rc == GR_OK && hipMemset(f->slots[s], 0, f->slot_bytes) != hipSuccess)
+        // on the copy stream, ahead of the slot's uploads (which run there)
+        if (rc == GR_OK && hipMemsetAsync(f->slots[s], 0, f->slot_bytes, ctx->copy) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_field_create: memset failed");
         if (rc == GR_OK && hipEventCreateWithFlags(&f->ready[s], hipEventDisableTiming) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_field_create: event creation failed");
@@ -960,7 +963,9 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     {
         if (rc != GR_OK) break;
         rc = dalloc(a.slot, a.bytes);
-        if (rc == GR_OK && a.fill >= 0 && hipMemset(*a.slot, a.fill, a.bytes) != hipSuccess)
+        // filled on the compute stream, ahead of the seed kernel and every launch of the job
+        // (the compute stream does not wait for work on the null stream)
+        if (rc == GR_OK && a.fill >= 0 && hipMemsetAsync(*a.slot, a.fill, a.bytes, ctx->compute) != hipSuccess)
             rc = set_err(GR_E_HIP, "gr_job_create: memset failed");
     }
     for (int k = 0; k < 3; ++k)
@@ -1085,6 +1090,14 @@ template <class T> static int grow(T *& buf, size_t & cap, size_t need)
     if (rc == GR_OK) cap = need;
     return rc;
 }
+
+// pinned staging ring of launch and reorder tables (defined with the batched launches)
+static int stage_reserve(gr_ctx * ctx, int k, size_t bytes);
+static int stage_commit(gr_ctx * ctx, int k, void * dst, size_t bytes, hipStream_t s);
+static int stage_upload(gr_ctx * ctx, void * dst, const void * src, size_t bytes, hipStream_t s);
+static_assert(sizeof(gr::SortJob) % 16 == 0 && sizeof(gr::GatherEntry) % 16 == 0 &&
+                  sizeof(gr::BatchItem) % 16 == 0,
+              "tables staged by launch_stage_copy are whole 16-byte words");
 
 // A count slot for job j (host-coherent pinned memory, device-visible): taken from the free list,
 // which grows by 4 KiB pages.
@@ -1280,7 +1293,7 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
         ctx->stmp_bytes = need;
     }
     if ((rc = grow(ctx->d_sjobs, ctx->sjobs_cap, tab.size())) != GR_OK) return rc;
-    GR_HIP(hipMemcpyAsync(ctx->d_sjobs, tab.data(), tab.size() * sizeof(gr::SortJob), hipMemcpyHostToDevice, s));
+    if ((rc = stage_upload(ctx, ctx->d_sjobs, tab.data(), tab.size() * sizeof(gr::SortJob), s)) != GR_OK) return rc;
     size_t tb = ctx->stmp_bytes;
     if (gr::launch_bucket_sort(ctx->d_sjobs, uint32_t(tab.size()), total, shift, nb, keys, ctx->svals[0],
                                ctx->bcount, ctx->bstart, ctx->stmp, tb, ctx->svals[1], s) != 0)
@@ -1313,7 +1326,7 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     if (g.size() + back.size() > 65535) return set_err(GR_E_INVALID, "gr_jobs_reorder: too many arrays in one call");
     if ((rc = grow(ctx->d_gtab, ctx->gtab_cap, g.size() + back.size())) != GR_OK) return rc;
     g.insert(g.end(), back.begin(), back.end());
-    GR_HIP(hipMemcpyAsync(ctx->d_gtab, g.data(), g.size() * sizeof(gr::GatherEntry), hipMemcpyHostToDevice, s));
+    if ((rc = stage_upload(ctx, ctx->d_gtab, g.data(), g.size() * sizeof(gr::GatherEntry), s)) != GR_OK) return rc;
     const uint32_t ng = uint32_t(g.size() - back.size());
     if (gr::launch_gather_multi(ctx->d_gtab, ng, maxn, s) != 0 ||
         (!back.empty() && gr::launch_gather_multi(ctx->d_gtab + ng, uint32_t(back.size()), maxb, s) != 0))
@@ -1592,7 +1605,6 @@ static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, hipE
     return GR_OK;
 }
 
-static int stage_reserve(gr_ctx * ctx, int k, size_t bytes);
 static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out);
 static int wait_ring_reads(gr_job * j, uint64_t nsaves, hipStream_t s);
 
@@ -1615,10 +1627,7 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     if ((rc = stage_reserve(ctx, k, tbytes)) != GR_OK) return rc;
     if ((rc = grow(ctx->d_stage, ctx->d_stage_cap, tbytes)) != GR_OK) return rc;
     time_table(P, nullptr, static_cast<float *>(ctx->stage[k].h));
-    GR_HIP(hipMemcpyAsync(ctx->d_stage, ctx->stage[k].h, tbytes, hipMemcpyHostToDevice, s));
-    GR_HIP(hipEventRecord(ctx->stage[k].done, s));
-    ctx->stage[k].pending = true;
-    ctx->stage_i = 1 - k;
+    if ((rc = stage_commit(ctx, k, ctx->d_stage, tbytes, s)) != GR_OK) return rc;
     P.ttab = reinterpret_cast<const float *>(ctx->d_stage);
     GR_HIP(hipEventRecord(j->ev0, s));
     if ((rc = kt_mark(j->ctx, s, false)) != GR_OK) return rc;
@@ -1739,8 +1748,10 @@ static void plan_span(const gr_job_desc & d, const gr_span & sp, uint32_t nslice
     }
 }
 
-// pinned host staging of a batch launch (double-buffered so the host can fill the next launch's
-// tables while the previous copy is still queued)
+// Pinned host staging of launch tables: a ring of NSTAGE blocks, so the host fills the next
+// launch's tables while earlier ones are still queued. A block is handed to the device by
+// launch_stage_copy on the compute stream (a kernel, ordered with the kernels around it), and
+// reused only once that copy has completed.
 static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
 {
     gr_ctx::Stage & st = ctx->stage[k];
@@ -1751,18 +1762,50 @@ static int stage_reserve(gr_ctx * ctx, int k, size_t bytes)
     }
     if (bytes <= st.cap) return GR_OK;
     if (st.h) (void)hipHostFree(st.h);
-    st.h = nullptr;
+    st.h = st.dh = nullptr;
     st.cap = 0;
     const size_t want = std::max<size_t>(bytes, 64 << 10);
-    if (hipHostMalloc(&st.h, want, hipHostMallocDefault) != hipSuccess)
+    // coherent: the copy kernel reads what the host wrote since the block's last use, never a
+    // line the device cached then
+    if (hipHostMalloc(&st.h, want, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     {
         (void)hipGetLastError();
         return set_err(GR_E_NOMEM, "pinned staging of %zu bytes failed", want);
+    }
+    if (hipHostGetDevicePointer(&st.dh, st.h, 0) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        (void)hipHostFree(st.h);
+        st.h = nullptr;
+        return set_err(GR_E_HIP, "pinned staging has no device address");
     }
     st.cap = want;
     // the host only needs to know that the copy has read the staging block: no fence
     if (!st.done) GR_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming | hipEventDisableSystemFence));
     return GR_OK;
+}
+
+// Hands staging block k (its first `bytes` bytes, a multiple of 16) to device memory `dst` on
+// stream s and advances the ring.
+static int stage_commit(gr_ctx * ctx, int k, void * dst, size_t bytes, hipStream_t s)
+{
+    if (gr::launch_stage_copy(ctx->stage[k].dh, dst, bytes, s) != 0)
+        return set_err(GR_E_HIP, "staging copy of %zu bytes failed", bytes);
+    GR_HIP(hipEventRecord(ctx->stage[k].done, s));
+    ctx->stage[k].pending = true;
+    ctx->stage_i = (k + 1) % gr_ctx::NSTAGE;
+    return GR_OK;
+}
+
+// Copies a small host table (bytes a multiple of 16) to device memory `dst` through the staging
+// ring: the caller's buffer is free when this returns, and the copy is ordered on stream s.
+static int stage_upload(gr_ctx * ctx, void * dst, const void * src, size_t bytes, hipStream_t s)
+{
+    const int k = ctx->stage_i;
+    int rc = stage_reserve(ctx, k, bytes);
+    if (rc != GR_OK) return rc;
+    std::memcpy(ctx->stage[k].h, src, bytes);
+    return stage_commit(ctx, k, dst, bytes, s);
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -2058,12 +2101,11 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     if (!wptr.empty()) std::memcpy(h + off_w, wptr.data(), wptr.size() * sizeof(void *));
     if (!wedge.empty()) std::memcpy(h + off_e, wedge.data(), wedge.size() * sizeof(uint32_t));
     if (items_dirty && nitems) std::memcpy(h + off_i, ctx->h_items.data(), size_t(nitems) * sizeof(gr::BatchItem));
-    GR_HIP(hipMemcpyAsync(ctx->d_stage, h, dev_bytes, hipMemcpyHostToDevice, s));
-    if (items_dirty && nitems)
-        GR_HIP(hipMemcpyAsync(ctx->d_items, h + off_i, size_t(nitems) * sizeof(gr::BatchItem), hipMemcpyHostToDevice, s));
-    GR_HIP(hipEventRecord(ctx->stage[k].done, s));
-    ctx->stage[k].pending = true;
-    ctx->stage_i = 1 - k;
+    if (items_dirty && nitems &&
+        gr::launch_stage_copy(static_cast<char *>(ctx->stage[k].dh) + off_i, ctx->d_items,
+                              size_t(nitems) * sizeof(gr::BatchItem), s) != 0)
+        return set_err(GR_E_HIP, "staging copy of the batch items failed");
+    if ((rc = stage_commit(ctx, k, ctx->d_stage, dev_bytes, s)) != GR_OK) return rc;
     ctx->items_valid = true;
 
     // interpreter LDS of the launch: the deepest stack of its jobs' programs (batched jobs use the

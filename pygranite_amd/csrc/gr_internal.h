@@ -324,6 +324,9 @@ int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uin
                        uint32_t nb, uint32_t * keys, uint32_t * vals, uint32_t * counts,
                        uint32_t * starts, void * temp, size_t & temp_bytes, uint32_t * out,
                        void * stream);
+// copies `bytes` (a multiple of 16, both pointers 16-byte aligned) from device-visible pinned host
+// memory to device memory with a kernel on `stream`
+int launch_stage_copy(const void * src, void * dst, size_t bytes, void * stream);
 // all entries in one launch (maxn: largest entry n)
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3

@@ -333,6 +333,29 @@ __global__ void k_bucket_scatter(const uint32_t * __restrict__ keys, const uint3
     }
 }
 
+// Launch tables (kernel parameters, queues, time weights, items, sort and gather tables) move from
+// the host's pinned staging block to the device with this kernel rather than with a runtime copy:
+// it runs in the compute stream's own queue, between the kernels that precede and follow it, so
+// the step kernel that reads the tables sees every byte of them.
+__global__ void k_stage_copy(const uint4 * __restrict__ src, uint4 * __restrict__ dst, uint64_t n16)
+{
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+int launch_stage_copy(const void * src, void * dst, size_t bytes, void * stream)
+{
+    if (bytes == 0) return 0;
+    if ((bytes & 15) || (reinterpret_cast<uintptr_t>(src) & 15) || (reinterpret_cast<uintptr_t>(dst) & 15))
+        return -1;
+    const uint64_t n16 = bytes / 16;
+    const uint32_t grid = uint32_t(std::min<uint64_t>((n16 + 255) / 256, 256));
+    hipLaunchKernelGGL(k_stage_copy, dim3(grid), dim3(256), 0, hipStream_t(stream),
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 __global__ void k_fill_u32(uint32_t * __restrict__ p, uint64_t n, uint32_t v)
 {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;

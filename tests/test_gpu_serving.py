@@ -137,6 +137,7 @@ def test_job_bytes_matches_device_memory(gpu_available):
         return free.value
 
     ctx = capi.Context(0)
+    alive = []  # no job is freed before the last measurement: the allocator would reuse its memory
     for curv, exprs, cap in ((3, ("sqrt(u*u+v*v+w*w)",), 8), (0, (), 1)):
         d = capi.JobDesc()
         d.dim, d.integrator, d.curvature, d.windfield_mode, d.dt, d.dtd = 3, 1, curv, 1, 0.01, 1.0
@@ -153,8 +154,10 @@ def test_job_bytes_matches_device_memory(gpu_available):
         job = capi.Job(ctx, d, np.ones((n, 3), np.float32), save_capacity=cap)
         ctx.synchronize()
         used = before - free_bytes()
-        job.close()
+        alive.append(job)
         # per array at most a 2 MiB allocation granule of rounding
         assert want <= used <= want + 24 * (2 << 20), (want, used)
         assert capi.job_bytes(d, n, cap, reorder=True) > want
+    for job in alive:
+        job.close()
     ctx.close()
