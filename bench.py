@@ -1202,17 +1202,28 @@ def main():
             "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
             "algorithmic_bytes_per_launch": ps_per_launch * B16}
     if counters:
-        # measured per-launch counters of the same command (rocprofv3 --pmc, summarised under
-        # profiles/): actual HBM bytes and VALU instructions against the launch time of this run
+        # measured per-launch counters of the same launches (rocprofv3 --pmc restricted to the
+        # step kernel, summarised under profiles/): HBM bytes and VALU instructions, scaled by the
+        # integrated particle-steps per launch when this run's launches did other work
+        scale = ps_per_launch / counters["integrated_particle_steps_per_launch_pmc_run"]
+        traffic = counters["hbm_bytes_per_launch"] * scale
+        roof["traffic"] = traffic
         roof["traffic_source"] = counters.get("source")
-        roof["hbm_actual_GBs"] = counters["hbm_bytes_per_launch"] / (hv["avg_launch_ms"] / 1e3) / 1e9
-        if counters.get("valu_insts_per_launch"):
-            rate = counters["valu_insts_per_launch"] / (hv["avg_launch_ms"] / 1e3)
-            peak = VALU_PEAK_WAVE_INSTS
-            roof["valu"] = {"wave_insts_per_launch": counters["valu_insts_per_launch"],
-                            "achieved_G_per_s": rate / 1e9, "peak_G_per_s": peak / 1e9,
-                            "frac": rate / peak,
-                            "peak_is": "256 CUs x 2.4 GHz x one wave64 VALU instruction per clock"}
+        roof["traffic_scaled_by"] = scale
+        roof["hbm_actual_GBs"] = traffic / (hv["avg_launch_ms"] / 1e3) / 1e9
+        roof["hbm_actual_frac"] = roof["hbm_actual_GBs"] / HBM_PEAK_GBS
+        roof["traffic_over_algorithmic"] = traffic / (ps_per_launch * B16)
+        valu = counters["valu_insts_per_launch"] * scale
+        rate = valu / (hv["avg_launch_ms"] / 1e3)
+        dense = counters["per_launch"][0]
+        roof["valu"] = {"wave_insts_per_launch": valu, "achieved_G_per_s": rate / 1e9,
+                        "peak_G_per_s": VALU_PEAK_WAVE_INSTS / 1e9, "frac": rate / VALU_PEAK_WAVE_INSTS,
+                        "frac_first_launch": dense["valu_frac"],
+                        "wave_insts_per_particle_step": counters["valu_insts_per_particle_step"],
+                        "peak_is": "256 CUs x 2.4 GHz x one wave64 VALU instruction per clock"}
+        roof["binding"] = ("VALU issue in the dense launches (frac_first_launch), one wavefront's "
+                           "dependent chain in the tail launches; HBM carries traffic / avg launch "
+                           "time = hbm_actual_GBs")
     line = {
         "metric": "RK4 3D particle-steps/s vs plain FP32 at <=1e-4 RMS err; p50/p99 job latency",
         "value": hv["psps"],
