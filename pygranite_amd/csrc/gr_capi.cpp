@@ -1397,6 +1397,12 @@ static void view(gr::FieldView & v, const gr_field * f, uint32_t back, uint32_t 
     v.ncomp = f->d.ncomp;
     v.address = f->d.address;
     v.dtype = f->d.dtype;
+    {
+        const uint32_t dim = f->d.ncomp == 3 ? 3 : 2;
+        const uint64_t bytes = gr::pair_nodes(f->d.nx, f->d.ny, f->d.nz, dim) * (f->d.ncomp == 3 ? 8 : 4) *
+                               dtype_size(f->d.dtype);
+        v.pair32 = bytes < (1ull << 32) ? 1u : 0u;
+    }
     v.zlev = f->zlev;
     v.zlut = f->zlut;
     v.nlut = f->nlut;

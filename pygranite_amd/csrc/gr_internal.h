@@ -69,6 +69,7 @@ struct FieldView
     uint32_t ncomp;
     uint32_t address;  // 0 border (zero outside), 1 clamp
     uint32_t dtype;    // gr_dtype
+    uint32_t pair32;   // the time-paired copy is smaller than 4 GiB: 32-bit byte offsets address it
     // stretched z axis (3D): node k at height zlev[k] (relative to Offset.Z); zlut[m] is the
     // last level at or below zlut_z0 + m / zlut_scale. Null for uniform spacing.
     uint32_t nlut;

@@ -539,6 +539,44 @@ __device__ __forceinline__ void load_pair_node(const void * pair, uint32_t node,
     }
 }
 
+// The same from the node's first byte `q` (16-byte aligned) of a paired copy smaller than 4 GiB:
+// the caller forms q from the copy's (wavefront-uniform) base and a 32-bit byte offset, so the
+// loads take the base + 32-bit offset + immediate form and a corner costs no 64-bit address
+// arithmetic.
+template <int DIM, class T>
+__device__ __forceinline__ void load_pair_node_at(const char * q, uint32_t (&w)[StencilCache<DIM, T>::NW])
+{
+    if constexpr (sizeof(T) == 4 && DIM == 3)
+    {
+        const uint4 a = GR_LOAD(uint4, q, 0);
+        const uint2 c = GR_LOAD(uint2, q + 16, 0);
+        w[0] = a.x;
+        w[1] = a.y;
+        w[2] = a.z;
+        w[3] = a.w;
+        w[4] = c.x;
+        w[5] = c.y;
+    }
+    else if constexpr (sizeof(T) == 2 && DIM == 2)
+    {
+        const uint2 h = GR_LOAD(uint2, q, 0);
+        w[0] = h.x;
+        w[1] = h.y;
+    }
+    else
+    {
+        const uint4 a = GR_LOAD(uint4, q, 0);
+#pragma unroll
+        for (int k = 0; k < StencilCache<DIM, T>::NW; ++k) w[k] = k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w;
+    }
+}
+
+// bytes of one node of the time-paired copy
+template <int DIM, class T> constexpr uint32_t pair_node_bytes()
+{
+    return sizeof(T) == 4 ? (DIM == 3 ? 32u : 16u) : (DIM == 3 ? 16u : 8u);
+}
+
 // back (x) and front (y) slice values of a cached BF16 / FP32 corner (FP16 goes through
 // combine_pair_h, which reads the words directly)
 template <int DIM, class T>
@@ -589,11 +627,26 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
         if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
         {
-#pragma unroll
-            for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
+            if (P.wind.pair32) // uniform: byte offsets of the copy fit 32 bits
             {
-                const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
-                load_pair_node<DIM, T>(W.pair, node, C.w[c]);
+                constexpr uint32_t NB = pair_node_bytes<DIM, T>();
+                const char * pb = static_cast<const char *>(W.pair);
+                const uint32_t o0 = base * NB, oy = o0 + PX * NB, oz = o0 + PXY * NB, oyz = oz + PX * NB;
+#pragma unroll
+                for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
+                {
+                    const uint32_t o = (c & 4) ? ((c & 2) ? oyz : oz) : ((c & 2) ? oy : o0);
+                    load_pair_node_at<DIM, T>(pb + o + ((c & 1) ? NB : 0u), C.w[c]);
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
+                {
+                    const uint32_t node = base + ((c & 1) ? 1u : 0u) + ((c & 2) ? PX : 0u) + ((c & 4) ? PXY : 0u);
+                    load_pair_node<DIM, T>(W.pair, node, C.w[c]);
+                }
             }
             C.base = base;
         }
