@@ -17,16 +17,18 @@ the headline counts integrated particle-steps: steps a particle began alive, cou
 device by the step kernels. The nominal count (particles x steps) is reported beside it.
 
 Forms, all on identical inputs:
-  plain_fp32   FP32 slices sampled slot by slot, particles in caller order, one job per launch
-               (the plain FP32 GPU form)
-  fp32_local   FP32 slices, time-paired layout, Morton-ordered particles
-  fp16         FP16 slices, time-paired layout, Morton-ordered particles
-  fp16_batched the fp16 form run by the persistent batched kernel, the particle set split into
-               16 jobs of 65,536 (value)
-Final positions of the FP16 batched and plain FP32 forms are compared with the FP64 CPU oracle
-on 16,384 particles: domain-relative error over particles whose termination agrees, and the
-termination mismatches classified (sensitive at the input precision, FP16 storage rounding,
-other).
+  plain_fp32    FP32 slices sampled slot by slot, particles in caller order, one job per launch
+                (the plain FP32 GPU form: the reference's arithmetic and layout)
+  fp32_local    FP32 slices, time-paired layout, Morton-ordered particles (bit for bit plain_fp32)
+  fp32k         fp32_local with Kahan-compensated positions
+  fp16          FP16 slices, time-paired layout, Morton-ordered particles
+  fp16_batched  the fp16 form run by the persistent batched kernel, the particle set split into
+                16 jobs of 65,536
+  fp32k_batched the fp32k form run by the batched kernel, 16 jobs
+Final positions of every form are compared with the FP64 CPU oracle on 16,384 particles:
+domain-relative error over particles whose termination agrees, and the termination mismatches
+classified from both step-by-step paths (divergence, threshold, other). The value is the fastest
+form other than plain_fp32 whose RMS error meets 1e-4; every form's rate is reported beside it.
 
 Also reported: the 2D double-gyre configuration on the GPU and as the CPU baseline in full
 (65,536 particles x 1000 Euler steps, all host threads and one), the CPU baseline on a bounded
@@ -48,6 +50,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 ERR_TARGET = 1e-4      # domain-relative RMS position error bound against the FP64 oracle
+FORMS = ("plain_fp32", "fp32_local", "fp32k", "fp16", "fp16_batched", "fp32k_batched")
+KAHAN_FORMS = ("fp32k", "fp32k_batched")
 # VALU issue ceiling: 256 CUs x 2.4 GHz x one wave64 (non-packed FP32) instruction per clock per CU
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 EXPRS = ["sqrt(u*u+v*v+w*w)"]
@@ -113,9 +117,9 @@ def make_field(capi, ctx, wind, dtype):
 
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
-               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0):
+               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False):
     def new_job():
-        j = capi.Job(ctx, build_desc(capi, gs, n), seeds, save_capacity=chunk)
+        j = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
         if cu_waves:
             j.set_option(capi.OPT_CU_WAVES, cu_waves)
@@ -960,17 +964,17 @@ def copy_calibration():
     print(json.dumps({"copies": 3, "bytes_read_each": a.numel() * 4, "bytes_written_each": a.numel() * 4}))
 
 
-def load_counters():
-    """Per-launch hardware counters of the dominant kernel for the default command, from the
-    committed rocprofv3 --pmc summary (profiles/counters_k_batch_abc3d.json): HBM bytes
+def load_counters(form):
+    """Per-launch hardware counters of a form's step kernel for the default command, from the
+    committed rocprofv3 --pmc summary (profiles/counters_<form>_abc3d.json): HBM bytes
     (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration) and VALU wave instructions. None
     when absent."""
-    p = os.path.join(ROOT, "profiles", "counters_k_batch_abc3d.json")
+    p = os.path.join(ROOT, "profiles", f"counters_{form}_abc3d.json")
     if not os.path.exists(p):
         return None
     with open(p) as fh:
         c = json.load(fh)
-    c["source"] = "profiles/counters_k_batch_abc3d.json (" + c.get("command", "rocprofv3 --pmc") + ")"
+    c["source"] = os.path.relpath(p, ROOT) + " (" + c.get("command", "rocprofv3 --pmc") + ")"
     return c
 
 
@@ -997,7 +1001,7 @@ def main():
     ap.add_argument("--regroup-min", type=int, default=65536,
                     help="regroup every K launches only while the live range (all jobs) holds at "
                          "least this many particles; smaller launches are latency-bound")
-    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp16,fp16_batched,fp32k_batched")
+    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp32k,fp16,fp16_batched,fp32k_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
@@ -1105,7 +1109,7 @@ def main():
 
     res = {}
     for name in forms:
-        reps = args.repeats if name.endswith("_batched") else 1
+        reps = args.repeats
         print(f"# form {name}: {args.warmup}+{args.steps} launches of {chunk} steps ({total}-step job, "
               f"{reps} timed run(s)), {args.particles} particles", file=sys.stderr, flush=True)
         kw = dict(total=total, repeats=reps)
@@ -1115,11 +1119,12 @@ def main():
             kw["cu_waves"] = args.cu_waves
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
-        elif name == "fp16":
+        elif name in ("fp16", "fp32k"):
             kw["cu_waves"] = args.cu_waves
             kw["reuse"] = args.reuse
-            r = run_single(capi, ctx, f16, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every, args.regroup_min, **kw)
+            r = run_single(capi, ctx, f16 if name == "fp16" else f32, seeds, gs, n, chunk, args.steps,
+                           args.warmup, True, True, args.reorder_every, args.regroup_min,
+                           kahan=name == "fp32k", **kw)
         elif name in ("fp16_batched", "fp32k_batched"):
             if not fixed:
                 autotune(name)
@@ -1144,21 +1149,21 @@ def main():
     if not args.no_error:
         print(f"# FP64 oracle error check ({args.error_particles} particles)", file=sys.stderr, flush=True)
         sub = min(args.error_particles, args.particles)
-        cmp_forms = {k: res[k] for k in ("fp16_batched", "fp32k_batched", "plain_fp32", "fp16", "fp32_local")
-                     if k in res}
+        cmp_forms = {k: res[k] for k in FORMS if k in res}
         rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched")}
         # every form is integrated over the same number of steps
         err = fp64_error(wind, seeds[:sub], gs, res[forms[0]]["steps_done"], cmp_forms, rounders,
-                         kahan_forms=("fp32k_batched",))
-    # the headline: the fastest batched form whose RMS position error against the FP64 oracle
-    # meets the 1e-4 (domain-relative) bound; the FP16 batched form when no error was measured
-    batched = [k for k in ("fp16_batched", "fp32k_batched") if k in res]
-    ok = [k for k in batched if err and err.get(k, {}).get("rms") is not None and err[k]["rms"] <= ERR_TARGET]
-    head = max(ok, key=lambda k: res[k]["psps"]) if ok else (batched[0] if batched else forms[-1])
+                         kahan_forms=("fp32k", "fp32k_batched"))
+    # the headline: the fastest form (plain_fp32, the reference's own form, is the baseline)
+    # whose RMS position error against the FP64 oracle meets the 1e-4 (domain-relative) bound;
+    # the fastest form when no error was measured
+    cands = [k for k in res if k != "plain_fp32"] or list(res)
+    ok = [k for k in cands if err and err.get(k, {}).get("rms") is not None and err[k]["rms"] <= ERR_TARGET]
+    head = max(ok or cands, key=lambda k: res[k]["psps"])
     hv = res[head]
     outputs = 3 * 4 + 4 + 4  # saved position, curvature, expression per particle-step
     # + Kahan compensation read and written with the positions
-    B16 = algorithmic_bytes(2 if "fp16" in head else 4, outputs) + (2 * 3 * 4 if "k_" in head else 0)
+    B16 = algorithmic_bytes(2 if "fp16" in head else 4, outputs) + (2 * 3 * 4 if head in KAHAN_FORMS else 0)
     # algorithmic bytes of the integrated particle-steps over the kernel's device time, both
     # from this run (HIP events around every step-kernel launch of the timed job)
     ps_per_launch = hv["work"] / max(1, hv["launches"])
@@ -1187,7 +1192,7 @@ def main():
         print("# CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wind, gs)
         cpu["cpu_model"] = cpu_model()
-    counters = load_counters()
+    counters = load_counters(head)
 
     plain = res.get("plain_fp32", {}).get("psps")
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -1197,7 +1202,9 @@ def main():
                            "saved outputs) of the launch's integrated particle-steps over its "
                            "average device time; most gathers are served by L2 / MALL",
             "kernel": ("k_batch (RK4 3D " + ("FP16" if "fp16" in head else "FP32 + Kahan") + " items)")
-                      if head.endswith("_batched") else "k_advance",
+                      if head.endswith("_batched") else
+                      "k_advance (RK4 3D " + ("FP16" if "fp16" in head else "FP32" + (" + Kahan" if head in KAHAN_FORMS else ""))
+                      + ", time-paired; narrow and four-wavefront builds)",
             "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
             "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
             "algorithmic_bytes_per_launch": ps_per_launch * B16}
@@ -1245,7 +1252,7 @@ def main():
                                f"0.005, {hv['steps_done']} steps, curvature + sqrt(u*u+v*v+w*w) saved "
                                "every step",
                    "form": head, "storage": "fp16" if head.startswith("fp16") else "fp32",
-                   "kahan": "k_" in head, "error_target_rms": ERR_TARGET,
+                   "kahan": head in KAHAN_FORMS, "error_target_rms": ERR_TARGET,
                    "meets_error_target": bool(err and err.get(head, {}).get("rms") is not None
                                               and err[head]["rms"] <= ERR_TARGET),
                    "particles": args.particles, "grid": n, "steps_done": hv["steps_done"],

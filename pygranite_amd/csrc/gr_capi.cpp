@@ -1472,6 +1472,11 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
     P.nvol = d.nvol;
     P.nconst = d.nconst;
     P.dt = d.dt;
+    {
+        const float two_dt = 2.f * d.dt, dtdt = d.dt * d.dt; // the kernel's float denominators
+        P.inv_2dt = 1.0 / double(two_dt);
+        P.inv_dtdt = 1.0 / double(dtdt);
+    }
     P.sphere_radius = d.sphere_radius;
     P.max_length = d.max_length;
     for (int k = 0; k < 3; ++k)

@@ -123,6 +123,11 @@ struct StepParams
     uint32_t refmask; // reference-vector slots read by the programs (bit per slot)
 
     float dt, sphere_radius, max_length;
+    // 1 / (2 dt) and 1 / (dt dt) rounded to FP64, the two float denominators (2.f * dt, dt * dt)
+    // of the curvature's finite differences: float(double(x) * r) is the correctly rounded
+    // x / denominator for every float x (the product's error, 2^-52, is below the distance of any
+    // float quotient from a rounding boundary), at the cost of one FP64 multiply
+    double inv_2dt, inv_dtdt;
     float h[3], inv_h[3], off[3], dom[3];
     double off64[3];
 

@@ -1010,8 +1010,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 #pragma unroll
                         for (int d = 0; d < 3; ++d)
                         {
-                            d1[d] = (np[d] - q[d]) / (2.f * dt);
-                            d2[d] = ((q[d] + np[d]) - 2.f * p[d]) / (dt * dt);
+                            // (np - q) / (2 dt) and ((q + np) - 2 p) / (dt dt), each correctly
+                            // rounded: one FP64 multiply by the denominator's FP64 reciprocal
+                            d1[d] = float(double(np[d] - q[d]) * P.inv_2dt);
+                            d2[d] = float(double((q[d] + np[d]) - 2.f * p[d]) * P.inv_dtdt);
                         }
                         const float ldp = sqrtf((d1[0] * d1[0] + d1[1] * d1[1]) + d1[2] * d1[2]);
                         const float ldp3 = (ldp * ldp) * ldp;
