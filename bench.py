@@ -117,10 +117,12 @@ def make_field(capi, ctx, wind, dtype):
 
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
-               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False):
+               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False, quad=-1):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
+        if quad >= 0:
+            j.set_option(capi.OPT_QUAD, quad)
         if cu_waves:
             j.set_option(capi.OPT_CU_WAVES, cu_waves)
         if reuse > 0:
@@ -1001,6 +1003,9 @@ def main():
     ap.add_argument("--regroup-min", type=int, default=65536,
                     help="regroup every K launches only while the live range (all jobs) holds at "
                          "least this many particles; smaller launches are latency-bound")
+    ap.add_argument("--quad", type=int, default=-1,
+                    help="single-job forms: live-particle limit for quad launches (GR_OPT_QUAD; -1: the "
+                         "library default)")
     ap.add_argument("--forms", default="plain_fp32,fp32_local,fp32k,fp16,fp16_batched,fp32k_batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -1117,11 +1122,13 @@ def main():
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, False, False, **kw)
         elif name == "fp32_local":
             kw["cu_waves"] = args.cu_waves
+            kw["quad"] = args.quad
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
         elif name in ("fp16", "fp32k"):
             kw["cu_waves"] = args.cu_waves
             kw["reuse"] = args.reuse
+            kw["quad"] = args.quad
             r = run_single(capi, ctx, f16 if name == "fp16" else f32, seeds, gs, n, chunk, args.steps,
                            args.warmup, True, True, args.reorder_every, args.regroup_min,
                            kahan=name == "fp32k", **kw)

@@ -317,6 +317,12 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
  * 0: only launches whose wavefronts all fit on the device at once do. Results do not depend on
  * it. */
 #define GR_OPT_REUSE 5u
+/* GR_OPT_QUAD (0 .. 2^29 - 1, default 128 x compute units): launches of a job whose live range
+ * holds at most this many particles, in the time-paired layout (a copy smaller than 4 GiB) with a
+ * plain feature set, run four lanes per particle, each lane interpolating one velocity component
+ * (a quarter of the lerps per lane: a shorter step chain for launches of few particles); 0:
+ * never. Results do not depend on it. */
+#define GR_OPT_QUAD 6u
 int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
 
 /* Reorders the resident particle state by the Morton code of each particle's current cell

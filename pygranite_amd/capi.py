@@ -35,7 +35,7 @@ EXPORTS = [
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
-OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE, OPT_REUSE = 1, 2, 3, 4, 5
+OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE, OPT_REUSE, OPT_QUAD = 1, 2, 3, 4, 5, 6
 CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP, CTX_BATCH_CU_WAVES, CTX_BATCH_REUSE = 1, 2, 3, 4, 5
 
 

@@ -33,6 +33,14 @@ __global__ void __launch_bounds__(256) k_advance_narrow(const StepParams P, uint
     advance_block<DIM, INTEG, T, MODE, true, SPEC>(P, nxcd);
 }
 
+// The time-paired kernel with a quad of lanes per particle (velocity's Q = 4) and the stencil
+// reuse, for launches of few particles: the plain feature sets only.
+template <int DIM, int INTEG, class T, int MODE, int SPEC>
+__global__ void __launch_bounds__(256) k_advance_quad(const StepParams P, uint32_t nxcd)
+{
+    advance_block<DIM, INTEG, T, MODE, true, SPEC, 4>(P, nxcd);
+}
+
 typedef void (*AdvanceFn)(const StepParams, uint32_t);
 
 // lean kernels exist generic (spec < 0) and for the plain feature set with curvature Off (0) or
@@ -63,9 +71,21 @@ template <int DIM, int INTEG, class T, int MODE, int SPEC> struct KNarrowFn
     static AdvanceFn fn() { return k_advance_narrow<DIM, INTEG, T, MODE, SPEC>; }
 };
 
+// quad kernels exist for the specialised feature sets only (null otherwise)
 template <int DIM, int INTEG, class T>
-AdvanceFn pick_mode(uint32_t mode, int spec, bool wide, bool narrow)
+AdvanceFn pick_quad(int spec)
 {
+    if (spec == 0) return k_advance_quad<DIM, INTEG, T, M_PAIRED, 0>;
+    if (spec == 3) return k_advance_quad<DIM, INTEG, T, M_PAIRED, 3>;
+    if (spec == 16) return k_advance_quad<DIM, INTEG, T, M_PAIRED, 16>;
+    if (spec == 19) return k_advance_quad<DIM, INTEG, T, M_PAIRED, 19>;
+    return nullptr;
+}
+
+template <int DIM, int INTEG, class T>
+AdvanceFn pick_mode(uint32_t mode, int spec, bool wide, bool narrow, bool quad)
+{
+    if (mode == M_PAIRED && quad) return pick_quad<DIM, INTEG, T>(spec);
     if (mode == M_PAIRED && narrow) return SpecPick<KNarrowFn, DIM, INTEG, T, M_PAIRED>::get(spec);
     if constexpr (std::is_same<T, __half>::value)
         if (mode == M_PAIRED && wide) return SpecPick<KW4Fn, DIM, INTEG, T, M_PAIRED>::get(spec);
@@ -76,18 +96,22 @@ AdvanceFn pick_mode(uint32_t mode, int spec, bool wide, bool narrow)
 
 // the kernel of storage type T for a launch (wide / narrow: see launch_advance)
 template <class T>
-AdvanceFn pick_advance_t(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow)
+AdvanceFn pick_advance_t(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
+                         bool quad)
 {
     if (dim == 3)
-        return integrator == GR_RK4 ? pick_mode<3, GR_RK4, T>(mode, spec, wide, narrow)
-                                    : pick_mode<3, GR_EULER, T>(mode, spec, wide, narrow);
-    return integrator == GR_RK4 ? pick_mode<2, GR_RK4, T>(mode, spec, wide, narrow)
-                                : pick_mode<2, GR_EULER, T>(mode, spec, wide, narrow);
+        return integrator == GR_RK4 ? pick_mode<3, GR_RK4, T>(mode, spec, wide, narrow, quad)
+                                    : pick_mode<3, GR_EULER, T>(mode, spec, wide, narrow, quad);
+    return integrator == GR_RK4 ? pick_mode<2, GR_RK4, T>(mode, spec, wide, narrow, quad)
+                                : pick_mode<2, GR_EULER, T>(mode, spec, wide, narrow, quad);
 }
 
 // one per storage type, each in its own translation unit
-AdvanceFn pick_advance_f32(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
-AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
-AdvanceFn pick_advance_bf16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow);
+AdvanceFn pick_advance_f32(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
+                           bool quad);
+AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
+                           bool quad);
+AdvanceFn pick_advance_bf16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
+                            bool quad);
 
 } // namespace gr

@@ -192,6 +192,7 @@ struct gr_job
     uint32_t cu_waves{0}; // cap on the step kernel's wavefronts per CU (0: none)
     uint32_t specialize{1}; // use the compile-time feature sets when they apply
     uint32_t reuse{1};      // stencil-reuse build for every launch (0: narrow launches only)
+    uint64_t quad_max{0};   // launches of at most this many live particles run a quad per particle
     float * p[3]{};
     float * q[3]{};
     float * c[3]{};
@@ -952,6 +953,9 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     j->run_n = n;
     j->expr_cols = expr_cols;
     j->cap = save_capacity ? save_capacity : 1;
+    // quad launches below 128 live particles per compute unit (measured on the ABC workload:
+    // 64 .. 512 per CU tried, 128 best for FP32 and FP16 storage)
+    j->quad_max = 128ull * uint64_t(std::max(1, ctx->cus));
     j->full = (d.dim == 2 && (d.border != GR_BLOCK || d.sphere_radius > 0.f)) ||
               d.abort_mode != GR_ABORT_TIME || d.uplift_mode != GR_MODE_OFF ||
               d.constants_mode != GR_MODE_OFF || d.volume_mode != GR_MODE_OFF || d.nvol > 0 ||
@@ -1050,6 +1054,10 @@ extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
             return GR_OK;
         case GR_OPT_SPECIALIZE: j->specialize = value ? 1 : 0; return GR_OK;
         case GR_OPT_REUSE: j->reuse = value ? 1 : 0; return GR_OK;
+        case GR_OPT_QUAD:
+            if (value < 0 || value >= (1ll << 29)) return set_err(GR_E_INVALID, "quad limit must be 0 .. 2^29 - 1");
+            j->quad_max = uint64_t(value);
+            return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown option %u", option);
     }
 }
@@ -1619,6 +1627,14 @@ static int finish_advance(gr_job * j, const gr_window * w, uint64_t nsaves, hipE
 static void time_table(const gr::StepParams & P, const uint32_t * edges, float * out);
 static int wait_ring_reads(gr_job * j, uint64_t nsaves, hipStream_t s);
 
+// A launch runs a quad of lanes per particle when its features are a specialised plain set in
+// the time-paired layout (a paired copy smaller than 4 GiB) and its live range is at most the
+// job's quad limit.
+static bool use_quad(const gr_job * j, const gr::StepParams & P, int spec, uint32_t mode)
+{
+    return spec >= 0 && mode == gr::MODE_PAIRED && P.wind.pair32 && j->quad_max && j->run_n <= j->quad_max;
+}
+
 extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
 {
     if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
@@ -1652,7 +1668,7 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, j->ctx->lds_per_cu);
     const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
     if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, j->ctx->nxcd, j->run_n, wide, narrow,
-                           lds, s) != 0)
+                           use_quad(j, P, spec, mode), lds, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     hipEvent_t done = nullptr;
     if ((rc = kt_mark(j->ctx, s, true, &done)) != GR_OK) return rc;
@@ -1691,7 +1707,7 @@ extern "C" int gr_job_advance_noalloc(gr_job * j, const gr_window * w)
     const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, ctx->lds_per_cu);
     const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
     if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, ctx->nxcd, j->run_n, wide, narrow,
-                           lds, s) != 0)
+                           use_quad(j, P, spec, mode), lds, s) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     GR_HIP(hipEventRecord(ctx->noalloc_done, s));
     j->timed = false;

@@ -281,8 +281,10 @@ inline uint32_t launch_lds_bytes(uint32_t block, uint32_t lds_cols, uint32_t cu_
 // fit at two per SIMD (selects the FP16 build that reuses stencils across RK4 stages)
 // spec: the kernel's compile-time feature set (lean_spec, or -1 for the generic build);
 // lds_bytes: as for launch_batch
+// quad: the launch runs a quad of lanes per particle (k_advance_quad; spec >= 0 and the paired
+// mode only, fewer than 2^29 particles)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode, int spec,
-                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow,
+                   uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, bool quad,
                    uint32_t lds_bytes, void * stream);
 // Scratch of the no-allocation entry points (gr_k_noalloc.hip): NOALLOC_MAX_STEPS steps of time
 // table in a file-scope device array. noalloc_setup takes its address on the current device

@@ -4,9 +4,10 @@
 namespace gr
 {
 
-AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow)
+AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
+                           bool quad)
 {
-    return pick_advance_t<__half>(dim, integrator, mode, spec, wide, narrow);
+    return pick_advance_t<__half>(dim, integrator, mode, spec, wide, narrow, quad);
 }
 
 } // namespace gr

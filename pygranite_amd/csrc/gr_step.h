@@ -604,7 +604,18 @@ __device__ __forceinline__ void decode_pair_node(const uint32_t (&w)[StencilCach
     }
 }
 
-template <int DIM, class T, int MODE, bool CACHE, int SPEC>
+// Value of lane D of the calling lane's quad (lanes 4k .. 4k + 3), one DPP move.
+template <int D> __device__ __forceinline__ float quad_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), D * 0x55, 0xf, 0xf, false));
+}
+
+// Q = 4 (quad launches): the four lanes of a quad hold one particle and compute the same state;
+// lane i of the quad interpolates velocity component min(i, DIM - 1) alone -- its back and front
+// values at the eight (four) corners, one lerp tree -- and the components are exchanged inside
+// the quad. The same operations on the same values as the one-lane form, a quarter of the lerps
+// per lane: for launches of few particles, whose time is one lane's step chain.
+template <int DIM, class T, int MODE, bool CACHE, int SPEC, int Q = 1>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
                                          float t, float * v, f16_t one, StencilCache<DIM, T> & C)
 {
@@ -625,6 +636,61 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         if (DIM == 3) split(arg[2], P.wind.nz, wmode, k, S.wz);
         const uint32_t PX = P.wind.nx + 3, PXY = PX * (P.wind.ny + 3);
         const uint32_t base = uint32_t(k + 1) * PXY + uint32_t(j + 1) * PX + uint32_t(i + 1);
+        if constexpr (Q == 4)
+        {
+            // this lane's component: its back value at byte comp * ES of a node, its front value
+            // DIM * ES further; cached as raw words C.w[c][0] (back) and C.w[c][1] (front)
+            constexpr uint32_t NB = pair_node_bytes<DIM, T>(), ES = sizeof(T), FO = DIM * ES;
+            constexpr int NCOR = StencilCache<DIM, T>::NCOR;
+            const uint32_t qi = threadIdx.x & 3u, comp = qi < uint32_t(DIM) ? qi : uint32_t(DIM - 1);
+            if (!CACHE || base != C.base)
+            {
+                const char * pb = static_cast<const char *>(W.pair);
+                auto word = [](const char * q) -> uint32_t {
+                    if constexpr (ES == 4) return GR_LOAD(uint32_t, q, 0);
+                    else return uint32_t(GR_LOAD(uint16_t, q, 0));
+                };
+                // quad launches address the paired copy with 32-bit byte offsets only (the host
+                // runs them for copies smaller than 4 GiB)
+                const uint32_t o0 = base * NB + comp * ES, oy = o0 + PX * NB, oz = o0 + PXY * NB, oyz = oz + PX * NB;
+#pragma unroll
+                for (int c = 0; c < NCOR; ++c)
+                {
+                    const char * q = pb + ((c & 4) ? ((c & 2) ? oyz : oz) : ((c & 2) ? oy : o0)) + ((c & 1) ? NB : 0u);
+                    C.w[c][0] = word(q);
+                    C.w[c][1] = word(q + FO);
+                }
+                C.base = base;
+            }
+            float vb[8][1], vf[8][1];
+#pragma unroll
+            for (int c = 0; c < NCOR; ++c)
+            {
+                if constexpr (ES == 4)
+                {
+                    vb[c][0] = __uint_as_float(C.w[c][0]);
+                    vf[c][0] = __uint_as_float(C.w[c][1]);
+                }
+                else if constexpr (std::is_same<T, __half>::value)
+                {
+                    vb[c][0] = float(__builtin_bit_cast(f16_t, uint16_t(C.w[c][0])));
+                    vf[c][0] = float(__builtin_bit_cast(f16_t, uint16_t(C.w[c][1])));
+                }
+                else
+                {
+                    vb[c][0] = __uint_as_float(C.w[c][0] << 16);
+                    vf[c][0] = __uint_as_float(C.w[c][1] << 16);
+                }
+            }
+            float b1[1], f1[1];
+            combine<DIM, 1>(vb, S, b1);
+            combine<DIM, 1>(vf, S, f1);
+            const float vm = lerp(b1[0], f1[0], t);
+            v[0] = quad_lane<0>(vm);
+            v[1] = quad_lane<1>(vm);
+            if (DIM == 3) v[2] = quad_lane<2>(vm);
+            return;
+        }
         if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
         {
             if (P.wind.pair32) // uniform: byte offsets of the copy fit 32 bits
@@ -748,9 +814,11 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
 
 // Advances one particle by the launch's steps; returns the number of steps it integrated
 // (steps begun alive, the aborting step included).
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1>
 __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint32_t pid)
 {
+    // quad launches: every lane of the quad keeps the particle's state, the first one stores it
+    const bool writer = Q == 1 || (threadIdx.x & 3u) == 0;
     constexpr bool FULL = MODE == M_FULL;
     static_assert(!FULL || SPEC < 0, "the full kernel reads every flag at run time");
     typedef Feat<SPEC> F;
@@ -840,22 +908,22 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 const float t1 = tt.y, t2 = tt.z;
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, p, t, k1, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, p, t, k1, one, sc);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t1, k2, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t1, k2, one, sc);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t1, k3, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t1, k3, one, sc);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, x, t2, k4, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t2, k4, one, sc);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -867,7 +935,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                velocity<DIM, T, MODE, CACHE, SPEC>(P, W, p, t, vc, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, p, t, vc, one, sc);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -1108,7 +1176,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
 
         // copyParticleInfo, tpp:997-1049. Columns after a particle's stop step hold its final
         // state and are filled when read (k_fill_frozen).
-        if (s == next_save && live)
+        if (s == next_save && live && writer)
         {
             auto rp = G(P.ring_pos) + (uint64_t(col) * P.n + pid) * DIM;
 #pragma unroll
@@ -1128,6 +1196,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
         if (status != GR_STATUS_TIME) break; // stopped: nothing more to compute or save
     }
 
+    if (!writer) return 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) G(P.p[d])[pid] = p[d];
     if (curv_pos)
@@ -1177,13 +1246,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint
     return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1>
 __device__ __forceinline__ void advance_block(const StepParams & P, uint32_t nxcd)
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
-    const uint32_t pid = b * blockDim.x + threadIdx.x;
+    const uint32_t pid = (b * blockDim.x + threadIdx.x) / Q; // a quad per particle when Q = 4
     uint32_t w = 0;
-    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE, SPEC>(P, pid);
+    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE, SPEC, Q>(P, pid);
     if (P.work) // block sizes are multiples of 64
         wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
 }

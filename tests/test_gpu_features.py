@@ -291,7 +291,7 @@ def test_specialized_kernels_equal_generic(gpu_available, dim, dtype, paired, cu
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
     if dim == 3:
-        assert (outs[0][2] != 0).sum() > 50  # particles stopped along the way (3D border)
+        assert (outs[0][-4 if curv else -3] != 0).sum() > 50  # particles stopped along the way (3D border)
     f.close()
     ctx.close()
 
@@ -355,5 +355,67 @@ def test_async_column_reads_overlapping_launches(gpu_available, reorder):
             np.testing.assert_array_equal(u, v)
     a.close()
     b.close()
+    f.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,dtype,curv,kahan", [(3, capi.F32, 3, 0), (3, capi.F16, 3, 1), (3, capi.BF16, 0, 0),
+                                                  (2, capi.F32, 0, 1), (2, capi.F16, 3, 0)])
+def test_quad_launches_equal_one_lane(gpu_available, dim, dtype, curv, kahan):
+    """Launches with four lanes per particle (GR_OPT_QUAD above the live range: every launch of
+    the job) give every output bit for bit equal to one lane per particle (GR_OPT_QUAD = 0), over
+    single-job advances through two windows of a regrouped job that save every step."""
+    ctx = capi.Context(0)
+    if dim == 3:
+        src, gs = abc(24, 4)
+        f = capi.Field(ctx, 24, 24, 24, 3, dtype, nslots=4)
+        seeds = seeds3(3000)
+    else:
+        src, gs = gyre(64, 32, 4)
+        f = capi.Field(ctx, 64, 32, 1, 2, dtype, nslots=4, address=1)
+        seeds = seeds2(3000)
+    for sl in range(4):
+        f.upload(sl, [src[sl, c] for c in range(dim)])
+    outs = []
+    for quad in (0, 1 << 20):
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode = dim, 1, curv, 1
+        d.dt, d.dtd = 0.01, 1.0
+        d.kahan = kahan
+        for k in range(3):
+            d.grid_scale[k] = gs[k % len(gs)]
+            d.domain[k] = float(np.float32(src.shape[-1 - k] if k < dim else 1) * np.float32(gs[k % len(gs)]))
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v)*2-x", dim=dim, curvature=curv)
+        job = capi.Job(ctx, d, seeds, save_capacity=200)
+        job.set_option(capi.OPT_QUAD, quad)
+        job.reorder()
+        counter, t = 0, np.float32(0)
+        o = []
+        for wb in (0, 1):
+            w = capi.Window()
+            w.wind, w.wind_back, w.wind_front = f.h, wb, wb + 1
+            w.time_back, w.time_front = float(wb), float(wb + 1)
+            w.time_start = w.time_start64 = float(t)
+            w.counter_start, w.nsteps = counter, 100
+            job.advance(w)
+            # this window's columns, read before the particles are regrouped again
+            o += [job.read_saved(capi.OUT_POSITIONS, counter, 100), job.read_saved(capi.OUT_EXPR0, counter, 100)]
+            if curv:
+                o.append(job.read_saved(capi.OUT_CURVATURE, counter, 100))
+            for _ in range(100):
+                t = np.float32(t + np.float32(0.01))
+            counter += 100
+            job.reorder()
+        o += [job.read(capi.OUT_STATUS), job.read(capi.OUT_ABORT_STEP), job.read(capi.OUT_POSITIONS)]
+        if curv:
+            o.append(job.read(capi.OUT_TOTAL_CURVATURE))
+        outs.append(o)
+        job.close()
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    if dim == 3:
+        assert (outs[0][-4 if curv else -3] != 0).sum() > 50  # particles stopped along the way (3D border)
     f.close()
     ctx.close()
