@@ -13,6 +13,7 @@ yet. Using an API name there fails loudly with the build command.
 from __future__ import annotations
 
 import importlib
+import importlib.util
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,6 +41,10 @@ def __getattr__(name: str):
         raise AttributeError(name)
     if name == "pygranite":
         return native()
+    # `from pygranite_amd import build` asks for the attribute before importing the submodule:
+    # pure-Python submodules must resolve without the native module
+    if importlib.util.find_spec(f"{__name__}.{name}") is not None:
+        return importlib.import_module(f".{name}", __name__)
     mod = native()
     try:
         return getattr(mod, name)

@@ -50,6 +50,9 @@ def clean_tree(tmp_path_factory):
 def test_unbuilt_tree_imports_build_and_refuses_api(clean_tree):
     assert not os.path.exists(os.path.join(clean_tree, "pygranite_amd", "libgranite_hip.so"))
     r = _py("import pygranite_amd, pygranite_amd.build, pygranite_amd.synthetic, pygranite_amd.capi\n"
+            "from pygranite_amd import build, serving\n"
+            "assert callable(build.build)\n"
+            "import __graft_entry__\n"
             "try:\n"
             "    pygranite_amd.IntegratorSettings\n"
             "except ImportError as e:\n"
