@@ -25,6 +25,8 @@ Forms, all on identical inputs:
   fp16_batched  the fp16 form run by the persistent batched kernel, the particle set split into
                 16 jobs of 65,536
   fp32k_batched the fp32k form run by the batched kernel, 16 jobs
+  fp32_tile     fp32_local with the per-workgroup LDS field tile (GR_OPT_TILE, --tile-bytes)
+  fp16_tile     fp16 with the LDS field tile
 Final positions of every form are compared with the FP64 CPU oracle on 16,384 particles:
 domain-relative error over particles whose termination agrees, and the termination mismatches
 classified from both step-by-step paths (divergence, threshold, other). The value is the fastest
@@ -50,7 +52,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 ERR_TARGET = 1e-4      # domain-relative RMS position error bound against the FP64 oracle
-FORMS = ("plain_fp32", "fp32_local", "fp32k", "fp16", "fp16_batched", "fp32k_batched")
+FORMS = ("plain_fp32", "fp32_local", "fp32k", "fp16", "fp16_batched", "fp32k_batched", "fp32_tile", "fp16_tile")
 KAHAN_FORMS = ("fp32k", "fp32k_batched")
 # VALU issue ceiling: 256 CUs x 2.4 GHz x one wave64 (non-packed FP32) instruction per clock per CU
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
@@ -117,12 +119,16 @@ def make_field(capi, ctx, wind, dtype):
 
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
-               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False, quad=-1):
+               regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False, quad=-1, tile=0,
+               tile_halo=1):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
         if quad >= 0:
             j.set_option(capi.OPT_QUAD, quad)
+        if tile:
+            j.set_option(capi.OPT_TILE, tile)
+            j.set_option(capi.OPT_TILE_HALO, tile_halo)
         if cu_waves:
             j.set_option(capi.OPT_CU_WAVES, cu_waves)
         if reuse > 0:
@@ -181,6 +187,9 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
         kernel_ms, launches = ctx.take_kernel_ms()  # HIP events around every step-kernel launch
         r = {"wall_s": wall, "kernel_ms": kernel_ms, "launches": launches,
              "steps_done": st["sch"].counter, "work": work}
+        if tile:
+            tiled, untiled = ctx.take_tile_stats()
+            r["tile_workgroup_steps"] = {"with_tile": tiled, "without_tile": untiled}
         if rep == repeats - 1:
             job = st["job"]
             r.update(positions=job.read(capi.OUT_POSITIONS), status=job.read(capi.OUT_STATUS),
@@ -1006,7 +1015,10 @@ def main():
     ap.add_argument("--quad", type=int, default=-1,
                     help="single-job forms: live-particle limit for quad launches (GR_OPT_QUAD; -1: the "
                          "library default)")
-    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp32k,fp16,fp16_batched,fp32k_batched")
+    ap.add_argument("--forms", default="plain_fp32,fp32_local,fp32k,fp16,fp16_batched,fp32k_batched,fp32_tile,fp16_tile")
+    ap.add_argument("--tile-bytes", type=int, default=32768,
+                    help="LDS field tile per workgroup of the *_tile forms (GR_OPT_TILE)")
+    ap.add_argument("--tile-halo", type=int, default=1, help="cells around the tile's box (GR_OPT_TILE_HALO)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the lite batch/serving runs")
@@ -1125,6 +1137,12 @@ def main():
             kw["quad"] = args.quad
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, **kw)
+        elif name in ("fp32_tile", "fp16_tile"):
+            kw["cu_waves"] = args.cu_waves
+            kw["quad"] = args.quad
+            r = run_single(capi, ctx, f32 if name == "fp32_tile" else f16, seeds, gs, n, chunk, args.steps,
+                           args.warmup, True, True, args.reorder_every, args.regroup_min,
+                           tile=args.tile_bytes, tile_halo=args.tile_halo, **kw)
         elif name in ("fp16", "fp32k"):
             kw["cu_waves"] = args.cu_waves
             kw["reuse"] = args.reuse
@@ -1157,7 +1175,7 @@ def main():
         print(f"# FP64 oracle error check ({args.error_particles} particles)", file=sys.stderr, flush=True)
         sub = min(args.error_particles, args.particles)
         cmp_forms = {k: res[k] for k in FORMS if k in res}
-        rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched")}
+        rounders = {k: synthetic.round_fp16 for k in ("fp16", "fp16_batched", "fp16_tile")}
         # every form is integrated over the same number of steps
         err = fp64_error(wind, seeds[:sub], gs, res[forms[0]]["steps_done"], cmp_forms, rounders,
                          kahan_forms=("fp32k", "fp32k_batched"))
@@ -1270,6 +1288,7 @@ def main():
                    "single_job_cu_waves": args.cu_waves},
         "forms_psps": {k: v["psps"] for k, v in res.items()},
         "forms_device_ms": {k: v["kernel_ms"] for k, v in res.items()},
+        "tile_workgroup_steps": {k: v["tile_workgroup_steps"] for k, v in res.items() if "tile_workgroup_steps" in v},
         "forms_steps_done": {k: v["steps_done"] for k, v in res.items()},
         "repeat_walls_s": hv["repeat_walls_s"],
         "autotune": tune,

@@ -91,6 +91,11 @@ int gr_ctx_synchronize(gr_ctx * ctx);
  * the compute stream. No reference counterpart: it is the work measure of throughput figures
  * (dead particles cost no sampling). */
 int gr_ctx_take_work(gr_ctx * ctx, uint64_t * particle_steps);
+/* Workgroup-steps of field-tile launches (GR_OPT_TILE) since the last call: with a tile
+ * (*tiled) and without one, because the box of the workgroup's live cells did not fit
+ * (*untiled); resets both. Synchronises the compute stream. No counterpart in the reference
+ * (its stencils come from the texture unit, source/granite/TrajectoryIntegrator.icc:22-40). */
+int gr_ctx_take_tile_stats(gr_ctx * ctx, uint64_t * tiled, uint64_t * untiled);
 /* Summed device time (HIP events on the compute stream) and number of the step-kernel launches
  * (gr_job_advance, gr_batch_run) since the previous call, then resets both. Waits for the last
  * launch; launches are not serialised by the timing. No reference counterpart. */
@@ -323,6 +328,14 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
  * (a quarter of the lerps per lane: a shorter step chain for launches of few particles); 0:
  * never. Results do not depend on it. */
 #define GR_OPT_QUAD 6u
+/* GR_OPT_TILE (0 or 1024 .. 65536 bytes, default 0): 3D time-paired launches (not quad ones) run
+ * the kernel in which every workgroup, each step, stages in LDS the box of time-paired nodes
+ * around its live particles' cells (this many bytes of LDS per workgroup, beyond the
+ * interpreter's) and samples the stencils inside it from there; stencils outside the box, and
+ * steps whose box does not fit, load from the copy. 0: never. GR_OPT_TILE_HALO (0 .. 8, default
+ * 1): cells added to the box on every side. Results do not depend on either. */
+#define GR_OPT_TILE 7u
+#define GR_OPT_TILE_HALO 8u
 int gr_job_set_option(gr_job * job, uint32_t option, int64_t value);
 
 /* Reorders the resident particle state by the Morton code of each particle's current cell

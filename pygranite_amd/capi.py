@@ -25,7 +25,7 @@ EXPR_ALLOW_ABS = 1
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
     "gr_abi_version", "gr_last_error", "gr_device_count", "gr_ctx_create", "gr_ctx_destroy",
-    "gr_ctx_synchronize", "gr_ctx_info", "gr_ctx_take_work", "gr_ctx_take_kernel_ms", "gr_ctx_set_option",
+    "gr_ctx_synchronize", "gr_ctx_info", "gr_ctx_take_work", "gr_ctx_take_tile_stats", "gr_ctx_take_kernel_ms", "gr_ctx_set_option",
     "gr_ctx_get_option", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
@@ -36,6 +36,7 @@ EXPORTS = [
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
 ]
 OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE, OPT_REUSE, OPT_QUAD = 1, 2, 3, 4, 5, 6
+OPT_TILE, OPT_TILE_HALO = 7, 8
 CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP, CTX_BATCH_CU_WAVES, CTX_BATCH_REUSE = 1, 2, 3, 4, 5
 
 
@@ -128,6 +129,7 @@ def lib():
         "gr_ctx_synchronize": ([vp], i32),
         "gr_ctx_info": ([vp, ctypes.c_char_p, ctypes.c_size_t, P(u64), P(i32)], i32),
         "gr_ctx_take_work": ([vp, P(u64)], i32),
+        "gr_ctx_take_tile_stats": ([vp, P(u64), P(u64)], i32),
         "gr_ctx_take_kernel_ms": ([vp, P(ctypes.c_double), P(u64)], i32),
         "gr_ctx_set_option": ([vp, u32, ctypes.c_int64], i32),
         "gr_ctx_get_option": ([vp, u32, P(ctypes.c_int64)], i32),
@@ -303,6 +305,12 @@ class Context:
         ms, n = ctypes.c_double(0), ctypes.c_uint64(0)
         _check(lib().gr_ctx_take_kernel_ms(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
+
+    def take_tile_stats(self):
+        """(workgroup-steps with a field tile, without one) since the last call (resets both)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().gr_ctx_take_tile_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def take_work(self) -> int:
         """Integrated particle-steps of every launch since the previous call (then reset)."""

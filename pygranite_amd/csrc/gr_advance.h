@@ -113,5 +113,7 @@ AdvanceFn pick_advance_f16(uint32_t dim, uint32_t integrator, uint32_t mode, int
                            bool quad);
 AdvanceFn pick_advance_bf16(uint32_t dim, uint32_t integrator, uint32_t mode, int spec, bool wide, bool narrow,
                             bool quad);
+// the 3D time-paired kernels with the field tile (gr_k_adv_tile.hip)
+AdvanceFn pick_advance_tile(uint32_t dtype, uint32_t integrator, int spec);
 
 } // namespace gr

@@ -81,6 +81,7 @@ struct gr_ctx
     hipStream_t copy{nullptr};
     unsigned long long * dcount{nullptr}; // alive count scratch
     unsigned long long * dwork{nullptr};  // integrated particle-steps, WORK_SLOTS x WORK_STRIDE
+    unsigned long long * dtile{nullptr};  // field-tile workgroup-steps (with, without), same slots
     uint32_t * d_culut{nullptr};          // cu_key -> queue of the batched kernel
     uint32_t ncu{1};                      // compute units found (dense indices in d_culut)
     // persistent batched launches
@@ -193,6 +194,8 @@ struct gr_job
     uint32_t specialize{1}; // use the compile-time feature sets when they apply
     uint32_t reuse{1};      // stencil-reuse build for every launch (0: narrow launches only)
     uint64_t quad_max{0};   // launches of at most this many live particles run a quad per particle
+    uint32_t tile_bytes{0}; // LDS field tile per workgroup of 3D time-paired launches (0: none)
+    uint32_t tile_halo{1};  // cells added to the tile's box on every side
     float * p[3]{};
     float * q[3]{};
     float * c[3]{};
@@ -328,13 +331,16 @@ extern "C" int gr_ctx_create(int device, gr_ctx ** out)
         hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&c->dcount), sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&c->dwork), size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess ||
-        hipMemsetAsync(c->dwork, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8, c->compute) != hipSuccess)
+        hipMemsetAsync(c->dwork, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8, c->compute) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&c->dtile), size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8) != hipSuccess ||
+        hipMemsetAsync(c->dtile, 0, size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE * 8, c->compute) != hipSuccess)
     {
         (void)hipGetLastError();
         if (c->compute) (void)hipStreamDestroy(c->compute);
         if (c->copy) (void)hipStreamDestroy(c->copy);
         dfree(c->dcount);
         dfree(c->dwork);
+        dfree(c->dtile);
         delete c;
         return set_err(GR_E_HIP, "gr_ctx_create: stream / buffer creation failed");
     }
@@ -375,6 +381,7 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     dfree(c->dcount);
     dfree(c->dwork);
+    dfree(c->dtile);
     dfree(c->d_culut);
     dfree(c->d_stage);
     dfree(c->d_items);
@@ -521,6 +528,25 @@ extern "C" int gr_ctx_take_work(gr_ctx * c, uint64_t * particle_steps)
     uint64_t sum = 0;
     for (uint32_t k = 0; k < gr::WORK_SLOTS; ++k) sum += h[size_t(k) * gr::WORK_STRIDE];
     *particle_steps = sum;
+    return GR_OK;
+}
+
+extern "C" int gr_ctx_take_tile_stats(gr_ctx * c, uint64_t * tiled, uint64_t * untiled)
+{
+    if (!c || !tiled || !untiled) return set_err(GR_E_INVALID, "gr_ctx_take_tile_stats: null argument");
+    GR_HIP(hipSetDevice(c->device));
+    std::vector<unsigned long long> h(size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE);
+    GR_HIP(hipMemcpyAsync(h.data(), c->dtile, h.size() * 8, hipMemcpyDeviceToHost, c->compute));
+    GR_HIP(hipMemsetAsync(c->dtile, 0, h.size() * 8, c->compute));
+    GR_HIP(hipStreamSynchronize(c->compute));
+    uint64_t a = 0, b = 0;
+    for (uint32_t k = 0; k < gr::WORK_SLOTS; ++k)
+    {
+        a += h[size_t(k) * gr::WORK_STRIDE];
+        b += h[size_t(k) * gr::WORK_STRIDE + 1];
+    }
+    *tiled = a;
+    *untiled = b;
     return GR_OK;
 }
 
@@ -1057,6 +1083,15 @@ extern "C" int gr_job_set_option(gr_job * j, uint32_t option, int64_t value)
         case GR_OPT_QUAD:
             if (value < 0 || value >= (1ll << 29)) return set_err(GR_E_INVALID, "quad limit must be 0 .. 2^29 - 1");
             j->quad_max = uint64_t(value);
+            return GR_OK;
+        case GR_OPT_TILE:
+            if (value != 0 && (value < 1024 || value > 65536))
+                return set_err(GR_E_INVALID, "tile bytes must be 0 or 1024 .. 65536");
+            j->tile_bytes = uint32_t(value);
+            return GR_OK;
+        case GR_OPT_TILE_HALO:
+            if (value < 0 || value > 8) return set_err(GR_E_INVALID, "tile halo must be 0 .. 8");
+            j->tile_halo = uint32_t(value);
             return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown option %u", option);
     }
@@ -1635,6 +1670,28 @@ static bool use_quad(const gr_job * j, const gr::StepParams & P, int spec, uint3
     return spec >= 0 && mode == gr::MODE_PAIRED && P.wind.pair32 && j->quad_max && j->run_n <= j->quad_max;
 }
 
+// A 3D time-paired launch runs the field-tile kernel when the job asks for a tile, the launch is
+// not a quad one, the padded grid's node coordinates fit 16 bits (the box reduction packs them)
+// and the interpreter's columns (rounded up to 16 bytes) plus the tile fit 64 KiB of LDS. Then
+// sets P.tile_* and raises *lds (the launch's dynamic LDS) to cover the tile area.
+static bool tile_setup(const gr_job * j, gr::StepParams & P, uint32_t mode, bool quad, uint32_t block,
+                       uint32_t * lds)
+{
+    if (!j->tile_bytes || quad || mode != gr::MODE_PAIRED || j->d.dim != 3 || P.wind.nx + 3 > 0xffffu ||
+        P.wind.ny + 3 > 0xffffu || P.wind.nz + 3 > 0xffffu)
+        return false;
+    const uint32_t lds_cols = j->d.nprog ? j->expr_cols : 0u;
+    const uint32_t off = (lds_cols * block * uint32_t(sizeof(float)) + 15u) & ~15u;
+    if (off + j->tile_bytes > 65536u) return false;
+    const uint32_t nb = P.wind.dtype == GR_F32 ? 32u : 16u; // bytes per 3D paired node
+    P.tile_off = off;
+    P.tile_cap = (j->tile_bytes - gr::TILE_AREA_HEADER) / nb;
+    P.tile_halo = j->tile_halo;
+    P.tile_stats = j->ctx->dtile;
+    *lds = std::max(*lds, off + j->tile_bytes);
+    return true;
+}
+
 extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
 {
     if (!j || !w || !w->wind) return set_err(GR_E_INVALID, "gr_job_advance: null argument");
@@ -1665,10 +1722,12 @@ extern "C" int gr_job_advance(gr_job * j, const gr_window * w)
     const bool narrow = j->reuse || j->run_n <= cus * 4 * 2 * 64;
     const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
     const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
-    const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, j->ctx->lds_per_cu);
     const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
+    const bool quad = use_quad(j, P, spec, mode);
+    uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, j->ctx->lds_per_cu);
+    const bool tile = tile_setup(j, P, mode, quad, blk, &lds);
     if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, j->ctx->nxcd, j->run_n, wide, narrow,
-                           use_quad(j, P, spec, mode), lds, s) != 0)
+                           quad, lds, s, tile) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     hipEvent_t done = nullptr;
     if ((rc = kt_mark(j->ctx, s, true, &done)) != GR_OK) return rc;
@@ -1704,10 +1763,12 @@ extern "C" int gr_job_advance_noalloc(gr_job * j, const gr_window * w)
     const bool narrow = j->reuse || j->run_n <= cus * 4 * 2 * 64;
     const uint32_t lds_cols = j->d.nprog ? (mode == gr::MODE_FULL ? gr::EXPR_REF_SLOTS : 0u) + j->expr_cols : 0u;
     const uint32_t blk = (j->block == 0 || j->block > 256) ? 256u : j->block;
-    const uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, ctx->lds_per_cu);
     const int spec = j->specialize ? gr::lean_spec(P, mode) : -1;
+    const bool quad = use_quad(j, P, spec, mode);
+    uint32_t lds = gr::launch_lds_bytes(blk, lds_cols, j->cu_waves, ctx->lds_per_cu);
+    const bool tile = tile_setup(j, P, mode, quad, blk, &lds);
     if (gr::launch_advance(P, j->d.dim, j->d.integrator, mode, spec, blk, ctx->nxcd, j->run_n, wide, narrow,
-                           use_quad(j, P, spec, mode), lds, s) != 0)
+                           quad, lds, s, tile) != 0)
         return set_err(GR_E_HIP, "step kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
     GR_HIP(hipEventRecord(ctx->noalloc_done, s));
     j->timed = false;

@@ -1,5 +1,5 @@
 // Internal data contract between the host engine (gr_capi.cpp) and the gfx950 kernels
-// (gr_kernels.hip). Replaces the by-value SimulationData kernel argument of the reference
+// (gr_step.h, compiled in the gr_k_*.hip translation units). Replaces the by-value SimulationData kernel argument of the reference
 // (source/granite/SimulationData.hpp:21-70) with a POD of raw device pointers and uniform
 // scalars; fields are node-interleaved arrays instead of texture objects.
 #pragma once
@@ -192,7 +192,17 @@ struct StepParams
 
     // integrated particle-steps (steps begun alive) are added here, or null
     unsigned long long * work;
+
+    // field tile of the tiled time-paired kernel (k_advance_tile): byte offset of its area in the
+    // workgroup's dynamic LDS (after the interpreter's columns; 16-byte aligned), the paired-copy
+    // nodes it holds, and the cells added on every side of the workgroup's stage-1 cells
+    uint32_t tile_off, tile_cap, tile_halo;
+    // workgroup-steps with a tile (slot element 0) and without one (element 1), counted in
+    // WORK_SLOTS slots WORK_STRIDE apart like the work counter; null: not counted
+    unsigned long long * tile_stats;
 };
+// bytes at the start of the tile area before its nodes (the box reduction's words)
+enum : uint32_t { TILE_AREA_HEADER = 128 };
 
 // LDS of the expression interpreter, in columns of one float per lane of the workgroup: the
 // full kernel stages reference slots 4..15 (EXPR_REF_SLOTS columns), and every kernel holds the
@@ -283,9 +293,11 @@ inline uint32_t launch_lds_bytes(uint32_t block, uint32_t lds_cols, uint32_t cu_
 // lds_bytes: as for launch_batch
 // quad: the launch runs a quad of lanes per particle (k_advance_quad; spec >= 0 and the paired
 // mode only, fewer than 2^29 particles)
+// tile: the 3D time-paired kernel with the workgroup's field tile (k_advance_tile; P.tile_* set,
+// lds_bytes covering the tile area; not with quad)
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode, int spec,
                    uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, bool quad,
-                   uint32_t lds_bytes, void * stream);
+                   uint32_t lds_bytes, void * stream, bool tile = false);
 // Scratch of the no-allocation entry points (gr_k_noalloc.hip): NOALLOC_MAX_STEPS steps of time
 // table in a file-scope device array. noalloc_setup takes its address on the current device
 // (once per device, at context creation); noalloc_scratch returns it (null before setup).

@@ -12,9 +12,11 @@ AdvanceFn pick_advance_f32(uint32_t dim, uint32_t integrator, uint32_t mode, int
 
 int launch_advance(const StepParams & P, uint32_t dim, uint32_t integrator, uint32_t mode, int spec,
                    uint32_t block, uint32_t nxcd, uint64_t nrun, bool wide, bool narrow, bool quad,
-                   uint32_t lds_bytes, void * stream)
+                   uint32_t lds_bytes, void * stream, bool tile)
 {
-    AdvanceFn fn = P.wind.dtype == GR_F16    ? pick_advance_f16(dim, integrator, mode, spec, wide, narrow, quad)
+    AdvanceFn fn = tile ? (dim == 3 && mode == M_PAIRED && !quad ? pick_advance_tile(P.wind.dtype, integrator, spec)
+                                                                 : nullptr)
+                   : P.wind.dtype == GR_F16  ? pick_advance_f16(dim, integrator, mode, spec, wide, narrow, quad)
                    : P.wind.dtype == GR_BF16 ? pick_advance_bf16(dim, integrator, mode, spec, wide, narrow, quad)
                                              : pick_advance_f32(dim, integrator, mode, spec, wide, narrow, quad);
     if (!fn) return -1;

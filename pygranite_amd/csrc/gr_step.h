@@ -604,6 +604,160 @@ __device__ __forceinline__ void decode_pair_node(const uint32_t (&w)[StencilCach
     }
 }
 
+// --------------------------------------------------------------------------------------------
+// field tile (k_advance_tile)
+// --------------------------------------------------------------------------------------------
+// Every step, a workgroup of neighbouring (cell-ordered) particles stages in LDS the box of
+// time-paired nodes that covers the stencils of its live particles: the padded base nodes of
+// their stage-1 samples, widened by tile_halo cells on every side (the later RK4 stages move a
+// fraction of a cell), plus the +1 corner. A stencil inside the box reads its corners from LDS;
+// any other (a stage that left the box, or a step whose box holds more than tile_cap nodes)
+// loads them from the paired copy as the untiled kernels do. The words are the same either
+// way, so results do not depend on the tile. The box keeps the copy's node layout
+// (pair_node_bytes per node), x fastest.
+//
+// LDS area at tile_off: two sets (step parity) of four 32-bit words per wavefront for the box
+// reduction, then the nodes.
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x2_t lds_u2;
+typedef __attribute__((address_space(3))) u32x4_t lds_u4;
+enum : uint32_t { TILE_SLOT_BYTES = 2 * 4 * 4 * 4, TILE_MAX_WAVES = 4 };
+static_assert(TILE_SLOT_BYTES == TILE_AREA_HEADER, "tile area header");
+
+struct TileBox
+{
+    uint32_t lo[3];  // first padded node of the box on each axis
+    uint32_t ext[3]; // nodes on each axis; {1, 1, 1} when the step has no tile (nothing fits)
+};
+
+// the raw words of a node of the LDS box (same words as load_pair_node_at)
+template <int DIM, class T>
+__device__ __forceinline__ void lds_pair_node(const lds_char * q, uint32_t (&w)[StencilCache<DIM, T>::NW])
+{
+    if constexpr (sizeof(T) == 4 && DIM == 3)
+    {
+        const u32x4_t a = *(const lds_u4 *)q;
+        const u32x2_t c = *(const lds_u2 *)(q + 16);
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = c.x; w[5] = c.y;
+    }
+    else if constexpr (sizeof(T) == 2 && DIM == 2)
+    {
+        const u32x2_t h = *(const lds_u2 *)q;
+        w[0] = h.x;
+        w[1] = h.y;
+    }
+    else
+    {
+        const u32x4_t a = *(const lds_u4 *)q;
+#pragma unroll
+        for (int k = 0; k < StencilCache<DIM, T>::NW; ++k) w[k] = k == 0 ? a.x : k == 1 ? a.y : k == 2 ? a.z : a.w;
+    }
+}
+
+// per-half minimum of two packed 16-bit values
+__device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b)
+{
+    const uint32_t lo = min(a & 0xffffu, b & 0xffffu), hi = min(a >> 16, b >> 16);
+    return lo | (hi << 16);
+}
+
+// Forms this step's box and stages it (every thread of the workgroup calls it, with the same
+// `parity` = step & 1). Returns false when no particle of the workgroup is alive (the workgroup
+// is done). `live`: the lane's particle is alive at the start of the step; p: its position.
+template <int DIM, class T, int SPEC>
+__device__ __forceinline__ bool tile_step(const StepParams & P, const void * pair, const float * p, bool live,
+                                          uint32_t parity, TileBox & tb)
+{
+    constexpr uint32_t NB = pair_node_bytes<DIM, T>();
+    lds_char * area = (lds_char *)gr_dyn_lds + P.tile_off;
+    lds_u32 * slots = (lds_u32 *)area + parity * (TILE_MAX_WAVES * 4);
+    lds_char * nodes = area + TILE_SLOT_BYTES;
+    const uint32_t n[3] = {P.wind.nx, P.wind.ny, P.wind.nz};
+    // per axis, packed: low half the smallest padded base node of the lane's wavefront, high half
+    // 0xffff minus the largest (the minimum of both halves over lanes gives the range)
+    uint32_t m[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    if (live)
+    {
+        float arg[3] = {0.f, 0.f, 0.f};
+        texel_arg<DIM, SPEC>(P, p, arg);
+        const uint32_t wmode = Feat<SPEC>::weights(P);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+        {
+            int i;
+            float w;
+            split(arg[d], n[d], wmode, i, w);
+            const uint32_t b = uint32_t(i + 1); // in [0, n + 1]
+            m[d] = b | ((0xffffu - b) << 16);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[d] = min_u16x2(m[d], uint32_t(__shfl_xor(int(m[d]), o, 64)));
+    const uint32_t wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63u) >> 6;
+    if ((threadIdx.x & 63u) == 0)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) slots[wave * 4 + d] = m[d];
+    __syncthreads();
+    uint32_t r[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu};
+    for (uint32_t k = 0; k < nwaves; ++k)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) r[d] = min_u16x2(r[d], slots[k * 4 + d]);
+    if ((r[0] & 0xffffu) + (r[0] >> 16) > 0xffffu) return false; // smallest > largest: none alive
+    uint32_t vol = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        if (d < DIM)
+        {
+            const uint32_t lo = r[d] & 0xffffu, hi = 0xffffu - (r[d] >> 16), last = n[d] + 2;
+            const uint32_t a = lo > P.tile_halo ? lo - P.tile_halo : 0u;
+            const uint32_t b = min(hi + 1u + P.tile_halo, last);
+            tb.lo[d] = a;
+            tb.ext[d] = b - a + 1u;
+        }
+        else
+        {
+            tb.lo[d] = 0;
+            tb.ext[d] = 1;
+        }
+        vol *= tb.ext[d];
+    }
+    const bool fits = vol <= P.tile_cap;
+    if (P.tile_stats && threadIdx.x == 0)
+        atomicAdd(P.tile_stats + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE + (fits ? 0 : 1), 1ull);
+    if (!fits)
+    {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) tb.ext[d] = 1; // no tile this step: every stencil loads globally
+        return true;
+    }
+    // stage the box: consecutive threads copy consecutive nodes of a row (coalesced)
+    const uint32_t PX = n[0] + 3, PY = DIM == 3 ? n[1] + 3 : 1u;
+    const char * src = static_cast<const char *>(pair);
+    for (uint32_t idx = threadIdx.x; idx < vol; idx += blockDim.x)
+    {
+        const uint32_t row = idx / tb.ext[0], x = idx - row * tb.ext[0];
+        const uint32_t z = row / tb.ext[1], y = row - z * tb.ext[1];
+        const uint64_t node = (uint64_t(tb.lo[2] + z) * PY + (tb.lo[1] + y)) * PX + (tb.lo[0] + x);
+        const char * s = src + node * NB;
+        lds_char * dst = nodes + idx * NB;
+        if constexpr (NB == 8)
+            *(lds_u2 *)dst = GR_LOAD(u32x2_t, s, 0);
+        else
+        {
+            *(lds_u4 *)dst = GR_LOAD(u32x4_t, s, 0);
+            if constexpr (NB == 32) *(lds_u2 *)(dst + 16) = GR_LOAD(u32x2_t, s + 16, 0); // not the pad
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
 // Value of lane D of the calling lane's quad (lanes 4k .. 4k + 3), one DPP move.
 template <int D> __device__ __forceinline__ float quad_lane(float v)
 {
@@ -615,9 +769,11 @@ template <int D> __device__ __forceinline__ float quad_lane(float v)
 // values at the eight (four) corners, one lerp tree -- and the components are exchanged inside
 // the quad. The same operations on the same values as the one-lane form, a quarter of the lerps
 // per lane: for launches of few particles, whose time is one lane's step chain.
-template <int DIM, class T, int MODE, bool CACHE, int SPEC, int Q = 1>
+// TL: the workgroup's field tile `tb` (tile_step) serves the stencils inside it.
+template <int DIM, class T, int MODE, bool CACHE, int SPEC, int Q = 1, bool TL = false>
 __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, const float * x,
-                                         float t, float * v, f16_t one, StencilCache<DIM, T> & C)
+                                         float t, float * v, f16_t one, StencilCache<DIM, T> & C,
+                                         const TileBox & tb)
 {
     float arg[3] = {0.f, 0.f, 0.f};
     texel_arg<DIM, SPEC>(P, x, arg);
@@ -693,7 +849,21 @@ __device__ __forceinline__ void velocity(const StepParams & P, const Win & W, co
         }
         if (!CACHE || base != C.base) // divergent: lanes whose cell is unchanged load nothing
         {
-            if (P.wind.pair32) // uniform: byte offsets of the copy fit 32 bits
+            // tile coordinates of the base node (unsigned: below the box wraps above it)
+            const uint32_t ti = uint32_t(i + 1) - tb.lo[0], tj = uint32_t(j + 1) - tb.lo[1],
+                           tk = uint32_t(k + 1) - tb.lo[2];
+            if (TL && ti < tb.ext[0] - 1u && tj < tb.ext[1] - 1u && (DIM == 2 || tk < tb.ext[2] - 1u))
+            {
+                constexpr uint32_t NB = pair_node_bytes<DIM, T>();
+                const uint32_t EX = tb.ext[0] * NB, EXY = EX * tb.ext[1];
+                const lds_char * q0 = (const lds_char *)gr_dyn_lds + P.tile_off + TILE_SLOT_BYTES +
+                                      ((tk * tb.ext[1] + tj) * tb.ext[0] + ti) * NB;
+#pragma unroll
+                for (int c = 0; c < StencilCache<DIM, T>::NCOR; ++c)
+                    lds_pair_node<DIM, T>(q0 + ((c & 1) ? NB : 0u) + ((c & 2) ? EX : 0u) + ((c & 4) ? EXY : 0u),
+                                          C.w[c]);
+            }
+            else if (P.wind.pair32) // uniform: byte offsets of the copy fit 32 bits
             {
                 constexpr uint32_t NB = pair_node_bytes<DIM, T>();
                 const char * pb = static_cast<const char *>(W.pair);
@@ -814,7 +984,10 @@ __device__ __forceinline__ float global_coord(const StepParams & P, float x, int
 
 // Advances one particle by the launch's steps; returns the number of steps it integrated
 // (steps begun alive, the aborting step included).
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1>
+// TL (k_advance_tile): every thread of the workgroup runs the step loop, also for a stopped
+// particle or none (pid >= n), because the field tile is staged by the whole workgroup each step;
+// such lanes only take part in the staging and store nothing.
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1, bool TL = false>
 __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint32_t pid)
 {
     // quad launches: every lane of the quad keeps the particle's state, the first one stores it
@@ -835,36 +1008,39 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     // A stopped particle's state never changes again (tpp:452-455 copies it unchanged) and its
     // saved columns after the stop are left to the reader, which fills them from the final
     // state: nothing to do.
-    int32_t status = G(P.status)[pid];
-    if (status != GR_STATUS_TIME) return 0;
+    int32_t status = (TL && pid >= P.n) ? int32_t(GR_STATUS_DOMAIN) : G(P.status)[pid];
+    const bool entered = status == GR_STATUS_TIME;
+    if (!TL && !entered) return 0;
+    // TL: a lane without a live particle reads particle 0's state (in bounds) and never stores
+    const uint32_t lp = (TL && !entered) ? 0u : pid;
     f16_t one = f16_t(1.0f);
     asm volatile("" : "+v"(one)); // opaque 1.0 for the mixed-precision lerps (lerp_h)
 
     float p[3] = {0.f, 0.f, 0.f}, q[3] = {0.f, 0.f, 0.f}, cmp[3] = {0.f, 0.f, 0.f};
     float ld[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) p[d] = G(P.p[d])[pid];
+    for (int d = 0; d < DIM; ++d) p[d] = G(P.p[d])[lp];
     const bool curv_on = curvm != GR_CURV_OFF;
     const bool curv_pos = curvm == GR_CURV_TOTAL || curvm == GR_CURV_INDIVIDUAL_AND_TOTAL;
     if (curv_pos)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) q[d] = G(P.q[d])[pid];
+        for (int d = 0; d < DIM; ++d) q[d] = G(P.q[d])[lp];
     if (curvm == GR_CURV_FAST_TOTAL)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) ld[d] = G(P.ld[d])[pid];
+        for (int d = 0; d < DIM; ++d) ld[d] = G(P.ld[d])[lp];
     if (kahan)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) cmp[d] = G(P.c[d])[pid];
-    int32_t abort_step = G(P.abort_step)[pid];
-    float total = curv_on ? G(P.total)[pid] : 0.f;
-    float curv = curvm == GR_CURV_INDIVIDUAL_AND_TOTAL ? G(P.curv)[pid] : 0.f;
-    float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? G(P.len)[pid] : 0.f;
+        for (int d = 0; d < DIM; ++d) cmp[d] = G(P.c[d])[lp];
+    int32_t abort_step = G(P.abort_step)[lp];
+    float total = curv_on ? G(P.total)[lp] : 0.f;
+    float curv = curvm == GR_CURV_INDIVIDUAL_AND_TOTAL ? G(P.curv)[lp] : 0.f;
+    float len = (FULL && P.abort_mode != GR_ABORT_TIME) ? G(P.len)[lp] : 0.f;
     float ex[GR_MAX_PROGRAMS];
 #pragma unroll
-    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) ex[e] = e < int(P.nprog) ? G(P.expr[e])[pid] : 0.f;
+    for (int e = 0; e < GR_MAX_PROGRAMS; ++e) ex[e] = e < int(P.nprog) ? G(P.expr[e])[lp] : 0.f;
     float vo[GR_MAX_VOLUMES];
 #pragma unroll
-    for (int k = 0; k < GR_MAX_VOLUMES; ++k) vo[k] = (FULL && k < int(P.nvol)) ? G(P.vol[k])[pid] : 0.f;
+    for (int k = 0; k < GR_MAX_VOLUMES; ++k) vo[k] = (FULL && k < int(P.nvol)) ? G(P.vol[k])[lp] : 0.f;
 
     const float dt = P.dt;
     const float half_dt = 0.5f * dt;
@@ -879,6 +1055,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
     float sb_prev = 0.f;
     StencilCache<DIM, T> sc;
     sc.base = 0xffffffffu; // no stencil loaded (also after every window change)
+    TileBox tb = {{0u, 0u, 0u}, {1u, 1u, 1u}};
     for (uint32_t s = 0; s < P.nsteps; ++s)
     {
         if (s == edge) // next window: slices c+1, c+2 (wave-uniform branch)
@@ -891,6 +1068,9 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             edge = wi + 1 < P.nwin ? GR_CONST(uint32_t, P.win_edge, wi) : 0xffffffffu;
             sc.base = 0xffffffffu;
         }
+        if constexpr (TL)
+            if (!tile_step<DIM, T, SPEC>(P, W.pair, p, status == GR_STATUS_TIME, s & 1u, tb))
+                break; // no live particle left in the workgroup (the same decision in every lane)
         const bool live = status == GR_STATUS_TIME; // alive at the start of step s
         if (live)
         {
@@ -908,22 +1088,22 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                 const float t1 = tt.y, t2 = tt.z;
                 float k1[3] = {0.f, 0.f, 0.f}, k2[3] = {0.f, 0.f, 0.f}, k3[3] = {0.f, 0.f, 0.f}, k4[3] = {0.f, 0.f, 0.f};
                 float x[3] = {0.f, 0.f, 0.f};
-                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, p, t, k1, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q, TL>(P, W, p, t, k1, one, sc, tb);
                 float k1c[3] = {k1[0], k1[1], k1[2]};
                 if (sph) sphere(P, p[1], k1c, k1);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k1[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t1, k2, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q, TL>(P, W, x, t1, k2, one, sc, tb);
                 float k2c[3] = {k2[0], k2[1], k2[2]};
                 if (sph) sphere(P, x[1], k2c, k2);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k2[d] * half_dt;
-                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t1, k3, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q, TL>(P, W, x, t1, k3, one, sc, tb);
                 float k3c[3] = {k3[0], k3[1], k3[2]};
                 if (sph) sphere(P, x[1], k3c, k3);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) x[d] = p[d] + k3[d] * dt;
-                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, x, t2, k4, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q, TL>(P, W, x, t2, k4, one, sc, tb);
                 float k4c[3] = {k4[0], k4[1], k4[2]};
                 if (sph) sphere(P, x[1], k4c, k4);
 #pragma unroll
@@ -935,7 +1115,7 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             }
             else
             {
-                velocity<DIM, T, MODE, CACHE, SPEC, Q>(P, W, p, t, vc, one, sc);
+                velocity<DIM, T, MODE, CACHE, SPEC, Q, TL>(P, W, p, t, vc, one, sc, tb);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) v[d] = vc[d];
                 if (sph) sphere(P, p[1], vc, v);
@@ -1193,10 +1373,10 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
             col = col + 1 == P.ring_cap ? 0u : col + 1;
             next_save += P.save_interval;
         }
-        if (status != GR_STATUS_TIME) break; // stopped: nothing more to compute or save
+        if (!TL && status != GR_STATUS_TIME) break; // stopped: nothing more to compute or save
     }
 
-    if (!writer) return 0;
+    if (!writer || (TL && !entered)) return 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) G(P.p[d])[pid] = p[d];
     if (curv_pos)
@@ -1246,13 +1426,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks, uint
     return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1>
+template <int DIM, int INTEG, class T, int MODE, bool CACHE = false, int SPEC = -1, int Q = 1, bool TL = false>
 __device__ __forceinline__ void advance_block(const StepParams & P, uint32_t nxcd)
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, nxcd);
     const uint32_t pid = (b * blockDim.x + threadIdx.x) / Q; // a quad per particle when Q = 4
     uint32_t w = 0;
-    if (pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE, SPEC, Q>(P, pid);
+    if (TL || pid < P.n) w = advance_one<DIM, INTEG, T, MODE, CACHE, SPEC, Q, TL>(P, pid);
     if (P.work) // block sizes are multiples of 64
         wave_add_work(P.work + size_t(blockIdx.x % WORK_SLOTS) * WORK_STRIDE, w);
 }
