@@ -675,7 +675,7 @@ def atmosphere_workload(capi, ctx, args):
         f.close()
     if not args.no_error:
         import oracle
-        sub = 256
+        sub = min(args.atmo_error_particles, n)
         wind_h = wind_t.cpu().numpy()
         del wind_t
         st = dict(Space=0, Integrator=1, DeltaT=float(dt), DataTimeDistance=3600.0,
@@ -800,7 +800,9 @@ def serving_workload(capi, ctx, args):
     1-4); sizes as in the batched configuration (particles log-uniform [1e3, 4e6], steps
     uniform [100, 5000]), RK4:Euler 3:1, dt 60 s; seeds uniform in x-y and log-uniform in height,
     made on the device at admission. Jobs are admitted by HBM budget and advanced in rounds of
-    the persistent batched kernel. Prints one JSON line with p50/p99 latency and throughput."""
+    the persistent batched kernel. Prints one JSON line with p50/p99 latency and throughput; with
+    --serve-sweep, one run per listed rate on the same resident datasets and the saturation rate
+    (the highest listed rate whose queue drained within 5 % of the arrival window)."""
     import torch
     from pygranite_amd import serving, synthetic
     dev = torch.device("cuda")
@@ -819,43 +821,59 @@ def serving_workload(capi, ctx, args):
                float(zlev[-1])]
         ds[f"d{k}"] = serving.Dataset(f, gs, dom, 3600.0, S, zlev)
     torch.cuda.empty_cache()
-    arr = serving.poisson_arrivals(4, args.serve_rate, args.serve_duration)
-    u = synthetic.uniform01(5, 5 * len(arr)).reshape(len(arr), 5)
-    reqs = []
-    for i, t in enumerate(arr):
-        n = int(round(math.exp(math.log(1e3) + u[i, 0] * (math.log(args.batch_pmax) - math.log(1e3)))))
-        dsn = f"d{int(u[i, 1] * 4)}"
-        dom, z1 = ds[dsn].domain, float(ds[dsn].zlevels[1])
+    workload = ("4 datasets 256x256x96 FP16 x 48 slices; Poisson arrivals; particles "
+                f"log-uniform [1e3, {args.batch_pmax:.0e}], steps uniform [100, {args.batch_smax}]")
 
-        def seed_fn(n=n, dom=dom, z1=z1, s=int(u[i, 4] * 2**31)):
-            g = torch.Generator(device=dev)
-            g.manual_seed(s)
-            r = torch.rand((n, 3), generator=g, device=dev, dtype=torch.float64)
-            xyz = torch.empty((n, 3), device=dev, dtype=torch.float32)
-            xyz[:, 0] = (r[:, 0] * dom[0]).float()
-            xyz[:, 1] = (r[:, 1] * dom[1]).float()
-            xyz[:, 2] = (z1 * torch.exp(r[:, 2] * math.log(dom[2] / z1))).float()
-            return xyz
+    def one_rate(rate, duration):
+        arr = serving.poisson_arrivals(4, rate, duration)
+        u = synthetic.uniform01(5, 5 * len(arr)).reshape(len(arr), 5)
+        reqs = []
+        for i, t in enumerate(arr):
+            n = int(round(math.exp(math.log(1e3) + u[i, 0] * (math.log(args.batch_pmax) - math.log(1e3)))))
+            dsn = f"d{int(u[i, 1] * 4)}"
+            dom, z1 = ds[dsn].domain, float(ds[dsn].zlevels[1])
 
-        reqs.append(serving.Request(rid=i, arrival=float(t), dataset=dsn, seeds=None, n=n,
-                                    seed_fn=seed_fn, steps=int(100 + u[i, 2] * (args.batch_smax - 100)),
-                                    integrator=0 if u[i, 3] < 0.25 else 1, dt=60.0))
-    offered = sum(r.n * r.steps for r in reqs) / args.serve_duration
-    print(f"# serving: {len(reqs)} requests over {args.serve_duration} s, offered "
-          f"{offered:.3e} nominal particle-steps/s", file=sys.stderr, flush=True)
-    srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum,
-                         reorder_every=args.serve_reorder_every)
-    ctx.take_work()
-    srv.run(reqs, progress=lambda m: print("#   " + m, file=sys.stderr, flush=True))
-    rep = srv.report()
-    rep["integrated_particle_steps"] = ctx.take_work()
-    rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]
-    rep.update(arrival_rate_per_s=args.serve_rate, duration_s=args.serve_duration,
-               regroup_every_rounds=args.serve_reorder_every,
-               offered_nominal_particle_steps_per_s=offered,
-               workload="4 datasets 256x256x96 FP16 x 48 slices; Poisson arrivals; particles "
-                        f"log-uniform [1e3, {args.batch_pmax:.0e}], steps uniform [100, {args.batch_smax}]")
-    print(json.dumps(rep), flush=True)
+            def seed_fn(n=n, dom=dom, z1=z1, s=int(u[i, 4] * 2**31)):
+                g = torch.Generator(device=dev)
+                g.manual_seed(s)
+                r = torch.rand((n, 3), generator=g, device=dev, dtype=torch.float64)
+                xyz = torch.empty((n, 3), device=dev, dtype=torch.float32)
+                xyz[:, 0] = (r[:, 0] * dom[0]).float()
+                xyz[:, 1] = (r[:, 1] * dom[1]).float()
+                xyz[:, 2] = (z1 * torch.exp(r[:, 2] * math.log(dom[2] / z1))).float()
+                return xyz
+
+            reqs.append(serving.Request(rid=i, arrival=float(t), dataset=dsn, seeds=None, n=n,
+                                        seed_fn=seed_fn, steps=int(100 + u[i, 2] * (args.batch_smax - 100)),
+                                        integrator=0 if u[i, 3] < 0.25 else 1, dt=60.0))
+        offered = sum(r.n * r.steps for r in reqs) / duration
+        print(f"# serving: lambda {rate:g}/s, {len(reqs)} requests over {duration} s, offered "
+              f"{offered:.3e} nominal particle-steps/s", file=sys.stderr, flush=True)
+        srv = serving.Server(ctx, ds, hbm_budget=int(args.serve_budget_gb * (1 << 30)), quantum=args.serve_quantum,
+                             reorder_every=args.serve_reorder_every)
+        ctx.take_work()
+        srv.run(reqs, progress=lambda m: print("#   " + m, file=sys.stderr, flush=True))
+        rep = srv.report()
+        rep["integrated_particle_steps"] = ctx.take_work()
+        rep["integrated_particle_steps_per_s"] = rep["integrated_particle_steps"] / rep["wall_s"]
+        rep.update(arrival_rate_per_s=rate, duration_s=duration,
+                   regroup_every_rounds=args.serve_reorder_every,
+                   offered_nominal_particle_steps_per_s=offered,
+                   drain_after_last_arrival_s=rep["wall_s"] - float(arr[-1]) if len(arr) else 0.0,
+                   kept_up=bool(rep["wall_s"] <= 1.05 * duration))
+        return rep
+
+    if args.serve_sweep:
+        rates = [float(x) for x in args.serve_sweep.split(",")]
+        reps = [one_rate(r, args.serve_duration) for r in rates]
+        ok = [r["arrival_rate_per_s"] for r in reps if r["kept_up"]]
+        print(json.dumps({"workload": workload, "runs": reps, "saturation_rate_per_s": max(ok) if ok else None,
+                          "saturation_rule": "highest swept rate whose run ended within 5 % of its arrival window"}),
+              flush=True)
+    else:
+        rep = one_rate(args.serve_rate, args.serve_duration)
+        rep["workload"] = workload
+        print(json.dumps(rep), flush=True)
     for d in ds.values():
         d.field.close()
 
@@ -1032,6 +1050,8 @@ def main():
                     help="e2e workload: particles of the ABC run with every step saved")
     ap.add_argument("--e2e-no-atmosphere", action="store_true")
     ap.add_argument("--atmo-particles", type=int, default=1 << 24)
+    ap.add_argument("--atmo-error-particles", type=int, default=4096,
+                    help="particles of the atmosphere workload checked against the FP64 oracle")
     ap.add_argument("--batch-block", type=int, default=0, help="batched kernel workgroup size (0: default)")
     ap.add_argument("--batch-waves", type=int, default=-1, help="batched kernel register budget (0 or 4)")
     ap.add_argument("--batch-group", type=int, default=0,
@@ -1046,6 +1066,8 @@ def main():
     ap.add_argument("--batch-pmax", type=float, default=4e6)
     ap.add_argument("--batch-smax", type=int, default=5000)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-sweep", default="",
+                    help="comma-separated arrival rates: one serving run each and the saturation rate")
     ap.add_argument("--serve-duration", type=float, default=120.0)
     ap.add_argument("--serve-quantum", type=int, default=250)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
