@@ -620,7 +620,7 @@ def atmosphere_workload(capi, ctx, args):
     z = synthetic.loguniform_heights(22, n, float(zlev[1]), float(zlev[-1]))
     seeds = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
     expr = "w*1000/(abs(u)+abs(v)+1)"
-    dt, chunk = np.float32(60.0), args.chunk
+    dt, chunk = np.float32(60.0), args.chunk or 120  # steps per launch (two 60-step windows)
     total_steps = 1440
     out = {"workload": f"atmosphere {nx}x{ny}x{nz} stretched z, {S} slices, {n} particles, RK4 dt 60 s, "
                        f"{total_steps} steps, Kahan + FP64 time, expression saved every step"}
@@ -752,8 +752,13 @@ def batch_workload(capi, ctx, args):
     out = {"workload": f"{nj} jobs (particles log-uniform [1e3, {pmax:.0e}], steps uniform [100, "
                        f"{smax}], half 2D double gyre / half 3D ABC, Euler:RK4 1:3)",
            "nominal_particle_steps": int(sum(x[2] * x[3] for x in specs))}
-    for form in ("batched_fp16", "sequential_plain_fp32"):
+    # the sequential plain launches run first, with the context's default launch options (the
+    # autotuner's choice for the batched form is not theirs)
+    for form in ("sequential_plain_fp32", "batched_fp16"):
         dtype = capi.F16 if form == "batched_fp16" else capi.F32
+        # plain launches sample slots separately and never reuse stencils: they run the batched
+        # kernel's build without the stencil registers (more wavefronts per SIMD)
+        ctx.set_option(capi.CTX_BATCH_REUSE, 1 if form == "batched_fp16" else 0)
         f2, f3 = make_field2(capi, ctx, g2, dtype), make_field(capi, ctx, g3, dtype)
         if form == "batched_fp16" and not args.no_autotune:
             tune = ctx.autotune(f3, desc(3, 1), n=65536, njobs=16, steps=50, trials=1)
@@ -783,7 +788,11 @@ def batch_workload(capi, ctx, args):
         work = ctx.take_work()
         kms, nl = ctx.take_kernel_ms()
         out[form] = {"wall_s": wall, "device_s": kms / 1e3, "launches": nl,
-                     "integrated_particle_steps": work, "integrated_particle_steps_per_s": work / wall}
+                     "integrated_particle_steps": work, "integrated_particle_steps_per_s": work / wall,
+                     "launch_options": {"block": ctx.get_option(capi.CTX_BATCH_BLOCK),
+                                        "group": ctx.get_option(capi.CTX_BATCH_GROUP),
+                                        "cu_waves": ctx.get_option(capi.CTX_BATCH_CU_WAVES),
+                                        "reuse": ctx.get_option(capi.CTX_BATCH_REUSE)}}
         print(f"#   {form}: {wall:.2f} s wall, {kms / 1e3:.2f} s device, {work / wall:.3e} "
               f"integrated particle-steps/s", file=sys.stderr, flush=True)
         for j, _, _ in jobs:

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -180,9 +181,14 @@ struct gr_field
     float zlut_z0{0.f}, zlut_scale{0.f};
 };
 
+// serial numbers of jobs: a job created where a closed one lived gets a new number, so caches
+// keyed by job (the batched launch's item list) never mistake one for the other
+static std::atomic<uint64_t> g_job_serial{0};
+
 struct gr_job
 {
     gr_ctx * ctx{nullptr};
+    uint64_t serial{++g_job_serial};
     gr_job_desc d{};
     uint64_t n{0};
     uint32_t full{0};
@@ -2066,9 +2072,11 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
         for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls_of[i];
-        key.insert(key.end(), {i, uint64_t(uintptr_t(j)), j->run_n, uint64_t(uintptr_t(spans[i].wind)),
-                               uint64_t(spans[i].wind->d.dtype), cls_of[i],
-                               uint64_t(int64_t(j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1))});
+        // the item variant (dimension, integrator, storage type, feature set) and the job's serial
+        const int var = gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype,
+                                          j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1);
+        key.insert(key.end(), {i, j->serial, j->run_n, uint64_t(uintptr_t(spans[i].wind)), cls_of[i],
+                               uint64_t(int64_t(var))});
     }
     const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
     if (items_dirty)

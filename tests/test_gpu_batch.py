@@ -502,3 +502,37 @@ def test_reorder_with_wide_keys_changes_nothing(gpu_available):
     assert any((s != 0).any() for _, s, _ in b)  # particles stopped (z or y edges), ranges shrank
     f.close()
     ctx.close()
+
+
+def test_new_job_in_a_closed_jobs_place_gets_its_own_items(gpu_available):
+    """A batched launch reuses its item list when its spans are unchanged. A job created after
+    another was closed (and possibly at the same address) with the same particle count and field
+    but a different integrator or dimension must not run the closed job's item variant: an RK4
+    job after an Euler job of the same size equals the RK4 job run alone on a fresh context."""
+    wind, gs = synthetic.abc_flow(24, 4)
+    seeds = synthetic.uniform_seeds(31, 3000, [0.05] * 3, [2 * math.pi - 0.05] * 3)
+
+    def field(ctx):
+        f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=4)
+        for sl in range(4):
+            f.upload(sl, [wind[sl, c] for c in range(3)])
+        return f
+
+    ctx = capi.Context(0)
+    f = field(ctx)
+    for integ in (0, 1, 0, 1):
+        job = capi.Job(ctx, make_desc(3, integ, gs, (24, 24, 24), 0.01), seeds)
+        ctx.batch_run([(job, f, dict(max_steps=60))], timed=False)
+        pos = job.read(capi.OUT_POSITIONS)
+        job.close()
+    f.close()
+    ctx.close()
+    ctx2 = capi.Context(0)
+    f2 = field(ctx2)
+    ref = capi.Job(ctx2, make_desc(3, 1, gs, (24, 24, 24), 0.01), seeds)
+    ctx2.batch_run([(ref, f2, dict(max_steps=60))], timed=False)
+    want = ref.read(capi.OUT_POSITIONS)
+    ref.close()
+    f2.close()
+    ctx2.close()
+    np.testing.assert_array_equal(pos, want)
