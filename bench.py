@@ -912,7 +912,9 @@ def e2e_workload(args):
     wind, gs = synthetic.abc_flow(128, 16)
     n = args.e2e_particles
     seeds = np.ascontiguousarray(synthetic.uniform_seeds(1, 1 << 20, [0.0] * 3, [2 * math.pi] * 3)[:n])
-    for storage in ("Float16", "Float32"):
+    cases = [("abc_float16", "Float16", None), ("abc_float32", "Float32", None),
+             ("abc_float32_regroup_edges_only", "Float32", 0)]
+    for name, storage, regroup in cases:
         st = pg.IntegratorSettings()
         st.Space, st.Integrator = pg.Space.Space3D, pg.Integrator.ClassicRungeKutta
         st.DeltaT, st.DataTimeDistance, st.GridScale, st.Offset = 0.005, 1.0, gs, [0.0, 0.0, 0.0]
@@ -920,6 +922,8 @@ def e2e_workload(args):
         st.CurvatureMode = pg.CurvatureMode.IndividualAndTotalCurvature
         st.AdditionalCompute = ["sqrt(u*u+v*v+w*w)"]
         st.WindfieldStorage = getattr(pg.StorageType, storage)
+        if regroup is not None:
+            st.RegroupInterval = regroup  # regroup at window edges only
         t0 = time.perf_counter()
         integ = pg.TrajectoryIntegrator(st, Loader(wind), pg.TrajectorySet(seeds))
         t1 = time.perf_counter()
@@ -931,8 +935,9 @@ def e2e_workload(args):
         t3 = time.perf_counter()
         nbytes = traj.nbytes + curv.nbytes + ex.nbytes
         work = integrated(res, 2000)
-        out[f"abc_{storage.lower()}"] = {
+        out[name] = {
             "particles": n, "steps": 2000, "saved_columns": int(traj.shape[1]),
+            "regroup_interval": int(st.RegroupInterval),
             "output_bytes": int(nbytes), "construct_s": t1 - t0, "compute_s": t2 - t1,
             "device_s": res.deviceTime(), "export_s": t3 - t2,
             "integrated_particle_steps": work,
@@ -940,7 +945,7 @@ def e2e_workload(args):
             "compute_particle_steps_per_s": work / (t2 - t1),
             "kernel_only_particle_steps_per_s": work / max(1e-9, res.deviceTime()),
             "output_GB_per_s_end_to_end": nbytes / (t3 - t0) / 1e9}
-        print(f"#   e2e ABC {storage}: construct {t1 - t0:.2f} s, compute {t2 - t1:.2f} s (device "
+        print(f"#   e2e {name}: construct {t1 - t0:.2f} s, compute {t2 - t1:.2f} s (device "
               f"{res.deviceTime():.3f} s), export {t3 - t2:.2f} s, {nbytes / 1e9:.1f} GB out",
               file=sys.stderr, flush=True)
         del traj, curv, ex, res, integ

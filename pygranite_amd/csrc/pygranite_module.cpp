@@ -107,7 +107,10 @@ struct IntegratorSettings
     bool SortParticles{true}; // Morton-reorder particle state at window edges (results unchanged)
     // and also every RegroupInterval steps inside a window while at least RegroupMinLive particles
     // are live (0: window edges only); launches of many live particles gather faster when a
-    // wavefront's particles are neighbours, and they drift apart within a window
+    // wavefront's particles are neighbours, and they drift apart within a window. Applies to runs
+    // that save no columns (SaveInterval 0): with saves, compute() is bound by the column reads
+    // and the extra chunks only add work (ABC, 2^18 particles, every step saved: 0.41 s with
+    // in-window regrouping, 0.29 s without)
     uint32_t RegroupInterval{64};
     uint64_t RegroupMinLive{65536};
     // stretched z axis (3D): height of every z node relative to Offset.Z, strictly increasing
@@ -822,7 +825,7 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
             // chunks of RegroupInterval steps, each followed by a regrouping, while many
             // particles are live
             bool regroup = false;
-            if (s_.SortParticles && s_.RegroupInterval && n >= 4096 && !s_.MinimumAliveParticles)
+            if (s_.SortParticles && s_.RegroupInterval && SI == 0 && n >= 4096 && !s_.MinimumAliveParticles)
             {
                 uint64_t live = 0;
                 check(gr_job_live_range(job.j, &live));
