@@ -1265,7 +1265,7 @@ def main():
             "kernel": ("k_batch (RK4 3D " + ("FP16" if "fp16" in head else "FP32 + Kahan") + " items)")
                       if head.endswith("_batched") else
                       "k_advance (RK4 3D " + ("FP16" if "fp16" in head else "FP32" + (" + Kahan" if head in KAHAN_FORMS else ""))
-                      + ", time-paired; narrow and four-wavefront builds)",
+                      + ", time-paired; stencil-reuse and quad builds)",
             "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
             "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
             "algorithmic_bytes_per_launch": ps_per_launch * B16}
