@@ -1091,8 +1091,9 @@ def main():
                     help="skip the batched-kernel autotuner (default: tune before the batched form)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path shards by "
-              "particles with no exchange during integration; see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path would shard by "
+              "particles (each GPU holding the window's slices and N/G particles, no exchange "
+              "during integration), and no run on several GPUs is measured here", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "copy":
