@@ -120,7 +120,7 @@ def make_field(capi, ctx, wind, dtype):
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
                regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False, quad=-1, tile=0,
-               tile_halo=1):
+               tile_halo=1, edge_min=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
@@ -157,9 +157,9 @@ def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder
             job.advance(w)
             sch.advance(k, edge, t)
             left -= k
-            if edge and reorder:
-                job.reorder()  # regroup by cell at every window edge (inside the timing)
-                regrouped = True
+            if edge and reorder and job.live_range() >= edge_min:
+                job.reorder()  # regroup by cell at window edges (inside the timing) while at
+                regrouped = True  # least edge_min particles are live
         st["count"] += 1
         if reorder and reorder_every and not regrouped and st["count"] % reorder_every == 0 and st["large"]:
             if job.live_range() >= regroup_min:
@@ -213,7 +213,7 @@ def median_run(runs):
 
 
 def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder_every=0, regroup_min=0,
-                total=2000, repeats=1, kahan=False):
+                total=2000, repeats=1, kahan=False, edge_min=0):
     parts = np.array_split(np.arange(seeds.shape[0]), njobs)
 
     def new_jobs():
@@ -242,8 +242,8 @@ def run_batched(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, njobs, reorder
         if (reorder_every and state["large"] and state["launches"] % reorder_every == 0
                 and sum(j.live_range() for j in jobs) < regroup_min):
             state["large"] = False
-        if e["window"] != state["window"] or (reorder_every and state["large"]
-                                               and state["launches"] % reorder_every == 0):
+        edge = e["window"] != state["window"] and sum(j.live_range() for j in jobs) >= edge_min
+        if edge or (reorder_every and state["large"] and state["launches"] % reorder_every == 0):
             ctx.reorder(jobs)  # every job regrouped by cell in one pass: at window edges, and
             # every reorder_every launches while the live range is large
         state.update(window=e["window"], time=e["time"], time64=e["time64"], counter=e["counter"])
@@ -1041,6 +1041,8 @@ def main():
     ap.add_argument("--reorder-every", type=int, default=1,
                     help="regroup particles by cell every K launches as well as at window edges "
                          "(0: at window edges only)")
+    ap.add_argument("--edge-regroup-min", type=int, default=4096,
+                    help="regroup at window edges only while at least this many particles are live")
     ap.add_argument("--regroup-min", type=int, default=65536,
                     help="regroup every K launches only while the live range (all jobs) holds at "
                          "least this many particles; smaller launches are latency-bound")
@@ -1173,26 +1175,26 @@ def main():
             kw["cu_waves"] = args.cu_waves
             kw["quad"] = args.quad
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
-                           args.reorder_every, args.regroup_min, **kw)
+                           args.reorder_every, args.regroup_min, edge_min=args.edge_regroup_min, **kw)
         elif name in ("fp32_tile", "fp16_tile"):
             kw["cu_waves"] = args.cu_waves
             kw["quad"] = args.quad
             r = run_single(capi, ctx, f32 if name == "fp32_tile" else f16, seeds, gs, n, chunk, args.steps,
                            args.warmup, True, True, args.reorder_every, args.regroup_min,
-                           tile=args.tile_bytes, tile_halo=args.tile_halo, **kw)
+                           tile=args.tile_bytes, tile_halo=args.tile_halo, edge_min=args.edge_regroup_min, **kw)
         elif name in ("fp16", "fp32k"):
             kw["cu_waves"] = args.cu_waves
             kw["reuse"] = args.reuse
             kw["quad"] = args.quad
             r = run_single(capi, ctx, f16 if name == "fp16" else f32, seeds, gs, n, chunk, args.steps,
                            args.warmup, True, True, args.reorder_every, args.regroup_min,
-                           kahan=name == "fp32k", **kw)
+                           kahan=name == "fp32k", edge_min=args.edge_regroup_min, **kw)
         elif name in ("fp16_batched", "fp32k_batched"):
             if not fixed:
                 autotune(name)
             r = run_batched(capi, ctx, f16 if name == "fp16_batched" else f32, seeds, gs, n, chunk, args.steps,
                             args.warmup, args.jobs, args.reorder_every, args.regroup_min,
-                            kahan=name == "fp32k_batched", **kw)
+                            kahan=name == "fp32k_batched", edge_min=args.edge_regroup_min, **kw)
         else:
             raise SystemExit(f"unknown form {name}")
         r["options"] = {"block": ctx.get_option(capi.CTX_BATCH_BLOCK), "group": ctx.get_option(capi.CTX_BATCH_GROUP),
@@ -1320,6 +1322,7 @@ def main():
                    "particles": args.particles, "grid": n, "steps_done": hv["steps_done"],
                    "steps_per_launch": chunk, "jobs": args.jobs if head.endswith("_batched") else 1,
                    "regroup_every_launches": args.reorder_every, "regroup_min_live": args.regroup_min,
+                   "edge_regroup_min_live": args.edge_regroup_min,
                    "batch_block": hv["options"]["block"], "batch_waves": hv["options"]["waves"],
                    "batch_group": hv["options"]["group"], "batch_cu_waves": hv["options"]["cu_waves"],
                    "single_job_cu_waves": args.cu_waves},

@@ -810,8 +810,14 @@ std::unique_ptr<TrajectorySet> TrajectoryIntegrator::compute()
         w.time_front64 = double(window + 1) * double(s_.DataTimeDistance);
 
         // particles moved during the last window: regroup them by cell (all saved columns
-        // have been read at this point)
-        if (s_.SortParticles && n >= 4096) check(gr_job_reorder(job.j));
+        // have been read at this point) while at least 4,096 are live -- fewer are stepped by a
+        // handful of wavefronts whose time is their own step chain, not their gathers
+        if (s_.SortParticles && n >= 4096)
+        {
+            uint64_t live = 0;
+            check(gr_job_live_range(job.j, &live));
+            if (live >= 4096) check(gr_job_reorder(job.j));
+        }
 
         // run the window in chunks (each fits the column ring); with MinimumAliveParticles the
         // alive count gates every single step (tpp:973-976)
