@@ -120,10 +120,12 @@ def make_field(capi, ctx, wind, dtype):
 
 def run_single(capi, ctx, f, seeds, gs, n, chunk, steps, warmup, paired, reorder, reorder_every=0,
                regroup_min=0, total=2000, repeats=1, cu_waves=0, reuse=0, kahan=False, quad=-1, tile=0,
-               tile_halo=1, edge_min=0):
+               tile_halo=1, edge_min=0, block=0):
     def new_job():
         j = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=chunk)
         j.set_option(capi.OPT_PAIRED, 1 if paired else 0)
+        if block:
+            j.set_launch(block)
         if quad >= 0:
             j.set_option(capi.OPT_QUAD, quad)
         if tile:
@@ -1163,6 +1165,21 @@ def main():
               file=sys.stderr, flush=True)
         tune[name] = {"candidates": t, "chosen": chosen}
 
+    def autotune_single(name, f, kahan):
+        """Launch configuration of a single-job form from the single-job autotuner (workgroup
+        size x lanes per particle x LDS field tile) on a probe job of the form's description
+        and particle count over the form's own field, before any timing of the form."""
+        probe = capi.Job(ctx, build_desc(capi, gs, n, kahan=kahan), seeds, save_capacity=1)
+        t = probe.autotune(f, steps=min(chunk, 100), trials=2)
+        probe.close()
+        best = min(t, key=lambda c: c["ms"])
+        print(f"# autotune {name}: " + ", ".join(f"b{c['block']}/l{c['lanes']}/t{c['tile_bytes'] // 1024}k "
+                                                f"{c['ms']:.3f} ms" for c in t)
+              + f" -> block {best['block']}, lanes {best['lanes']}, tile {best['tile_bytes']}",
+              file=sys.stderr, flush=True)
+        tune[name] = {"candidates": t, "chosen": best}
+        return dict(block=best["block"], tile=best["tile_bytes"], quad=(1 << 29) - 1 if best["lanes"] == 4 else -1)
+
     res = {}
     for name in forms:
         reps = args.repeats
@@ -1174,6 +1191,8 @@ def main():
         elif name == "fp32_local":
             kw["cu_waves"] = args.cu_waves
             kw["quad"] = args.quad
+            if not args.no_autotune:
+                kw.update(autotune_single(name, f32, False))
             r = run_single(capi, ctx, f32, seeds, gs, n, chunk, args.steps, args.warmup, True, True,
                            args.reorder_every, args.regroup_min, edge_min=args.edge_regroup_min, **kw)
         elif name in ("fp32_tile", "fp16_tile"):
@@ -1186,6 +1205,8 @@ def main():
             kw["cu_waves"] = args.cu_waves
             kw["reuse"] = args.reuse
             kw["quad"] = args.quad
+            if not args.no_autotune:
+                kw.update(autotune_single(name, f16 if name == "fp16" else f32, name == "fp32k"))
             r = run_single(capi, ctx, f16 if name == "fp16" else f32, seeds, gs, n, chunk, args.steps,
                            args.warmup, True, True, args.reorder_every, args.regroup_min,
                            kahan=name == "fp32k", edge_min=args.edge_regroup_min, **kw)

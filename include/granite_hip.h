@@ -429,6 +429,27 @@ int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_desc * desc, u
                       uint32_t njobs, uint32_t steps, uint32_t trials, gr_tune_entry * results,
                       uint32_t max_results, uint32_t * nresults);
 
+/* Autotuner of single-job launches (gr_job_advance; no counterpart in the reference, which
+ * launches one fixed configuration, TrajectoryIntegrator.tpp:790-796). Times every candidate --
+ * workgroup size 64 / 128 / 256 x lanes per particle 1 / 4 (GR_OPT_QUAD) x LDS field tile
+ * 0 / 16 / 32 KiB (GR_OPT_TILE; 3D time-paired launches with one lane per particle) -- on a
+ * probe job of `job`'s description with min(n, 2^20) seeds spread over the domain (regrouped by
+ * cell), advanced `steps` steps through `wind`'s slots 0 and 1 from time 0 (best of `trials`
+ * runs each, all from identical state), and sets the fastest candidate's options on `job`:
+ * its block size, its tile, and, when four lanes per particle won, a quad limit covering the
+ * whole job (one lane keeps the job's limit, which serves its last few live particles). The
+ * job's particle state is not touched; results do not depend on the options. Jobs with
+ * uplift, constants or additional volumes are refused (the probe has no per-particle data).
+ * results[0 .. *nresults) receive every candidate's device time (may be NULL). */
+typedef struct {
+    uint32_t block;
+    uint32_t lanes;      /* lanes per particle: 1 or 4 */
+    uint32_t tile_bytes; /* 0: no field tile */
+    float ms;
+} gr_job_tune_entry;
+int gr_job_autotune(gr_job * job, gr_field * wind, uint32_t steps, uint32_t trials,
+                    gr_job_tune_entry * results, uint32_t max_results, uint32_t * nresults);
+
 #ifdef __cplusplus
 }
 #endif

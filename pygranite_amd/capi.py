@@ -34,6 +34,7 @@ EXPORTS = [
     "gr_job_read_saved_async", "gr_job_read_wait",
     "gr_job_count_alive", "gr_job_live_range", "gr_job_saved_columns", "gr_job_last_advance_ms", "gr_job_set_launch",
     "gr_job_set_option", "gr_job_reorder", "gr_jobs_reorder", "gr_batch_run", "gr_batch_autotune",
+    "gr_job_autotune",
 ]
 OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE, OPT_REUSE, OPT_QUAD = 1, 2, 3, 4, 5, 6
 OPT_TILE, OPT_TILE_HALO = 7, 8
@@ -107,6 +108,11 @@ class TuneEntry(ctypes.Structure):
                 ("ms", ctypes.c_float), ("cu_waves", ctypes.c_uint32)]
 
 
+class JobTuneEntry(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("lanes", ctypes.c_uint32), ("tile_bytes", ctypes.c_uint32),
+                ("ms", ctypes.c_float)]
+
+
 _lib = None
 
 
@@ -163,6 +169,7 @@ def lib():
         "gr_jobs_reorder": ([P(vp), u32], i32),
         "gr_batch_run": ([vp, P(Span), u32, u32, P(SpanEnd), P(ctypes.c_float)], i32),
         "gr_batch_autotune": ([vp, vp, P(JobDesc), u32, u32, u32, u32, P(TuneEntry), u32, P(u32)], i32),
+        "gr_job_autotune": ([vp, vp, u32, u32, P(JobTuneEntry), u32, P(u32)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -401,6 +408,16 @@ class Job:
 
     def set_option(self, option: int, value: int):
         _check(lib().gr_job_set_option(self.h, option, value))
+
+    def autotune(self, field, steps: int = 50, trials: int = 2):
+        """Runs the single-job autotuner (gr_job_autotune) on a probe job of this job's
+        description over `field`; sets the fastest block size, lanes per particle and field tile
+        on this job and returns every candidate's device time."""
+        res = (JobTuneEntry * 64)()
+        cnt = ctypes.c_uint32(0)
+        _check(lib().gr_job_autotune(self.h, field.h, steps, trials, res, 64, ctypes.byref(cnt)))
+        return [{"block": r.block, "lanes": r.lanes, "tile_bytes": r.tile_bytes, "ms": r.ms}
+                for r in res[:cnt.value]]
 
     def reorder(self):
         _check(lib().gr_job_reorder(self.h))

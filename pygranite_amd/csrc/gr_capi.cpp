@@ -2767,3 +2767,119 @@ extern "C" int gr_batch_autotune(gr_ctx * ctx, gr_field * wind, const gr_job_des
     if (nresults) *nresults = nres;
     return rc2;
 }
+
+// --------------------------------------------------------------------------------------------
+// autotuner of single-job launches
+// --------------------------------------------------------------------------------------------
+
+// Runs every candidate launch configuration of gr_job_advance (workgroup size x lanes per
+// particle x LDS field tile) on one probe job with the caller's description, from identical
+// state each time, and keeps the fastest as the job's options.
+extern "C" int gr_job_autotune(gr_job * job, gr_field * wind, uint32_t steps, uint32_t trials,
+                               gr_job_tune_entry * results, uint32_t max_results, uint32_t * nresults)
+{
+    if (!job || !wind) return set_err(GR_E_INVALID, "gr_job_autotune: null argument");
+    if (steps == 0) return set_err(GR_E_INVALID, "gr_job_autotune: steps must be > 0");
+    const gr_job_desc & d = job->d;
+    if (d.uplift_mode || d.constants_mode || d.nconst || d.nvol || d.volume_mode)
+        return set_err(GR_E_INVALID, "gr_job_autotune: jobs with uplift, constants or volumes are not tuned");
+    if (d.windfield_mode == 1 && wind->d.nslots < 2)
+        return set_err(GR_E_INVALID, "gr_job_autotune: a dynamic wind field needs two slots");
+    if (trials == 0) trials = 2;
+    gr_ctx * ctx = job->ctx;
+    GR_HIP(hipSetDevice(ctx->device));
+    int rc;
+    // park the context's measurement state (kernel time, work and tile counters)
+    if ((rc = kt_fold(ctx)) != GR_OK) return rc;
+    const double kms_saved = ctx->kms_acc;
+    const uint64_t kl_saved = ctx->klaunch_acc;
+    const size_t nctr = size_t(gr::WORK_SLOTS) * gr::WORK_STRIDE;
+    std::vector<unsigned long long> work_saved(nctr), tile_saved(nctr);
+    GR_HIP(hipMemcpyAsync(work_saved.data(), ctx->dwork, nctr * 8, hipMemcpyDeviceToHost, ctx->compute));
+    GR_HIP(hipMemcpyAsync(tile_saved.data(), ctx->dtile, nctr * 8, hipMemcpyDeviceToHost, ctx->compute));
+    GR_HIP(hipStreamSynchronize(ctx->compute));
+
+    // probe seeds: a deterministic low-discrepancy spread over the domain (inset by 1e-3)
+    const uint32_t dim = d.dim;
+    const uint32_t n = uint32_t(std::min<uint64_t>(job->n, 1ull << 20));
+    std::vector<float> seeds(size_t(n) * dim);
+    const double alpha[3] = {0.7548776662466927, 0.5698402909980532, 0.4142135623730950};
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < dim; ++k)
+        {
+            double u = std::fmod(0.5 + alpha[k] * double(i + 1), 1.0);
+            u = 1e-3 + u * (1.0 - 2e-3);
+            seeds[size_t(i) * dim + k] = float(d.offset[k] + u * double(d.domain[k]));
+        }
+
+    struct Cand { uint32_t block, lanes, tile; };
+    std::vector<Cand> cands;
+    const bool tile_ok = dim == 3 && d.windfield_mode == 1 && job->paired;
+    for (uint32_t b : {64u, 128u, 256u})
+    {
+        cands.push_back(Cand{b, 4u, 0u});
+        for (uint32_t t : {0u, 16384u, 32768u})
+            if (t == 0 || tile_ok) cands.push_back(Cand{b, 1u, t});
+    }
+    gr_window w;
+    std::memset(&w, 0, sizeof(w));
+    w.wind = wind;
+    w.wind_back = 0;
+    w.wind_front = d.windfield_mode == 1 ? 1 : 0;
+    w.time_back = 0.f;
+    w.time_front = d.dtd;
+    w.time_back64 = 0.0;
+    w.time_front64 = double(d.dtd);
+    w.counter_start = 0;
+    w.nsteps = steps;
+    float best = 3.4e38f;
+    size_t best_i = 0;
+    uint32_t nres = 0;
+    for (size_t ci = 0; ci < cands.size() && rc == GR_OK; ++ci)
+    {
+        float tmin = 3.4e38f;
+        for (uint32_t t = 0; t < trials && rc == GR_OK; ++t)
+        {
+            gr_job * pj = nullptr;
+            rc = gr_job_create(ctx, &d, seeds.data(), n, steps, &pj);
+            if (rc != GR_OK) break;
+            pj->paired = job->paired;
+            pj->specialize = job->specialize;
+            pj->reuse = job->reuse;
+            pj->cu_waves = job->cu_waves;
+            pj->tile_halo = job->tile_halo;
+            pj->block = cands[ci].block;
+            pj->quad_max = cands[ci].lanes == 4 ? uint64_t(n) : 0u;
+            pj->tile_bytes = cands[ci].tile;
+            if (n >= 4096) rc = gr_job_reorder(pj);
+            float ms = 0.f;
+            if (rc == GR_OK) rc = gr_job_advance(pj, &w);
+            if (rc == GR_OK) rc = gr_job_last_advance_ms(pj, &ms);
+            if (rc == GR_OK) tmin = std::min(tmin, ms);
+            const int rd = gr_job_destroy(pj);
+            if (rc == GR_OK) rc = rd;
+        }
+        if (rc != GR_OK) break;
+        if (results && nres < max_results)
+            results[nres++] = gr_job_tune_entry{cands[ci].block, cands[ci].lanes, cands[ci].tile, tmin};
+        if (tmin < best)
+        {
+            best = tmin;
+            best_i = ci;
+        }
+    }
+    // restore the measurement state; set the winner (nothing on failure)
+    int rc2 = kt_fold(ctx);
+    ctx->kms_acc = kms_saved;
+    ctx->klaunch_acc = kl_saved;
+    if (hipMemcpyAsync(ctx->dwork, work_saved.data(), nctr * 8, hipMemcpyHostToDevice, ctx->compute) != hipSuccess ||
+        hipMemcpyAsync(ctx->dtile, tile_saved.data(), nctr * 8, hipMemcpyHostToDevice, ctx->compute) != hipSuccess ||
+        hipStreamSynchronize(ctx->compute) != hipSuccess)
+        rc2 = set_err(GR_E_HIP, "gr_job_autotune: restoring the counters failed");
+    if (rc != GR_OK) return rc;
+    job->block = cands[best_i].block;
+    job->tile_bytes = cands[best_i].tile;
+    if (cands[best_i].lanes == 4) job->quad_max = std::max<uint64_t>(job->quad_max, job->n);
+    if (nresults) *nresults = nres;
+    return rc2;
+}

@@ -536,3 +536,48 @@ def test_new_job_in_a_closed_jobs_place_gets_its_own_items(gpu_available):
     f2.close()
     ctx2.close()
     np.testing.assert_array_equal(pos, want)
+
+
+def test_single_job_autotune_sets_a_candidate_and_changes_nothing(gpu_available):
+    """gr_job_autotune times every single-job launch configuration (workgroup size x lanes per
+    particle x LDS field tile) on a probe job, sets the fastest on the job and leaves the job's
+    state, the context's work counter and its tile statistics as they were; a tuned job's
+    results equal an untuned twin's bit for bit. Jobs with per-particle data are refused."""
+    wind, gs = synthetic.abc_flow(24, 4)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 24, 24, 24, 3, capi.F16, nslots=4)
+    for sl in range(4):
+        f.upload(sl, [wind[sl, c] for c in range(3)])
+    seeds = synthetic.uniform_seeds(41, 12000, [0.05] * 3, [2 * math.pi - 0.05] * 3)
+    d = make_desc(3, 1, gs, (24, 24, 24), 0.01, curvature=3, exprs=("sqrt(u*u+v*v+w*w)",), save_interval=1)
+    tuned, twin = (capi.Job(ctx, d, seeds, save_capacity=100) for _ in range(2))
+    ctx.take_work()
+    ctx.take_tile_stats()
+    cands = tuned.autotune(f, steps=40, trials=1)
+    assert len(cands) == 12  # 3 workgroup sizes x (4 lanes, or 1 lane with tiles of 0 / 16 / 32 KiB)
+    assert {(c["block"], c["lanes"], c["tile_bytes"]) for c in cands} == \
+        {(b, 4, 0) for b in (64, 128, 256)} | {(b, 1, t) for b in (64, 128, 256) for t in (0, 16384, 32768)}
+    assert all(c["ms"] > 0 for c in cands)
+    assert ctx.take_work() == 0 and ctx.take_tile_stats() == (0, 0)
+    outs = []
+    for j in (tuned, twin):
+        j.reorder()
+        w = capi.Window()
+        w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+        w.time_back, w.time_front, w.time_back64, w.time_front64 = 0.0, 1.0, 0.0, 1.0
+        w.time_start = w.time_start64 = 0.0
+        w.counter_start, w.nsteps = 0, 100
+        j.advance(w)
+        outs.append([j.read_saved(capi.OUT_POSITIONS, 0, 100), j.read_saved(capi.OUT_EXPR0, 0, 100),
+                     j.read(capi.OUT_STATUS), j.read(capi.OUT_TOTAL_CURVATURE)])
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    dv = make_desc(3, 1, gs, (24, 24, 24), 0.01)
+    dv.uplift_mode = 1
+    withlift = capi.Job(ctx, dv, seeds[:100])
+    with pytest.raises(capi.GraniteError):
+        withlift.autotune(f, steps=10, trials=1)
+    for j in (tuned, twin, withlift):
+        j.close()
+    f.close()
+    ctx.close()
