@@ -1603,6 +1603,16 @@ static int prepare_advance(gr_job * j, const gr_window * w, gr::StepParams & P, 
     for (uint32_t e = 0; e < d.nprog; ++e)
         if (gr::lower_program(d.prog[e], P.prog[e], P.refmask) != GR_OK)
             return set_err(GR_E_INVALID, "program %u is malformed", e);
+    for (uint32_t e = 0; e < d.nprog; ++e)
+        if (P.prog[e].n == 1 && ((P.prog[e].code[0] >> 4) & 0xfu) == gr::DC_DOT) P.dotmask |= 1u << e;
+    if (P.dotmask & 1u)
+    {
+        // slots 4 5 6 are u v w: arg holds the slot pairs (4, 4), (5, 5) [, (6, 6)], 3 bits each
+        const uint32_t w = P.prog[0].code[0], terms = w & 3u, arg = w >> 8;
+        const uint32_t uvw = 4u | 4u << 3 | 5u << 6 | 5u << 9 | 6u << 12 | 6u << 15;
+        const uint32_t mask = terms == 3 ? 0x3ffffu : 0xfffu;
+        if ((terms == 3 || d.dim == 2) && (arg & mask) == (uvw & mask)) P.dotmask |= gr::DOT_UVW;
+    }
 
     hipStream_t s = j->ctx->compute;
     // wait for the slices this window reads

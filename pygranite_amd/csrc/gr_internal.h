@@ -121,6 +121,10 @@ struct StepParams
     uint32_t weights;    // gr_weights
     uint32_t nprog, nvol, nconst;
     uint32_t refmask; // reference-vector slots read by the programs (bit per slot)
+    // bit e: program e is a single DOT instruction (a sum of two or three products of register
+    // slots, optionally under a square root), evaluated without the interpreter's dispatch loop;
+    // DOT_UVW: program 0's DOT is u*u + v*v [+ w*w] (the squared speed, in that order)
+    uint32_t dotmask;
 
     float dt, sphere_radius, max_length;
     // 1 / (2 dt) and 1 / (dt dt) rounded to FP64, the two float denominators (2.f * dt, dt * dt)
@@ -201,6 +205,7 @@ struct StepParams
     // WORK_SLOTS slots WORK_STRIDE apart like the work counter; null: not counted
     unsigned long long * tile_stats;
 };
+enum : uint32_t { DOT_UVW = 1u << 16 };
 // bytes at the start of the tile area before its nodes (the box reduction's words)
 enum : uint32_t { TILE_AREA_HEADER = 128 };
 

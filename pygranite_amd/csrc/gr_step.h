@@ -333,6 +333,18 @@ __device__ __forceinline__ float apply_op(uint32_t k, float a, float b)
 // eight would be expanded into a chain of eight compares and selects per read.
 typedef float refvec_t __attribute__((ext_vector_type(9)));
 
+// A program that is one DOT instruction, (r0 * r1) + (r2 * r3) [+ (r4 * r5)] over register
+// slots r_k = arg >> 3k & 7 [then its square root]: the instruction run_program would execute,
+// without the dispatch loop (same operations, same result).
+__device__ __forceinline__ float run_dot(uint32_t w, const refvec_t rv)
+{
+    const uint32_t sub = w & 0xfu, arg = w >> 8;
+    float x = (rv[arg & 7u] * rv[(arg >> 3) & 7u]) + (rv[(arg >> 6) & 7u] * rv[(arg >> 9) & 7u]);
+    if ((sub & 3) == 3) x = x + (rv[(arg >> 12) & 7u] * rv[(arg >> 15) & 7u]);
+    if (sub & 4) x = sqrtf(x);
+    return x;
+}
+
 // rv: reference slots x y z cv u v w of the step; R: this lane's LDS reference column (volumes
 // and constants, full kernel only)
 template <bool FULL>
@@ -1336,8 +1348,17 @@ __device__ __forceinline__ uint32_t advance_one(const StepParams & P, const uint
                     }
                     const refvec_t rv = {gx, gy, gz, curvm == GR_CURV_INDIVIDUAL_AND_TOTAL ? curv : 0.f,
                                          vc[0], vc[1], DIM == 3 ? vc[2] : 0.f, 0.f, 0.f};
-                    for (uint32_t e = 0; e < P.nprog; ++e)
-                        ex[e] = run_program<FULL>(P.prog[e], rv, R, s_stk, lane, bs);
+                    if (P.nprog == 1 && (P.dotmask & DOT_UVW)) // the squared speed [square root]
+                    {
+                        float x = (vc[0] * vc[0]) + (vc[1] * vc[1]);
+                        if (DIM == 3) x = x + (vc[2] * vc[2]);
+                        ex[0] = (P.prog[0].code[0] & 4u) ? sqrtf(x) : x;
+                    }
+                    else if (P.nprog == 1 && (P.dotmask & 1u)) // another single-DOT program
+                        ex[0] = run_dot(P.prog[0].code[0], rv);
+                    else
+                        for (uint32_t e = 0; e < P.nprog; ++e)
+                            ex[e] = run_program<FULL>(P.prog[e], rv, R, s_stk, lane, bs);
                 }
 
                 if (kahan)

@@ -139,3 +139,18 @@ def test_regrouping_inside_windows_changes_nothing(pg, gpu_available, save_inter
     sub = {k: (v[:64] if isinstance(v, np.ndarray) else v) for k, v in regrouped.items()}
     assert_bitexact(sub, ref, fields=("trajectories", "curvatures") if save_interval else ("trajectories",),
                     label="regrouped ")
+
+
+@pytest.mark.parametrize("expr", ["sqrt(u*u+v*v+w*w)", "u*u+v*v+w*w", "x*u+y*v+z*w", "sqrt(v*v+u*u+w*w)",
+                                  "u*u+v*v"])
+def test_single_dot_programs_bitexact(pg, gpu_available, expr):
+    """Programs of one DOT instruction skip the interpreter's dispatch loop (the squared speed
+    u*u + v*v [+ w*w] reads the velocity registers directly): every saved value equals the FP32
+    oracle's evaluation bit for bit, as the interpreted programs do."""
+    st, wind, gs = c1_settings(steps=120)
+    st = dict(st, AdditionalCompute=[expr])
+    seeds = synthetic.uniform_seeds(23, 3000, [0.0] * 3, [2 * math.pi] * 3)
+    gpu = run_product(pg, st, wind, seeds)
+    ref = oracle.run(st, wind, seeds, real64=False)
+    assert_bitexact(gpu, ref, fields=("trajectories",), label=expr + " ")
+    np.testing.assert_array_equal(gpu["compute"][0], ref["compute"][0].astype(np.float32))
