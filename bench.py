@@ -1278,6 +1278,8 @@ def main():
         cpu = cpu_baseline(wind, gs)
         cpu["cpu_model"] = cpu_model()
     counters = load_counters(head)
+    # the measured counterpart of the published peak: a 1 GiB device-to-device streaming copy
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 5)
 
     plain = res.get("plain_fp32", {}).get("psps")
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -1290,6 +1292,7 @@ def main():
                       if head.endswith("_batched") else
                       "k_advance (RK4 3D " + ("FP16" if "fp16" in head else "FP32" + (" + Kahan" if head in KAHAN_FORMS else ""))
                       + ", time-paired; stencil-reuse and quad builds)",
+            "peak_measured_copy_GBs": copy_gbs,
             "bytes_per_particle_step": B16, "avg_launch_ms": hv["avg_launch_ms"],
             "launches": hv["launches"], "integrated_particle_steps_per_launch": ps_per_launch,
             "algorithmic_bytes_per_launch": ps_per_launch * B16}

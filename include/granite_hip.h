@@ -100,6 +100,11 @@ int gr_ctx_take_tile_stats(gr_ctx * ctx, uint64_t * tiled, uint64_t * untiled);
  * (gr_job_advance, gr_batch_run) since the previous call, then resets both. Waits for the last
  * launch; launches are not serialised by the timing. No reference counterpart. */
 int gr_ctx_take_kernel_ms(gr_ctx * ctx, double * ms, uint64_t * launches);
+/* Device-to-device copy bandwidth of the context's GPU in GB/s (bytes read plus bytes written per
+ * second), the best of `trials` copies of `bytes` (a multiple of 16) between two buffers by a
+ * grid-stride streaming kernel: the measured counterpart of the published HBM peak. No
+ * counterpart in the reference. Allocates and frees the two buffers. */
+int gr_ctx_copy_bandwidth(gr_ctx * ctx, uint64_t bytes, uint32_t trials, double * gbs);
 /* device name and HBM bytes of the context's GPU */
 int gr_ctx_info(gr_ctx * ctx, char * name, size_t name_len, uint64_t * hbm_bytes,
                 int * compute_units);

@@ -25,7 +25,7 @@ EXPR_ALLOW_ABS = 1
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = [
     "gr_abi_version", "gr_last_error", "gr_device_count", "gr_ctx_create", "gr_ctx_destroy",
-    "gr_ctx_synchronize", "gr_ctx_info", "gr_ctx_take_work", "gr_ctx_take_tile_stats", "gr_ctx_take_kernel_ms", "gr_ctx_set_option",
+    "gr_ctx_synchronize", "gr_ctx_info", "gr_ctx_take_work", "gr_ctx_take_tile_stats", "gr_ctx_copy_bandwidth", "gr_ctx_take_kernel_ms", "gr_ctx_set_option",
     "gr_ctx_get_option", "gr_field_create", "gr_field_destroy",
     "gr_field_upload", "gr_field_upload_device", "gr_field_wait", "gr_field_bytes",
     "gr_field_set_z_levels",
@@ -136,6 +136,7 @@ def lib():
         "gr_ctx_info": ([vp, ctypes.c_char_p, ctypes.c_size_t, P(u64), P(i32)], i32),
         "gr_ctx_take_work": ([vp, P(u64)], i32),
         "gr_ctx_take_tile_stats": ([vp, P(u64), P(u64)], i32),
+        "gr_ctx_copy_bandwidth": ([vp, u64, u32, P(ctypes.c_double)], i32),
         "gr_ctx_take_kernel_ms": ([vp, P(ctypes.c_double), P(u64)], i32),
         "gr_ctx_set_option": ([vp, u32, ctypes.c_int64], i32),
         "gr_ctx_get_option": ([vp, u32, P(ctypes.c_int64)], i32),
@@ -312,6 +313,12 @@ class Context:
         ms, n = ctypes.c_double(0), ctypes.c_uint64(0)
         _check(lib().gr_ctx_take_kernel_ms(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, trials: int = 5) -> float:
+        """Device-to-device copy bandwidth in GB/s (read + write), best of `trials`."""
+        v = ctypes.c_double()
+        _check(lib().gr_ctx_copy_bandwidth(self.h, nbytes, trials, ctypes.byref(v)))
+        return v.value
 
     def take_tile_stats(self):
         """(workgroup-steps with a field tile, without one) since the last call (resets both)."""

@@ -556,6 +556,47 @@ extern "C" int gr_ctx_take_tile_stats(gr_ctx * c, uint64_t * tiled, uint64_t * u
     return GR_OK;
 }
 
+extern "C" int gr_ctx_copy_bandwidth(gr_ctx * c, uint64_t bytes, uint32_t trials, double * gbs)
+{
+    if (!c || !gbs) return set_err(GR_E_INVALID, "gr_ctx_copy_bandwidth: null argument");
+    if (bytes == 0 || (bytes & 15)) return set_err(GR_E_INVALID, "gr_ctx_copy_bandwidth: bytes must be a positive multiple of 16");
+    if (trials == 0) trials = 5;
+    GR_HIP(hipSetDevice(c->device));
+    void * a = nullptr;
+    void * b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GR_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemsetAsync(a, 0, bytes, c->compute) != hipSuccess)
+        rc = set_err(GR_E_HIP, "gr_ctx_copy_bandwidth: allocation failed");
+    // enough workgroups to keep every CU busy: 16 per CU
+    const uint32_t grid = uint32_t(std::max(1, c->cus)) * 16u;
+    float best = 3.4e38f;
+    for (uint32_t t = 0; rc == GR_OK && t <= trials; ++t) // the first copy warms up
+    {
+        if (hipEventRecord(e0, c->compute) != hipSuccess || gr::launch_stream_copy(a, b, bytes, grid, c->compute) != 0 ||
+            hipEventRecord(e1, c->compute) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        {
+            rc = set_err(GR_E_HIP, "gr_ctx_copy_bandwidth: copy failed");
+            break;
+        }
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = set_err(GR_E_HIP, "gr_ctx_copy_bandwidth: timing failed");
+        else if (t > 0)
+            best = std::min(best, ms);
+    }
+    (void)hipStreamSynchronize(c->compute);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    if (rc != GR_OK) return rc;
+    *gbs = 2.0 * double(bytes) / (double(best) * 1e-3) / 1e9;
+    return GR_OK;
+}
+
 extern "C" int gr_ctx_info(gr_ctx * c, char * name, size_t name_len, uint64_t * hbm_bytes,
                            int * compute_units)
 {

@@ -352,6 +352,8 @@ int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uin
 // copies `bytes` (a multiple of 16, both pointers 16-byte aligned) from device-visible pinned host
 // memory to device memory with a kernel on `stream`
 int launch_stage_copy(const void * src, void * dst, size_t bytes, void * stream);
+// grid-stride 16-byte copy of `bytes` (a multiple of 16) with `grid` workgroups of 256
+int launch_stream_copy(const void * src, void * dst, size_t bytes, uint32_t grid, void * stream);
 // all entries in one launch (maxn: largest entry n)
 int launch_gather_multi(const GatherEntry * tab, uint32_t nent, uint32_t maxn, void * stream);
 // out[orig[i] * w + c] = in[i * w + c] for 4-byte elements, w = 1..3

@@ -344,6 +344,24 @@ __global__ void k_stage_copy(const uint4 * __restrict__ src, uint4 * __restrict_
         dst[i] = src[i];
 }
 
+// Streaming copy for measuring the device's copy bandwidth (gr_ctx_copy_bandwidth): 16 bytes
+// per lane per iteration, a grid of many workgroups per CU.
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4 * __restrict__ src, uint4 * __restrict__ dst,
+                                                     uint64_t n16)
+{
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+int launch_stream_copy(const void * src, void * dst, size_t bytes, uint32_t grid, void * stream)
+{
+    if (bytes == 0 || (bytes & 15)) return -1;
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, hipStream_t(stream),
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), uint64_t(bytes / 16));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_stage_copy(const void * src, void * dst, size_t bytes, void * stream)
 {
     if (bytes == 0) return 0;

@@ -62,3 +62,16 @@ def test_integrators_on_two_threads(pg, gpu_available):
         for f in ("trajectories", "status", "abort_step", "curvatures", "total_curvature"):
             np.testing.assert_array_equal(got[k][f], alone[k][f], err_msg=f)
         np.testing.assert_array_equal(got[k]["compute"][0], alone[k]["compute"][0])
+
+
+def test_copy_bandwidth_measures_the_device(gpu_available):
+    """The streaming-copy measurement returns a device-to-device rate between 1 TB/s and the
+    published 8 TB/s peak plus margin, and refuses sizes that are not whole 16-byte units."""
+    from pygranite_amd import capi
+    ctx = capi.Context(0)
+    gbs = ctx.copy_bandwidth(1 << 28, 3)
+    assert 1000.0 < gbs < 10000.0, gbs
+    import pytest
+    with pytest.raises(capi.GraniteError):
+        ctx.copy_bandwidth(100, 1)
+    ctx.close()
