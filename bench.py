@@ -383,13 +383,23 @@ def cpu_baseline(wind, gs):
     r = oracle.run(st, wind[:7], seeds, real64=False, nthreads=threads)
     el = time.perf_counter() - t0
     # integrated particle-steps, as on the GPU: steps begun alive, the aborting step included
-    alive = r["status"] == 0
-    work = int(np.where(alive, r["steps"], r["abort_step"].astype(np.int64) + 1).sum())
+    def integrated(res):
+        alive = res["status"] == 0
+        return int(np.where(alive, res["steps"], res["abort_step"].astype(np.int64) + 1).sum())
+
+    work = integrated(r)
+    # one thread on a sixteenth of the sample (about the same wall time)
+    n1 = n // 16
+    t0 = time.perf_counter()
+    r1 = oracle.run(st, wind[:7], seeds[:n1], real64=False, nthreads=1)
+    el1 = time.perf_counter() - t0
+    w1 = integrated(r1)
     return {"value": work / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port", "nominal_value": n * r["steps"] / el,
             "sample": f"{n} particles x {r['steps']} RK4 steps of the same 3D ABC workload with "
                       f"curvature + expression saved every step (FP32 oracle restatement, "
-                      f"{threads} threads, {el:.1f} s; {work} integrated particle-steps)"}
+                      f"{threads} threads, {el:.1f} s; {work} integrated particle-steps)",
+            "one_thread": {"value": w1 / el1, "particles": n1, "seconds": el1, "integrated_particle_steps": w1}}
 
 
 def cpu_model():
