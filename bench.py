@@ -1086,6 +1086,8 @@ def main():
                     help="batched kernel compute units per queue (0: default / autotuned)")
     ap.add_argument("--batch-cu-waves", type=int, default=-1,
                     help="batched kernel wavefronts per CU (0: register-limited; default: autotuned)")
+    ap.add_argument("--batch-uniform", type=int, default=-1,
+                    help="per-variant batched kernels for stencil-reuse launches of one variant (0 or 1)")
     ap.add_argument("--reuse", type=int, default=-1,
                     help="stencil-reuse build for every launch: batched (context) and single-job forms")
     ap.add_argument("--cu-waves", type=int, default=0,
@@ -1137,6 +1139,8 @@ def main():
         ctx.set_option(capi.CTX_BATCH_GROUP, args.batch_group)
     if args.reuse >= 0:
         ctx.set_option(capi.CTX_BATCH_REUSE, args.reuse)
+    if args.batch_uniform >= 0:
+        ctx.set_option(capi.CTX_BATCH_UNIFORM, args.batch_uniform)
     if args.workload == "atmosphere":
         atmosphere_workload(capi, ctx, args)
         return

@@ -331,7 +331,8 @@ int gr_job_set_launch(gr_job * job, uint32_t block);
  * holds at most this many particles, in the time-paired layout (a copy smaller than 4 GiB) with a
  * plain feature set, run four lanes per particle, each lane interpolating one velocity component
  * (a quarter of the lerps per lane: a shorter step chain for launches of few particles); 0:
- * never. Results do not depend on it. */
+ * never. In gr_batch_run the count is the live ranges of all the launch's spans together.
+ * Results do not depend on it. */
 #define GR_OPT_QUAD 6u
 /* GR_OPT_TILE (0 or 1024 .. 65536 bytes, default 0): 3D time-paired launches (not quad ones) run
  * the kernel in which every workgroup, each step, stages in LDS the box of time-paired nodes
@@ -407,12 +408,26 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
  *   default 0 = as many as the registers allow), applied by reserving LDS per workgroup.
  * GR_CTX_BATCH_REUSE: 1 (default) = every batched launch uses the build that keeps each lane's
  *   last time-paired stencil across RK4 stages (fewer gathers, more registers); 0 = only launches
- *   whose wavefronts all fit on the device at once. */
+ *   whose wavefronts all fit on the device at once.
+ * GR_CTX_BATCH_QUAD_ITEMS (read-only): items of the last gr_batch_run that ran a quad of lanes
+ *   per particle. A launch whose spans hold at most a job's quad limit (GR_OPT_QUAD) of live
+ *   particles in all runs the jobs that have a specialised feature set in the time-paired
+ *   layout this way, in items of 16 particles, with the stencil-reuse build.
+ * GR_CTX_BATCH_UNIFORM: 1 (default) = a stencil-reuse launch whose spans all share one
+ *   specialised variant (dimension, integrator, storage type, feature set) runs that variant's
+ *   own kernel, held to four wavefronts per SIMD; 0 = always the kernel of every variant (whose
+ *   register demand is the largest variant's).
+ * GR_CTX_BATCH_LAST_KERNEL (read-only): build of the last gr_batch_run launch: 0 none (no
+ *   items), 1 the default build, 2 the 128-VGPR build, 3 the stencil-reuse build, 4 a
+ *   per-variant kernel. */
 #define GR_CTX_BATCH_BLOCK 1u
 #define GR_CTX_BATCH_WAVES 2u
 #define GR_CTX_BATCH_GROUP 3u
 #define GR_CTX_BATCH_CU_WAVES 4u
 #define GR_CTX_BATCH_REUSE 5u
+#define GR_CTX_BATCH_QUAD_ITEMS 6u
+#define GR_CTX_BATCH_UNIFORM 7u
+#define GR_CTX_BATCH_LAST_KERNEL 8u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 

@@ -37,10 +37,12 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
 # kernel translation units (compiled in parallel): the single-job step kernels per storage type
-# (the FP32 one also holds their launcher), the batched kernel's three builds, and the support
-# kernels; all share gr_step.h
+# (the FP32 one also holds their launcher), the batched kernel's three builds and its
+# per-variant kernels (2D, 3D), the support kernels, the allocation-free entry point's kernels and
+# the field-tile kernels; all share gr_step.h
 HIP_SOURCES = ("gr_k_adv_f32.hip", "gr_k_adv_f16.hip", "gr_k_adv_bf16.hip", "gr_k_batch.hip",
-               "gr_k_batch_w4.hip", "gr_k_batch_narrow.hip", "gr_k_support.hip", "gr_k_noalloc.hip", "gr_k_adv_tile.hip")
+               "gr_k_batch_w4.hip", "gr_k_batch_narrow.hip", "gr_k_batch_uni2.hip", "gr_k_batch_uni3.hip",
+               "gr_k_support.hip", "gr_k_noalloc.hip", "gr_k_adv_tile.hip")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 

@@ -39,6 +39,9 @@ EXPORTS = [
 OPT_BLOCK, OPT_PAIRED, OPT_CU_WAVES, OPT_SPECIALIZE, OPT_REUSE, OPT_QUAD = 1, 2, 3, 4, 5, 6
 OPT_TILE, OPT_TILE_HALO = 7, 8
 CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP, CTX_BATCH_CU_WAVES, CTX_BATCH_REUSE = 1, 2, 3, 4, 5
+CTX_BATCH_QUAD_ITEMS = 6  # read-only: quad items of the last batched launch
+CTX_BATCH_UNIFORM = 7  # 1: per-variant kernels for stencil-reuse launches of one variant
+CTX_BATCH_LAST_KERNEL = 8  # read-only: 0 none, 1 default, 2 128-VGPR, 3 stencil-reuse, 4 per-variant
 
 
 class FieldDesc(ctypes.Structure):

@@ -87,6 +87,8 @@ struct gr_ctx
     uint32_t ncu{1};                      // compute units found (dense indices in d_culut)
     // persistent batched launches
     std::vector<gr::BatchItem> h_items;  // the last item list (reused while the spans match)
+    uint64_t items_quad{0};               // its quad items
+    uint32_t batch_last_kernel{0};        // build of the last batched launch (GR_CTX_BATCH_LAST_KERNEL)
     std::vector<uint32_t> qstart;         // its queue starts
     std::vector<uint64_t> items_key;
     bool items_valid{false};
@@ -128,6 +130,7 @@ struct gr_ctx
     uint32_t batch_group{32}; // compute units per queue of the batched kernel
     uint32_t batch_cu_waves{0}; // cap on the batched kernel's wavefronts per CU (0: none)
     uint32_t batch_reuse{1};    // stencil-reuse build: 1 always (default), 0 for narrow launches only
+    uint32_t batch_uniform{1};  // per-variant kernels for narrow launches of one variant (default 1)
     // device time of every step-kernel launch (event pairs, summed by gr_ctx_take_kernel_ms)
     std::vector<hipEvent_t> kev;
     size_t kpending{0};
@@ -454,6 +457,12 @@ extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
             if (value != 0 && value != 1) return set_err(GR_E_INVALID, "batch reuse must be 0 or 1");
             c->batch_reuse = uint32_t(value);
             return GR_OK;
+        case GR_CTX_BATCH_UNIFORM:
+            if (value != 0 && value != 1) return set_err(GR_E_INVALID, "batch uniform must be 0 or 1");
+            c->batch_uniform = uint32_t(value);
+            return GR_OK;
+        case GR_CTX_BATCH_QUAD_ITEMS:
+        case GR_CTX_BATCH_LAST_KERNEL: return set_err(GR_E_INVALID, "context option %u is read-only", option);
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -468,6 +477,9 @@ extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
         case GR_CTX_BATCH_GROUP: *value = c->batch_group; return GR_OK;
         case GR_CTX_BATCH_CU_WAVES: *value = c->batch_cu_waves; return GR_OK;
         case GR_CTX_BATCH_REUSE: *value = c->batch_reuse; return GR_OK;
+        case GR_CTX_BATCH_QUAD_ITEMS: *value = int64_t(c->items_valid ? c->items_quad : 0); return GR_OK;
+        case GR_CTX_BATCH_UNIFORM: *value = c->batch_uniform; return GR_OK;
+        case GR_CTX_BATCH_LAST_KERNEL: *value = c->batch_last_kernel; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -2117,17 +2129,30 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     const uint32_t nq = std::max<uint32_t>(1, std::min<uint32_t>((ctx->ncu + group - 1) / group, gr::MAX_QUEUES));
     std::vector<uint64_t> key;
     key.push_back(nq);
-    std::vector<uint32_t> cls_of(nspans, 0);
+    std::vector<uint32_t> cls_of(nspans, 0), var_of(nspans, 0), isz(nspans, gr::BATCH_ITEM);
+    // Launches of few live particles (in all their spans together, at most a job's quad limit)
+    // run a quad of lanes per particle for the jobs with a specialised feature set in the
+    // time-paired layout (as single-job launches do), in items of QUAD_ITEM particles.
+    uint64_t live_total = 0;
+    for (uint32_t i = 0; i < nspans; ++i)
+        if (span_steps[i]) live_total += spans[i].job->run_n;
+    bool any_quad = false;
     for (uint32_t i = 0; i < nspans; ++i)
     {
         if (!span_steps[i]) continue;
         gr_job * j = spans[i].job;
         for (uint32_t v = span_steps[i]; v; v >>= 1) ++cls_of[i];
-        // the item variant (dimension, integrator, storage type, feature set) and the job's serial
-        const int var = gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype,
-                                          j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1);
-        key.insert(key.end(), {i, j->serial, j->run_n, uint64_t(uintptr_t(spans[i].wind)), cls_of[i],
-                               uint64_t(int64_t(var))});
+        // the item variant (dimension, integrator, storage type, feature set, quad) and the job's
+        // serial
+        const int spec = j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1;
+        var_of[i] = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype, spec));
+        if (spec >= 0 && params[i].wind.pair && params[i].wind.pair32 && j->quad_max && live_total <= j->quad_max)
+        {
+            var_of[i] += gr::BATCH_QUAD;
+            isz[i] = gr::QUAD_ITEM;
+            any_quad = true;
+        }
+        key.insert(key.end(), {i, j->serial, j->run_n, uint64_t(uintptr_t(spans[i].wind)), cls_of[i], var_of[i]});
     }
     const bool items_dirty = !(ctx->items_valid && key == ctx->items_key);
     if (items_dirty)
@@ -2162,7 +2187,7 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
                 span_base[i] = uint32_t((cr * nfld + fr) * PB);
             }
         auto bucket_of = [&](uint32_t i, uint64_t f, uint64_t n) -> uint64_t {
-            const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, n - f));
+            const uint32_t cnt = uint32_t(std::min<uint64_t>(isz[i], n - f));
             const double pos = (double(f) + 0.5 * cnt) / double(n); // in [0, 1)
             const double qp = pos * double(nq);
             const uint32_t q = std::min<uint32_t>(nq - 1, uint32_t(qp));
@@ -2175,25 +2200,24 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
         {
             if (!span_steps[i]) continue;
             const uint64_t n = spans[i].job->run_n;
-            for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM) ++count[bucket_of(i, f, n) + 1];
-            total += (n + gr::BATCH_ITEM - 1) / gr::BATCH_ITEM;
+            for (uint64_t f = 0; f < n; f += isz[i]) ++count[bucket_of(i, f, n) + 1];
+            total += (n + isz[i] - 1) / isz[i];
         }
         for (uint64_t b = 0; b < nbuckets; ++b) count[b + 1] += count[b];
         ctx->h_items.resize(total);
+        ctx->items_quad = 0;
         ctx->qstart.assign(nq + 1, 0u);
         for (uint32_t q = 0; q < nq; ++q) ctx->qstart[q + 1] = count[uint64_t(q + 1) * ncls * nfld * PB];
         for (uint32_t i = 0; i < nspans; ++i)
         {
             if (!span_steps[i]) continue;
-            gr_job * j = spans[i].job;
-            const uint32_t var = uint32_t(gr::batch_variant(j->d.dim, j->d.integrator, spans[i].wind->d.dtype,
-                                                            j->specialize ? gr::lean_spec(params[i], gr::MODE_PAIRED) : -1));
-            const uint64_t n = j->run_n;
-            for (uint64_t f = 0; f < n; f += gr::BATCH_ITEM)
+            const uint64_t n = spans[i].job->run_n;
+            for (uint64_t f = 0; f < n; f += isz[i])
             {
-                const uint32_t cnt = uint32_t(std::min<uint64_t>(gr::BATCH_ITEM, n - f));
-                ctx->h_items[count[bucket_of(i, f, n)]++] = gr::BatchItem{i, uint32_t(f), cnt, var};
+                const uint32_t cnt = uint32_t(std::min<uint64_t>(isz[i], n - f));
+                ctx->h_items[count[bucket_of(i, f, n)]++] = gr::BatchItem{i, uint32_t(f), cnt, var_of[i]};
             }
+            if (var_of[i] >= gr::BATCH_QUAD) ctx->items_quad += (n + isz[i] - 1) / isz[i];
         }
         ctx->items_key = key;
         ctx->items_valid = false; // until the copy below is queued
@@ -2260,9 +2284,29 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     uint32_t lds_cols = 0;
     for (uint32_t i = 0; i < nspans; ++i)
         if (span_steps[i] && spans[i].job->d.nprog) lds_cols = std::max(lds_cols, spans[i].job->expr_cols);
+    // narrow: every item's wavefront fits on the device at two per SIMD (and quad items, which
+    // only the narrow builds run)
+    const bool narrow = ctx->batch_reuse || any_quad || uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
+    // a narrow launch whose spans all have one specialised variant runs that variant's kernel
+    uint32_t uni = 0;
+    if (narrow && ctx->batch_uniform)
+        for (uint32_t i = 0; i < nspans; ++i)
+        {
+            if (!span_steps[i]) continue;
+            const uint32_t base = var_of[i] >= gr::BATCH_QUAD ? var_of[i] - gr::BATCH_QUAD : var_of[i];
+            if (base < 12 || (uni && uni != base))
+            {
+                uni = 0;
+                break;
+            }
+            uni = base;
+        }
     int per_cu = 0;
     const uint32_t lds = gr::launch_lds_bytes(block, lds_cols, ctx->batch_cu_waves, ctx->lds_per_cu);
-    if (gr::batch_occupancy(block, ctx->batch_waves, lds, &per_cu) != 0 || per_cu <= 0) per_cu = 1;
+    if ((uni ? gr::batch_uni_occupancy(uni, block, lds, &per_cu)
+             : gr::batch_occupancy(block, ctx->batch_waves, lds, &per_cu)) != 0 ||
+        per_cu <= 0)
+        per_cu = 1;
     // enough workgroups to fill the device, no more than one wavefront per item
     const uint32_t wpb = block / 64u;
     const uint32_t grid = std::max<uint32_t>(
@@ -2273,11 +2317,12 @@ extern "C" int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans
     // packet between consecutive launches)
     if (ms) GR_HIP(hipEventRecord(ctx->bev0, s));
     if (nitems && (rc = kt_mark(ctx, s, false)) != GR_OK) return rc;
-    // narrow: every item's wavefront fits on the device at two per SIMD
-    const bool narrow = ctx->batch_reuse || uint64_t(nitems) <= uint64_t(std::max(1, ctx->cus)) * 4 * 2;
-    if (nitems && gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
-                                   block, ctx->batch_waves, narrow, lds, s) != 0)
+    if (nitems && (uni ? gr::launch_batch_uni(uni, dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork,
+                                              grid, block, lds, s)
+                       : gr::launch_batch(dparams, ctx->d_items, dq, nq, ctx->d_culut, group, ctx->dwork, grid,
+                                          block, ctx->batch_waves, narrow, lds, s)) != 0)
         return set_err(GR_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    ctx->batch_last_kernel = !nitems ? 0u : uni ? 4u : narrow ? 3u : ctx->batch_waves >= 4 ? 2u : 1u;
     hipEvent_t done = nullptr;
     if (nitems && (rc = kt_mark(ctx, s, true, &done)) != GR_OK) return rc;
     if (ms) GR_HIP(hipEventRecord(ctx->bev1, s));

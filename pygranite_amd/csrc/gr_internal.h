@@ -231,14 +231,14 @@ inline int lean_spec(const StepParams & P, uint32_t mode)
     return int(P.curvature) + ((P.flags & F_KAHAN) ? 16 : 0);
 }
 
-// one work item of the persistent batched kernel: `count` (<= 64, one wavefront) particles of
-// job `job`
+// one work item of the persistent batched kernel: `count` particles of job `job` from `first`
+// on (<= 64, one wavefront; <= QUAD_ITEM for quad items)
 struct BatchItem
 {
     uint32_t job;
     uint32_t first;
     uint32_t count;
-    uint32_t variant; // batch_variant(dim, integrator, dtype)
+    uint32_t variant; // batch_variant(dim, integrator, dtype, spec), + BATCH_QUAD for quad items
 };
 // BATCH_ITEM: particles per item (one wavefront). Queue data of a launch (qdata): nq counters
 // QUEUE_STRIDE words apart (zero at launch), nq + 1 queue starts, ceil(nq / 32) mask words (bit
@@ -256,7 +256,9 @@ enum : uint32_t
 // item variant: 12 * specialisation (0 generic, 1 plain with curvature off, 2 plain with
 // individual-and-total curvature) + 6 * (dim == 3) + 3 * (RK4) + storage type
 int batch_variant(uint32_t dim, uint32_t integrator, uint32_t dtype, int spec);
-enum : uint32_t { BATCH_VARIANTS = 60 };
+// quad items: variant BATCH_QUAD + v (v a specialised variant, 12 .. 59), QUAD_ITEM particles
+// each, run only by the stencil-reuse (narrow) build
+enum : uint32_t { BATCH_VARIANTS = 60, BATCH_QUAD = 64, QUAD_ITEM = 16 };
 // culut[cu_key] = dense index of that CU (0xffffffff: none); a CU's queue is index / group;
 // work: the work counter slots
 // waves: 0 / 1 = the compiler's register allocation, 4 = at most 128 VGPRs (4 wavefronts/SIMD);
@@ -267,6 +269,13 @@ enum : uint32_t { BATCH_VARIANTS = 60 };
 int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
                  const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
                  uint32_t block, uint32_t waves, bool narrow, uint32_t lds_bytes, void * stream);
+// the per-variant batched kernel of specialised variant `var` (12 .. 59): for launches whose
+// items all have that variant (or its quad form); four wavefronts per SIMD, stencil reuse.
+// Returns -2 for a variant without one.
+int launch_batch_uni(uint32_t var, const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
+                     const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
+                     uint32_t block, uint32_t lds_bytes, void * stream);
+int batch_uni_occupancy(uint32_t var, uint32_t block, uint32_t lds_bytes, int * blocks_per_cu);
 // writes cu_key() of the CU running each of n workgroups
 int launch_cu_probe(uint32_t * out, uint32_t n, void * stream);
 int batch_occupancy(uint32_t block, uint32_t waves, uint32_t lds_bytes, int * blocks_per_cu);

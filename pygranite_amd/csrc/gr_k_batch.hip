@@ -35,6 +35,23 @@ int launch_batch(const StepParams * jobs, const BatchItem * items, uint32_t * qd
     return fn(jobs, items, qdata, nq, culut, group, work, grid, block, lds_bytes, stream);
 }
 
+int launch_batch_uni(uint32_t var, const StepParams * jobs, const BatchItem * items, uint32_t * qdata, uint32_t nq,
+                     const uint32_t * culut, uint32_t group, unsigned long long * work, uint32_t grid,
+                     uint32_t block, uint32_t lds_bytes, void * stream)
+{
+    if (nq == 0 || grid == 0 || group == 0) return 0;
+    return (var % 12) >= 6 ? launch_batch_uni3(var, jobs, items, qdata, nq, culut, group, work, grid, block,
+                                               lds_bytes, stream)
+                           : launch_batch_uni2(var, jobs, items, qdata, nq, culut, group, work, grid, block,
+                                               lds_bytes, stream);
+}
+
+int batch_uni_occupancy(uint32_t var, uint32_t block, uint32_t lds_bytes, int * blocks_per_cu)
+{
+    return (var % 12) >= 6 ? occupancy_batch_uni3(var, block, lds_bytes, blocks_per_cu)
+                           : occupancy_batch_uni2(var, block, lds_bytes, blocks_per_cu);
+}
+
 int batch_occupancy(uint32_t block, uint32_t waves, uint32_t lds_bytes, int * blocks_per_cu)
 {
     return waves >= 4 ? occupancy_batch_w4(block, lds_bytes, blocks_per_cu)

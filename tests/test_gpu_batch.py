@@ -46,7 +46,10 @@ JOBS = [
 ]
 
 
-def test_batch_mixed_jobs_bitexact(gpu_available):
+@pytest.mark.parametrize("quad", [True, False])
+def test_batch_mixed_jobs_bitexact(gpu_available, quad):
+    """Every job of the batch equals the FP32 oracle, with quad items (the launch's few live
+    particles: a quad of lanes per particle for the specialised jobs) and without (quad limit 0)."""
     ctx = capi.Context(0)
     g2, gs2 = synthetic.double_gyre(96, 48, 6)
     g3, gs3 = synthetic.abc_flow(24, 6)
@@ -69,6 +72,8 @@ def test_batch_mixed_jobs_bitexact(gpu_available):
         seeds = synthetic.uniform_seeds(100 + idx, n, lo, hi)
         d = make_desc(dim, integ, gs, src.shape, step, curv, exprs)
         job = capi.Job(ctx, d, seeds)
+        if not quad:
+            job.set_option(capi.OPT_QUAD, 0)
         if idx % 2 == 0:
             job.reorder()  # batched items read reordered state too
         jobs.append(job)
@@ -77,6 +82,9 @@ def test_batch_mixed_jobs_bitexact(gpu_available):
         refs.append(oracle.run(st, ROUND[dt_](src), seeds, real64=False))
     steps, ms = ctx.batch_run(spans)
     assert ms > 0
+    # quad items for the four specialised jobs (curvature 0 or 3): ceil(n / 16) each
+    want_quad = sum(-(-n // 16) for (_, _, _, n, _, curv, _) in JOBS if curv in (0, 3)) if quad else 0
+    assert ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS) == want_quad
     for idx, (job, ref) in enumerate(zip(jobs, refs)):
         assert steps[idx] == ref["steps"], (idx, steps[idx], ref["steps"])
         pos = job.read(capi.OUT_POSITIONS)
@@ -95,7 +103,8 @@ def test_batch_mixed_jobs_bitexact(gpu_available):
 
 
 def test_batch_equals_single_job_advances(gpu_available):
-    """A span in the batch == the same job advanced window by window with gr_job_advance."""
+    """A span in the batch (quad items) == the same job advanced window by window with
+    gr_job_advance at one lane per particle."""
     ctx = capi.Context(0)
     src, gs = synthetic.abc_flow(32, 5)
     f = capi.Field(ctx, 32, 32, 32, 3, capi.F16, nslots=5)
@@ -105,8 +114,10 @@ def test_batch_equals_single_job_advances(gpu_available):
     d = make_desc(3, 1, gs, src.shape, 0.005, 3, ("sqrt(u*u+v*v+w*w)",), save_interval=10)
     a = capi.Job(ctx, d, seeds, save_capacity=200)
     b = capi.Job(ctx, d, seeds, save_capacity=200)
+    b.set_option(capi.OPT_QUAD, 0)  # one lane per particle
     steps, _ = ctx.batch_run([(a, f, dict(max_steps=700))])
     assert steps == [700]
+    assert ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS) == 4096 // 16  # a quad of lanes per particle
     # b: window by window under the same FP32 schedule
     t, counter, c, mwt, left = np.float32(0), 0, 0, np.float32(1.0), 700
     while left:
@@ -579,5 +590,126 @@ def test_single_job_autotune_sets_a_candidate_and_changes_nothing(gpu_available)
         withlift.autotune(f, steps=10, trials=1)
     for j in (tuned, twin, withlift):
         j.close()
+    f.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dtype,kahan", [(capi.F32, 1), (capi.F16, 0), (capi.BF16, 1)])
+def test_batch_quad_items_equal_one_lane_items(gpu_available, dtype, kahan):
+    """Quad items (a quad of lanes per particle) against one-lane items on the same jobs: 3D and
+    2D RK4 jobs with curvature, an expression and positions saved every step, through window
+    edges, in two consecutive launches -- every output and the device work counter equal."""
+    src3, gs3 = synthetic.abc_flow(24, 4)
+    g2, gs2 = synthetic.double_gyre(96, 48, 4)
+    ctx = capi.Context(0)
+    f3 = capi.Field(ctx, 24, 24, 24, 3, dtype, nslots=4)
+    f2 = capi.Field(ctx, 96, 48, 1, 2, dtype, nslots=4, address=1)
+    for s in range(4):
+        f3.upload(s, [src3[s, c] for c in range(3)])
+        f2.upload(s, [g2[s, c] for c in range(2)])
+    s3 = synthetic.uniform_seeds(31, 3000, [0.0] * 3, [2 * math.pi] * 3)
+    s2 = synthetic.uniform_seeds(32, 1999, [0.02, 0.02], [1.98, 0.98])
+    d3 = make_desc(3, 1, gs3, src3.shape, 0.01, 3, ("sqrt(u*u+v*v+w*w)*2-x",), save_interval=1)
+    d2 = make_desc(2, 1, gs2, g2.shape, 0.01, 3, ("u*v+y",), save_interval=1)
+    d3.kahan = d2.kahan = kahan
+
+    def run(quad):
+        jobs = [capi.Job(ctx, d3, s3, save_capacity=260), capi.Job(ctx, d2, s2, save_capacity=260)]
+        for j in jobs:
+            if not quad:
+                j.set_option(capi.OPT_QUAD, 0)
+        ctx.reorder(jobs)
+        ctx.take_work()
+        ends, _ = ctx.batch_run([(jobs[0], f3, dict(max_steps=130)), (jobs[1], f2, dict(max_steps=130))],
+                                ends=True)
+        nq = [ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS)]
+        spans = [(j, f, dict(first_window=e["window"], max_steps=130, time_start=e["time"],
+                             time_start64=e["time64"], counter_start=e["counter"]))
+                 for j, f, e in zip(jobs, (f3, f2), ends)]
+        steps, _ = ctx.batch_run(spans)
+        nq.append(ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS))
+        work = ctx.take_work()
+        out = []
+        for j in jobs:
+            out += [j.read(capi.OUT_POSITIONS), j.read(capi.OUT_STATUS), j.read(capi.OUT_ABORT_STEP),
+                    j.read(capi.OUT_TOTAL_CURVATURE), j.read_saved(capi.OUT_POSITIONS, 0, 260),
+                    j.read_saved(capi.OUT_CURVATURE, 0, 260), j.read_saved(capi.OUT_EXPR0, 0, 260)]
+            j.close()
+        return [e["steps"] for e in ends] + steps, nq, work, out
+
+    steps_q, nq_q, work_q, got = run(True)
+    steps_1, nq_1, work_1, ref = run(False)
+    assert steps_q == steps_1 == [130, 130, 130, 130]
+    assert nq_q[0] == math.ceil(3000 / 16) + math.ceil(1999 / 16) and nq_q[1] > 0 and nq_1 == [0, 0]
+    assert work_q == work_1 > 0
+    for a, b in zip(ref, got):
+        np.testing.assert_array_equal(a, b)
+    assert (ref[1] != 0).sum() > 50  # particles stopped on the way (3D border)
+    f3.close()
+    f2.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dtype,kahan,curv", [(capi.F16, 0, 3), (capi.F32, 1, 3), (capi.BF16, 1, 0), (capi.F32, 0, 0)])
+def test_batch_per_variant_kernel_equals_kernel_of_every_variant(gpu_available, dtype, kahan, curv):
+    """Spans of one specialised variant run that variant's own kernel (GR_CTX_BATCH_UNIFORM = 1)
+    or the kernel of every variant (0): every output, saved column and the work counter equal,
+    in wide launches (one lane per particle) and in launches of few live particles (quad
+    items); a launch that mixes variants runs the kernel of every variant."""
+    src, gs = synthetic.abc_flow(24, 4)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 24, 24, 24, 3, dtype, nslots=4)
+    for s in range(4):
+        f.upload(s, [src[s, c] for c in range(3)])
+    seeds = synthetic.uniform_seeds(41, 60000, [0.0] * 3, [2 * math.pi] * 3)
+    d = make_desc(3, 1, gs, src.shape, 0.01, curv, ("sqrt(u*u+v*v+w*w)*2-x",), save_interval=1)
+    d.kahan = kahan
+
+    def run(uniform):
+        ctx.set_option(capi.CTX_BATCH_UNIFORM, uniform)
+        jobs = [capi.Job(ctx, d, seeds[:40000], save_capacity=250), capi.Job(ctx, d, seeds[40000:], save_capacity=250)]
+        ctx.reorder(jobs)
+        ctx.take_work()
+        kernels, quads, counter, t, t64, win = [], [], 0, 0.0, 0.0, 0
+        out = []
+        for _ in range(5):
+            ends, _ = ctx.batch_run([(j, f, dict(first_window=win, max_steps=50, time_start=t, time_start64=t64,
+                                                 counter_start=counter)) for j in jobs], ends=True)
+            kernels.append(ctx.get_option(capi.CTX_BATCH_LAST_KERNEL))
+            quads.append(ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS))
+            for j in jobs:  # this launch's columns, in the order of its particles
+                out += [j.read_saved(capi.OUT_POSITIONS, counter, 50), j.read_saved(capi.OUT_EXPR0, counter, 50)]
+                if curv:
+                    out.append(j.read_saved(capi.OUT_CURVATURE, counter, 50))
+            e = ends[0]
+            counter, t, t64, win = e["counter"], e["time"], e["time64"], e["window"]
+            ctx.reorder(jobs)  # shrinks the live ranges: the last launches have few particles
+        work = ctx.take_work()
+        status = [j.read(capi.OUT_STATUS) for j in jobs]
+        for j in jobs:
+            out += [j.read(capi.OUT_POSITIONS), j.read(capi.OUT_ABORT_STEP)]
+            if curv:
+                out.append(j.read(capi.OUT_TOTAL_CURVATURE))
+            j.close()
+        return kernels, quads, work, counter, out + status
+
+    k1, q1, w1, c1, got = run(1)
+    k0, q0, w0, c0, ref = run(0)
+    assert c0 == c1 == 250
+    assert k1 == [4] * 5 and k0 == [3] * 5
+    assert q0 == q1 and q1[0] == 0 and q1[-1] > 0  # wide first launch, quad items at the end
+    assert w0 == w1 > 0
+    for a, b in zip(ref, got):
+        np.testing.assert_array_equal(a, b)
+    assert sum(int((st != 0).sum()) for st in ref[-2:]) > 10000  # many particles left the domain
+    # a launch mixing two variants (with and without Kahan) runs the kernel of every variant
+    ctx.set_option(capi.CTX_BATCH_UNIFORM, 1)
+    d2 = make_desc(3, 1, gs, src.shape, 0.01, curv, ("sqrt(u*u+v*v+w*w)*2-x",), save_interval=1)
+    d2.kahan = 1 - kahan
+    a, b = capi.Job(ctx, d, seeds[:1000], save_capacity=10), capi.Job(ctx, d2, seeds[:1000], save_capacity=10)
+    ctx.batch_run([(a, f, dict(max_steps=10)), (b, f, dict(max_steps=10))])
+    assert ctx.get_option(capi.CTX_BATCH_LAST_KERNEL) == 3
+    a.close()
+    b.close()
     f.close()
     ctx.close()
