@@ -428,6 +428,13 @@ int gr_batch_run(gr_ctx * ctx, const gr_span * spans, uint32_t nspans, uint32_t 
 #define GR_CTX_BATCH_QUAD_ITEMS 6u
 #define GR_CTX_BATCH_UNIFORM 7u
 #define GR_CTX_BATCH_LAST_KERNEL 8u
+/* GR_CTX_POOL_BYTES (>= 0, default 8 GiB): device memory of destroyed jobs' arrays that the
+ * context keeps for later jobs (by size class: powers of two from 4 KiB below 2 MiB, 2 MiB
+ * granules above), so that creating and destroying jobs does not call hipMalloc / hipFree (which
+ * waits for the device) each time; 0 frees at once. A failing allocation empties the cache and
+ * retries; gr_ctx_destroy frees it. GR_CTX_POOL_CACHED (read-only): bytes cached now. */
+#define GR_CTX_POOL_BYTES 9u
+#define GR_CTX_POOL_CACHED 10u
 int gr_ctx_set_option(gr_ctx * ctx, uint32_t option, int64_t value);
 int gr_ctx_get_option(gr_ctx * ctx, uint32_t option, int64_t * value);
 

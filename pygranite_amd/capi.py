@@ -42,6 +42,7 @@ CTX_BATCH_BLOCK, CTX_BATCH_WAVES, CTX_BATCH_GROUP, CTX_BATCH_CU_WAVES, CTX_BATCH
 CTX_BATCH_QUAD_ITEMS = 6  # read-only: quad items of the last batched launch
 CTX_BATCH_UNIFORM = 7  # 1: per-variant kernels for stencil-reuse launches of one variant
 CTX_BATCH_LAST_KERNEL = 8  # read-only: 0 none, 1 default, 2 128-VGPR, 3 stencil-reuse, 4 per-variant
+CTX_POOL_BYTES, CTX_POOL_CACHED = 9, 10  # cache of destroyed jobs' arrays: cap, bytes held (read-only)
 
 
 class FieldDesc(ctypes.Structure):

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <new>
 #include <string>
 #include <thread>
@@ -150,6 +151,13 @@ struct gr_ctx
     // end of the last no-allocation launch (created with the context, recorded by each such
     // launch): later uploads into the slots it read wait for it
     hipEvent_t noalloc_done{nullptr};
+    // Cache of released job arrays (state, shadows, index maps, rings), keyed by size class:
+    // a job created after another's release reuses its device memory instead of calling
+    // hipMalloc / hipFree (which waits for the device) again. At most pool_cap bytes are kept
+    // (GR_CTX_POOL_BYTES); a failing allocation empties the cache and retries.
+    std::multimap<size_t, void *> pool;
+    size_t pool_cached{0};
+    size_t pool_cap{size_t(8) << 30};
 };
 
 struct gr_field
@@ -263,6 +271,70 @@ struct gr_job
     std::vector<Read> read_pool;
     uint64_t next_ticket{1};
 };
+
+// Size class of a cached job array: powers of two from 4 KiB below 2 MiB (the runtime packs
+// such allocations), whole 2 MiB granules above.
+static size_t pool_class(size_t bytes)
+{
+    const size_t G = size_t(2) << 20;
+    if (bytes >= G) return (bytes + G - 1) / G * G;
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+static void pool_flush(gr_ctx * c)
+{
+    for (auto & e : c->pool) (void)hipFree(e.second);
+    c->pool.clear();
+    c->pool_cached = 0;
+}
+
+// Device memory of `bytes` (0: none) for a job array: a cached block of its size class, or a new
+// one. Callers fill what must start with defined contents.
+template <class T> static int pool_alloc(gr_ctx * c, T ** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return GR_OK;
+    const size_t cls = pool_class(bytes);
+    auto it = c->pool.find(cls);
+    if (it != c->pool.end())
+    {
+        *p = static_cast<T *>(it->second);
+        c->pool.erase(it);
+        c->pool_cached -= cls;
+        return GR_OK;
+    }
+    void * q = nullptr;
+    hipError_t e = hipMalloc(&q, cls);
+    if (e != hipSuccess && !c->pool.empty())
+    {
+        (void)hipGetLastError();
+        pool_flush(c);
+        e = hipMalloc(&q, cls);
+    }
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return set_err(GR_E_NOMEM, "hipMalloc(%zu) failed: %s", cls, hipGetErrorString(e));
+    }
+    *p = static_cast<T *>(q);
+    return GR_OK;
+}
+
+// Releases a job array that no queued work uses any more (the caller has drained the streams).
+static void pool_free(gr_ctx * c, void * p, size_t bytes)
+{
+    if (!p) return;
+    const size_t cls = pool_class(bytes);
+    if (c && c->pool_cached + cls <= c->pool_cap)
+    {
+        c->pool.emplace(cls, p);
+        c->pool_cached += cls;
+    }
+    else
+        (void)hipFree(p);
+}
 
 // --------------------------------------------------------------------------------------------
 // library / context
@@ -415,6 +487,7 @@ extern "C" int gr_ctx_destroy(gr_ctx * c)
     for (hipEvent_t & e : c->alive_ev)
         if (e) (void)hipEventDestroy(e);
     for (uint32_t * pg : c->alive_pages) (void)hipHostFree(pg);
+    pool_flush(c);
     (void)hipStreamDestroy(c->compute);
     (void)hipStreamDestroy(c->copy);
     delete c;
@@ -461,8 +534,14 @@ extern "C" int gr_ctx_set_option(gr_ctx * c, uint32_t option, int64_t value)
             if (value != 0 && value != 1) return set_err(GR_E_INVALID, "batch uniform must be 0 or 1");
             c->batch_uniform = uint32_t(value);
             return GR_OK;
+        case GR_CTX_POOL_BYTES:
+            if (value < 0) return set_err(GR_E_INVALID, "pool bytes must be >= 0");
+            c->pool_cap = size_t(value);
+            if (c->pool_cached > c->pool_cap) pool_flush(c);
+            return GR_OK;
         case GR_CTX_BATCH_QUAD_ITEMS:
-        case GR_CTX_BATCH_LAST_KERNEL: return set_err(GR_E_INVALID, "context option %u is read-only", option);
+        case GR_CTX_BATCH_LAST_KERNEL:
+        case GR_CTX_POOL_CACHED: return set_err(GR_E_INVALID, "context option %u is read-only", option);
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
 }
@@ -479,6 +558,8 @@ extern "C" int gr_ctx_get_option(gr_ctx * c, uint32_t option, int64_t * value)
         case GR_CTX_BATCH_REUSE: *value = c->batch_reuse; return GR_OK;
         case GR_CTX_BATCH_QUAD_ITEMS: *value = int64_t(c->items_valid ? c->items_quad : 0); return GR_OK;
         case GR_CTX_BATCH_UNIFORM: *value = c->batch_uniform; return GR_OK;
+        case GR_CTX_POOL_BYTES: *value = int64_t(c->pool_cap); return GR_OK;
+        case GR_CTX_POOL_CACHED: *value = int64_t(c->pool_cached); return GR_OK;
         case GR_CTX_BATCH_LAST_KERNEL: *value = c->batch_last_kernel; return GR_OK;
         default: return set_err(GR_E_INVALID, "unknown context option %u", option);
     }
@@ -866,47 +947,6 @@ static int field_pair(gr_field * f, uint32_t b, uint32_t fr, hipStream_t s, cons
 // --------------------------------------------------------------------------------------------
 // jobs
 // --------------------------------------------------------------------------------------------
-static void job_free(gr_job * j)
-{
-    if (!j) return;
-    for (int d = 0; d < 3; ++d)
-    {
-        dfree(j->p[d]);
-        dfree(j->q[d]);
-        dfree(j->c[d]);
-        dfree(j->ld[d]);
-        dfree(j->uplift[d]);
-        for (int k = 0; k < GR_MAX_CONSTANTS; ++k) dfree(j->cst[d][k]);
-    }
-    dfree(j->status);
-    dfree(j->abort_step);
-    dfree(j->total);
-    dfree(j->curv);
-    dfree(j->len);
-    for (float * e : j->expr) dfree(e);
-    for (float * v : j->vol) dfree(v);
-    dfree(j->ring_pos);
-    dfree(j->ring_curv);
-    dfree(j->ring_expr);
-    dfree(j->ring_vol);
-    dfree(j->scratch_aos);
-    dfree(j->scratch2);
-    dfree(j->orig);
-    for (void * a : j->shadow) dfree(a);
-    if (j->ev0) (void)hipEventDestroy(j->ev0);
-    if (j->ev1) (void)hipEventDestroy(j->ev1);
-    for (auto * v : {&j->reads, &j->read_pool})
-        for (gr_job::Read & r : *v)
-        {
-            if (r.host) (void)hipHostFree(r.host);
-            dfree(r.dstage);
-            if (r.ready) (void)hipEventDestroy(r.ready);
-            if (r.done) (void)hipEventDestroy(r.done);
-        }
-    // the slot is reused only after the compute stream has drained (gr_job_destroy)
-    if (j->h_alive && j->ctx) j->ctx->alive_free.push_back(j->h_alive);
-    delete j;
-}
 
 static int zalloc(float ** p, size_t n)
 {
@@ -957,6 +997,39 @@ static std::vector<JobArray> job_arrays(gr_job * j)
     add(j->scratch_aos, N * D * 4, -1, false);
     add(j->scratch2, N * D * 4, -1, false);
     return a;
+}
+
+static void job_free(gr_job * j)
+{
+    if (!j) return;
+    gr_ctx * c = j->ctx;
+    // the arrays of job_arrays (a reorder swaps state arrays with shadows of the same size), the
+    // index map and the shadows go back to the context's cache
+    for (const JobArray & a : job_arrays(j))
+    {
+        pool_free(c, *a.slot, a.bytes);
+        *a.slot = nullptr;
+    }
+    pool_free(c, j->orig, size_t(j->n) * 4);
+    for (void * a : j->shadow) pool_free(c, a, size_t(j->n) * 4);
+    for (int d = 0; d < 3; ++d)
+    {
+        dfree(j->uplift[d]);
+        for (int k = 0; k < GR_MAX_CONSTANTS; ++k) dfree(j->cst[d][k]);
+    }
+    if (j->ev0) (void)hipEventDestroy(j->ev0);
+    if (j->ev1) (void)hipEventDestroy(j->ev1);
+    for (auto * v : {&j->reads, &j->read_pool})
+        for (gr_job::Read & r : *v)
+        {
+            if (r.host) (void)hipHostFree(r.host);
+            dfree(r.dstage);
+            if (r.ready) (void)hipEventDestroy(r.ready);
+            if (r.done) (void)hipEventDestroy(r.done);
+        }
+    // the slot is reused only after the compute stream has drained (gr_job_destroy)
+    if (j->h_alive && c) c->alive_free.push_back(j->h_alive);
+    delete j;
 }
 
 extern "C" int gr_job_bytes(const gr_job_desc * desc, uint64_t n, uint32_t save_capacity,
@@ -1051,7 +1124,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     for (const JobArray & a : job_arrays(j))
     {
         if (rc != GR_OK) break;
-        rc = dalloc(a.slot, a.bytes);
+        rc = pool_alloc(ctx, a.slot, a.bytes);
         // filled on the compute stream, ahead of the seed kernel and every launch of the job
         // (the compute stream does not wait for work on the null stream)
         if (rc == GR_OK && a.fill >= 0 && hipMemsetAsync(*a.slot, a.fill, a.bytes, ctx->compute) != hipSuccess)
@@ -1097,6 +1170,7 @@ extern "C" int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float
     {
         if (g_err.empty()) set_err(rc, "gr_job_create failed");
         (void)hipGetLastError();
+        (void)hipStreamSynchronize(ctx->compute); // queued fills end before the arrays are reused
         job_free(j);
         return rc;
     }
@@ -1293,17 +1367,14 @@ extern "C" int gr_jobs_reorder(gr_job * const * jobs, uint32_t njobs)
     {
         if (!j->orig)
         {
-            if ((rc = dalloc(&j->orig, size_t(j->n) * 4)) != GR_OK) return rc;
-            std::vector<uint32_t> iota(j->n);
-            for (uint64_t i = 0; i < j->n; ++i) iota[i] = uint32_t(i);
-            GR_HIP(hipMemcpyAsync(j->orig, iota.data(), size_t(j->n) * 4, hipMemcpyHostToDevice, s));
-            GR_HIP(hipStreamSynchronize(s));
+            if ((rc = pool_alloc(ctx, &j->orig, size_t(j->n) * 4)) != GR_OK) return rc;
+            if (gr::launch_iota(j->orig, j->n, s) != 0) return set_err(GR_E_HIP, "index map kernel launch failed");
         }
         const size_t na = state_slots(j).size();
         while (j->shadow.size() < na)
         {
             void * b = nullptr;
-            if ((rc = dalloc(&b, size_t(j->n) * 4)) != GR_OK) return rc;
+            if ((rc = pool_alloc(ctx, &b, size_t(j->n) * 4)) != GR_OK) return rc;
             j->shadow.push_back(b);
         }
     }

@@ -371,6 +371,8 @@ int launch_scatter32(const void * in, void * out, const uint32_t * orig, uint64_
 // fills the saved-column entries of stopped particles (columns saved after their stop step)
 // with their final state: ring holds `cap` columns of n x w floats, columns [first, first +
 // count) are completed; src[0..w-1] are the final values; off64 is added when local != 0
+// p[i] = i for i < n (n < 2^32)
+int launch_iota(uint32_t * p, uint64_t n, void * stream);
 int launch_fill_frozen(float * ring, uint32_t cap, uint64_t first, uint32_t count, uint64_t n,
                        uint32_t w, const int32_t * status, const int32_t * abort_step,
                        const float * const * src, uint64_t save_interval, uint32_t local,

@@ -380,6 +380,20 @@ __global__ void k_fill_u32(uint32_t * __restrict__ p, uint64_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_iota_u32(uint32_t * __restrict__ p, uint64_t n)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = uint32_t(i);
+}
+
+int launch_iota(uint32_t * p, uint64_t n, void * stream)
+{
+    if (n == 0) return 0;
+    const uint32_t block = 256;
+    hipLaunchKernelGGL(k_iota_u32, dim3(uint32_t((n + block - 1) / block)), dim3(block), 0, hipStream_t(stream), p, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_bucket_sort(const SortJob * jobs, uint32_t njobs, uint64_t total, uint32_t shift,
                        uint32_t nb, uint32_t * keys, uint32_t * vals, uint32_t * counts,
                        uint32_t * starts, void * temp, size_t & temp_bytes, uint32_t * out,

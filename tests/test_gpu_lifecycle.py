@@ -75,3 +75,76 @@ def test_copy_bandwidth_measures_the_device(gpu_available):
     with pytest.raises(capi.GraniteError):
         ctx.copy_bandwidth(100, 1)
     ctx.close()
+
+
+def test_released_job_memory_is_reused_with_identical_results(gpu_available):
+    """A destroyed job's arrays (state, ring, index map, shadows) go to the context's cache and
+    the next job of the same sizes takes them: its results equal those of the same job on fresh
+    memory (cache off), whatever the reused arrays held; the cache keeps at most its cap."""
+    import math
+    from pygranite_amd import synthetic
+
+    src, gs = synthetic.abc_flow(24, 4)
+    seeds = synthetic.uniform_seeds(3, 50000, [0.0] * 3, [2 * math.pi] * 3)
+
+    def desc():
+        d = capi.JobDesc()
+        d.dim, d.integrator, d.curvature, d.windfield_mode, d.dt, d.dtd = 3, 1, 3, 1, 0.01, 1.0
+        for k in range(3):
+            d.grid_scale[k] = gs[k]
+            d.domain[k] = float(np.float32(24) * np.float32(gs[k]))
+        d.save_interval = 1
+        d.nprog = 1
+        d.prog[0] = capi.compile_expr("sqrt(u*u+v*v+w*w)", dim=3, curvature=3)
+        return d
+
+    def run(ctx, f):
+        job = capi.Job(ctx, desc(), seeds, save_capacity=60)
+        job.reorder()
+        out, t, counter = [], np.float32(0), 0
+        for steps in (60, 40):  # two launches in window 0 (slices 0 and 1)
+            w = capi.Window()
+            w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+            w.time_back, w.time_front = 0.0, 1.0
+            w.time_back64, w.time_front64 = 0.0, 1.0
+            w.time_start = w.time_start64 = float(t)
+            w.counter_start, w.nsteps = counter, steps
+            job.advance(w)
+            out += [job.read_saved(capi.OUT_POSITIONS, counter, steps), job.read_saved(capi.OUT_EXPR0, counter, steps)]
+            for _ in range(steps):
+                t = np.float32(t + np.float32(0.01))
+            counter += steps
+            job.reorder()
+        out += [job.read(capi.OUT_POSITIONS), job.read(capi.OUT_STATUS), job.read(capi.OUT_ABORT_STEP),
+                job.read(capi.OUT_TOTAL_CURVATURE)]
+        job.close()
+        return out
+
+    outs = {}
+    for cap in (0, 8 << 30):
+        ctx = capi.Context(0)
+        ctx.set_option(capi.CTX_POOL_BYTES, cap)
+        assert ctx.get_option(capi.CTX_POOL_BYTES) == cap
+        f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=3)
+        for sl in range(3):
+            f.upload(sl, [src[sl, c] for c in range(3)])
+        first = run(ctx, f)
+        held = ctx.get_option(capi.CTX_POOL_CACHED)
+        second = run(ctx, f)  # on the first job's memory when the cache is on
+        if cap:
+            assert held >= 50000 * 4 * 20  # the first job's arrays, shadows and rings
+            assert ctx.get_option(capi.CTX_POOL_CACHED) == held  # taken and given back
+        else:
+            assert held == 0 and ctx.get_option(capi.CTX_POOL_CACHED) == 0
+        for a, b in zip(first, second):
+            np.testing.assert_array_equal(a, b)
+        outs[cap] = first
+        with pytest.raises(capi.GraniteError):
+            ctx.set_option(capi.CTX_POOL_CACHED, 0)
+        ctx.set_option(capi.CTX_POOL_BYTES, 0)  # a cap below the cached bytes empties the cache
+        assert ctx.get_option(capi.CTX_POOL_CACHED) == 0
+        f.close()
+        ctx.close()
+    for a, b in zip(outs[0], outs[8 << 30]):
+        np.testing.assert_array_equal(a, b)
+    assert (outs[0][-3] != 0).sum() > 1000  # particles stopped on the way
