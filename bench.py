@@ -640,6 +640,7 @@ def atmosphere_workload(capi, ctx, args):
     for name, dtype in (("fp32", capi.F32), ("fp16", capi.F16), ("bf16", capi.BF16)):
         f = capi.Field(ctx, nx, ny, nz, 3, dtype=dtype, nslots=S, address=0)
         f.set_z_levels(zlev)
+        torch.cuda.synchronize()  # the engine's copies are not ordered after torch's stream
         for s in range(S):
             f.upload_device(s, [wind_t[s, c].data_ptr() for c in range(3)])
         ctx.synchronize()
@@ -834,6 +835,7 @@ def serving_workload(capi, ctx, args):
         wind_t, gs, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=k + 1, device=dev)
         f = capi.Field(ctx, nx, ny, nz, 3, dtype=capi.F16, nslots=S, address=0)
         f.set_z_levels(zlev)
+        torch.cuda.synchronize()  # the engine's copies are not ordered after torch's stream
         for sl in range(S):
             f.upload_device(sl, [wind_t[sl, c].data_ptr() for c in range(3)])
         ctx.synchronize()

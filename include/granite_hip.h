@@ -131,7 +131,8 @@ int gr_field_destroy(gr_field * field);
  * conversion run on the copy stream. With async = 0 the call returns after the slot is
  * resident. Kernels that read the slot wait for it on the device. */
 int gr_field_upload(gr_field * field, uint32_t slot, const float * const * comps, int async);
-/* Same from device memory (comps[c] are device pointers, FP32). */
+/* Same from device memory (comps[c] are device pointers, FP32), copied on the copy stream: the
+ * work that writes them must be complete, and they must stay unchanged until gr_field_wait. */
 int gr_field_upload_device(gr_field * field, uint32_t slot, const float * const * dcomps);
 /* Blocks until the slot's last upload finished. */
 int gr_field_wait(gr_field * field, uint32_t slot);
@@ -243,7 +244,9 @@ typedef enum {
 
 /* Creates a job for `n` particles; `seeds` are n*dim floats (AoS, the last row of the
  * reference's TrajectorySet, cpp:424-442) in host memory or in device memory of the context's
- * GPU (then copied on the compute stream, in order, without waiting). `save_capacity` = saved
+ * GPU (then copied on the compute stream, in order, without waiting: the work that writes them
+ * must be complete, and they must stay unchanged until the job's first launch or
+ * gr_ctx_synchronize). `save_capacity` = saved
  * columns held on the device between reads. The state is initialised on the compute stream: a
  * call does not wait for launches already queued there. */
 int gr_job_create(gr_ctx * ctx, const gr_job_desc * desc, const float * seeds, uint64_t n,

@@ -402,9 +402,15 @@ class Job:
         self.ctx = ctx
         self.desc = desc
         self.dim = desc.dim
+        self._seeds = None
         if hasattr(seeds, "data_ptr") and getattr(seeds, "is_cuda", False):
             if str(seeds.dtype) != "torch.float32" or not seeds.is_contiguous():
                 raise ValueError("device seeds must be a contiguous float32 tensor")
+            import torch
+            # the engine copies device seeds on its own stream, ordered after nothing of torch's:
+            # the tensor's producer must be done, and the tensor must outlive the queued copy
+            torch.cuda.current_stream(seeds.device).synchronize()
+            self._seeds = seeds
             ptr, self.n = seeds.data_ptr(), int(seeds.shape[0])
         else:
             seeds = np.ascontiguousarray(seeds, dtype=np.float32)
@@ -502,6 +508,7 @@ class Job:
         if self.h:
             lib().gr_job_destroy(self.h)
             self.h = ctypes.c_void_p()
+        self._seeds = None
 
     def __del__(self):
         try:

@@ -148,3 +148,66 @@ def test_released_job_memory_is_reused_with_identical_results(gpu_available):
     for a, b in zip(outs[0], outs[8 << 30]):
         np.testing.assert_array_equal(a, b)
     assert (outs[0][-3] != 0).sum() > 1000  # particles stopped on the way
+
+
+_TORCH_SEEDS = r"""
+import math, sys
+import numpy as np
+import torch
+torch.zeros(1, device="cuda")  # torch's runtime serves the process, as in bench.py
+sys.path.insert(0, sys.argv[1])
+from pygranite_amd import capi, synthetic
+
+src, gs = synthetic.abc_flow(24, 3)
+ctx = capi.Context(0)
+f = capi.Field(ctx, 24, 24, 24, 3, capi.F32, nslots=2)
+for sl in range(2):
+    f.upload(sl, [src[sl, c] for c in range(3)])
+d = capi.JobDesc()
+d.dim, d.integrator, d.curvature, d.windfield_mode, d.dt, d.dtd = 3, 1, 0, 1, 0.01, 1.0
+for k in range(3):
+    d.grid_scale[k] = gs[k]
+    d.domain[k] = float(np.float32(24) * np.float32(gs[k]))
+n = 1 << 20
+g = torch.Generator(device="cuda")
+g.manual_seed(7)
+t = torch.rand((n, 3), generator=g, device="cuda", dtype=torch.float64)
+for _ in range(8):  # a queue of torch kernels ahead of the seeds' last write
+    t = torch.sin(t) * 0.5 + 0.5
+host = (t * (2 * math.pi)).float().cpu().numpy()
+seeds_t = (t * (2 * math.pi)).float().contiguous()  # written again, still queued
+a = capi.Job(ctx, d, seeds_t)
+del seeds_t, t
+junk = [torch.full((n, 3), -1e30, device="cuda") for _ in range(4)]  # reuse freed blocks
+b = capi.Job(ctx, d, host)
+for job in (a, b):
+    w = capi.Window()
+    w.wind, w.wind_back, w.wind_front = f.h, 0, 1
+    w.time_back, w.time_front = 0.0, 1.0
+    w.time_back64, w.time_front64 = 0.0, 1.0
+    w.counter_start, w.nsteps = 0, 20
+    job.advance(w)
+pa, pb = a.read(capi.OUT_POSITIONS), b.read(capi.OUT_POSITIONS)
+sa, sb = a.read(capi.OUT_STATUS), b.read(capi.OUT_STATUS)
+print("equal", bool(np.array_equal(pa, pb) and np.array_equal(sa, sb)), int((sb != 0).sum()), flush=True)
+del junk
+a.close()
+b.close()
+f.close()
+ctx.close()
+"""
+
+
+def test_device_seeds_from_torch_equal_host_seeds(gpu_available):
+    """Seeds produced by torch kernels just before gr_job_create, and the tensor dropped and its
+    memory overwritten right after, give the same job as the same seeds from host memory: the
+    binding waits for torch's stream and keeps the tensor until the job is closed. Runs in a
+    process of its own whose HIP runtime torch loads first (as bench.py does)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _TORCH_SEEDS, root], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-3000:]
+    line = [l for l in r.stdout.splitlines() if l.startswith("equal")][-1].split()
+    assert line[1] == "True", r.stdout
