@@ -23,7 +23,7 @@ def _clean_copy(dst):
                          capture_output=True, check=True).stdout.decode()
     n = 0
     for rel in filter(None, out.split("\0")):
-        if rel.startswith(("_work/", "gpurun_out/")) or rel.endswith(".so"):
+        if rel.endswith(".so"):  # ignored working folders are already left out by git
             continue
         src = os.path.join(ROOT, rel)
         if not os.path.isfile(src):
