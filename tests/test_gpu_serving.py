@@ -161,3 +161,53 @@ def test_job_bytes_matches_device_memory(gpu_available):
     for job in alive:
         job.close()
     ctx.close()
+
+
+def test_serving_on_atmosphere_datasets(gpu_available):
+    """The serving configuration's kind of dataset: atmosphere-like FP16 fields on a stretched z
+    axis (48x40x20, hourly slices, dt 60 s), seeds uniform in x-y and log-uniform in height.
+    Every request gives the same result alone and in the mix (jobs cross window edges), and one
+    of them equals the FP32 oracle on the same FP16-rounded slices and levels."""
+    nx, ny, nz, S = 48, 40, 20, 6
+    ctx = capi.Context(0)
+    ds, raw = {}, {}
+    for k in range(2):
+        wind, gs, zlev = synthetic.atmosphere(nx, ny, nz, S, seed=11 + k, dx=3000.0, dz0=20.0, ratio=1.08)
+        f = capi.Field(ctx, nx, ny, nz, 3, capi.F16, nslots=S)
+        f.set_z_levels(zlev)
+        for s in range(S):
+            f.upload(s, [wind[s, c] for c in range(3)])
+        dom = [float(np.float32(nx) * np.float32(gs[0])), float(np.float32(ny) * np.float32(gs[1])), float(zlev[-1])]
+        ds[f"a{k}"] = serving.Dataset(f, list(gs), dom, 3600.0, S, zlev)
+        raw[f"a{k}"] = (wind, gs, zlev, dom)
+
+    def reqs():
+        out = []
+        for i in range(6):
+            wind, gs, zlev, dom = raw[f"a{i % 2}"]
+            n = [500, 6000, 90, 15000, 2500, 4096][i]
+            xy = synthetic.uniform_seeds(40 + i, n, [0.0, 0.0], [dom[0], dom[1]])
+            z = synthetic.loguniform_heights(50 + i, n, float(zlev[1]), float(zlev[-1]))
+            seeds = np.ascontiguousarray(np.concatenate([xy, z[:, None]], axis=1), np.float32)
+            out.append(serving.Request(rid=i, arrival=0.005 * i, dataset=f"a{i % 2}", seeds=seeds,
+                                       steps=[100, 130, 60, 120, 80, 110][i],
+                                       integrator=0 if i == 2 else 1, dt=60.0))
+        return out
+
+    mix = {r.rid: r for r in serving.Server(ctx, ds, quantum=50).run(reqs())}
+    assert len(mix) == 6
+    for r in reqs():
+        alone = serving.Server(ctx, ds, quantum=1000).run([r])[0]
+        assert alone.steps_done == mix[r.rid].steps_done == r.steps
+        np.testing.assert_array_equal(alone.positions, mix[r.rid].positions)
+        np.testing.assert_array_equal(alone.status, mix[r.rid].status)
+    r = reqs()[3]
+    wind, gs, zlev, dom = raw[r.dataset]
+    st = dict(Space=0, Integrator=r.integrator, DeltaT=r.dt, DataTimeDistance=3600.0, GridScale=list(gs),
+              Offset=[0, 0, 0], MaximumSimulationTime=(r.steps - 0.5) * r.dt, SaveInterval=0)
+    ref = oracle.run(st, synthetic.round_fp16(wind), r.seeds, real64=False, zlevels=zlev)
+    assert ref["steps"] == r.steps
+    np.testing.assert_array_equal(mix[r.rid].positions, ref["trajectories"][:, -1].astype(np.float32))
+    np.testing.assert_array_equal(mix[r.rid].status, ref["status"])
+    assert (ref["status"] != 0).sum() > 10 and (ref["status"] == 0).sum() > 10  # stops and survivors
+    ctx.close()
