@@ -139,10 +139,12 @@ __device__ __forceinline__ float load_comp(const void * b, uint32_t dtype, uint6
 __device__ __forceinline__ float lerp(float a, float b, float t) { return __builtin_fmaf(t, b - a, a); }
 
 // texel coordinate -> (index, weight). Clamping to [-1, n] leaves every tap outside the grid
-// and keeps the conversion defined (NaN -> -1).
+// and keeps the conversion defined (NaN -> -1). One v_med3_f32: the median of (a, -1, n) is
+// fminf(fmaxf(a, -1), n) for every a (a NaN input gives the minimum of the other two, -1, as
+// fmaxf drops the NaN first), without the two min/max operations and their canonicalisation.
 __device__ __forceinline__ void split(float a, uint32_t n, uint32_t wmode, int & i, float & w)
 {
-    a = fminf(fmaxf(a, -1.f), float(n));
+    a = __builtin_amdgcn_fmed3f(a, -1.f, float(n));
     const float f = floorf(a);
     i = int(f);
     w = a - f;
