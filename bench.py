@@ -828,12 +828,14 @@ def serving_child(duration):
     cmd = [sys.executable, os.path.abspath(__file__), "--workload", "serving", "--serve-rate", "50",
            "--serve-duration", f"{duration:g}"]
     try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=max(120.0, 6 * duration + 90))
+        # the child's progress lines go straight to this process's stderr; its record to a pipe
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True,
+                           timeout=max(120.0, 6 * duration + 90))
     except subprocess.TimeoutExpired:
         return {"error": "timed out", "command": " ".join(cmd[1:])}
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
     if r.returncode != 0 or not lines:
-        return {"error": f"exit {r.returncode}: {r.stderr[-500:]}", "command": " ".join(cmd[1:])}
+        return {"error": f"exit {r.returncode}", "command": " ".join(cmd[1:])}
     rec = json.loads(lines[-1])
     rec["command"] = "python3 bench.py " + " ".join(cmd[2:])
     return rec
