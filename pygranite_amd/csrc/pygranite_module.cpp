@@ -335,6 +335,8 @@ struct CtxHolder
         if (n == 0)
             throw std::runtime_error("ERROR < RUNTIME | pygranite_amd needs a HIP GPU (none visible) >");
         check(gr_ctx_create(0, &c));
+        // one job per integrator: no later job would take a destroyed one's arrays
+        check(gr_ctx_set_option(c, GR_CTX_POOL_BYTES, 0));
     }
     CtxHolder(const CtxHolder &) = delete;
     CtxHolder & operator=(const CtxHolder &) = delete;
