@@ -713,3 +713,47 @@ def test_batch_per_variant_kernel_equals_kernel_of_every_variant(gpu_available, 
     b.close()
     f.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("dtype,integ", [(capi.F32, 1), (capi.F16, 0), (capi.BF16, 1)])
+def test_batch_per_variant_kernel_2d(gpu_available, dtype, integ):
+    """The 2D per-variant kernels (double gyre, clamp addressing): two jobs of one variant run
+    their own kernel or the kernel of every variant with equal results, in a wide launch and in
+    a launch of few live particles (quad items), and equal the FP32 oracle."""
+    g2, gs2 = synthetic.double_gyre(96, 48, 4)
+    ctx = capi.Context(0)
+    f = capi.Field(ctx, 96, 48, 1, 2, dtype, nslots=4, address=1)
+    for s in range(4):
+        f.upload(s, [g2[s, c] for c in range(2)])
+    seeds = synthetic.uniform_seeds(61, 50000, [0.02, 0.02], [1.98, 0.98])
+    d = make_desc(2, integ, gs2, g2.shape, 0.01)
+
+    def run(uniform, quad):
+        ctx.set_option(capi.CTX_BATCH_UNIFORM, uniform)
+        jobs = [capi.Job(ctx, d, seeds[:30000]), capi.Job(ctx, d, seeds[30000:])]
+        for j in jobs:
+            j.set_option(capi.OPT_QUAD, (1 << 29) - 1 if quad else 0)
+        ctx.batch_run([(j, f, dict(max_steps=250)) for j in jobs])
+        kernel, nq = ctx.get_option(capi.CTX_BATCH_LAST_KERNEL), ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS)
+        out = [x for j in jobs for x in (j.read(capi.OUT_POSITIONS), j.read(capi.OUT_STATUS), j.read(capi.OUT_ABORT_STEP))]
+        for j in jobs:
+            j.close()
+        return kernel, nq, out
+
+    ref = None
+    for uniform in (1, 0):
+        for quad in (False, True):
+            kernel, nq, out = run(uniform, quad)
+            assert kernel == (4 if uniform else 3)
+            assert (nq > 0) == quad
+            if ref is None:
+                ref = out
+            for a, b in zip(ref, out):
+                np.testing.assert_array_equal(a, b)
+    st = oracle_settings(2, integ, gs2, 0.01, 2.495)
+    want = oracle.run(st, ROUND[dtype](g2), seeds, real64=False)
+    assert want["steps"] == 250
+    np.testing.assert_array_equal(np.concatenate([ref[0], ref[3]]), want["trajectories"][:, -1].astype(np.float32))
+    np.testing.assert_array_equal(np.concatenate([ref[1], ref[4]]), want["status"])
+    f.close()
+    ctx.close()
