@@ -211,3 +211,23 @@ def test_device_seeds_from_torch_equal_host_seeds(gpu_available):
     assert r.returncode == 0, r.stderr[-3000:]
     line = [l for l in r.stdout.splitlines() if l.startswith("equal")][-1].split()
     assert line[1] == "True", r.stdout
+
+
+def test_context_option_validation(gpu_available):
+    """Context options reject values outside their ranges and the read-only ones reject writes;
+    defaults as documented."""
+    ctx = capi.Context(0)
+    assert ctx.get_option(capi.CTX_BATCH_UNIFORM) == 1
+    assert ctx.get_option(capi.CTX_POOL_BYTES) == 8 << 30
+    assert ctx.get_option(capi.CTX_POOL_CACHED) == 0
+    assert ctx.get_option(capi.CTX_BATCH_LAST_KERNEL) == 0
+    assert ctx.get_option(capi.CTX_BATCH_QUAD_ITEMS) == 0
+    for opt, bad in ((capi.CTX_BATCH_UNIFORM, 2), (capi.CTX_POOL_BYTES, -1), (capi.CTX_BATCH_LAST_KERNEL, 0),
+                     (capi.CTX_BATCH_QUAD_ITEMS, 0), (capi.CTX_POOL_CACHED, 0), (99, 0)):
+        with pytest.raises(capi.GraniteError):
+            ctx.set_option(opt, bad)
+    ctx.set_option(capi.CTX_BATCH_UNIFORM, 0)
+    assert ctx.get_option(capi.CTX_BATCH_UNIFORM) == 0
+    with pytest.raises(capi.GraniteError):
+        ctx.get_option(99)
+    ctx.close()
